@@ -1,0 +1,282 @@
+/*
+ * qiron.h — C ABI of the MI355X (gfx950) state-vector engine.
+ *
+ * This is the drop-in boundary for quant-iron's state-vector hot path. It
+ * replaces the OpenCL seam `execute_on_gpu` (src/components/operator.rs:24-148)
+ * and the CPU bodies behind it, and exports one entry point per public
+ * operation of that path:
+ *   - the `State` gate methods (src/components/state.rs:1019-2345) and the
+ *     `Operator::apply` impls behind them (src/components/operator.rs:279-2285),
+ *   - `Circuit::execute` / `Gate::apply` (src/circuit.rs:160-172,
+ *     src/components/gate.rs:99-122),
+ *   - `PauliString` / `SumOp` apply, exp and expectation
+ *     (src/components/pauli_string.rs:139-287, 453-507),
+ *   - `State::measure` / `measure_n` (src/components/state.rs:525-784).
+ *
+ * Conventions (all follow the reference):
+ *   - A state is 2^n complex amplitudes, interleaved (re, im); qubit k is bit k
+ *     of the amplitude index (little-endian).
+ *   - Errors mirror `enum Error` (src/errors.rs:4-97): every call returns a
+ *     `qi_status` and, when `err` is non-NULL, fills the variant's payload.
+ *   - The reference returns a new `State` from every operation; the engine is
+ *     in place on a device-resident handle. `qi_state_clone` is the explicit copy.
+ *   - All calls are synchronous with respect to the host unless stated.
+ *   - Calls on one context are serialised by a per-context mutex (the reference
+ *     serialises all GPU work through one global mutex, operator.rs:32).
+ *
+ * Plain C: no C++ or torch types cross this boundary.
+ */
+#ifndef QIRON_H
+#define QIRON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QI_ABI_VERSION 1
+#define QI_MAX_TARGETS 4
+#define QI_MAX_CONTROLS 62
+#define QI_MAX_QUBITS 40
+
+/* ---- status codes: one per `enum Error` variant (src/errors.rs:4-97) ---- */
+typedef enum qi_status {
+    QI_OK = 0,
+    QI_ERR_INVALID_NUMBER_OF_MEASUREMENTS = 1, /* InvalidNumberOfMeasurements(a)      */
+    QI_ERR_OVERLAPPING_CONTROL_AND_TARGET = 2, /* OverlappingControlAndTargetQubits(a,b) */
+    QI_ERR_INVALID_NUMBER_OF_QUBITS = 3,       /* InvalidNumberOfQubits(a)            */
+    QI_ERR_INVALID_QUBIT_INDEX = 4,            /* InvalidQubitIndex(a, b)             */
+    QI_ERR_STATE_NOT_NORMALISED = 5,           /* StateVectorNotNormalised            */
+    QI_ERR_NON_UNITARY_MATRIX = 6,             /* NonUnitaryMatrix                    */
+    QI_ERR_INVALID_NUMBER_OF_INPUTS = 7,       /* InvalidNumberOfInputs(a, b)         */
+    QI_ERR_MISMATCHED_PARAMETERS = 8,          /* MismatchedNumberOfParameters{a,b}   */
+    QI_ERR_UNKNOWN = 9,                        /* UnknownError                        */
+    QI_ERR_DEVICE = 10,                        /* replaces OpenCLError(String)        */
+    QI_ERR_LOCK = 11,                          /* GpuContextLockError                 */
+    QI_ERR_CIRCUIT_MACRO = 12,                 /* CircuitMacroError(String)           */
+    QI_ERR_INVALID_INPUT_VALUE = 13,           /* InvalidInputValue(a)                */
+    QI_ERR_ZERO_NORM = 14,                     /* ZeroNorm                            */
+    QI_ERR_INVALID_PAULI_COEFFICIENT = 15,     /* InvalidPauliStringCoefficient(re,im)*/
+    /* engine-only codes (no reference variant) */
+    QI_ERR_OUT_OF_MEMORY = 16,                 /* device allocation failed            */
+    QI_ERR_INVALID_ARGUMENT = 17,              /* NULL pointer, bad enum, size limit   */
+    QI_ERR_PRECISION_MISMATCH = 18             /* two states of different precision   */
+} qi_status;
+
+typedef struct qi_error {
+    int32_t code;     /* qi_status                                        */
+    uint64_t a, b;    /* usize payloads, in the order of the Rust variant */
+    double re, im;    /* Complex payload (InvalidPauliStringCoefficient)  */
+    char msg[256];    /* Display text of the variant (errors.rs #[error]) */
+} qi_error;
+
+typedef struct qi_ctx_s *qi_ctx_t;
+typedef struct qi_state_s *qi_state_t;
+
+typedef enum qi_prec { QI_C64 = 0, QI_C128 = 1 } qi_prec;
+
+typedef enum qi_init {
+    QI_INIT_ZERO = 0,  /* State::new_zero  (state.rs:165-178) */
+    QI_INIT_BASIS = 1, /* State::new_basis_n (state.rs:194-210) */
+    QI_INIT_PLUS = 2,  /* State::new_plus  (state.rs:225-237) */
+    QI_INIT_MINUS = 3, /* State::new_minus (state.rs:240-275) */
+    QI_INIT_GHZ = 4    /* State::new_ghz   (state.rs:277-300) */
+} qi_init;
+
+/* Gate kinds: one per `Operator` impl (operator.rs) plus the non-operator
+ * `Gate` variants (gate.rs:13-52). */
+typedef enum qi_gate_kind {
+    QI_GATE_H = 0,         /* Hadamard      operator.rs:279-439  */
+    QI_GATE_X = 1,         /* Pauli::X      operator.rs:443-637  */
+    QI_GATE_Y = 2,         /* Pauli::Y      operator.rs:443-637  */
+    QI_GATE_Z = 3,         /* Pauli::Z      operator.rs:443-637  */
+    QI_GATE_I = 4,         /* Identity      operator.rs:1096-1138 */
+    QI_GATE_S = 5,         /* PhaseS        operator.rs:1144-1235 */
+    QI_GATE_T = 6,         /* PhaseT        operator.rs:1237-1334 */
+    QI_GATE_SDAG = 7,      /* PhaseSdag     operator.rs:1336-1427 */
+    QI_GATE_TDAG = 8,      /* PhaseTdag     operator.rs:1429-1526 */
+    QI_GATE_P = 9,         /* PhaseShift(params[0]) operator.rs:1528-1643 */
+    QI_GATE_RX = 10,       /* RotateX(params[0])  operator.rs:1645-1786 */
+    QI_GATE_RY = 11,       /* RotateY(params[0])  operator.rs:1788-1927 */
+    QI_GATE_RZ = 12,       /* RotateZ(params[0])  operator.rs:1929-2056 */
+    QI_GATE_U2 = 13,       /* Unitary2(matrix)    operator.rs:2058-2284 */
+    QI_GATE_CNOT = 14,     /* CNOT          operator.rs:643-700   */
+    QI_GATE_SWAP = 15,     /* SWAP          operator.rs:706-835   */
+    QI_GATE_MATCHGATE = 16,/* Matchgate(theta, phi1, phi2) operator.rs:852-1025 */
+    QI_GATE_TOFFOLI = 17,  /* Toffoli       operator.rs:1031-1090 */
+    QI_GATE_MEASURE = 18,  /* Gate::Measurement (gate.rs:103-105)   */
+    QI_GATE_PAULI_STRING = 19,    /* Gate::PauliString -> apply_normalised (gate.rs:115-117) */
+    QI_GATE_PAULI_EVOLUTION = 20, /* Gate::PauliTimeEvolution -> apply_exp_neg_i_dt (gate.rs:118-120) */
+    QI_GATE_KIND_COUNT = 21
+} qi_gate_kind;
+
+typedef enum qi_basis {
+    QI_BASIS_COMPUTATIONAL = 0, /* MeasurementBasis::Computational (measurement.rs:76-86) */
+    QI_BASIS_X = 1,
+    QI_BASIS_Y = 2,
+    QI_BASIS_CUSTOM = 3         /* MeasurementBasis::Custom(matrix) */
+} qi_basis;
+
+/* One gate = one `Gate` value. `targets`/`controls` carry the index slices of
+ * `Gate::Operator(op, targets, controls)`; `ntargets` may exceed what the
+ * operator expects (validation then reports InvalidNumberOfQubits(ntargets),
+ * as validate_qubits does, operator.rs:214-273), but at most QI_MAX_TARGETS
+ * entries are read.
+ *   QI_GATE_MEASURE:  controls[0..ncontrols) are the measured qubits (empty =
+ *                     all), `basis`, custom basis in `matrix`, and the injected
+ *                     uniform draw in params[0] (replaces rand::rng(), state.rs:601).
+ *   QI_GATE_PAULI_STRING / QI_GATE_PAULI_EVOLUTION: x_mask/z_mask (Y = X&Z),
+ *                     coefficient (params[0], params[1]); evolution time params[2].
+ *   QI_GATE_U2:       matrix = [[m00, m01], [m10, m11]] row-major, each (re, im). */
+typedef struct qi_gate {
+    int32_t kind;
+    uint32_t ntargets;
+    uint32_t ncontrols;
+    uint32_t basis;
+    uint64_t targets[QI_MAX_TARGETS];
+    uint64_t controls[QI_MAX_CONTROLS];
+    double params[3];
+    double matrix[8];
+    uint64_t x_mask, z_mask;
+} qi_gate;
+
+/* PauliString as masks: qubit q carries X if x&~z, Z if z&~x, Y if x&z; the
+ * product runs over distinct qubits, so the HashMap iteration order of the
+ * reference (pauli_string.rs:147) cannot change the result. */
+typedef struct qi_pauli_term {
+    uint64_t x_mask, z_mask;
+    double c_re, c_im;
+} qi_pauli_term;
+
+/* Executor options for qi_apply_gates. */
+typedef enum qi_exec_mode {
+    QI_EXEC_PER_GATE = 0,  /* one streaming kernel per gate                  */
+    QI_EXEC_TILED = 1,     /* LDS-tiled passes: all gates on a qubit set per HBM pass */
+    QI_EXEC_FUSED_MFMA = 2,/* dense 2^k x 2^k blocks applied as complex GEMM on MFMA */
+    QI_EXEC_AUTO = 3       /* tiled unless the state is too small for tiles   */
+} qi_exec_mode;
+
+typedef struct qi_exec_opts {
+    int32_t mode;        /* qi_exec_mode                                    */
+    int32_t tile_qubits; /* LDS tile width k for QI_EXEC_TILED (0 = default) */
+    int32_t block_qubits;/* dense block width k (<= 6) for QI_EXEC_FUSED_MFMA */
+    int32_t use_graph;   /* capture the launch sequence in a hipGraph         */
+} qi_exec_opts;
+
+/* ---------------- context ---------------- */
+/* Replaces GpuContext::new + the global Lazy<Mutex<..>> (gpu_context.rs:94-168). */
+qi_status qi_ctx_create(int device, qi_ctx_t *out, qi_error *err);
+qi_status qi_ctx_destroy(qi_ctx_t ctx);
+qi_status qi_ctx_synchronize(qi_ctx_t ctx, qi_error *err);
+/* Device name and HBM size, for reports. */
+qi_status qi_ctx_info(qi_ctx_t ctx, char *name, size_t name_len, uint64_t *hbm_bytes,
+                      int *cu_count, qi_error *err);
+
+/* ---------------- states ---------------- */
+/* Allocates 2^n amplitudes (8 or 16 bytes each) on the context's device. */
+qi_status qi_state_create(qi_ctx_t ctx, uint32_t n, qi_prec prec, qi_state_t *out, qi_error *err);
+qi_status qi_state_destroy(qi_state_t s);
+/* `State::clone` (derive(Clone), state.rs:74). */
+qi_status qi_state_clone(qi_state_t src, qi_state_t *out, qi_error *err);
+/* dst <- src (same n and precision). */
+qi_status qi_state_copy(qi_state_t dst, qi_state_t src, qi_error *err);
+uint32_t qi_state_num_qubits(qi_state_t s);
+int32_t qi_state_precision(qi_state_t s);
+/* Device address of the amplitude array (interop; owned by the handle). */
+void *qi_state_device_ptr(qi_state_t s);
+/* Constructors: new_zero / new_basis_n / new_plus / new_minus / new_ghz
+ * (state.rs:165-300); `basis` is used by QI_INIT_BASIS only. */
+qi_status qi_state_init(qi_state_t s, qi_init kind, uint64_t basis, qi_error *err);
+/* Host -> device from 2^(n+1) doubles. check_norm != 0 applies `State::new`'s
+ * normalisation check (|sum|a|^2 - 1| <= f64::EPSILON * len, state.rs:99-127). */
+qi_status qi_state_upload(qi_state_t s, const double *amps, int check_norm, qi_error *err);
+/* Device -> host into 2^(n+1) doubles (c64 amplitudes are widened). */
+qi_status qi_state_download(qi_state_t s, double *amps, qi_error *err);
+
+/* ---------------- gates ---------------- */
+/* `Operator::apply` for one gate, in place (operator.rs:151-190 and the impls
+ * it dispatches to; also `Gate::apply`, gate.rs:99-122, for the non-operator
+ * kinds). Validation order and payloads follow validate_qubits
+ * (operator.rs:214-273) and the per-gate checks (CNOT :674-679, Toffoli
+ * :1064-1072, Matchgate :903-905, Unitary2::new :2092-2118). */
+qi_status qi_apply_gate(qi_state_t s, const qi_gate *g, qi_error *err);
+/* `Circuit::execute` (circuit.rs:160-172): all gates in list order, in place,
+ * no per-gate host synchronisation. `n_circuit` is the circuit's qubit count
+ * (InvalidNumberOfQubits(n_state) when it differs). opts may be NULL
+ * (= QI_EXEC_AUTO). The list is validated before any amplitude changes. */
+qi_status qi_apply_gates(qi_state_t s, uint32_t n_circuit, const qi_gate *gates, size_t ngates,
+                         const qi_exec_opts *opts, qi_error *err);
+
+/* ---------------- Pauli algebra ---------------- */
+/* out <- PauliString::apply(in) (pauli_string.rs:139-151). in == out allowed. */
+qi_status qi_pauli_apply(qi_state_t in, qi_state_t out, const qi_pauli_term *t, qi_error *err);
+/* out <- PauliString::apply_normalised(in) (pauli_string.rs:165-168). */
+qi_status qi_pauli_apply_normalised(qi_state_t in, qi_state_t out, const qi_pauli_term *t,
+                                    qi_error *err);
+/* s <- PauliString::apply_exp_factor(s, factor) (pauli_string.rs:237-262);
+ * factor = 1 gives apply_exp (:199-223). */
+qi_status qi_pauli_exp(qi_state_t s, const qi_pauli_term *t, double factor_re, double factor_im,
+                       qi_error *err);
+/* s <- PauliString::apply_exp_neg_i_dt(s, dt) (pauli_string.rs:281-287). */
+qi_status qi_pauli_exp_neg_i_dt(qi_state_t s, const qi_pauli_term *t, double dt, qi_error *err);
+/* out <- SumOp::apply(in) (pauli_string.rs:453-466). in != out. */
+qi_status qi_sumop_apply(qi_state_t in, qi_state_t out, const qi_pauli_term *terms, size_t nterms,
+                         qi_error *err);
+/* SumOp::expectation_value (pauli_string.rs:485-507): FP64 compensated sums. */
+qi_status qi_sumop_expectation(qi_state_t s, const qi_pauli_term *terms, size_t nterms,
+                               double *re, double *im, qi_error *err);
+/* trotter_evolve_state (time_evolution.rs:140-166); order 1 = First, 2 = Second. */
+qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt,
+                            uint64_t steps, int order, qi_error *err);
+
+/* ---------------- reductions and arithmetic (FP64 accumulation) ---------------- */
+/* State::inner_product (state.rs:890-917): sum conj(a_i) b_i. */
+qi_status qi_inner_product(qi_state_t a, qi_state_t b, double *re, double *im, qi_error *err);
+/* sum |a_i|^2 */
+qi_status qi_norm_sqr(qi_state_t s, double *out, qi_error *err);
+/* State::normalise (state.rs:924-945), in place. */
+qi_status qi_normalise(qi_state_t s, qi_error *err);
+/* State::fs_fidelity (state.rs:492-498). */
+qi_status qi_fidelity(qi_state_t a, qi_state_t b, double *out, qi_error *err);
+/* State * Complex (state.rs:2687-2705), in place. */
+qi_status qi_state_scale(qi_state_t s, double re, double im, qi_error *err);
+/* y <- y + alpha * x (State Add, state.rs:2773-2800, with a scalar). */
+qi_status qi_state_axpy(qi_state_t y, double alpha_re, double alpha_im, qi_state_t x,
+                        qi_error *err);
+
+/* ---------------- measurement ---------------- */
+/* Outcome probabilities over `qubits` (bit b of the outcome = qubit qubits[b]),
+ * the histogram of State::measure (state.rs:559-588), FP64, 2^m entries. */
+qi_status qi_measure_probs(qi_state_t s, const uint64_t *qubits, uint32_t m, double *probs,
+                           qi_error *err);
+/* State::measure (state.rs:525-730) with the uniform draw injected:
+ * outcome = first i with u < C_i (sequential cumsum of normalised
+ * probabilities), else the last bin. The collapsed state is written to
+ * `out_state` (may equal s); `*outcome` holds the outcome bits. */
+qi_status qi_measure(qi_state_t s, qi_basis basis, const double *custom_u, const uint64_t *qubits,
+                     uint32_t m, double uniform, qi_state_t out_state, uint64_t *outcome,
+                     qi_error *err);
+/* State::measure_n (state.rs:750-784): `shots` independent measurements of the
+ * same state with injected uniforms; outcomes only (the collapsed state of a
+ * given outcome is qi_measure_collapse). */
+qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u,
+                       const uint64_t *qubits, uint32_t m, const double *uniforms, size_t shots,
+                       uint64_t *outcomes, qi_error *err);
+/* Collapse onto a given computational outcome and renormalise (state.rs:622-654). */
+qi_status qi_measure_collapse(qi_state_t src, qi_state_t dst, const uint64_t *qubits, uint32_t m,
+                              uint64_t outcome, qi_error *err);
+
+/* ---------------- timing hooks (for benchmarks) ---------------- */
+/* Device time of the last qi_apply_gates call's launch sequence, from HIP
+ * events recorded on the context's stream (milliseconds). */
+qi_status qi_last_exec_ms(qi_ctx_t ctx, double *ms, qi_error *err);
+/* Number of launch steps (kernels or host-orchestrated gates) the last
+ * qi_apply_gates call planned. */
+qi_status qi_last_exec_steps(qi_ctx_t ctx, uint64_t *steps, qi_error *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QIRON_H */

@@ -1,0 +1,890 @@
+// api.hip — the C ABI (include/qiron.h): validation, error payloads, state
+// handles and the host-side orchestration of the kernels. Every entry point
+// locks its context's mutex and runs on the context's single in-order stream.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+
+#include "qi_internal.h"
+#include "exec_plan.h"
+
+namespace qi {
+
+// ----------------------------------------------------------------- errors --
+// Display strings of errors.rs:4-97.
+Status make_err(int code, uint64_t a, uint64_t b, double re, double im, const char *detail) {
+    Status s;
+    s.code = code; s.a = a; s.b = b; s.re = re; s.im = im;
+    char buf[256];
+    switch (code) {
+    case QI_ERR_INVALID_NUMBER_OF_MEASUREMENTS: snprintf(buf, sizeof buf, "Invalid number of measurements: %llu", (unsigned long long)a); break;
+    case QI_ERR_OVERLAPPING_CONTROL_AND_TARGET:
+        snprintf(buf, sizeof buf, "Control qubit index %llu overlaps with target qubit index %llu", (unsigned long long)a, (unsigned long long)b); break;
+    case QI_ERR_INVALID_NUMBER_OF_QUBITS: snprintf(buf, sizeof buf, "Invalid number of qubits: %llu", (unsigned long long)a); break;
+    case QI_ERR_INVALID_QUBIT_INDEX:
+        snprintf(buf, sizeof buf, "Invalid qubit index: %llu for %llu qubits", (unsigned long long)a, (unsigned long long)b); break;
+    case QI_ERR_STATE_NOT_NORMALISED: snprintf(buf, sizeof buf, "State vector is not normalised"); break;
+    case QI_ERR_NON_UNITARY_MATRIX: snprintf(buf, sizeof buf, "Non-unitary matrix"); break;
+    case QI_ERR_UNKNOWN: snprintf(buf, sizeof buf, "An unknown error occurred"); break;
+    case QI_ERR_DEVICE: snprintf(buf, sizeof buf, "Device error: %s", detail ? detail : "unknown"); break;
+    case QI_ERR_LOCK: snprintf(buf, sizeof buf, "Failed to lock GPU context"); break;
+    case QI_ERR_ZERO_NORM: snprintf(buf, sizeof buf, "The state cannot be normalised because it has zero norm."); break;
+    case QI_ERR_INVALID_PAULI_COEFFICIENT: snprintf(buf, sizeof buf, "Invalid Pauli String coefficient: %g+%gi", re, im); break;
+    case QI_ERR_OUT_OF_MEMORY: snprintf(buf, sizeof buf, "Device out of memory%s%s", detail ? ": " : "", detail ? detail : ""); break;
+    case QI_ERR_INVALID_ARGUMENT: snprintf(buf, sizeof buf, "Invalid argument%s%s", detail ? ": " : "", detail ? detail : ""); break;
+    case QI_ERR_PRECISION_MISMATCH: snprintf(buf, sizeof buf, "States differ in precision"); break;
+    default: snprintf(buf, sizeof buf, "error %d", code); break;
+    }
+    s.msg = buf;
+    return s;
+}
+
+void fill_err(qi_error *err, const Status &st) {
+    if (!err) return;
+    memset(err, 0, sizeof(*err));
+    err->code = st.code; err->a = st.a; err->b = st.b; err->re = st.re; err->im = st.im;
+    snprintf(err->msg, sizeof err->msg, "%s", st.msg.c_str());
+}
+
+Status ensure(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return Status{};
+    if (b.p) {
+        QI_HIP_TRY(hipDeviceSynchronize());  // earlier launches may still read it
+        QI_HIP_TRY(hipFree(b.p));
+        b.p = nullptr; b.bytes = 0;
+    }
+    size_t want = bytes < 4096 ? 4096 : bytes;
+    hipError_t e = hipMalloc(&b.p, want);
+    if (e != hipSuccess) { b.p = nullptr; return make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)); }
+    b.bytes = want;
+    return Status{};
+}
+
+// ------------------------------------------------------------- validation --
+// validate_qubits, operator.rs:214-273.
+static Status validate_qubits(uint32_t n, const uint64_t *t, uint32_t nt, const uint64_t *c, uint32_t nc, uint32_t expected) {
+    if (nt != expected) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, nt);
+    for (uint32_t i = 0; i < nt; i++)
+        if (t[i] >= n) return make_err(QI_ERR_INVALID_QUBIT_INDEX, t[i], n);
+    for (uint32_t j = 0; j < nc; j++) {
+        if (c[j] >= n) return make_err(QI_ERR_INVALID_QUBIT_INDEX, c[j], n);
+        for (uint32_t i = 0; i < nt; i++)
+            if (c[j] == t[i]) return make_err(QI_ERR_OVERLAPPING_CONTROL_AND_TARGET, c[j], t[i]);
+    }
+    if (expected > 1)
+        for (uint32_t i = 0; i < nt; i++)
+            for (uint32_t j = i + 1; j < nt; j++)
+                if (t[i] == t[j]) return make_err(QI_ERR_INVALID_QUBIT_INDEX, t[i], n);
+    return Status{};
+}
+
+// Unitary2::new, operator.rs:2092-2118 (tolerance 2 * f64::EPSILON).
+static Status check_unitary(const double *m) {
+    const double tol = DBL_EPSILON * 2.0;
+    double ar = m[0], ai = m[1], br = m[2], bi = m[3], cr = m[4], ci = m[5], dr = m[6], di = m[7];
+    double n0 = (ar * ar + ai * ai) + (br * br + bi * bi);
+    double n1 = (cr * cr + ci * ci) + (dr * dr + di * di);
+    if (std::fabs(n0 - 1.0) > tol || std::fabs(n1 - 1.0) > tol) return make_err(QI_ERR_NON_UNITARY_MATRIX);
+    // a * conj(c) + b * conj(d)
+    double xr = (ar * cr - ai * -ci) + (br * dr - bi * -di);
+    double xi = (ar * -ci + ai * cr) + (br * -di + bi * dr);
+    if (xr * xr + xi * xi > tol * tol) return make_err(QI_ERR_NON_UNITARY_MATRIX);
+    return Status{};
+}
+
+static Status check_mask(uint32_t n, uint64_t mask) {
+    uint64_t bad = n >= 64 ? 0 : mask >> n;
+    if (bad) return make_err(QI_ERR_INVALID_QUBIT_INDEX, n + (uint64_t)__builtin_ctzll(bad), n);
+    return Status{};
+}
+
+// State::measure argument checks, state.rs:525-557.
+static Status check_measure(uint32_t n, const uint64_t *q, uint32_t m) {
+    if (m > n) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, n);
+    for (uint32_t i = 0; i < m; i++)
+        if (q[i] >= n) return make_err(QI_ERR_INVALID_QUBIT_INDEX, q[i], n);
+    return Status{};
+}
+
+static Status validate_gate(uint32_t n, const qi_gate &g) {
+    uint32_t nt = g.ntargets, nc = g.ncontrols;
+    if (nc > QI_MAX_CONTROLS) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "too many controls");
+    Status s;
+    switch (g.kind) {
+    case QI_GATE_H: case QI_GATE_X: case QI_GATE_Y: case QI_GATE_Z: case QI_GATE_I: case QI_GATE_S:
+    case QI_GATE_T: case QI_GATE_SDAG: case QI_GATE_TDAG: case QI_GATE_P: case QI_GATE_RX: case QI_GATE_RY:
+    case QI_GATE_RZ:
+        return validate_qubits(n, g.targets, nt, g.controls, nc, 1);
+    case QI_GATE_U2:
+        if (!(s = check_unitary(g.matrix)).ok()) return s;
+        return validate_qubits(n, g.targets, nt, g.controls, nc, 1);
+    case QI_GATE_CNOT:  // operator.rs:674-679
+        if (!(s = validate_qubits(n, g.targets, nt, g.controls, nc, 1)).ok()) return s;
+        if (nc != 1) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, nc);
+        return Status{};
+    case QI_GATE_TOFFOLI:  // operator.rs:1062-1072
+        if (!(s = validate_qubits(n, g.targets, nt, g.controls, nc, 1)).ok()) return s;
+        if (nc != 2 || g.controls[0] == g.controls[1]) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, nc);
+        return Status{};
+    case QI_GATE_SWAP:
+        return validate_qubits(n, g.targets, nt, g.controls, nc, 2);
+    case QI_GATE_MATCHGATE:  // operator.rs:900-905
+        if (!(s = validate_qubits(n, g.targets, nt, g.controls, nc, 1)).ok()) return s;
+        if (g.targets[0] == uint64_t(n) - 1) return make_err(QI_ERR_INVALID_QUBIT_INDEX, g.targets[0], n);
+        return Status{};
+    case QI_GATE_MEASURE:
+        if (g.basis > QI_BASIS_CUSTOM) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "measurement basis");
+        if (g.basis == QI_BASIS_CUSTOM && !(s = check_unitary(g.matrix)).ok()) return s;
+        if (!(g.params[0] >= 0.0 && g.params[0] < 1.0)) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)");
+        return check_measure(n, g.controls, nc);
+    case QI_GATE_PAULI_STRING:
+        return check_mask(n, g.x_mask | g.z_mask);
+    case QI_GATE_PAULI_EVOLUTION:  // pauli_string.rs:281-287
+        if (g.params[1] != 0.0) return make_err(QI_ERR_INVALID_PAULI_COEFFICIENT, 0, 0, g.params[0], g.params[1]);
+        return check_mask(n, g.x_mask | g.z_mask);
+    default:
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "unknown gate kind");
+    }
+}
+
+static bool is_operator(int kind) { return kind >= QI_GATE_H && kind <= QI_GATE_TOFFOLI; }
+
+// ------------------------------------------------------------ complex math --
+struct cxd { double re, im; };
+static cxd cmul(cxd a, cxd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+// num-complex exp / cosh / sinh (from_polar form)
+static cxd cexp_(cxd z) { double e = std::exp(z.re); return {e * std::cos(z.im), e * std::sin(z.im)}; }
+static cxd ccosh_(cxd z) { return {std::cosh(z.re) * std::cos(z.im), std::sinh(z.re) * std::sin(z.im)}; }
+static cxd csinh_(cxd z) { return {std::sinh(z.re) * std::cos(z.im), std::cosh(z.re) * std::sin(z.im)}; }
+
+// ---------------------------------------------------------------- helpers --
+static Status norm_sqr(State &s, double *out) {
+    double r[2];
+    Status e = reduce_state(s, nullptr, RK_NORM, r, s.ctx->stream);
+    *out = r[0];
+    return e;
+}
+
+// State::normalise, state.rs:924-945 (norm in FP64 on the device).
+static Status normalise(State &s) {
+    double ns;
+    Status e = norm_sqr(s, &ns);
+    if (!e.ok()) return e;
+    double norm = std::sqrt(ns);
+    if (norm == 0.0) return make_err(QI_ERR_ZERO_NORM);
+    if (norm == 1.0) return Status{};
+    return launch_div_real(s, norm, s.ctx->stream);
+}
+
+// State::new's check (state.rs:117-122) on a device state.
+static Status check_normalised(State &s) {
+    double ns;
+    Status e = norm_sqr(s, &ns);
+    if (!e.ok()) return e;
+    double tol = DBL_EPSILON * double(uint64_t(1) << s.n);
+    if (std::fabs(ns - 1.0) > tol) return make_err(QI_ERR_STATE_NOT_NORMALISED);
+    return Status{};
+}
+
+static Status alloc_state(Ctx &C, uint32_t n, int prec, State &out) {
+    out.ctx = &C; out.n = n; out.prec = prec; out.d = nullptr;
+    hipError_t e = hipMalloc(&out.d, out.bytes());
+    if (e != hipSuccess) { out.d = nullptr; return make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)); }
+    return Status{};
+}
+
+struct TmpState {
+    State s;
+    ~TmpState() { if (s.d) (void)hipFree(s.d); }
+};
+
+static Status apply_1q_list(State &s, int kind, const uint64_t *q, uint32_t m, const double *mat) {
+    for (uint32_t k = 0; k < m; k++) {
+        qi_gate g{};
+        g.kind = kind; g.ntargets = 1; g.targets[0] = q[k];
+        if (mat) memcpy(g.matrix, mat, sizeof g.matrix);
+        Lowered L;
+        Status e = lower_gate(s.n, g, L);
+        if (!e.ok()) return e;
+        if (!(e = launch_gate(s, L, s.ctx->stream)).ok()) return e;
+    }
+    return Status{};
+}
+
+// The basis change before and after a computational measurement
+// (state.rs:670-728).
+static Status basis_in(State &w, int basis, const double *u, const uint64_t *q, uint32_t m) {
+    if (basis == QI_BASIS_X) return apply_1q_list(w, QI_GATE_H, q, m, nullptr);
+    if (basis == QI_BASIS_Y) {
+        Status e = apply_1q_list(w, QI_GATE_SDAG, q, m, nullptr);
+        if (!e.ok()) return e;
+        return apply_1q_list(w, QI_GATE_H, q, m, nullptr);
+    }
+    if (basis == QI_BASIS_CUSTOM) return apply_1q_list(w, QI_GATE_U2, q, m, u);
+    return Status{};
+}
+static Status basis_out(State &w, int basis, const double *u, const uint64_t *q, uint32_t m) {
+    if (basis == QI_BASIS_X) return apply_1q_list(w, QI_GATE_H, q, m, nullptr);
+    if (basis == QI_BASIS_Y) {
+        Status e = apply_1q_list(w, QI_GATE_H, q, m, nullptr);
+        if (!e.ok()) return e;
+        return apply_1q_list(w, QI_GATE_S, q, m, nullptr);
+    }
+    if (basis == QI_BASIS_CUSTOM) {  // calculate_adjoint: conjugate transpose
+        double ud[8] = {u[0], -u[1], u[4], -u[5], u[2], -u[3], u[6], -u[7]};
+        Status e = check_unitary(ud);
+        if (!e.ok()) return e;
+        return apply_1q_list(w, QI_GATE_U2, q, m, ud);
+    }
+    return Status{};
+}
+
+// State::measure with an injected uniform; `dst` may be `s`.
+static Status measure_one(State &s, int basis, const double *u, const uint64_t *qubits, uint32_t m, double uniform,
+                          State &dst, uint64_t *outcome) {
+    std::vector<uint64_t> all;
+    if (m == 0) { for (uint32_t i = 0; i < s.n; i++) all.push_back(i); qubits = all.data(); m = s.n; }
+    Status e = check_measure(s.n, qubits, m);
+    if (!e.ok()) return e;
+    if (basis == QI_BASIS_CUSTOM && !(e = check_unitary(u)).ok()) return e;
+    if (!(uniform >= 0.0 && uniform < 1.0)) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)");
+    hipStream_t st = s.ctx->stream;
+    if (dst.d != s.d && !(e = launch_copy(dst, s, st)).ok()) return e;
+    if (!(e = basis_in(dst, basis, u, qubits, m)).ok()) return e;
+    if (!(e = measure_sample_collapse(dst, dst, qubits, m, &uniform, 1, outcome, true, st)).ok()) return e;
+    if (!(e = check_normalised(dst)).ok()) return e;
+    return basis_out(dst, basis, u, qubits, m);
+}
+
+// apply_exp_factor (pauli_string.rs:237-262) on the device.
+static Status pauli_exp(State &s, const qi_pauli_term &t, cxd factor) {
+    cxd alpha = cmul(cxd{t.c_re, t.c_im}, factor);
+    if ((t.x_mask | t.z_mask) == 0) {
+        cxd e = cexp_(alpha);
+        return launch_scale(s, e.re, e.im, s.ctx->stream);
+    }
+    Status e = check_mask(s.n, t.x_mask | t.z_mask);
+    if (!e.ok()) return e;
+    cxd ch = ccosh_(alpha), sh = csinh_(alpha);
+    return launch_pauli_exp(s, t, ch.re, ch.im, sh.re, sh.im, s.ctx->stream);
+}
+
+static Status pauli_apply_normalised(State &in, State &out, const qi_pauli_term &t) {
+    Status e = check_mask(in.n, t.x_mask | t.z_mask);
+    if (!e.ok()) return e;
+    qi_pauli_term u = t;
+    u.c_re = 1.0; u.c_im = 0.0;  // apply_operators drops the coefficient
+    if (!(e = launch_pauli_apply(in, out, u, in.ctx->stream)).ok()) return e;
+    return normalise(out);
+}
+
+// Gate::apply for one validated gate.
+static Status apply_validated(State &s, const qi_gate &g) {
+    switch (g.kind) {
+    case QI_GATE_MEASURE: {
+        uint64_t o;
+        return measure_one(s, (int)g.basis, g.matrix, g.controls, g.ncontrols, g.params[0], s, &o);
+    }
+    case QI_GATE_PAULI_STRING: {
+        qi_pauli_term t{g.x_mask, g.z_mask, g.params[0], g.params[1]};
+        return pauli_apply_normalised(s, s, t);
+    }
+    case QI_GATE_PAULI_EVOLUTION: {
+        qi_pauli_term t{g.x_mask, g.z_mask, g.params[0], g.params[1]};
+        return pauli_exp(s, t, cxd{0.0, -g.params[2]});
+    }
+    default: {
+        Lowered L;
+        Status e = lower_gate(s.n, g, L);
+        if (!e.ok()) return e;
+        return launch_gate(s, L, s.ctx->stream);
+    }
+    }
+}
+
+static Status device_check(hipStream_t st) {
+    QI_HIP_TRY(hipGetLastError());
+    QI_HIP_TRY(hipStreamSynchronize(st));
+    return Status{};
+}
+
+// Runs a launch sequence, optionally captured into a hipGraph that is cached
+// for an identical call (same gates, state, options).
+template <typename F>
+static Status run_maybe_graph(Ctx &C, State &s, bool use_graph, const void *key, size_t key_len, F &&launch) {
+    hipStream_t st = C.stream;
+    QI_HIP_TRY(hipEventRecord(C.ev0, st));
+    if (!use_graph) {
+        Status e = launch();
+        if (!e.ok()) return e;
+    } else {
+        bool hit = C.graph_exec && C.graph_state == (uint64_t)(uintptr_t)s.d && C.graph_key.size() == key_len &&
+                   memcmp(C.graph_key.data(), key, key_len) == 0;
+        if (!hit) {
+            if (C.graph_exec) { (void)hipGraphExecDestroy(C.graph_exec); C.graph_exec = nullptr; }
+            hipGraph_t graph = nullptr;
+            QI_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            Status e = launch();
+            hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (!e.ok()) { if (graph) (void)hipGraphDestroy(graph); return e; }
+            if (ce != hipSuccess) return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&C.graph_exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { C.graph_exec = nullptr; return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ie)); }
+            C.graph_key.assign((const unsigned char *)key, (const unsigned char *)key + key_len);
+            C.graph_state = (uint64_t)(uintptr_t)s.d;
+        }
+        QI_HIP_TRY(hipGraphLaunch(C.graph_exec, st));
+    }
+    QI_HIP_TRY(hipEventRecord(C.ev1, st));
+    Status e = device_check(st);
+    if (!e.ok()) return e;
+    float ms = 0.f;
+    QI_HIP_TRY(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+    C.last_exec_ms = ms;
+    return Status{};
+}
+
+// Plans the launch sequence of a gate list: runs of operator gates go through
+// the chosen executor, other Gate variants (measurement, Pauli strings) become
+// host-orchestrated steps between them.
+static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, const qi_exec_opts &o, ExecPlan &plan) {
+    size_t k = 0;
+    while (k < ng) {
+        if (!is_operator(gates[k].kind)) { plan.steps.push_back(ExecStep::host(k)); k++; continue; }
+        size_t j = k;
+        std::vector<Lowered> seg;
+        while (j < ng && is_operator(gates[j].kind)) {
+            Lowered L;
+            Status e = lower_gate(s.n, gates[j], L);
+            if (!e.ok()) return e;
+            if (L.kind != LK_NOP) seg.push_back(L);
+            j++;
+        }
+        Status e;
+        if (mode == QI_EXEC_TILED) e = plan_tiled(s, seg, o.tile_qubits, plan);
+        else if (mode == QI_EXEC_FUSED_MFMA) e = plan_fused(s, seg, o.block_qubits, plan);
+        else for (const Lowered &L : seg) plan.steps.push_back(ExecStep::gate(L));
+        if (!e.ok()) return e;
+        k = j;
+    }
+    return Status{};
+}
+
+static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates) {
+    Ctx &C = *s.ctx;
+    hipStream_t st = C.stream;
+    const TileOp *dops = (const TileOp *)C.plan.p;
+    const unsigned char *dmats = (const unsigned char *)C.plan.p + plan.mats_offset();
+    for (const ExecStep &step : plan.steps) {
+        Status e;
+        switch (step.type) {
+        case ExecStep::GATE: e = launch_gate(s, step.L, st); break;
+        case ExecStep::TILE: e = launch_tile_pass(s, step.tile, dops, st); break;
+        case ExecStep::FUSED: e = launch_fused_block(s, step.fused, dmats + step.fused.mat_off, st); break;
+        default: e = apply_validated(s, gates[step.host_gate]); break;
+        }
+        if (!e.ok()) return e;
+    }
+    return Status{};
+}
+
+static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, size_t ng, const qi_exec_opts *opts) {
+    if (n_circuit != s.n) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, s.n);  // circuit.rs:161-163
+    if (ng > 0 && !gates) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gates is NULL");
+    for (size_t k = 0; k < ng; k++) {
+        Status e = validate_gate(s.n, gates[k]);
+        if (!e.ok()) return e;
+    }
+    qi_exec_opts o{};
+    o.mode = QI_EXEC_AUTO;
+    if (opts) o = *opts;
+    if (o.mode < QI_EXEC_PER_GATE || o.mode > QI_EXEC_AUTO) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "exec mode");
+    int mode = o.mode;
+    if (mode == QI_EXEC_AUTO) mode = s.n >= 12 ? QI_EXEC_TILED : QI_EXEC_PER_GATE;
+    bool all_ops = true;
+    for (size_t k = 0; k < ng; k++) all_ops = all_ops && is_operator(gates[k].kind);
+    Ctx &C = *s.ctx;
+    ExecPlan plan;
+    Status e = build_plan(s, gates, ng, mode, o, plan);
+    if (!e.ok()) return e;
+    C.last_plan_steps = plan.steps.size();
+    if (plan.blob_bytes() > 0) {
+        if (!(e = ensure(C.plan, plan.blob_bytes())).ok()) return e;
+        QI_HIP_TRY(hipStreamSynchronize(C.stream));  // the previous plan may still be in use
+        std::vector<unsigned char> blob(plan.blob_bytes(), 0);
+        if (!plan.tile_ops.empty()) memcpy(blob.data(), plan.tile_ops.data(), plan.tile_ops.size() * sizeof(TileOp));
+        if (!plan.mats.empty()) memcpy(blob.data() + plan.mats_offset(), plan.mats.data(), plan.mats.size());
+        QI_HIP_TRY(hipMemcpy(C.plan.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    }
+    bool use_graph = o.use_graph && all_ops;
+    std::vector<unsigned char> key;
+    if (use_graph) {
+        key.resize(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 16);
+        memcpy(key.data(), &o, sizeof o);
+        memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
+        uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
+        uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
+        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
+        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate) + 8, &pp, 8);
+    }
+    auto launch = [&]() -> Status { return launch_plan(s, plan, gates); };
+    return run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
+}
+
+static Status check_pair(const State &a, const State &b) {
+    if (a.n != b.n) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, a.n);
+    if (a.prec != b.prec) return make_err(QI_ERR_PRECISION_MISMATCH);
+    if (a.ctx != b.ctx) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "states on different contexts");
+    return Status{};
+}
+
+}  // namespace qi
+
+using namespace qi;
+
+#define QI_RET(st)               \
+    do {                         \
+        Status s_ = (st);        \
+        fill_err(err, s_);       \
+        return (qi_status)s_.code; \
+    } while (0)
+
+#define QI_NULLCHK(p)                                                                             \
+    do {                                                                                          \
+        if (!(p)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, #p " is NULL"));          \
+    } while (0)
+
+#define QI_LOCK(ctxp) std::lock_guard<std::mutex> lk_((ctxp)->mu)
+
+extern "C" {
+
+qi_status qi_ctx_create(int device, qi_ctx_t *out, qi_error *err) {
+    QI_NULLCHK(out);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, e != hipSuccess ? hipGetErrorString(e) : "no HIP device"));
+    if (device < 0 || device >= ndev) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "device index"));
+    std::unique_ptr<qi_ctx_s> c(new qi_ctx_s());
+    Ctx &C = c->c;
+    C.device = device;
+    if ((e = hipSetDevice(device)) != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if ((e = hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking)) != hipSuccess)
+        QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if ((e = hipEventCreate(&C.ev0)) != hipSuccess || (e = hipEventCreate(&C.ev1)) != hipSuccess)
+        QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if ((e = hipHostMalloc((void **)&C.host_pinned, 64 * sizeof(double))) != hipSuccess)
+        QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    C.host_pinned_words = 64;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) C.cu_count = prop.multiProcessorCount;
+    *out = c.release();
+    QI_RET(Status{});
+}
+
+qi_status qi_ctx_destroy(qi_ctx_t ctx) {
+    if (!ctx) return QI_OK;
+    Ctx &C = ctx->c;
+    (void)hipSetDevice(C.device);
+    (void)hipStreamSynchronize(C.stream);
+    if (C.graph_exec) (void)hipGraphExecDestroy(C.graph_exec);
+    for (DevBuf *b : {&C.red, &C.red_out, &C.ops, &C.probs, &C.aux, &C.plan})
+        if (b->p) (void)hipFree(b->p);
+    if (C.host_pinned) (void)hipHostFree(C.host_pinned);
+    if (C.ev0) (void)hipEventDestroy(C.ev0);
+    if (C.ev1) (void)hipEventDestroy(C.ev1);
+    if (C.stream) (void)hipStreamDestroy(C.stream);
+    delete ctx;
+    return QI_OK;
+}
+
+qi_status qi_ctx_synchronize(qi_ctx_t ctx, qi_error *err) {
+    QI_NULLCHK(ctx);
+    QI_LOCK(&ctx->c);
+    QI_RET(device_check(ctx->c.stream));
+}
+
+qi_status qi_ctx_info(qi_ctx_t ctx, char *name, size_t name_len, uint64_t *hbm_bytes, int *cu_count, qi_error *err) {
+    QI_NULLCHK(ctx);
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, ctx->c.device);
+    if (e != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if (name && name_len) snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    QI_RET(Status{});
+}
+
+qi_status qi_state_create(qi_ctx_t ctx, uint32_t n, qi_prec prec, qi_state_t *out, qi_error *err) {
+    QI_NULLCHK(ctx);
+    QI_NULLCHK(out);
+    if (n == 0) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, 0));  // state.rs:166-168
+    if (n > QI_MAX_QUBITS) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "more qubits than the engine supports"));
+    if (prec != QI_C64 && prec != QI_C128) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "precision"));
+    QI_LOCK(&ctx->c);
+    std::unique_ptr<qi_state_s> s(new qi_state_s());
+    Status e = alloc_state(ctx->c, n, prec, s->s);
+    if (!e.ok()) QI_RET(e);
+    *out = s.release();
+    QI_RET(Status{});
+}
+
+qi_status qi_state_destroy(qi_state_t s) {
+    if (!s) return QI_OK;
+    {
+        QI_LOCK(s->s.ctx);
+        (void)hipStreamSynchronize(s->s.ctx->stream);
+        if (s->s.d) (void)hipFree(s->s.d);
+    }
+    delete s;
+    return QI_OK;
+}
+
+qi_status qi_state_clone(qi_state_t src, qi_state_t *out, qi_error *err) {
+    QI_NULLCHK(src);
+    QI_NULLCHK(out);
+    QI_LOCK(src->s.ctx);
+    std::unique_ptr<qi_state_s> s(new qi_state_s());
+    Status e = alloc_state(*src->s.ctx, src->s.n, src->s.prec, s->s);
+    if (!e.ok()) QI_RET(e);
+    if (!(e = launch_copy(s->s, src->s, src->s.ctx->stream)).ok()) { (void)hipFree(s->s.d); QI_RET(e); }
+    if (!(e = device_check(src->s.ctx->stream)).ok()) { (void)hipFree(s->s.d); QI_RET(e); }
+    *out = s.release();
+    QI_RET(Status{});
+}
+
+qi_status qi_state_copy(qi_state_t dst, qi_state_t src, qi_error *err) {
+    QI_NULLCHK(dst);
+    QI_NULLCHK(src);
+    Status e = check_pair(dst->s, src->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(src->s.ctx);
+    if (!(e = launch_copy(dst->s, src->s, src->s.ctx->stream)).ok()) QI_RET(e);
+    QI_RET(device_check(src->s.ctx->stream));
+}
+
+uint32_t qi_state_num_qubits(qi_state_t s) { return s ? s->s.n : 0; }
+int32_t qi_state_precision(qi_state_t s) { return s ? s->s.prec : -1; }
+void *qi_state_device_ptr(qi_state_t s) { return s ? s->s.d : nullptr; }
+
+qi_status qi_state_init(qi_state_t s, qi_init kind, uint64_t basis, qi_error *err) {
+    QI_NULLCHK(s);
+    State &S = s->s;
+    if (kind < QI_INIT_ZERO || kind > QI_INIT_GHZ) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "init kind"));
+    if (kind == QI_INIT_BASIS && basis >= (uint64_t(1) << S.n)) QI_RET(make_err(QI_ERR_INVALID_QUBIT_INDEX, basis, S.n));  // state.rs:196-198
+    QI_LOCK(S.ctx);
+    Status e = launch_init(S, kind, basis, S.ctx->stream);
+    if (!e.ok()) QI_RET(e);
+    QI_RET(device_check(S.ctx->stream));
+}
+
+qi_status qi_state_upload(qi_state_t s, const double *amps, int check_norm, qi_error *err) {
+    QI_NULLCHK(s);
+    QI_NULLCHK(amps);
+    State &S = s->s;
+    uint64_t dim = uint64_t(1) << S.n;
+    if (check_norm) {  // State::new, state.rs:117-122
+        double ns = 0.0;
+        for (uint64_t i = 0; i < 2 * dim; i++) ns += amps[i] * amps[i];
+        if (std::fabs(ns - 1.0) > DBL_EPSILON * double(dim)) QI_RET(make_err(QI_ERR_STATE_NOT_NORMALISED));
+    }
+    QI_LOCK(S.ctx);
+    hipStream_t st = S.ctx->stream;
+    if (S.prec == QI_C128) {
+        hipError_t e = hipMemcpyAsync(S.d, amps, S.bytes(), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    } else {
+        std::vector<float> f(2 * dim);
+        for (uint64_t i = 0; i < 2 * dim; i++) f[i] = (float)amps[i];
+        hipError_t e = hipMemcpyAsync(S.d, f.data(), S.bytes(), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+        QI_RET(device_check(st));
+    }
+    QI_RET(device_check(st));
+}
+
+qi_status qi_state_download(qi_state_t s, double *amps, qi_error *err) {
+    QI_NULLCHK(s);
+    QI_NULLCHK(amps);
+    State &S = s->s;
+    uint64_t dim = uint64_t(1) << S.n;
+    QI_LOCK(S.ctx);
+    hipStream_t st = S.ctx->stream;
+    if (S.prec == QI_C128) {
+        hipError_t e = hipMemcpyAsync(amps, S.d, S.bytes(), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+        QI_RET(device_check(st));
+    }
+    std::vector<float> f(2 * dim);
+    hipError_t e = hipMemcpyAsync(f.data(), S.d, S.bytes(), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    Status ds = device_check(st);
+    if (!ds.ok()) QI_RET(ds);
+    for (uint64_t i = 0; i < 2 * dim; i++) amps[i] = f[i];
+    QI_RET(Status{});
+}
+
+qi_status qi_apply_gate(qi_state_t s, const qi_gate *g, qi_error *err) {
+    QI_NULLCHK(s);
+    QI_NULLCHK(g);
+    State &S = s->s;
+    Status e = validate_gate(S.n, *g);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(S.ctx);
+    if (!(e = apply_validated(S, *g)).ok()) QI_RET(e);
+    QI_RET(device_check(S.ctx->stream));
+}
+
+qi_status qi_apply_gates(qi_state_t s, uint32_t n_circuit, const qi_gate *gates, size_t ngates, const qi_exec_opts *opts,
+                         qi_error *err) {
+    QI_NULLCHK(s);
+    QI_LOCK(s->s.ctx);
+    QI_RET(apply_gates(s->s, n_circuit, gates, ngates, opts));
+}
+
+qi_status qi_pauli_apply(qi_state_t in, qi_state_t out, const qi_pauli_term *t, qi_error *err) {
+    QI_NULLCHK(in); QI_NULLCHK(out); QI_NULLCHK(t);
+    Status e = check_pair(in->s, out->s);
+    if (!e.ok()) QI_RET(e);
+    if (!(e = check_mask(in->s.n, t->x_mask | t->z_mask)).ok()) QI_RET(e);
+    QI_LOCK(in->s.ctx);
+    if (!(e = launch_pauli_apply(in->s, out->s, *t, in->s.ctx->stream)).ok()) QI_RET(e);
+    QI_RET(device_check(in->s.ctx->stream));
+}
+
+qi_status qi_pauli_apply_normalised(qi_state_t in, qi_state_t out, const qi_pauli_term *t, qi_error *err) {
+    QI_NULLCHK(in); QI_NULLCHK(out); QI_NULLCHK(t);
+    Status e = check_pair(in->s, out->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(in->s.ctx);
+    if (!(e = pauli_apply_normalised(in->s, out->s, *t)).ok()) QI_RET(e);
+    QI_RET(device_check(in->s.ctx->stream));
+}
+
+qi_status qi_pauli_exp(qi_state_t s, const qi_pauli_term *t, double f_re, double f_im, qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(t);
+    QI_LOCK(s->s.ctx);
+    Status e = pauli_exp(s->s, *t, cxd{f_re, f_im});
+    if (!e.ok()) QI_RET(e);
+    QI_RET(device_check(s->s.ctx->stream));
+}
+
+qi_status qi_pauli_exp_neg_i_dt(qi_state_t s, const qi_pauli_term *t, double dt, qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(t);
+    if (t->c_im != 0.0) QI_RET(make_err(QI_ERR_INVALID_PAULI_COEFFICIENT, 0, 0, t->c_re, t->c_im));
+    QI_LOCK(s->s.ctx);
+    Status e = pauli_exp(s->s, *t, cxd{0.0, -dt});
+    if (!e.ok()) QI_RET(e);
+    QI_RET(device_check(s->s.ctx->stream));
+}
+
+qi_status qi_sumop_apply(qi_state_t in, qi_state_t out, const qi_pauli_term *terms, size_t nterms, qi_error *err) {
+    QI_NULLCHK(in); QI_NULLCHK(out);
+    if (nterms && !terms) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "terms is NULL"));
+    if (in == out) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "in and out must differ"));
+    Status e = check_pair(in->s, out->s);
+    if (!e.ok()) QI_RET(e);
+    for (size_t k = 0; k < nterms; k++)
+        if (!(e = check_mask(in->s.n, terms[k].x_mask | terms[k].z_mask)).ok()) QI_RET(e);
+    QI_LOCK(in->s.ctx);
+    hipStream_t st = in->s.ctx->stream;
+    if (nterms == 0) {  // state.clone() * 0.0 (pauli_string.rs:454-456)
+        if (!(e = launch_copy(out->s, in->s, st)).ok()) QI_RET(e);
+        if (!(e = launch_scale(out->s, 0.0, 0.0, st)).ok()) QI_RET(e);
+    } else if (!(e = launch_sumop_apply(in->s, out->s, terms, nterms, st)).ok()) {
+        QI_RET(e);
+    }
+    QI_RET(device_check(st));
+}
+
+qi_status qi_sumop_expectation(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double *re, double *im,
+                               qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(re); QI_NULLCHK(im);
+    if (nterms && !terms) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "terms is NULL"));
+    *re = 0.0; *im = 0.0;
+    if (nterms == 0) QI_RET(Status{});  // pauli_string.rs:486-488
+    Status e;
+    for (size_t k = 0; k < nterms; k++)
+        if (!(e = check_mask(s->s.n, terms[k].x_mask | terms[k].z_mask)).ok()) QI_RET(e);
+    QI_LOCK(s->s.ctx);
+    QI_RET(sumop_expectation(s->s, terms, nterms, re, im, s->s.ctx->stream));
+}
+
+qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt, uint64_t steps, int order,
+                            qi_error *err) {
+    QI_NULLCHK(s);
+    if (nterms == 0) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, 0));  // time_evolution.rs:147-149
+    QI_NULLCHK(terms);
+    if (order != 1 && order != 2) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "trotter order"));
+    Status e;
+    for (size_t k = 0; k < nterms; k++)
+        if (!(e = check_mask(s->s.n, terms[k].x_mask | terms[k].z_mask)).ok()) QI_RET(e);
+    QI_LOCK(s->s.ctx);
+    State &S = s->s;
+    for (uint64_t step = 0; step < steps; step++) {
+        if (order == 1) {
+            for (size_t k = 0; k < nterms; k++)
+                if (!(e = pauli_exp(S, terms[k], cxd{0.0, -dt})).ok()) QI_RET(e);
+        } else {
+            cxd half{0.0, -dt / 2.0};
+            for (size_t k = 0; k < nterms; k++)
+                if (!(e = pauli_exp(S, terms[k], half)).ok()) QI_RET(e);
+            for (size_t k = nterms; k-- > 0;)
+                if (!(e = pauli_exp(S, terms[k], half)).ok()) QI_RET(e);
+        }
+    }
+    QI_RET(device_check(S.ctx->stream));
+}
+
+qi_status qi_inner_product(qi_state_t a, qi_state_t b, double *re, double *im, qi_error *err) {
+    QI_NULLCHK(a); QI_NULLCHK(b); QI_NULLCHK(re); QI_NULLCHK(im);
+    Status e = check_pair(a->s, b->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(a->s.ctx);
+    double r[2];
+    if (!(e = reduce_state(a->s, &b->s, RK_INNER, r, a->s.ctx->stream)).ok()) QI_RET(e);
+    *re = r[0]; *im = r[1];
+    QI_RET(Status{});
+}
+
+qi_status qi_norm_sqr(qi_state_t s, double *out, qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(out);
+    QI_LOCK(s->s.ctx);
+    QI_RET(norm_sqr(s->s, out));
+}
+
+qi_status qi_normalise(qi_state_t s, qi_error *err) {
+    QI_NULLCHK(s);
+    QI_LOCK(s->s.ctx);
+    Status e = normalise(s->s);
+    if (!e.ok()) QI_RET(e);
+    QI_RET(device_check(s->s.ctx->stream));
+}
+
+qi_status qi_fidelity(qi_state_t a, qi_state_t b, double *out, qi_error *err) {
+    // fs_fidelity, state.rs:492-498: |<a/|a| | b/|b|>|^2
+    QI_NULLCHK(a); QI_NULLCHK(b); QI_NULLCHK(out);
+    Status e = check_pair(a->s, b->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(a->s.ctx);
+    double na, nb, r[2];
+    if (!(e = norm_sqr(a->s, &na)).ok()) QI_RET(e);
+    if (!(e = norm_sqr(b->s, &nb)).ok()) QI_RET(e);
+    if (std::sqrt(na) == 0.0 || std::sqrt(nb) == 0.0) QI_RET(make_err(QI_ERR_ZERO_NORM));
+    if (!(e = reduce_state(a->s, &b->s, RK_INNER, r, a->s.ctx->stream)).ok()) QI_RET(e);
+    double scale = std::sqrt(na) * std::sqrt(nb);
+    double ir = r[0] / scale, ii = r[1] / scale;
+    *out = ir * ir + ii * ii;
+    QI_RET(Status{});
+}
+
+qi_status qi_state_scale(qi_state_t s, double re, double im, qi_error *err) {
+    QI_NULLCHK(s);
+    QI_LOCK(s->s.ctx);
+    Status e = launch_scale(s->s, re, im, s->s.ctx->stream);
+    if (!e.ok()) QI_RET(e);
+    QI_RET(device_check(s->s.ctx->stream));
+}
+
+qi_status qi_state_axpy(qi_state_t y, double re, double im, qi_state_t x, qi_error *err) {
+    QI_NULLCHK(y); QI_NULLCHK(x);
+    Status e = check_pair(y->s, x->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(y->s.ctx);
+    if (!(e = launch_axpy(y->s, re, im, x->s, y->s.ctx->stream)).ok()) QI_RET(e);
+    QI_RET(device_check(y->s.ctx->stream));
+}
+
+qi_status qi_measure_probs(qi_state_t s, const uint64_t *qubits, uint32_t m, double *probs, qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(probs);
+    State &S = s->s;
+    std::vector<uint64_t> all;
+    if (m == 0) { for (uint32_t i = 0; i < S.n; i++) all.push_back(i); qubits = all.data(); m = S.n; }
+    QI_NULLCHK(qubits);
+    Status e = check_measure(S.n, qubits, m);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(S.ctx);
+    double *hm = nullptr;
+    if (!(e = measure_probs_device(S, qubits, m, &hm, S.ctx->stream)).ok()) QI_RET(e);
+    hipError_t he = hipMemcpyAsync(probs, hm, (size_t(1) << m) * sizeof(double), hipMemcpyDeviceToHost, S.ctx->stream);
+    if (he != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(he)));
+    QI_RET(device_check(S.ctx->stream));
+}
+
+qi_status qi_measure(qi_state_t s, qi_basis basis, const double *custom_u, const uint64_t *qubits, uint32_t m,
+                     double uniform, qi_state_t out_state, uint64_t *outcome, qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(out_state); QI_NULLCHK(outcome);
+    if (m && !qubits) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "qubits is NULL"));
+    if (basis < QI_BASIS_COMPUTATIONAL || basis > QI_BASIS_CUSTOM) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "basis"));
+    if (basis == QI_BASIS_CUSTOM && !custom_u) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "custom basis is NULL"));
+    Status e = check_pair(s->s, out_state->s);
+    if (!e.ok()) QI_RET(e);
+    QI_LOCK(s->s.ctx);
+    if (!(e = measure_one(s->s, basis, custom_u, qubits, m, uniform, out_state->s, outcome)).ok()) QI_RET(e);
+    QI_RET(device_check(s->s.ctx->stream));
+}
+
+qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u, const uint64_t *qubits, uint32_t m,
+                       const double *uniforms, size_t shots, uint64_t *outcomes, qi_error *err) {
+    QI_NULLCHK(s);
+    if (shots == 0) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_MEASUREMENTS, 0));  // state.rs:756-758
+    QI_NULLCHK(uniforms); QI_NULLCHK(outcomes);
+    if (m && !qubits) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "qubits is NULL"));
+    if (basis < QI_BASIS_COMPUTATIONAL || basis > QI_BASIS_CUSTOM) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "basis"));
+    if (basis == QI_BASIS_CUSTOM && !custom_u) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "custom basis is NULL"));
+    State &S = s->s;
+    std::vector<uint64_t> all;
+    if (m == 0) { for (uint32_t i = 0; i < S.n; i++) all.push_back(i); qubits = all.data(); m = S.n; }
+    Status e = check_measure(S.n, qubits, m);
+    if (!e.ok()) QI_RET(e);
+    if (basis == QI_BASIS_CUSTOM && !(e = check_unitary(custom_u)).ok()) QI_RET(e);
+    for (size_t k = 0; k < shots; k++)
+        if (!(uniforms[k] >= 0.0 && uniforms[k] < 1.0)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)"));
+    QI_LOCK(S.ctx);
+    // every shot measures the same state: one histogram and scan, many searches
+    TmpState w;
+    State *src = &S;
+    if (basis != QI_BASIS_COMPUTATIONAL) {
+        if (!(e = alloc_state(*S.ctx, S.n, S.prec, w.s)).ok()) QI_RET(e);
+        if (!(e = launch_copy(w.s, S, S.ctx->stream)).ok()) QI_RET(e);
+        if (!(e = basis_in(w.s, basis, custom_u, qubits, m)).ok()) QI_RET(e);
+        src = &w.s;
+    }
+    QI_RET(measure_sample_collapse(*src, *src, qubits, m, uniforms, shots, outcomes, false, S.ctx->stream));
+}
+
+qi_status qi_measure_collapse(qi_state_t src, qi_state_t dst, const uint64_t *qubits, uint32_t m, uint64_t outcome,
+                              qi_error *err) {
+    QI_NULLCHK(src); QI_NULLCHK(dst);
+    Status e = check_pair(src->s, dst->s);
+    if (!e.ok()) QI_RET(e);
+    State &S = src->s;
+    std::vector<uint64_t> all;
+    if (m == 0) { for (uint32_t i = 0; i < S.n; i++) all.push_back(i); qubits = all.data(); m = S.n; }
+    if (!(e = check_measure(S.n, qubits, m)).ok()) QI_RET(e);
+    if (m < 64 && outcome >= (uint64_t(1) << m)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "outcome out of range"));
+    // the outcome's bits must agree on repeated qubits
+    for (uint32_t a = 0; a < m; a++)
+        for (uint32_t b = a + 1; b < m; b++)
+            if (qubits[a] == qubits[b] && (((outcome >> a) ^ (outcome >> b)) & 1))
+                QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "outcome inconsistent on repeated qubits"));
+    QI_LOCK(S.ctx);
+    if (!(e = collapse_outcome(S, dst->s, qubits, m, outcome, S.ctx->stream)).ok()) QI_RET(e);
+    QI_RET(check_normalised(dst->s));
+}
+
+qi_status qi_last_exec_ms(qi_ctx_t ctx, double *ms, qi_error *err) {
+    QI_NULLCHK(ctx); QI_NULLCHK(ms);
+    *ms = ctx->c.last_exec_ms;
+    QI_RET(Status{});
+}
+
+qi_status qi_last_exec_steps(qi_ctx_t ctx, uint64_t *steps, qi_error *err) {
+    QI_NULLCHK(ctx); QI_NULLCHK(steps);
+    *steps = ctx->c.last_plan_steps;
+    QI_RET(Status{});
+}
+
+}  // extern "C"

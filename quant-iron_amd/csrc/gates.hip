@@ -1,0 +1,353 @@
+// gates.hip — streaming per-gate kernels (one HBM pass per gate) and the small
+// elementwise kernels (init, scale, axpy, copy).
+//
+// Each gate kernel enumerates exactly the amplitudes the gate changes: the
+// index space with the target and control bits removed is walked linearly and
+// the fixed bits are re-inserted (controls = 1, target = 0 or 1), so a CNOT
+// reads and writes half the state and a Toffoli a quarter, as the minimal-touch
+// byte count of each gate requires. Consecutive threads own consecutive free
+// indices, so every wave's loads are contiguous runs.
+//
+// Arithmetic: the per-element formulas keep the reference's operand order
+// (operator.rs, e.g. Hadamard :396-412, RotateX :1756-1757, RotateY :1897-1898,
+// Unitary2 :2253-2254, Matchgate :1000-1006) and this file is compiled
+// without FMA contraction, so a complex128 state gets the reference's results
+// bit for bit (up to the sign of zero). All constants are computed on the host.
+#include "qi_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace qi {
+
+namespace {
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+template <typename T> __device__ __forceinline__ T cmul(T a, T b) {
+    T r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T> __device__ __forceinline__ T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T, typename R> __device__ __forceinline__ T cscale(R f, T a) { T r; r.x = f * a.x; r.y = f * a.y; return r; }
+template <typename T> __device__ __forceinline__ T mk(decltype(T::x) re, decltype(T::x) im) { T r; r.x = re; r.y = im; return r; }
+
+// Insert a zero bit at every set bit of `fixed` (ascending), turning a free
+// index into a full index. `fixed` is uniform across the launch.
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t k, uint64_t fixed) {
+    while (fixed) {
+        unsigned p = __builtin_ctzll(fixed);
+        uint64_t lo = k & ((uint64_t(1) << p) - 1);
+        k = ((k >> p) << (p + 1)) | lo;
+        fixed &= fixed - 1;
+    }
+    return k;
+}
+
+constexpr int kThreads = 256;
+constexpr int kItems = 4;  // independent pairs per thread (memory-level parallelism)
+
+template <typename R> struct GateK {
+    uint64_t items;   // number of free indices
+    uint64_t fixed;   // bits removed from the free index
+    uint64_t set;     // bits forced to 1 in the base index
+    uint64_t tb;      // target bit
+    uint64_t pmask;   // pair partner = base ^ pmask
+    uint64_t tb2;     // second target bit (swap / matchgate)
+    uint64_t cmask;   // control mask tested per element (matchgate only)
+    R c[10];
+};
+
+// Pair kernels: H, X, Y, RX, RY, U2 and RZ (two phases).
+template <typename R, int KIND>
+__global__ __launch_bounds__(kThreads) void k_pair(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
+    using T = typename V2<R>::T;
+    const uint64_t first = (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
+    uint64_t i0[kItems];
+    T a[kItems], b[kItems];
+#pragma unroll
+    for (int u = 0; u < kItems; u++) {
+        uint64_t k = first + uint64_t(u) * kThreads;
+        i0[u] = k < g.items ? (insert_zeros(k, g.fixed) | g.set) : ~uint64_t(0);
+        if (k < g.items) { a[u] = psi[i0[u]]; b[u] = psi[i0[u] ^ g.pmask]; }
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; u++) {
+        if (i0[u] == ~uint64_t(0)) continue;
+        T na, nb;
+        if constexpr (KIND == LK_H) {
+            R s = g.c[0];
+            na = cscale<T>(s, cadd(a[u], b[u]));
+            nb = cscale<T>(s, csub(a[u], b[u]));
+        } else if constexpr (KIND == LK_X) {
+            na = b[u]; nb = a[u];
+        } else if constexpr (KIND == LK_Y) {
+            na = cmul(mk<T>(g.c[0], g.c[1]), b[u]);   // (-i) * b
+            nb = cmul(mk<T>(g.c[2], g.c[3]), a[u]);   // ( i) * a
+        } else if constexpr (KIND == LK_RX) {
+            na = csub(cscale<T>(g.c[0], a[u]), cmul(mk<T>(g.c[1], g.c[2]), b[u]));
+            nb = cadd(cmul(mk<T>(g.c[3], g.c[4]), a[u]), cscale<T>(g.c[0], b[u]));
+        } else if constexpr (KIND == LK_RY) {
+            na = csub(cscale<T>(g.c[0], a[u]), cscale<T>(g.c[1], b[u]));
+            nb = cadd(cscale<T>(g.c[1], a[u]), cscale<T>(g.c[0], b[u]));
+        } else if constexpr (KIND == LK_U2) {
+            na = cadd(cmul(mk<T>(g.c[0], g.c[1]), a[u]), cmul(mk<T>(g.c[2], g.c[3]), b[u]));
+            nb = cadd(cmul(mk<T>(g.c[4], g.c[5]), a[u]), cmul(mk<T>(g.c[6], g.c[7]), b[u]));
+        } else {  // LK_RZ
+            na = cmul(a[u], mk<T>(g.c[0], g.c[1]));
+            nb = cmul(b[u], mk<T>(g.c[2], g.c[3]));
+        }
+        psi[i0[u]] = na;
+        psi[i0[u] ^ g.pmask] = nb;
+    }
+}
+
+// Target-1 diagonal kernels: Z (negate) and S/T/Sdag/Tdag/P (phase).
+template <typename R, int KIND>
+__global__ __launch_bounds__(kThreads) void k_diag(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
+    using T = typename V2<R>::T;
+    const uint64_t first = (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
+    uint64_t idx[kItems];
+    T a[kItems];
+#pragma unroll
+    for (int u = 0; u < kItems; u++) {
+        uint64_t k = first + uint64_t(u) * kThreads;
+        idx[u] = k < g.items ? (insert_zeros(k, g.fixed) | g.set) : ~uint64_t(0);
+        if (k < g.items) a[u] = psi[idx[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kItems; u++) {
+        if (idx[u] == ~uint64_t(0)) continue;
+        T r;
+        if constexpr (KIND == LK_Z) { r.x = -a[u].x; r.y = -a[u].y; }
+        else r = cmul(a[u], mk<T>(g.c[0], g.c[1]));
+        psi[idx[u]] = r;
+    }
+}
+
+// Matchgate on (q1, q1+1); controls are tested on i01 for the rotation and on
+// i11 for the phase, as the reference does (operator.rs:993-1004).
+template <typename R>
+__global__ __launch_bounds__(kThreads) void k_match(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
+    using T = typename V2<R>::T;
+    const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (k >= g.items) return;
+    uint64_t l = insert_zeros(k, g.fixed);
+    uint64_t i01 = l | g.tb, i10 = l | g.tb2, i11 = l | g.tb | g.tb2;
+    if ((i01 & g.cmask) == g.cmask) {
+        T a01 = psi[i01], a10 = psi[i10];
+        psi[i01] = csub(cscale<T>(g.c[0], a01), cmul(mk<T>(g.c[2], g.c[3]), a10));
+        psi[i10] = cadd(cscale<T>(g.c[1], a01), cmul(mk<T>(g.c[4], g.c[5]), a10));
+    }
+    if ((i11 & g.cmask) == g.cmask) psi[i11] = cmul(psi[i11], mk<T>(g.c[6], g.c[7]));
+}
+
+template <typename R> GateK<R> make_args(const Lowered &L, uint32_t n) {
+    GateK<R> g{};
+    for (int q = 0; q < 8; q++) g.c[q] = R(L.c[q]);
+    g.tb = uint64_t(1) << L.t0;
+    g.tb2 = uint64_t(1) << L.t1;
+    uint64_t fixed = L.ctrl_mask | g.tb;
+    if (L.kind == LK_SWAP) fixed |= g.tb2;
+    if (L.kind == LK_MATCH) { fixed = g.tb | g.tb2; g.cmask = L.ctrl_mask; }
+    g.fixed = fixed;
+    g.set = L.ctrl_mask;
+    g.pmask = g.tb;
+    if (L.kind == LK_Z || L.kind == LK_PHASE) g.set |= g.tb;
+    if (L.kind == LK_SWAP) { g.set |= g.tb; g.pmask = g.tb | g.tb2; }
+    if (L.kind == LK_MATCH) g.set = 0;
+    g.items = uint64_t(1) << (n - __builtin_popcountll(fixed));
+    return g;
+}
+
+template <typename R> Status launch_t(State &s, const Lowered &L, hipStream_t st) {
+    using T = typename V2<R>::T;
+    T *psi = reinterpret_cast<T *>(s.d);
+    GateK<R> g = make_args<R>(L, s.n);
+    const uint64_t per_block = uint64_t(kThreads) * kItems;
+    dim3 grid((unsigned)((g.items + per_block - 1) / per_block)), block(kThreads);
+    switch (L.kind) {
+    case LK_H: hipLaunchKernelGGL((k_pair<R, LK_H>), grid, block, 0, st, psi, g); break;
+    case LK_X: hipLaunchKernelGGL((k_pair<R, LK_X>), grid, block, 0, st, psi, g); break;
+    case LK_Y: hipLaunchKernelGGL((k_pair<R, LK_Y>), grid, block, 0, st, psi, g); break;
+    case LK_RX: hipLaunchKernelGGL((k_pair<R, LK_RX>), grid, block, 0, st, psi, g); break;
+    case LK_RY: hipLaunchKernelGGL((k_pair<R, LK_RY>), grid, block, 0, st, psi, g); break;
+    case LK_U2: hipLaunchKernelGGL((k_pair<R, LK_U2>), grid, block, 0, st, psi, g); break;
+    case LK_RZ: hipLaunchKernelGGL((k_pair<R, LK_RZ>), grid, block, 0, st, psi, g); break;
+    case LK_Z: hipLaunchKernelGGL((k_diag<R, LK_Z>), grid, block, 0, st, psi, g); break;
+    case LK_PHASE: hipLaunchKernelGGL((k_diag<R, LK_PHASE>), grid, block, 0, st, psi, g); break;
+    case LK_SWAP:  // exchange base|tb with base|tb2: the X pair kernel with partner i ^ (tb|tb2)
+        hipLaunchKernelGGL((k_pair<R, LK_X>), grid, block, 0, st, psi, g); break;
+    case LK_MATCH: {
+        dim3 mg((unsigned)((g.items + kThreads - 1) / kThreads));
+        hipLaunchKernelGGL((k_match<R>), mg, block, 0, st, psi, g);
+    } break;
+    default: return Status{};
+    }
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+// ------------------------------------------------------ elementwise kernels --
+template <typename R>
+__global__ __launch_bounds__(256) void k_init(typename V2<R>::T *psi, uint64_t dim, int kind, uint64_t basis,
+                                              double amp) {
+    using T = typename V2<R>::T;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        T v; v.x = 0; v.y = 0;
+        switch (kind) {
+        case QI_INIT_ZERO: if (i == 0) v.x = 1; break;
+        case QI_INIT_BASIS: if (i == basis) v.x = 1; break;
+        case QI_INIT_PLUS: v.x = R(amp); break;
+        case QI_INIT_MINUS: v.x = (__popcll(i) & 1) ? -R(amp) : R(amp); break;
+        case QI_INIT_GHZ: if (i == 0 || i == dim - 1) v.x = R(amp); break;
+        }
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scale(typename V2<R>::T *psi, uint64_t dim, R re, R im) {
+    using T = typename V2<R>::T;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        psi[i] = cmul(psi[i], mk<T>(re, im));
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_div(typename V2<R>::T *psi, uint64_t dim, R f) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        auto v = psi[i];
+        v.x = v.x / f; v.y = v.y / f;
+        psi[i] = v;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_axpy(typename V2<R>::T *y, const typename V2<R>::T *x, uint64_t dim,
+                                              R re, R im) {
+    using T = typename V2<R>::T;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        y[i] = cadd(y[i], cmul(x[i], mk<T>(re, im)));
+}
+
+}  // namespace
+
+int pick_blocks(uint64_t items, int per_block) {
+    uint64_t b = (items + per_block - 1) / per_block;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+Status lower_gate(uint32_t n, const qi_gate &g, Lowered &L) {
+    (void)n;
+    L = Lowered{};
+    L.t0 = (uint32_t)g.targets[0];
+    L.t1 = g.ntargets > 1 ? (uint32_t)g.targets[1] : 0;
+    for (uint32_t k = 0; k < g.ncontrols; k++) L.ctrl_mask |= uint64_t(1) << g.controls[k];
+    const double r2 = 1.0 / std::sqrt(2.0);
+    const double h = g.params[0] / 2.0;
+    switch (g.kind) {
+    case QI_GATE_H: L.kind = LK_H; L.c[0] = r2; break;
+    case QI_GATE_X: case QI_GATE_CNOT: case QI_GATE_TOFFOLI: L.kind = LK_X; break;
+    case QI_GATE_Y:  // -i_complex = (-0.0, -1.0), i_complex = (0, 1) (operator.rs:489)
+        L.kind = LK_Y; L.c[0] = -0.0; L.c[1] = -1.0; L.c[2] = 0.0; L.c[3] = 1.0; break;
+    case QI_GATE_Z: L.kind = LK_Z; break;
+    case QI_GATE_I: L.kind = LK_NOP; break;
+    case QI_GATE_S: L.kind = LK_PHASE; L.c[0] = 0.0; L.c[1] = 1.0; break;
+    case QI_GATE_SDAG: L.kind = LK_PHASE; L.c[0] = 0.0; L.c[1] = -1.0; break;
+    case QI_GATE_T: L.kind = LK_PHASE; L.c[0] = r2; L.c[1] = r2; break;
+    case QI_GATE_TDAG: L.kind = LK_PHASE; L.c[0] = r2; L.c[1] = -r2; break;
+    case QI_GATE_P: L.kind = LK_PHASE; L.c[0] = std::cos(g.params[0]); L.c[1] = std::sin(g.params[0]); break;
+    case QI_GATE_RX: {  // i*sin, (-i)*sin as Complex * f64 (operator.rs:1756-1757)
+        double ch = std::cos(h), sh = std::sin(h);
+        L.kind = LK_RX; L.c[0] = ch; L.c[1] = 0.0 * sh; L.c[2] = 1.0 * sh; L.c[3] = -0.0 * sh; L.c[4] = -1.0 * sh;
+    } break;
+    case QI_GATE_RY: L.kind = LK_RY; L.c[0] = std::cos(h); L.c[1] = std::sin(h); break;
+    case QI_GATE_RZ:
+        L.kind = LK_RZ; L.c[0] = std::cos(h); L.c[1] = -std::sin(h); L.c[2] = std::cos(h); L.c[3] = std::sin(h); break;
+    case QI_GATE_U2: L.kind = LK_U2; for (int q = 0; q < 8; q++) L.c[q] = g.matrix[q]; break;
+    case QI_GATE_SWAP: L.kind = LK_SWAP; break;
+    case QI_GATE_MATCHGATE: {
+        double th = g.params[0] / 2.0, ch = std::cos(th), sh = std::sin(th);
+        // exp(i phi) = from_polar(exp(0), phi) (num-complex)
+        double e1r = std::exp(0.0) * std::cos(g.params[1]), e1i = std::exp(0.0) * std::sin(g.params[1]);
+        double e2r = std::exp(0.0) * std::cos(g.params[2]), e2i = std::exp(0.0) * std::sin(g.params[2]);
+        L.kind = LK_MATCH; L.t1 = L.t0 + 1;
+        L.c[0] = ch; L.c[1] = sh;
+        L.c[2] = e1r * sh; L.c[3] = e1i * sh;  // exp_i_phi1 * sin_theta_half
+        L.c[4] = e1r * ch; L.c[5] = e1i * ch;  // exp_i_phi1 * cos_theta_half
+        L.c[6] = e2r; L.c[7] = e2i;
+    } break;
+    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+    }
+    return Status{};
+}
+
+Status launch_gate(State &s, const Lowered &L, hipStream_t st) {
+    if (L.kind == LK_NOP) return Status{};
+    return s.prec == QI_C128 ? launch_t<double>(s, L, st) : launch_t<float>(s, L, st);
+}
+
+Status launch_init(State &s, int kind, uint64_t basis, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << s.n;
+    double amp = 0.0;
+    if (kind == QI_INIT_PLUS || kind == QI_INIT_MINUS) amp = 1.0 / std::sqrt(double(dim));  // state.rs:233
+    if (kind == QI_INIT_GHZ) amp = 0.70710678118654752440;                                    // FRAC_1_SQRT_2
+    int blocks = pick_blocks(dim, 256);
+    if (s.prec == QI_C128)
+        hipLaunchKernelGGL(k_init<double>, dim3(blocks), dim3(256), 0, st, (double2 *)s.d, dim, kind, basis, amp);
+    else
+        hipLaunchKernelGGL(k_init<float>, dim3(blocks), dim3(256), 0, st, (float2 *)s.d, dim, kind, basis, amp);
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_scale(State &s, double re, double im, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << s.n;
+    int blocks = pick_blocks(dim, 256);
+    if (s.prec == QI_C128)
+        hipLaunchKernelGGL(k_scale<double>, dim3(blocks), dim3(256), 0, st, (double2 *)s.d, dim, re, im);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(blocks), dim3(256), 0, st, (float2 *)s.d, dim, (float)re, (float)im);
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_div_real(State &s, double f, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << s.n;
+    int blocks = pick_blocks(dim, 256);
+    if (s.prec == QI_C128)
+        hipLaunchKernelGGL(k_div<double>, dim3(blocks), dim3(256), 0, st, (double2 *)s.d, dim, f);
+    else
+        hipLaunchKernelGGL(k_div<float>, dim3(blocks), dim3(256), 0, st, (float2 *)s.d, dim, (float)f);
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_axpy(State &y, double re, double im, const State &x, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << y.n;
+    int blocks = pick_blocks(dim, 256);
+    if (y.prec == QI_C128)
+        hipLaunchKernelGGL(k_axpy<double>, dim3(blocks), dim3(256), 0, st, (double2 *)y.d, (const double2 *)x.d, dim, re, im);
+    else
+        hipLaunchKernelGGL(k_axpy<float>, dim3(blocks), dim3(256), 0, st, (float2 *)y.d, (const float2 *)x.d, dim,
+                           (float)re, (float)im);
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_copy(State &dst, const State &src, hipStream_t st) {
+    QI_HIP_TRY(hipMemcpyAsync(dst.d, src.d, src.bytes(), hipMemcpyDeviceToDevice, st));
+    return Status{};
+}
+
+}  // namespace qi

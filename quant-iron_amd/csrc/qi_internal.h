@@ -1,0 +1,139 @@
+// qi_internal.h — shared declarations of the gfx950 state-vector engine.
+// Host orchestration (api.hip) talks to the kernel files through the launcher
+// functions declared here; nothing in this header crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/qiron.h"
+
+namespace qi {
+
+// ---------------------------------------------------------------- errors --
+struct Status {
+    int code = QI_OK;
+    uint64_t a = 0, b = 0;
+    double re = 0, im = 0;
+    std::string msg;
+    bool ok() const { return code == QI_OK; }
+};
+Status make_err(int code, uint64_t a = 0, uint64_t b = 0, double re = 0, double im = 0,
+                const char *detail = nullptr);
+void fill_err(qi_error *err, const Status &st);
+
+#define QI_HIP_TRY(expr)                                                          \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ::qi::make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e_)); \
+    } while (0)
+
+// --------------------------------------------------------- device buffers --
+struct Ctx;
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// Scratch that grows on demand (reductions, histograms, schedules).
+Status ensure(DevBuf &b, size_t bytes);
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    DevBuf red;      // reduction partials
+    DevBuf red_out;  // reduction results (device)
+    DevBuf ops;      // per-call op tables (tiles, Pauli terms)
+    DevBuf probs;    // measurement histograms / CDF
+    DevBuf aux;      // misc (sample outcomes, uniforms)
+    DevBuf plan;     // op tables and fused matrices of the last gate-list plan
+    size_t last_plan_steps = 0;
+    double *host_pinned = nullptr;  // pinned host words for results
+    size_t host_pinned_words = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_exec_ms = 0.0;
+    int cu_count = 0;
+    // cached graph of the last captured gate list
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<unsigned char> graph_key;
+    uint64_t graph_state = 0;
+};
+
+struct State {
+    Ctx *ctx = nullptr;
+    uint32_t n = 0;
+    int prec = QI_C128;
+    void *d = nullptr;  // 2^n complex amplitudes (float2 or double2)
+    size_t bytes() const { return (size_t(1) << n) * (prec == QI_C128 ? 16 : 8); }
+};
+
+// ------------------------------------------------------------ gate kernels --
+// A gate lowered for the streaming kernels. The host resolves all constants
+// (cos/sin of angles, phases) in FP64 with the host libm, so device results in
+// FP64 reproduce the reference's arithmetic.
+enum LoweredKind : int {
+    LK_H = 0, LK_X, LK_Y, LK_RX, LK_RY, LK_U2,      // pair kernels
+    LK_Z, LK_PHASE,                                  // target-1 diagonal kernels
+    LK_RZ,                                           // two-phase diagonal
+    LK_SWAP,                                         // permutation of two targets
+    LK_MATCH,                                        // matchgate
+    LK_NOP
+};
+
+struct Lowered {
+    int kind = LK_NOP;
+    uint32_t t0 = 0, t1 = 0;        // targets
+    uint64_t ctrl_mask = 0;         // controls (must be 1)
+    double c[8] = {0};              // coefficients (kind-specific, see gates.hip)
+};
+
+Status lower_gate(uint32_t n, const qi_gate &g, Lowered &out);
+Status launch_gate(State &s, const Lowered &L, hipStream_t st);
+
+// ------------------------------------------------------- misc kernels ------
+Status launch_init(State &s, int kind, uint64_t basis, hipStream_t st);
+Status launch_scale(State &s, double re, double im, hipStream_t st);
+Status launch_axpy(State &y, double re, double im, const State &x, hipStream_t st);
+Status launch_copy(State &dst, const State &src, hipStream_t st);
+Status launch_div_real(State &s, double f, hipStream_t st);
+
+// FP64 compensated reductions: partials in ctx.red, the result in ctx.red_out,
+// copied to pinned host memory before returning.
+enum RedKind : int { RK_NORM = 0, RK_INNER = 1 };
+Status reduce_state(State &a, const State *b, int kind, double *out2, hipStream_t st);
+
+// Pauli kernels (pauli.hip)
+Status launch_pauli_apply(const State &in, State &out, const qi_pauli_term &t, hipStream_t st);
+Status launch_pauli_exp(State &s, const qi_pauli_term &t, double ch_re, double ch_im, double sh_re,
+                        double sh_im, hipStream_t st);
+Status launch_sumop_apply(const State &in, State &out, const qi_pauli_term *terms, size_t nt,
+                          hipStream_t st);
+// Expectation of a list of terms; result (re, im) summed in FP64 compensated.
+Status sumop_expectation(State &s, const qi_pauli_term *terms, size_t nt, double *re, double *im,
+                         hipStream_t st);
+
+// Measurement kernels (measure.hip)
+Status measure_probs_device(State &s, const uint64_t *qubits, uint32_t m, double **dprobs,
+                            hipStream_t st);
+Status collapse_outcome(State &s, State &dst, const uint64_t *qubits, uint32_t m, uint64_t outcome, hipStream_t st);
+Status measure_sample_collapse(State &s, State &dst, const uint64_t *qubits, uint32_t m,
+                               const double *uniforms, size_t shots, uint64_t *outcomes_host,
+                               bool collapse, hipStream_t st);
+
+int pick_blocks(uint64_t items, int per_block);
+
+}  // namespace qi
+
+struct qi_ctx_s {
+    qi::Ctx c;
+};
+struct qi_state_s {
+    qi::State s;
+};
