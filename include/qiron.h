@@ -268,6 +268,31 @@ qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u,
 qi_status qi_measure_collapse(qi_state_t src, qi_state_t dst, const uint64_t *qubits, uint32_t m,
                               uint64_t outcome, qi_error *err);
 
+/* ---------------- batched circuits ---------------- */
+/* A batch of parametrised circuits of uneven qubit counts, each run from
+ * |0..0> (State::new_zero, state.rs:165-178) through Circuit::execute
+ * (circuit.rs:160-172) and reduced to SumOp::expectation_value
+ * (pauli_string.rs:485-507) of its own Hamiltonian. The whole batch is one
+ * hipGraph launch. No reference counterpart as a single call: it is the
+ * reference's per-circuit path applied to many circuits at once. */
+typedef struct qi_batch_s *qi_batch_t;
+qi_status qi_batch_create(qi_ctx_t ctx, qi_prec prec, qi_batch_t *out, qi_error *err);
+qi_status qi_batch_destroy(qi_batch_t b);
+/* Adds one circuit (operator gates only, validated as qi_apply_gate does).
+ * angle_slots[g] >= 0 makes gate g (RX/RY/RZ/P) take its angle from slot
+ * angle_slots[g] of the batch's angle table instead of params[0]; NULL = none.
+ * Each term's X/Y support must fit one LDS tile (at most 9 qubits outside the
+ * 4 lowest for complex64, 8 outside the 3 lowest for complex128). */
+qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ngates,
+                       const int32_t *angle_slots, const qi_pauli_term *terms, size_t nterms,
+                       qi_error *err);
+/* Replaces the angle table (one upload; the captured graph is reused). */
+qi_status qi_batch_set_angles(qi_batch_t b, const double *angles, size_t nangles, qi_error *err);
+/* Runs the batch; energies[2c], energies[2c+1] = Re, Im of circuit c's energy. */
+qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *err);
+qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uint64_t *nlevels,
+                        double *last_ms, qi_error *err);
+
 /* ---------------- timing hooks (for benchmarks) ---------------- */
 /* Device time of the last qi_apply_gates call's launch sequence, from HIP
  * events recorded on the context's stream (milliseconds). */

@@ -48,7 +48,24 @@ def build(verbose: bool = False) -> Path:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_host_tests(verbose)
     return OUT
+
+
+HOST_TEST_SRC = PKG.parent / "tests" / "cpp" / "test_state_api.cpp"
+HOST_TEST_BIN = PKG / "host" / "test_state_api"
+
+
+def build_host_tests(verbose: bool = False) -> Path:
+    """The C++ host-API test program (plain g++ against libqiron.so)."""
+    cmd = ["g++", "-O1", "-std=c++17", "-I", str(PKG.parent / "include"), str(HOST_TEST_SRC), "-o", str(HOST_TEST_BIN),
+           "-L", str(OUT.parent), "-lqiron", "-Wl,-rpath,$ORIGIN/../qiron"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host test build failed:\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {HOST_TEST_BIN}")
+    return HOST_TEST_BIN
 
 
 if __name__ == "__main__":

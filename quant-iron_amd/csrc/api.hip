@@ -108,7 +108,7 @@ static Status check_measure(uint32_t n, const uint64_t *q, uint32_t m) {
     return Status{};
 }
 
-static Status validate_gate(uint32_t n, const qi_gate &g) {
+Status validate_gate(uint32_t n, const qi_gate &g) {
     uint32_t nt = g.ntargets, nc = g.ncontrols;
     if (nc > QI_MAX_CONTROLS) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "too many controls");
     Status s;

@@ -1,0 +1,725 @@
+// batch.hip — thousands of parametrised circuits of uneven qubit counts, each
+// with its own SumOp, evaluated to energies in one hipGraph launch.
+//
+// Circuits that fit a workgroup's LDS (2^13 complex64 / 2^12 complex128
+// amplitudes) run LDS-resident: one workgroup builds |0..0>, applies every gate
+// and reduces the energy without touching HBM. Larger circuits live in one
+// device arena; their gates are planned into LDS-tiled passes (tiled.hip's
+// planner), and pass p of every circuit runs in one launch ("level" p), so a
+// level boundary is the only synchronisation a circuit needs. Energies are
+// then accumulated by tiled expectation passes (FP64, two-sum compensated per
+// block, folded in block order). Parametric gates carry a slot into a device
+// angle table, so new parameters are one upload and the graph is replayed.
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "qi_internal.h"
+#include "exec_plan.h"
+#include "reduce_util.h"
+
+#pragma clang fp contract(off)
+
+namespace qi {
+
+namespace {
+
+template <typename R> struct V2;
+template <> struct V2<float> { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+template <typename T> __device__ __forceinline__ T cmul(T a, T b) {
+    T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T> __device__ __forceinline__ T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T, typename R> __device__ __forceinline__ T cscale(R f, T a) { T r; r.x = f * a.x; r.y = f * a.y; return r; }
+template <typename T> __device__ __forceinline__ T mk(decltype(T::x) re, decltype(T::x) im) { T r; r.x = re; r.y = im; return r; }
+
+__device__ __forceinline__ uint32_t ins0(uint32_t k, unsigned p) { return ((k >> p) << (p + 1)) | (k & ((1u << p) - 1)); }
+
+__device__ __forceinline__ uint64_t insert_zeros64(uint64_t k, uint64_t fixed) {
+    while (fixed) {
+        unsigned p = __builtin_ctzll(fixed);
+        k = ((k >> p) << (p + 1)) | (k & ((uint64_t(1) << p) - 1));
+        fixed &= fixed - 1;
+    }
+    return k;
+}
+
+template <int KMAX>
+__device__ __forceinline__ uint64_t deposit_pos(uint32_t e, const uint8_t *pos, int k) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++)
+        if (j < k) r |= uint64_t((e >> j) & 1) << pos[j];
+    return r;
+}
+
+constexpr int kBT = 256;
+
+// Coefficients of one op; parametric ops take their angle from the table
+// (same formulas as lower_gate, evaluated on the device).
+struct OpC {
+    double c[8];
+};
+
+__device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ angles, double *c) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) c[q] = op.c[q];
+    const int slot = (int)op.pad;
+    if (slot < 0) return;
+    const double th = angles[slot];
+    double sh, ch;
+    switch (op.kind) {
+    case LK_RY: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = sh; break;
+    case LK_RZ: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = -sh; c[2] = ch; c[3] = sh; break;
+    case LK_RX: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = 0.0 * sh; c[2] = sh; c[3] = -0.0 * sh; c[4] = -sh; break;
+    case LK_PHASE: sincos(th, &sh, &ch); c[2] = ch; c[3] = sh; break;
+    default: break;
+    }
+}
+
+// Apply one op to a tile held in LDS (shared by the LDS-resident and the tiled kernels).
+template <typename R, int N>
+__device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp &op, const double *c, uint64_t base,
+                                           int nloc) {
+    using T = typename V2<R>::T;
+    const uint32_t tid = threadIdx.x, lcm = op.lcm;
+    const uint32_t nel = 1u << nloc;
+    switch (op.kind) {
+    case LK_Z: case LK_PHASE: case LK_RZ: {
+        T p0 = mk<T>(R(c[0]), R(c[1])), p1 = mk<T>(R(c[2]), R(c[3]));
+        for (uint32_t e = tid; e < nel; e += kBT) {
+            if ((e & lcm) != lcm) continue;
+            bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
+            T a = tile[e];
+            if (op.kind == LK_Z) { if (one) { a.x = -a.x; a.y = -a.y; } }
+            else if (op.kind == LK_PHASE) { if (one) a = cmul(a, p1); }
+            else a = cmul(a, one ? p1 : p0);
+            tile[e] = a;
+        }
+    } break;
+    case LK_SWAP: case LK_MATCH: {
+        const uint32_t b0 = 1u << op.lp0, b1 = 1u << op.lp1;
+        const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
+        for (uint32_t q = tid; q < (nel >> 2); q += kBT) {
+            uint32_t l = ins0(ins0(q, lo), hi);
+            if (op.kind == LK_SWAP) {
+                uint32_t i = l | b0, j = l | b1;
+                if ((i & lcm) != lcm) continue;
+                T a = tile[i], b = tile[j];
+                tile[i] = b; tile[j] = a;
+            } else {
+                uint32_t i01 = l | b0, i10 = l | b1, i11 = l | b0 | b1;
+                if ((i01 & lcm) == lcm) {
+                    T a01 = tile[i01], a10 = tile[i10];
+                    tile[i01] = csub(cscale<T>(R(c[0]), a01), cmul(mk<T>(R(c[2]), R(c[3])), a10));
+                    tile[i10] = cadd(cscale<T>(R(c[1]), a01), cmul(mk<T>(R(c[4]), R(c[5])), a10));
+                }
+                if ((i11 & lcm) == lcm) tile[i11] = cmul(tile[i11], mk<T>(R(c[6]), R(c[7])));
+            }
+        }
+    } break;
+    default: {
+        const uint32_t tb = 1u << op.lp0;
+        for (uint32_t q = tid; q < (nel >> 1); q += kBT) {
+            uint32_t i = ins0(q, op.lp0), j = i | tb;
+            if ((i & lcm) != lcm) continue;
+            T a = tile[i], b = tile[j], na, nb;
+            switch (op.kind) {
+            case LK_H: na = cscale<T>(R(c[0]), cadd(a, b)); nb = cscale<T>(R(c[0]), csub(a, b)); break;
+            case LK_X: na = b; nb = a; break;
+            case LK_Y: na = cmul(mk<T>(R(c[0]), R(c[1])), b); nb = cmul(mk<T>(R(c[2]), R(c[3])), a); break;
+            case LK_RX:
+                na = csub(cscale<T>(R(c[0]), a), cmul(mk<T>(R(c[1]), R(c[2])), b));
+                nb = cadd(cmul(mk<T>(R(c[3]), R(c[4])), a), cscale<T>(R(c[0]), b));
+                break;
+            case LK_RY:
+                na = csub(cscale<T>(R(c[0]), a), cscale<T>(R(c[1]), b));
+                nb = cadd(cscale<T>(R(c[1]), a), cscale<T>(R(c[0]), b));
+                break;
+            default:
+                na = cadd(cmul(mk<T>(R(c[0]), R(c[1])), a), cmul(mk<T>(R(c[2]), R(c[3])), b));
+                nb = cadd(cmul(mk<T>(R(c[4]), R(c[5])), a), cmul(mk<T>(R(c[6]), R(c[7])), b));
+                break;
+            }
+            tile[i] = na; tile[j] = nb;
+        }
+    } break;
+    }
+}
+
+// An energy term: coefficient with (-i)^{nY} folded into quarter turns.
+struct ETerm {
+    uint64_t x, z;
+    uint32_t q0, pad;
+    double c[2];
+};
+
+// sum over tile elements e of conj(psi_e) * c * i^{q(e)} * psi_{e ^ x}
+template <typename R, int KMAX>
+__device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const ETerm *__restrict__ terms, int nt,
+                                            uint64_t base, const uint8_t *pos, int nloc, DD &sr, DD &si) {
+    const uint32_t tid = threadIdx.x, nel = 1u << nloc;
+    for (uint32_t e = tid; e < nel; e += kBT) {
+        const uint64_t gi = base | deposit_pos<KMAX>(e, pos, nloc);
+        const double ar = tile[e].x, ai = tile[e].y;
+        double hr = 0.0, hi = 0.0;
+        for (int t = 0; t < nt; t++) {
+            const ETerm tm = terms[t];
+            // local partner: x lies inside the tile, so flipping its global
+            // bits flips the same local bits (tm.pad holds the local x mask)
+            const uint32_t le = e ^ tm.pad;
+            const double br = tile[le].x, bi = tile[le].y;
+            const unsigned q = (tm.q0 + 2u * (unsigned)__popcll(gi & tm.z)) & 3u;
+            // w = conj(a) * b
+            const double wr = ar * br + ai * bi, wi = ar * bi - ai * br;
+            // c * i^q
+            const double pr = q == 0 ? tm.c[0] : q == 1 ? -tm.c[1] : q == 2 ? -tm.c[0] : tm.c[1];
+            const double pi = q == 0 ? tm.c[1] : q == 1 ? tm.c[0] : q == 2 ? -tm.c[1] : -tm.c[0];
+            hr += wr * pr - wi * pi;
+            hi += wr * pi + wi * pr;
+        }
+        dd_acc(sr, hr);
+        dd_acc(si, hi);
+    }
+}
+
+// ------------------------------------------------------- LDS-resident path --
+struct SmallTask {
+    uint32_t n, op0, nops, t0, nt;
+    uint32_t out;  // energy slot
+};
+
+template <typename R, int KMAX>
+__global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict__ tasks, const TileOp *__restrict__ ops,
+                                                     const ETerm *__restrict__ terms, const double *__restrict__ angles,
+                                                     double *__restrict__ energies) {
+    using T = typename V2<R>::T;
+    __shared__ T tile[1 << KMAX];
+    __shared__ double sh[4 * 4];
+    __shared__ uint8_t pos[KMAX];
+    const SmallTask tk = tasks[blockIdx.x];
+    const uint32_t nel = 1u << tk.n;
+    if (threadIdx.x < KMAX) pos[threadIdx.x] = (uint8_t)threadIdx.x;
+    for (uint32_t e = threadIdx.x; e < nel; e += kBT) { T z; z.x = e == 0 ? R(1) : R(0); z.y = R(0); tile[e] = z; }
+    __syncthreads();
+    for (uint32_t o = 0; o < tk.nops; o++) {
+        const TileOp op = ops[tk.op0 + o];
+        double c[8];
+        resolve(op, angles, c);
+        tile_apply<R, (1 << KMAX)>(tile, op, c, 0, (int)tk.n);
+        __syncthreads();
+    }
+    DD v[2] = {dd_zero(), dd_zero()};
+    tile_energy<R, KMAX>(tile, terms + tk.t0, (int)tk.nt, 0, pos, (int)tk.n, v[0], v[1]);
+    block_reduce_dd<2>(v, sh);
+    if (threadIdx.x == 0) {
+        energies[2 * tk.out] = v[0].s + v[0].c;
+        energies[2 * tk.out + 1] = v[1].s + v[1].c;
+    }
+}
+
+// ------------------------------------------------------------- tiled path --
+struct BigTask {
+    uint64_t off;      // element offset of the circuit's state in the arena
+    uint64_t qmask;    // tile qubits
+    uint32_t block0;   // first block of this task in the launch
+    uint32_t op0, nops;
+    uint32_t init;     // 1: the state is |0..0> (first pass reads nothing)
+    uint32_t t0, nt;   // energy terms (expectation passes)
+    uint32_t part0;    // first partial slot (expectation passes)
+    uint32_t pad;
+    uint8_t pos[48];
+};
+
+__device__ __forceinline__ int find_task(const BigTask *__restrict__ tasks, int ntasks, uint32_t b) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename R, int K>
+__global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restrict__ arena, const BigTask *__restrict__ tasks,
+                                                    int ntasks, const TileOp *__restrict__ ops,
+                                                    const double *__restrict__ angles) {
+    using T = typename V2<R>::T;
+    constexpr int N = 1 << K, PER = N / kBT;
+    __shared__ T tile[N];
+    const int ti = find_task(tasks, ntasks, blockIdx.x);
+    const BigTask &tk = tasks[ti];
+    const uint64_t base = insert_zeros64(uint64_t(blockIdx.x - tk.block0), tk.qmask);
+    T *psi = arena + tk.off;
+    const uint64_t off_t = deposit_pos<8>(threadIdx.x, tk.pos, 8);
+    if (tk.init) {
+#pragma unroll
+        for (int it = 0; it < PER; it++) {
+            const uint32_t e = threadIdx.x + it * kBT;
+            T z; z.x = (base == 0 && e == 0) ? R(1) : R(0); z.y = R(0);
+            tile[e] = z;
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < PER; it++)
+            tile[threadIdx.x + it * kBT] = psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)];
+    }
+    __syncthreads();
+    for (uint32_t o = 0; o < tk.nops; o++) {
+        const TileOp op = ops[tk.op0 + o];
+        if ((base & op.gcm) != op.gcm) continue;
+        double c[8];
+        resolve(op, angles, c);
+        tile_apply<R, N>(tile, op, c, base, K);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int it = 0; it < PER; it++)
+        psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)] = tile[threadIdx.x + it * kBT];
+}
+
+template <typename R, int K>
+__global__ __launch_bounds__(kBT) void k_batch_expect(const typename V2<R>::T *__restrict__ arena,
+                                                      const BigTask *__restrict__ tasks, int ntasks,
+                                                      const ETerm *__restrict__ terms, double *__restrict__ part) {
+    using T = typename V2<R>::T;
+    constexpr int N = 1 << K, PER = N / kBT;
+    __shared__ T tile[N];
+    __shared__ double sh[4 * 4];
+    const int ti = find_task(tasks, ntasks, blockIdx.x);
+    const BigTask &tk = tasks[ti];
+    const uint32_t local = blockIdx.x - tk.block0;
+    const uint64_t base = insert_zeros64(uint64_t(local), tk.qmask);
+    const T *psi = arena + tk.off;
+    const uint64_t off_t = deposit_pos<8>(threadIdx.x, tk.pos, 8);
+#pragma unroll
+    for (int it = 0; it < PER; it++)
+        tile[threadIdx.x + it * kBT] = psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)];
+    __syncthreads();
+    DD v[2] = {dd_zero(), dd_zero()};
+    tile_energy<R, K>(tile, terms + tk.t0, (int)tk.nt, base, tk.pos, K, v[0], v[1]);
+    block_reduce_dd<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[(size_t(tk.part0) + local) * 4 + 0] = v[0].s; part[(size_t(tk.part0) + local) * 4 + 1] = v[0].c;
+        part[(size_t(tk.part0) + local) * 4 + 2] = v[1].s; part[(size_t(tk.part0) + local) * 4 + 3] = v[1].c;
+    }
+}
+
+// Fold each circuit's partials (all its expectation tiles, in order) into its energy.
+struct FoldTask {
+    uint32_t part0, nparts, out, pad;
+};
+
+__global__ __launch_bounds__(256) void k_batch_fold(const FoldTask *__restrict__ f, const double *__restrict__ part,
+                                                    double *__restrict__ energies) {
+    __shared__ double sh[4 * 4];
+    const FoldTask t = f[blockIdx.x];
+    DD v[2] = {dd_zero(), dd_zero()};
+    for (uint32_t k = threadIdx.x; k < t.nparts; k += blockDim.x) {
+        const double *p = part + (size_t(t.part0) + k) * 4;
+        v[0] = dd_add(v[0], DD{p[0], p[1]});
+        v[1] = dd_add(v[1], DD{p[2], p[3]});
+    }
+    block_reduce_dd<2>(v, sh);
+    if (threadIdx.x == 0) {
+        energies[2 * t.out] = v[0].s + v[0].c;
+        energies[2 * t.out + 1] = v[1].s + v[1].c;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host --
+struct Batch {
+    Ctx *ctx = nullptr;
+    int prec = QI_C64;
+    struct Circ {
+        uint32_t n;
+        std::vector<qi_gate> gates;
+        std::vector<qi_pauli_term> terms;
+        std::vector<int> slot_of_gate;  // angle slot per gate (-1: fixed)
+    };
+    std::vector<Circ> circs;
+    std::vector<double> angles;  // host copy of the angle table
+    // device
+    DevBuf d_ops, d_terms, d_small, d_big, d_fold, d_angles, d_part, d_energy, d_arena;
+    std::vector<std::pair<size_t, size_t>> levels;  // [task begin, task end) of each pass level
+    std::vector<uint32_t> level_blocks;
+    std::vector<std::pair<size_t, size_t>> elevels;
+    std::vector<uint32_t> elevel_blocks;
+    size_t nsmall = 0, nfold = 0;
+    int K = 13, KS = 13;
+    bool planned = false;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    hipGraphExec_t graph = nullptr;
+    double last_ms = 0.0;
+    ~Batch() {
+        for (DevBuf *b : {&d_ops, &d_terms, &d_small, &d_big, &d_fold, &d_angles, &d_part, &d_energy, &d_arena})
+            if (b->p) (void)hipFree(b->p);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (side) (void)hipStreamDestroy(side);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+    }
+};
+
+static ETerm make_eterm(const qi_pauli_term &p) {
+    ETerm t{};
+    t.x = p.x_mask; t.z = p.z_mask;
+    t.q0 = (3u * (unsigned)__builtin_popcountll(p.x_mask & p.z_mask)) & 3u;
+    t.c[0] = p.c_re; t.c[1] = p.c_im;
+    return t;
+}
+
+template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &v) {
+    Status e = ensure(b, std::max<size_t>(v.size() * sizeof(T), 16));
+    if (!e.ok()) return e;
+    if (!v.empty()) QI_HIP_TRY(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return Status{};
+}
+
+Status batch_plan(Batch &B) {
+    const bool c128 = B.prec == QI_C128;
+    B.K = c128 ? 12 : 13;
+    B.KS = B.K;  // LDS-resident when n <= KS
+    const int L = c128 ? 3 : 4;
+    std::vector<TileOp> ops;
+    std::vector<ETerm> eterms;
+    std::vector<SmallTask> small;
+    std::vector<std::vector<BigTask>> per_level, per_elevel;
+    std::vector<FoldTask> folds;
+    uint64_t arena_elems = 0;
+    uint32_t nparts = 0;
+    for (size_t ci = 0; ci < B.circs.size(); ci++) {
+        auto &C = B.circs[ci];
+        State fake;
+        fake.n = C.n; fake.prec = B.prec;
+        std::vector<Lowered> lw;
+        std::vector<int> lslot;
+        for (size_t g = 0; g < C.gates.size(); g++) {
+            Lowered Lg;
+            Status e = lower_gate(C.n, C.gates[g], Lg);
+            if (!e.ok()) return e;
+            if (Lg.kind == LK_NOP) continue;
+            lw.push_back(Lg);
+            lslot.push_back(C.slot_of_gate[g]);
+        }
+        // energy terms of this circuit
+        const uint32_t t0 = (uint32_t)eterms.size();
+        if ((int)C.n <= B.KS) {
+            SmallTask st{};
+            st.n = C.n; st.op0 = (uint32_t)ops.size(); st.out = (uint32_t)ci;
+            for (size_t g = 0; g < lw.size(); g++) {
+                TileOp op{};
+                op.kind = lw[g].kind;
+                for (int q = 0; q < 8; q++) op.c[q] = lw[g].c[q];
+                if (op.kind == LK_Z) { op.c[2] = -1.0; op.c[3] = 0.0; }
+                if (op.kind == LK_PHASE) { op.c[2] = lw[g].c[0]; op.c[3] = lw[g].c[1]; }
+                op.lp0 = (uint8_t)lw[g].t0; op.lp1 = (uint8_t)lw[g].t1;
+                op.lcm = (uint32_t)lw[g].ctrl_mask;
+                op.pad = (uint32_t)lslot[g];
+                ops.push_back(op);
+            }
+            st.nops = (uint32_t)lw.size();
+            for (auto &p : C.terms) { ETerm t = make_eterm(p); t.pad = (uint32_t)p.x_mask; eterms.push_back(t); }
+            st.t0 = t0; st.nt = (uint32_t)C.terms.size();
+            small.push_back(st);
+            continue;
+        }
+        // tiled: plan the passes with the general planner; each planned op
+        // records its source gate, which gives its angle slot
+        ExecPlan plan;
+        Status e = plan_tiled(fake, lw, B.K, plan);
+        if (!e.ok()) return e;
+        const uint64_t off = arena_elems;
+        arena_elems += uint64_t(1) << C.n;
+        const uint32_t tiles = uint32_t(1) << (C.n - B.K);
+        size_t level = 0;
+        for (const ExecStep &s : plan.steps) {
+            if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
+            BigTask bt{};
+            bt.off = off; bt.qmask = s.tile.qmask; bt.init = level == 0 ? 1 : 0;
+            memcpy(bt.pos, s.tile.pos, sizeof bt.pos);
+            bt.op0 = (uint32_t)ops.size(); bt.nops = (uint32_t)s.tile.nops;
+            for (int o = 0; o < s.tile.nops; o++) {
+                TileOp op = plan.tile_ops[s.tile.op0 + o];
+                op.pad = (uint32_t)lslot[op.pad];
+                ops.push_back(op);
+            }
+            bt.block0 = tiles;  // tile count; turned into offsets per level below
+            if (per_level.size() <= level) per_level.resize(level + 1);
+            per_level[level].push_back(bt);
+            level++;
+        }
+        if (plan.steps.empty()) {  // no gates: a pass that only writes |0..0>
+            BigTask bt{};
+            bt.off = off; bt.init = 1;
+            uint64_t q = 0;
+            for (int b = 0, j = 0; j < B.K; b++, j++) { q |= uint64_t(1) << b; bt.pos[j] = (uint8_t)b; }
+            bt.qmask = q; bt.block0 = tiles;
+            if (per_level.empty()) per_level.resize(1);
+            per_level[0].push_back(bt);
+        }
+        // expectation passes: each term goes to the first pass whose tile holds its x support
+        std::vector<qi_pauli_term> rest = C.terms;
+        uint32_t cparts = 0;
+        const uint32_t part_first = nparts;
+        size_t elevel = 0;
+        while (!rest.empty()) {
+            uint64_t q = (uint64_t(1) << L) - 1;
+            std::vector<qi_pauli_term> take, left;
+            for (auto &p : rest) {
+                if (__builtin_popcountll(q | p.x_mask) <= B.K) { q |= p.x_mask; take.push_back(p); }
+                else left.push_back(p);
+            }
+            for (int b = 0; __builtin_popcountll(q) < B.K; b++) q |= uint64_t(1) << b;
+            BigTask bt{};
+            bt.off = off; bt.qmask = q;
+            int j = 0;
+            uint32_t local[64];
+            for (int b = 0; b < (int)C.n; b++)
+                if ((q >> b) & 1) { local[b] = j; bt.pos[j++] = (uint8_t)b; }
+            bt.t0 = (uint32_t)eterms.size(); bt.nt = (uint32_t)take.size();
+            for (auto &p : take) {
+                ETerm t = make_eterm(p);
+                uint32_t lx = 0;
+                for (int b = 0; b < (int)C.n; b++)
+                    if ((p.x_mask >> b) & 1) lx |= 1u << local[b];
+                t.pad = lx;
+                eterms.push_back(t);
+            }
+            bt.part0 = nparts;
+            nparts += tiles;
+            cparts += tiles;
+            bt.block0 = tiles;
+            if (per_elevel.size() <= elevel) per_elevel.resize(elevel + 1);
+            per_elevel[elevel].push_back(bt);
+            elevel++;
+            rest.swap(left);
+        }
+        FoldTask f{};
+        f.part0 = part_first; f.nparts = cparts; f.out = (uint32_t)ci;
+        folds.push_back(f);
+    }
+    // flatten levels: block0 becomes the running block offset inside the level
+    std::vector<BigTask> big;
+    B.levels.clear(); B.level_blocks.clear(); B.elevels.clear(); B.elevel_blocks.clear();
+    auto flatten = [&](std::vector<std::vector<BigTask>> &lv, std::vector<std::pair<size_t, size_t>> &rng,
+                       std::vector<uint32_t> &blocks) {
+        for (auto &v : lv) {
+            size_t b0 = big.size();
+            uint32_t acc = 0;
+            for (auto &t : v) { uint32_t tiles = t.block0; t.block0 = acc; acc += tiles; big.push_back(t); }
+            rng.push_back({b0, big.size()});
+            blocks.push_back(acc);
+        }
+    };
+    flatten(per_level, B.levels, B.level_blocks);
+    flatten(per_elevel, B.elevels, B.elevel_blocks);
+    B.nsmall = small.size();
+    B.nfold = folds.size();
+    Status e;
+    if (!(e = upload_vec(B.d_ops, ops)).ok()) return e;
+    if (!(e = upload_vec(B.d_terms, eterms)).ok()) return e;
+    if (!(e = upload_vec(B.d_small, small)).ok()) return e;
+    if (!(e = upload_vec(B.d_big, big)).ok()) return e;
+    if (!(e = upload_vec(B.d_fold, folds)).ok()) return e;
+    if (!(e = upload_vec(B.d_angles, B.angles)).ok()) return e;
+    if (!(e = ensure(B.d_part, std::max<size_t>(nparts, 1) * 4 * sizeof(double))).ok()) return e;
+    if (!(e = ensure(B.d_energy, std::max<size_t>(B.circs.size(), 1) * 2 * sizeof(double))).ok()) return e;
+    const size_t esz = c128 ? 16 : 8;
+    if (!(e = ensure(B.d_arena, std::max<uint64_t>(arena_elems, 1) * esz)).ok()) return e;
+    if (!B.side) {
+        QI_HIP_TRY(hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking));
+        QI_HIP_TRY(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+        QI_HIP_TRY(hipEventCreateWithFlags(&B.join, hipEventDisableTiming));
+    }
+    if (B.graph) { (void)hipGraphExecDestroy(B.graph); B.graph = nullptr; }
+    B.planned = true;
+    return Status{};
+}
+
+template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st) {
+    using T = typename V2<R>::T;
+    const BigTask *big = (const BigTask *)B.d_big.p;
+    for (size_t l = 0; l < B.levels.size(); l++) {
+        auto r = B.levels[l];
+        hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kBT), 0, st, (T *)B.d_arena.p,
+                           big + r.first, (int)(r.second - r.first), (const TileOp *)B.d_ops.p,
+                           (const double *)B.d_angles.p);
+    }
+    for (size_t l = 0; l < B.elevels.size(); l++) {
+        auto r = B.elevels[l];
+        hipLaunchKernelGGL((k_batch_expect<R, K>), dim3(B.elevel_blocks[l]), dim3(kBT), 0, st, (const T *)B.d_arena.p,
+                           big + r.first, (int)(r.second - r.first), (const ETerm *)B.d_terms.p, (double *)B.d_part.p);
+    }
+    if (B.nfold)
+        hipLaunchKernelGGL(k_batch_fold, dim3((unsigned)B.nfold), dim3(256), 0, st, (const FoldTask *)B.d_fold.p,
+                           (const double *)B.d_part.p, (double *)B.d_energy.p);
+}
+
+static Status batch_launch(Batch &B, hipStream_t st) {
+    // fork: LDS-resident circuits on the side stream, tiled levels on `st`
+    QI_HIP_TRY(hipEventRecord(B.fork, st));
+    QI_HIP_TRY(hipStreamWaitEvent(B.side, B.fork, 0));
+    if (B.nsmall) {
+        if (B.prec == QI_C128)
+            hipLaunchKernelGGL((k_batch_small<double, 12>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
+                               (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
+                               (const double *)B.d_angles.p, (double *)B.d_energy.p);
+        else
+            hipLaunchKernelGGL((k_batch_small<float, 13>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
+                               (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
+                               (const double *)B.d_angles.p, (double *)B.d_energy.p);
+    }
+    if (B.prec == QI_C128) launch_levels<double, 12>(B, st);
+    else launch_levels<float, 13>(B, st);
+    QI_HIP_TRY(hipEventRecord(B.join, B.side));
+    QI_HIP_TRY(hipStreamWaitEvent(st, B.join, 0));
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status batch_run(Batch &B, double *energies, bool use_graph) {
+    Ctx &C = *B.ctx;
+    hipStream_t st = C.stream;
+    if (!B.planned) { Status e = batch_plan(B); if (!e.ok()) return e; }
+    QI_HIP_TRY(hipEventRecord(C.ev0, st));
+    if (use_graph) {
+        if (!B.graph) {
+            hipGraph_t g = nullptr;
+            QI_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            Status e = batch_launch(B, st);
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            if (!e.ok()) { if (g) (void)hipGraphDestroy(g); return e; }
+            if (ce != hipSuccess) return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&B.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) { B.graph = nullptr; return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ie)); }
+        }
+        QI_HIP_TRY(hipGraphLaunch(B.graph, st));
+    } else {
+        Status e = batch_launch(B, st);
+        if (!e.ok()) return e;
+    }
+    QI_HIP_TRY(hipEventRecord(C.ev1, st));
+    QI_HIP_TRY(hipMemcpyAsync(energies, B.d_energy.p, B.circs.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    QI_HIP_TRY(hipStreamSynchronize(st));
+    float ms = 0.f;
+    QI_HIP_TRY(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+    B.last_ms = ms;
+    C.last_exec_ms = ms;
+    return Status{};
+}
+
+}  // namespace qi
+
+using namespace qi;
+
+struct qi_batch_s {
+    qi::Batch b;
+};
+
+#define QB_RET(st)                 \
+    do {                           \
+        Status s_ = (st);          \
+        fill_err(err, s_);         \
+        return (qi_status)s_.code; \
+    } while (0)
+
+extern "C" {
+
+qi_status qi_batch_create(qi_ctx_t ctx, qi_prec prec, qi_batch_t *out, qi_error *err) {
+    if (!ctx || !out) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    if (prec != QI_C64 && prec != QI_C128) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "precision"));
+    auto *b = new qi_batch_s();
+    b->b.ctx = &ctx->c;
+    b->b.prec = prec;
+    *out = b;
+    QB_RET(Status{});
+}
+
+qi_status qi_batch_destroy(qi_batch_t b) {
+    if (!b) return QI_OK;
+    {
+        std::lock_guard<std::mutex> lk(b->b.ctx->mu);
+        (void)hipStreamSynchronize(b->b.ctx->stream);
+        delete b;
+    }
+    return QI_OK;
+}
+
+qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ngates, const int32_t *angle_slots,
+                       const qi_pauli_term *terms, size_t nterms, qi_error *err) {
+    if (!b || (ngates && !gates) || (nterms && !terms)) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    if (n == 0 || n > 34) QB_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, n));
+    Batch &B = b->b;
+    Batch::Circ C;
+    C.n = n;
+    const int K = B.prec == QI_C128 ? 12 : 13, L = B.prec == QI_C128 ? 3 : 4;
+    for (size_t g = 0; g < ngates; g++) {
+        if (gates[g].kind < QI_GATE_H || gates[g].kind > QI_GATE_TOFFOLI)
+            QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuits take operator gates only"));
+        Status v = validate_gate(n, gates[g]);
+        if (!v.ok()) QB_RET(v);
+        C.gates.push_back(gates[g]);
+        C.slot_of_gate.push_back(angle_slots ? angle_slots[g] : -1);
+    }
+    for (size_t t = 0; t < nterms; t++) {
+        uint64_t bad = (terms[t].x_mask | terms[t].z_mask) >> n;
+        if (bad) QB_RET(make_err(QI_ERR_INVALID_QUBIT_INDEX, n + (uint64_t)__builtin_ctzll(bad), n));
+        // a term's X/Y support must fit one tile next to the low qubits
+        if ((int)n > K && __builtin_popcountll(terms[t].x_mask | ((uint64_t(1) << L) - 1)) > K)
+            QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "Pauli term too wide for batched evaluation"));
+        C.terms.push_back(terms[t]);
+    }
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    B.circs.push_back(std::move(C));
+    B.planned = false;
+    QB_RET(Status{});
+}
+
+qi_status qi_batch_set_angles(qi_batch_t b, const double *angles, size_t nangles, qi_error *err) {
+    if (!b || (nangles && !angles)) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    B.angles.assign(angles, angles + nangles);
+    if (B.planned) {
+        if (B.d_angles.bytes < nangles * sizeof(double)) { B.planned = false; QB_RET(Status{}); }
+        hipError_t e = hipMemcpyAsync(B.d_angles.p, B.angles.data(), nangles * sizeof(double), hipMemcpyHostToDevice,
+                                      B.ctx->stream);
+        if (e != hipSuccess) QB_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+        e = hipStreamSynchronize(B.ctx->stream);
+        if (e != hipSuccess) QB_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    }
+    QB_RET(Status{});
+}
+
+qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *err) {
+    if (!b || !energies) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    // every angle slot must exist in the table
+    for (auto &C : B.circs)
+        for (int s : C.slot_of_gate)
+            if (s >= 0 && (size_t)s >= B.angles.size())
+                QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slot beyond the angle table"));
+    QB_RET(batch_run(B, energies, use_graph != 0));
+}
+
+qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uint64_t *nlevels, double *last_ms,
+                        qi_error *err) {
+    if (!b) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    if (ncircuits) *ncircuits = B.circs.size();
+    if (nsmall) *nsmall = B.nsmall;
+    if (nlevels) *nlevels = B.levels.size() + B.elevels.size();
+    if (last_ms) *last_ms = B.last_ms;
+    QB_RET(Status{});
+}
+
+}  // extern "C"

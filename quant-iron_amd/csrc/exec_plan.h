@@ -15,7 +15,7 @@ struct TileOp {
     uint8_t outside;     // diagonal gate whose target is outside the tile
     uint8_t gt;          // that target's qubit (tested on the tile base)
     uint32_t lcm;        // controls inside the tile (local bit mask)
-    uint32_t pad;
+    uint32_t pad;        // planner: source gate index; batched engine: angle slot (-1: none)
     uint64_t gcm;        // controls outside the tile (global bit mask)
     double c[8];         // coefficients as in Lowered (phases in c[2], c[3])
 };

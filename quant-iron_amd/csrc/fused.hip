@@ -115,11 +115,13 @@ __global__ __launch_bounds__(fused_threads<KB>()) void k_fused_block(typename MF
             }
         }
         __syncthreads();
+        // accumulator layout: lane holds column lane % 16; its 4 registers are
+        // rows 4*(lane/16) + r for FP32, but rows (lane/16) + 4*r for FP64
 #pragma unroll
         for (int cb = 0; cb < CB; cb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = wave * 16 + 4 * (lane >> 4) + r;
+                const int row = wave * 16 + (sizeof(R) == 8 ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
                 const int col = cb * 16 + (lane & 15);
                 R *w = reinterpret_cast<R *>(&v[(row & (D - 1)) * NC + col]);
                 w[row >= D ? 1 : 0] = acc[cb][r];

@@ -94,6 +94,9 @@ struct Lowered {
     double c[8] = {0};              // coefficients (kind-specific, see gates.hip)
 };
 
+// Argument checks of Operator::apply / Gate::apply (validate_qubits and the
+// per-operator checks, operator.rs:214-273 and siblings).
+Status validate_gate(uint32_t n, const qi_gate &g);
 Status lower_gate(uint32_t n, const qi_gate &g, Lowered &out);
 Status launch_gate(State &s, const Lowered &L, hipStream_t st);
 

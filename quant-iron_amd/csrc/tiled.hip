@@ -229,6 +229,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
             for (int b = 0; b < (int)s.n; b++)
                 if ((lc >> b) & 1) lcm |= 1u << local[b];
             op.lcm = lcm;
+            op.pad = (uint32_t)g;  // source gate (the batched engine maps it to an angle slot)
             plan.tile_ops.push_back(op);
         }
         p.nops = (int)members.size();

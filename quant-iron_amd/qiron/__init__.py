@@ -10,6 +10,7 @@ from .core import (  # noqa: F401
     FIRST,
     SECOND,
     Circuit,
+    CircuitBatch,
     CircuitBuilder,
     Context,
     Gate,

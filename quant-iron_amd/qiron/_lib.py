@@ -125,6 +125,12 @@ SIGNATURES = {
     "qi_measure": [P, C.c_int, DP, U64P, U32, D, P, U64P, ERR],
     "qi_measure_n": [P, C.c_int, DP, U64P, U32, DP, C.c_size_t, U64P, ERR],
     "qi_measure_collapse": [P, P, U64P, U32, U64, ERR],
+    "qi_batch_create": [P, C.c_int, C.POINTER(P), ERR],
+    "qi_batch_destroy": [P],
+    "qi_batch_add": [P, U32, C.c_void_p, C.c_size_t, C.POINTER(C.c_int32), C.c_void_p, C.c_size_t, ERR],
+    "qi_batch_set_angles": [P, DP, C.c_size_t, ERR],
+    "qi_batch_run": [P, DP, C.c_int, ERR],
+    "qi_batch_info": [P, U64P, U64P, U64P, DP, ERR],
     "qi_last_exec_ms": [P, DP, ERR],
     "qi_last_exec_steps": [P, U64P, ERR],
 }

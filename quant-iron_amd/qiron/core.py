@@ -878,3 +878,55 @@ class Subroutine:
                         den /= 2.0
             b.h_gate(qubits[i])
         return b.build()
+
+
+# ------------------------------------------------------------------- batches --
+class CircuitBatch:
+    """Many parametrised circuits, each with its own SumOp, evaluated to
+    energies in one engine launch (qi_batch_* in include/qiron.h)."""
+
+    def __init__(self, precision: str = "c64", ctx: Optional[Context] = None):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self._h = C.c_void_p()
+        self.count = 0
+        err = L.QiError()
+        _check(self.lib.qi_batch_create(self.ctx.handle, PRECISIONS[precision], C.byref(self._h), C.byref(err)), err)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.lib.qi_batch_destroy(self._h)
+        except Exception:
+            pass
+
+    def add(self, n: int, gates: np.ndarray, terms: np.ndarray, angle_slots: Optional[np.ndarray] = None) -> int:
+        g = np.ascontiguousarray(gates)
+        t = np.ascontiguousarray(terms)
+        slots = None
+        if angle_slots is not None:
+            slots = np.ascontiguousarray(np.asarray(angle_slots, dtype=np.int32))
+        err = L.QiError()
+        _check(self.lib.qi_batch_add(self._h, int(n), g.ctypes.data_as(C.c_void_p), g.size,
+                                     slots.ctypes.data_as(C.POINTER(C.c_int32)) if slots is not None else None,
+                                     t.ctypes.data_as(C.c_void_p), t.size, C.byref(err)), err)
+        self.count += 1
+        return self.count - 1
+
+    def set_angles(self, angles) -> None:
+        a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64))
+        err = L.QiError()
+        _check(self.lib.qi_batch_set_angles(self._h, L.dptr(a), a.size, C.byref(err)), err)
+
+    def run(self, use_graph: bool = True) -> np.ndarray:
+        out = np.zeros(2 * max(self.count, 1), dtype=np.float64)
+        err = L.QiError()
+        _check(self.lib.qi_batch_run(self._h, L.dptr(out), 1 if use_graph else 0, C.byref(err)), err)
+        return out[0::2][: self.count] + 1j * out[1::2][: self.count]
+
+    def info(self):
+        nc, ns, nl = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        ms = C.c_double()
+        err = L.QiError()
+        _check(self.lib.qi_batch_info(self._h, C.byref(nc), C.byref(ns), C.byref(nl), C.byref(ms), C.byref(err)), err)
+        return {"circuits": nc.value, "lds_resident": ns.value, "levels": nl.value, "last_ms": ms.value}
