@@ -288,10 +288,38 @@ qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ng
                        qi_error *err);
 /* Replaces the angle table (one upload; the captured graph is reused). */
 qi_status qi_batch_set_angles(qi_batch_t b, const double *angles, size_t nangles, qi_error *err);
+/* Replaces every circuit's term coefficients: (re, im) pairs in the order the
+ * terms were added, circuit by circuit (no replanning). */
+qi_status qi_batch_set_term_coeffs(qi_batch_t b, const double *coeffs, size_t nterms_total,
+                                   qi_error *err);
 /* Runs the batch; energies[2c], energies[2c+1] = Re, Im of circuit c's energy. */
 qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *err);
 qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uint64_t *nlevels,
                         double *last_ms, qi_error *err);
+
+/* ---------------- energy surrogate (MLP) ---------------- */
+/* An MLP n_in -> H -> H -> H -> 1 with GELU, BF16 weights on the matrix
+ * cores with FP32 accumulation and FP32 Adam masters. It screens ansatz
+ * parameters; exact simulation (qi_batch_*) stays the judge of every value.
+ * No counterpart in the reference. Parameter layout for get/set/gradients:
+ * W1 [k_in x H] (k_in = n_in rounded up to 16; rows >= n_in are zero), b1 [H],
+ * W2 [H x H], b2, W3 [H x H], b3, w4 [H], b4 [1]; row-major. */
+typedef struct qi_mlp_s *qi_mlp_t;
+qi_status qi_mlp_create(qi_ctx_t ctx, uint32_t n_in, uint32_t hidden, uint64_t seed, qi_mlp_t *out,
+                        qi_error *err);
+qi_status qi_mlp_destroy(qi_mlp_t m);
+qi_status qi_mlp_num_params(qi_mlp_t m, uint64_t *n, qi_error *err);
+qi_status qi_mlp_get_params(qi_mlp_t m, float *out, qi_error *err);
+qi_status qi_mlp_set_params(qi_mlp_t m, const float *in, qi_error *err);
+/* Gradient of the mean squared error over rows samples (rows % 16 == 0). */
+qi_status qi_mlp_gradients(qi_mlp_t m, const float *x, const float *t, size_t rows, float *grads,
+                           float *loss, qi_error *err);
+/* Uploads a training set once; qi_mlp_train_epoch runs Adam over shuffled
+ * mini-batches of `batch` rows (batch % 16 == 0). */
+qi_status qi_mlp_set_data(qi_mlp_t m, const float *x, const float *t, size_t rows, qi_error *err);
+qi_status qi_mlp_train_epoch(qi_mlp_t m, uint32_t batch, float lr, uint64_t shuffle_seed,
+                             float *mean_loss, qi_error *err);
+qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_error *err);
 
 /* ---------------- timing hooks (for benchmarks) ---------------- */
 /* Device time of the last qi_apply_gates call's launch sequence, from HIP

@@ -178,12 +178,15 @@ static Status normalise(State &s) {
     return launch_div_real(s, norm, s.ctx->stream);
 }
 
-// State::new's check (state.rs:117-122) on a device state.
+// State::new's check (state.rs:117-122) on a device state: |sum |a|^2 - 1|
+// <= eps * len, with eps the machine epsilon of the stored amplitudes
+// (f64::EPSILON for complex128 as in the reference, f32::EPSILON for
+// complex64, whose renormalised amplitudes carry FP32 rounding).
 static Status check_normalised(State &s) {
     double ns;
     Status e = norm_sqr(s, &ns);
     if (!e.ok()) return e;
-    double tol = DBL_EPSILON * double(uint64_t(1) << s.n);
+    double tol = (s.prec == QI_C128 ? DBL_EPSILON : (double)FLT_EPSILON) * double(uint64_t(1) << s.n);
     if (std::fabs(ns - 1.0) > tol) return make_err(QI_ERR_STATE_NOT_NORMALISED);
     return Status{};
 }

@@ -64,12 +64,12 @@ struct OpC {
     double c[8];
 };
 
-__device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ angles, double *c) {
+__device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ angles, int slot_base, double *c) {
 #pragma unroll
     for (int q = 0; q < 8; q++) c[q] = op.c[q];
     const int slot = (int)op.pad;
     if (slot < 0) return;
-    const double th = angles[slot];
+    const double th = angles[slot_base + slot];
     double sh, ch;
     switch (op.kind) {
     case LK_RY: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = sh; break;
@@ -189,7 +189,9 @@ __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const
 // ------------------------------------------------------- LDS-resident path --
 struct SmallTask {
     uint32_t n, op0, nops, t0, nt;
-    uint32_t out;  // energy slot
+    uint32_t out;       // energy slot
+    int32_t slot_base;  // first angle of this circuit in the table
+    uint32_t pad;
 };
 
 template <typename R, int KMAX>
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
     for (uint32_t o = 0; o < tk.nops; o++) {
         const TileOp op = ops[tk.op0 + o];
         double c[8];
-        resolve(op, angles, c);
+        resolve(op, angles, tk.slot_base, c);
         tile_apply<R, (1 << KMAX)>(tile, op, c, 0, (int)tk.n);
         __syncthreads();
     }
@@ -230,7 +232,7 @@ struct BigTask {
     uint32_t init;     // 1: the state is |0..0> (first pass reads nothing)
     uint32_t t0, nt;   // energy terms (expectation passes)
     uint32_t part0;    // first partial slot (expectation passes)
-    uint32_t pad;
+    int32_t slot_base; // first angle of this circuit in the table
     uint8_t pos[48];
 };
 
@@ -272,7 +274,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restric
         const TileOp op = ops[tk.op0 + o];
         if ((base & op.gcm) != op.gcm) continue;
         double c[8];
-        resolve(op, angles, c);
+        resolve(op, angles, tk.slot_base, c);
         tile_apply<R, N>(tile, op, c, base, K);
         __syncthreads();
     }
@@ -336,14 +338,28 @@ __global__ __launch_bounds__(256) void k_batch_fold(const FoldTask *__restrict__
 struct Batch {
     Ctx *ctx = nullptr;
     int prec = QI_C64;
-    struct Circ {
+    // a circuit shape shared by many circuits: gates and angle slots relative
+    // to each circuit's slot base
+    struct Program {
         uint32_t n;
         std::vector<qi_gate> gates;
-        std::vector<qi_pauli_term> terms;
-        std::vector<int> slot_of_gate;  // angle slot per gate (-1: fixed)
+        std::vector<int32_t> rel_slot;
+        // planned form (filled by batch_plan)
+        bool small = false;
+        uint32_t op0 = 0, nops = 0;           // small: op range
+        std::vector<TilePass> passes;         // tiled: passes (op0 relative to ops_base)
+        uint32_t ops_base = 0;
     };
+    struct Circ {
+        uint32_t prog;
+        int32_t slot_base;
+        std::vector<qi_pauli_term> terms;
+        std::vector<uint32_t> eterm_idx;  // where each term landed in the device table
+    };
+    std::vector<Program> progs;
     std::vector<Circ> circs;
     std::vector<double> angles;  // host copy of the angle table
+    std::vector<ETerm> eterms;   // host copy of the term table
     // device
     DevBuf d_ops, d_terms, d_small, d_big, d_fold, d_angles, d_part, d_energy, d_arena;
     std::vector<std::pair<size_t, size_t>> levels;  // [task begin, task end) of each pass level
@@ -382,37 +398,29 @@ template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &
     return Status{};
 }
 
+// Plans every program once (LDS-resident op list or tiled passes), then lays
+// out per-circuit tasks, expectation passes and the energy fold.
 Status batch_plan(Batch &B) {
     const bool c128 = B.prec == QI_C128;
     B.K = c128 ? 12 : 13;
-    B.KS = B.K;  // LDS-resident when n <= KS
+    B.KS = B.K;
     const int L = c128 ? 3 : 4;
     std::vector<TileOp> ops;
-    std::vector<ETerm> eterms;
-    std::vector<SmallTask> small;
-    std::vector<std::vector<BigTask>> per_level, per_elevel;
-    std::vector<FoldTask> folds;
-    uint64_t arena_elems = 0;
-    uint32_t nparts = 0;
-    for (size_t ci = 0; ci < B.circs.size(); ci++) {
-        auto &C = B.circs[ci];
-        State fake;
-        fake.n = C.n; fake.prec = B.prec;
+    for (auto &P : B.progs) {
         std::vector<Lowered> lw;
-        std::vector<int> lslot;
-        for (size_t g = 0; g < C.gates.size(); g++) {
+        std::vector<int32_t> lslot;
+        for (size_t g = 0; g < P.gates.size(); g++) {
             Lowered Lg;
-            Status e = lower_gate(C.n, C.gates[g], Lg);
+            Status e = lower_gate(P.n, P.gates[g], Lg);
             if (!e.ok()) return e;
             if (Lg.kind == LK_NOP) continue;
             lw.push_back(Lg);
-            lslot.push_back(C.slot_of_gate[g]);
+            lslot.push_back(P.rel_slot[g]);
         }
-        // energy terms of this circuit
-        const uint32_t t0 = (uint32_t)eterms.size();
-        if ((int)C.n <= B.KS) {
-            SmallTask st{};
-            st.n = C.n; st.op0 = (uint32_t)ops.size(); st.out = (uint32_t)ci;
+        P.small = (int)P.n <= B.KS;
+        P.passes.clear();
+        if (P.small) {
+            P.op0 = (uint32_t)ops.size();
             for (size_t g = 0; g < lw.size(); g++) {
                 TileOp op{};
                 op.kind = lw[g].kind;
@@ -424,73 +432,97 @@ Status batch_plan(Batch &B) {
                 op.pad = (uint32_t)lslot[g];
                 ops.push_back(op);
             }
-            st.nops = (uint32_t)lw.size();
-            for (auto &p : C.terms) { ETerm t = make_eterm(p); t.pad = (uint32_t)p.x_mask; eterms.push_back(t); }
-            st.t0 = t0; st.nt = (uint32_t)C.terms.size();
-            small.push_back(st);
+            P.nops = (uint32_t)lw.size();
             continue;
         }
-        // tiled: plan the passes with the general planner; each planned op
-        // records its source gate, which gives its angle slot
+        State fake;
+        fake.n = P.n; fake.prec = B.prec;
         ExecPlan plan;
         Status e = plan_tiled(fake, lw, B.K, plan);
         if (!e.ok()) return e;
-        const uint64_t off = arena_elems;
-        arena_elems += uint64_t(1) << C.n;
-        const uint32_t tiles = uint32_t(1) << (C.n - B.K);
-        size_t level = 0;
+        P.ops_base = (uint32_t)ops.size();
+        for (TileOp op : plan.tile_ops) { op.pad = (uint32_t)lslot[op.pad]; ops.push_back(op); }
         for (const ExecStep &s : plan.steps) {
             if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
-            BigTask bt{};
-            bt.off = off; bt.qmask = s.tile.qmask; bt.init = level == 0 ? 1 : 0;
-            memcpy(bt.pos, s.tile.pos, sizeof bt.pos);
-            bt.op0 = (uint32_t)ops.size(); bt.nops = (uint32_t)s.tile.nops;
-            for (int o = 0; o < s.tile.nops; o++) {
-                TileOp op = plan.tile_ops[s.tile.op0 + o];
-                op.pad = (uint32_t)lslot[op.pad];
-                ops.push_back(op);
+            P.passes.push_back(s.tile);
+        }
+    }
+    std::vector<SmallTask> small;
+    std::vector<std::vector<BigTask>> per_level, per_elevel;
+    std::vector<FoldTask> folds;
+    B.eterms.clear();
+    uint64_t arena_elems = 0;
+    uint32_t nparts = 0;
+    for (size_t ci = 0; ci < B.circs.size(); ci++) {
+        auto &C = B.circs[ci];
+        const auto &P = B.progs[C.prog];
+        C.eterm_idx.assign(C.terms.size(), 0);
+        if (P.small) {
+            SmallTask st{};
+            st.n = P.n; st.op0 = P.op0; st.nops = P.nops; st.out = (uint32_t)ci; st.slot_base = C.slot_base;
+            st.t0 = (uint32_t)B.eterms.size(); st.nt = (uint32_t)C.terms.size();
+            for (size_t k = 0; k < C.terms.size(); k++) {
+                ETerm t = make_eterm(C.terms[k]);
+                t.pad = (uint32_t)C.terms[k].x_mask;  // local = global inside a whole-state tile
+                C.eterm_idx[k] = (uint32_t)B.eterms.size();
+                B.eterms.push_back(t);
             }
+            small.push_back(st);
+            continue;
+        }
+        const uint64_t off = arena_elems;
+        arena_elems += uint64_t(1) << P.n;
+        const uint32_t tiles = uint32_t(1) << (P.n - B.K);
+        size_t level = 0;
+        for (const TilePass &tp : P.passes) {
+            BigTask bt{};
+            bt.off = off; bt.qmask = tp.qmask; bt.init = level == 0 ? 1 : 0; bt.slot_base = C.slot_base;
+            memcpy(bt.pos, tp.pos, sizeof bt.pos);
+            bt.op0 = P.ops_base + (uint32_t)tp.op0; bt.nops = (uint32_t)tp.nops;
             bt.block0 = tiles;  // tile count; turned into offsets per level below
             if (per_level.size() <= level) per_level.resize(level + 1);
             per_level[level].push_back(bt);
             level++;
         }
-        if (plan.steps.empty()) {  // no gates: a pass that only writes |0..0>
+        if (P.passes.empty()) {  // no gates: one pass that only writes |0..0>
             BigTask bt{};
             bt.off = off; bt.init = 1;
             uint64_t q = 0;
-            for (int b = 0, j = 0; j < B.K; b++, j++) { q |= uint64_t(1) << b; bt.pos[j] = (uint8_t)b; }
+            for (int j = 0; j < B.K; j++) { q |= uint64_t(1) << j; bt.pos[j] = (uint8_t)j; }
             bt.qmask = q; bt.block0 = tiles;
             if (per_level.empty()) per_level.resize(1);
             per_level[0].push_back(bt);
         }
-        // expectation passes: each term goes to the first pass whose tile holds its x support
-        std::vector<qi_pauli_term> rest = C.terms;
+        // expectation passes: each term goes to the first pass whose tile holds its X/Y support
+        std::vector<size_t> rest(C.terms.size());
+        for (size_t k = 0; k < rest.size(); k++) rest[k] = k;
         uint32_t cparts = 0;
         const uint32_t part_first = nparts;
         size_t elevel = 0;
         while (!rest.empty()) {
             uint64_t q = (uint64_t(1) << L) - 1;
-            std::vector<qi_pauli_term> take, left;
-            for (auto &p : rest) {
-                if (__builtin_popcountll(q | p.x_mask) <= B.K) { q |= p.x_mask; take.push_back(p); }
-                else left.push_back(p);
+            std::vector<size_t> take, left;
+            for (size_t k : rest) {
+                uint64_t x = C.terms[k].x_mask;
+                if (__builtin_popcountll(q | x) <= B.K) { q |= x; take.push_back(k); }
+                else left.push_back(k);
             }
             for (int b = 0; __builtin_popcountll(q) < B.K; b++) q |= uint64_t(1) << b;
             BigTask bt{};
             bt.off = off; bt.qmask = q;
             int j = 0;
             uint32_t local[64];
-            for (int b = 0; b < (int)C.n; b++)
+            for (int b = 0; b < (int)P.n; b++)
                 if ((q >> b) & 1) { local[b] = j; bt.pos[j++] = (uint8_t)b; }
-            bt.t0 = (uint32_t)eterms.size(); bt.nt = (uint32_t)take.size();
-            for (auto &p : take) {
-                ETerm t = make_eterm(p);
+            bt.t0 = (uint32_t)B.eterms.size(); bt.nt = (uint32_t)take.size();
+            for (size_t k : take) {
+                ETerm t = make_eterm(C.terms[k]);
                 uint32_t lx = 0;
-                for (int b = 0; b < (int)C.n; b++)
-                    if ((p.x_mask >> b) & 1) lx |= 1u << local[b];
+                for (int b = 0; b < (int)P.n; b++)
+                    if ((C.terms[k].x_mask >> b) & 1) lx |= 1u << local[b];
                 t.pad = lx;
-                eterms.push_back(t);
+                C.eterm_idx[k] = (uint32_t)B.eterms.size();
+                B.eterms.push_back(t);
             }
             bt.part0 = nparts;
             nparts += tiles;
@@ -524,7 +556,7 @@ Status batch_plan(Batch &B) {
     B.nfold = folds.size();
     Status e;
     if (!(e = upload_vec(B.d_ops, ops)).ok()) return e;
-    if (!(e = upload_vec(B.d_terms, eterms)).ok()) return e;
+    if (!(e = upload_vec(B.d_terms, B.eterms)).ok()) return e;
     if (!(e = upload_vec(B.d_small, small)).ok()) return e;
     if (!(e = upload_vec(B.d_big, big)).ok()) return e;
     if (!(e = upload_vec(B.d_fold, folds)).ok()) return e;
@@ -658,26 +690,51 @@ qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ng
     if (!b || (ngates && !gates) || (nterms && !terms)) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
     if (n == 0 || n > 34) QB_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, n));
     Batch &B = b->b;
-    Batch::Circ C;
-    C.n = n;
     const int K = B.prec == QI_C128 ? 12 : 13, L = B.prec == QI_C128 ? 3 : 4;
-    for (size_t g = 0; g < ngates; g++) {
-        if (gates[g].kind < QI_GATE_H || gates[g].kind > QI_GATE_TOFFOLI)
-            QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuits take operator gates only"));
-        Status v = validate_gate(n, gates[g]);
-        if (!v.ok()) QB_RET(v);
-        C.gates.push_back(gates[g]);
-        C.slot_of_gate.push_back(angle_slots ? angle_slots[g] : -1);
-    }
+    // slots relative to this circuit's first angle
+    int32_t base = INT32_MAX;
+    for (size_t g = 0; g < ngates; g++)
+        if (angle_slots && angle_slots[g] >= 0) base = std::min(base, angle_slots[g]);
+    if (base == INT32_MAX) base = 0;
+    std::vector<int32_t> rel(ngates, -1);
+    for (size_t g = 0; g < ngates; g++)
+        if (angle_slots && angle_slots[g] >= 0) rel[g] = angle_slots[g] - base;
     for (size_t t = 0; t < nterms; t++) {
         uint64_t bad = (terms[t].x_mask | terms[t].z_mask) >> n;
         if (bad) QB_RET(make_err(QI_ERR_INVALID_QUBIT_INDEX, n + (uint64_t)__builtin_ctzll(bad), n));
         // a term's X/Y support must fit one tile next to the low qubits
         if ((int)n > K && __builtin_popcountll(terms[t].x_mask | ((uint64_t(1) << L) - 1)) > K)
             QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "Pauli term too wide for batched evaluation"));
-        C.terms.push_back(terms[t]);
     }
     std::lock_guard<std::mutex> lk(B.ctx->mu);
+    // reuse the previous program when the circuit has the same shape
+    uint32_t prog = (uint32_t)B.progs.size();
+    if (!B.progs.empty()) {
+        const auto &P = B.progs.back();
+        if (P.n == n && P.gates.size() == ngates && P.rel_slot == rel &&
+            (ngates == 0 || memcmp(P.gates.data(), gates, ngates * sizeof(qi_gate)) == 0))
+            prog = (uint32_t)B.progs.size() - 1;
+    }
+    if (prog == B.progs.size()) {
+        Batch::Program P;
+        P.n = n;
+        for (size_t g = 0; g < ngates; g++) {
+            if (gates[g].kind < QI_GATE_H || gates[g].kind > QI_GATE_TOFFOLI)
+                QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuits take operator gates only"));
+            Status v = validate_gate(n, gates[g]);
+            if (!v.ok()) QB_RET(v);
+            if (rel[g] >= 0 && gates[g].kind != QI_GATE_RX && gates[g].kind != QI_GATE_RY && gates[g].kind != QI_GATE_RZ &&
+                gates[g].kind != QI_GATE_P)
+                QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slots apply to RX/RY/RZ/P only"));
+        }
+        P.gates.assign(gates, gates + ngates);
+        P.rel_slot = rel;
+        B.progs.push_back(std::move(P));
+    }
+    Batch::Circ C;
+    C.prog = prog;
+    C.slot_base = base;
+    C.terms.assign(terms, terms + nterms);
     B.circs.push_back(std::move(C));
     B.planned = false;
     QB_RET(Status{});
@@ -692,8 +749,34 @@ qi_status qi_batch_set_angles(qi_batch_t b, const double *angles, size_t nangles
         if (B.d_angles.bytes < nangles * sizeof(double)) { B.planned = false; QB_RET(Status{}); }
         hipError_t e = hipMemcpyAsync(B.d_angles.p, B.angles.data(), nangles * sizeof(double), hipMemcpyHostToDevice,
                                       B.ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(B.ctx->stream);
         if (e != hipSuccess) QB_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
-        e = hipStreamSynchronize(B.ctx->stream);
+    }
+    QB_RET(Status{});
+}
+
+// Rewrites the coefficients of every circuit's terms (re, im pairs, in the
+// order the terms were added) without replanning.
+qi_status qi_batch_set_term_coeffs(qi_batch_t b, const double *coeffs, size_t nterms_total, qi_error *err) {
+    if (!b || (nterms_total && !coeffs)) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    size_t total = 0;
+    for (auto &C : B.circs) total += C.terms.size();
+    if (total != nterms_total) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "term count differs from the batch"));
+    size_t k = 0;
+    for (auto &C : B.circs)
+        for (size_t t = 0; t < C.terms.size(); t++, k++) {
+            C.terms[t].c_re = coeffs[2 * k];
+            C.terms[t].c_im = coeffs[2 * k + 1];
+            if (B.planned) {
+                ETerm &e = B.eterms[C.eterm_idx[t]];
+                e.c[0] = coeffs[2 * k];
+                e.c[1] = coeffs[2 * k + 1];
+            }
+        }
+    if (B.planned) {
+        hipError_t e = hipMemcpy(B.d_terms.p, B.eterms.data(), B.eterms.size() * sizeof(ETerm), hipMemcpyHostToDevice);
         if (e != hipSuccess) QB_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
     }
     QB_RET(Status{});
@@ -704,10 +787,12 @@ qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *
     Batch &B = b->b;
     std::lock_guard<std::mutex> lk(B.ctx->mu);
     // every angle slot must exist in the table
-    for (auto &C : B.circs)
-        for (int s : C.slot_of_gate)
-            if (s >= 0 && (size_t)s >= B.angles.size())
-                QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slot beyond the angle table"));
+    for (auto &C : B.circs) {
+        int32_t top = -1;
+        for (int32_t r : B.progs[C.prog].rel_slot) top = std::max(top, r);
+        if (top >= 0 && size_t(C.slot_base) + size_t(top) >= B.angles.size())
+            QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slot beyond the angle table"));
+    }
     QB_RET(batch_run(B, energies, use_graph != 0));
 }
 

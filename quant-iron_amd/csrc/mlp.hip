@@ -1,0 +1,590 @@
+// mlp.hip — the energy surrogate: an MLP (n_in -> H -> H -> H -> 1, GELU)
+// trained and evaluated with hand-written kernels. Weights are BF16 for the
+// matrix cores (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation) with FP32
+// master copies for Adam; activations are stored BF16 (GEMM inputs) and FP32
+// (pre-activations, needed by the GELU derivative).
+//
+// GEMM tile: 64 x 64 per workgroup, four waves of 32 x 32, K in steps of 16
+// staged through LDS with K contiguous per row, so each lane's 4 consecutive
+// K values are one 8-byte LDS read in the operand layout of the instruction
+// (A: row = lane % 16, k = 4 (lane / 16) + v; B: col = lane % 16, same k;
+// D: row = 4 (lane / 16) + r, col = lane % 16).
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "qi_internal.h"
+
+namespace qi {
+
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16;
+
+__device__ __host__ __forceinline__ bf16 f2bf(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return (bf16)(u >> 16);
+}
+__device__ __host__ __forceinline__ float bf2f(bf16 h) {
+    uint32_t u = uint32_t(h) << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float z) {
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+enum Epi { EPI_GELU = 0, EPI_STORE = 1, EPI_GELU_BWD = 2 };
+
+constexpr int TM = 64, TN = 64, TK = 16, LDK = TK + 4;  // LDS row stride (bf16), padded
+
+// C[M x N] = op(A)[M x K] op(B)[K x N]; op(A) = A (row-major M x K, lda) or
+// A^T (A row-major K x M); op(B) = B (row-major K x N) or B^T (B row-major N x K).
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B, int ldb,
+                                              int M, int N, int K, float *__restrict__ Cf, bf16 *__restrict__ Cb,
+                                              int ldc, const float *__restrict__ bias, const float *__restrict__ Zin) {
+    __shared__ bf16 As[TM][LDK];
+    __shared__ bf16 Bs[TN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int wr = wave >> 1, wc = wave & 1;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        // ---- stage op(A) tile (64 x 16) as As[m][k]
+        if (!TA) {
+            const int m = tid >> 2, kq = (tid & 3) * 4;
+            const int gm = m0 + m;
+            uint64_t v = 0;
+            if (gm < M) v = *reinterpret_cast<const uint64_t *>(A + size_t(gm) * lda + k0 + kq);
+            *reinterpret_cast<uint64_t *>(&As[m][kq]) = v;
+        } else {
+            const int k = tid >> 4, mq = (tid & 15) * 4;
+            const int gm = m0 + mq;
+            bf16 v[4] = {0, 0, 0, 0};
+            if (gm + 3 < M) {
+                uint64_t w = *reinterpret_cast<const uint64_t *>(A + size_t(k0 + k) * lda + gm);
+                memcpy(v, &w, 8);
+            } else {
+                for (int q = 0; q < 4; q++)
+                    if (gm + q < M) v[q] = A[size_t(k0 + k) * lda + gm + q];
+            }
+            for (int q = 0; q < 4; q++) As[mq + q][k] = v[q];
+        }
+        // ---- stage op(B) tile (16 x 64) as Bs[n][k]
+        if (TB) {
+            const int n = tid >> 2, kq = (tid & 3) * 4;
+            const int gn = n0 + n;
+            uint64_t v = 0;
+            if (gn < N) v = *reinterpret_cast<const uint64_t *>(B + size_t(gn) * ldb + k0 + kq);
+            *reinterpret_cast<uint64_t *>(&Bs[n][kq]) = v;
+        } else {
+            const int k = tid >> 4, nq = (tid & 15) * 4;
+            const int gn = n0 + nq;
+            bf16 v[4] = {0, 0, 0, 0};
+            if (gn + 3 < N) {
+                uint64_t w = *reinterpret_cast<const uint64_t *>(B + size_t(k0 + k) * ldb + gn);
+                memcpy(v, &w, 8);
+            } else {
+                for (int q = 0; q < 4; q++)
+                    if (gn + q < N) v[q] = B[size_t(k0 + k) * ldb + gn + q];
+            }
+            for (int q = 0; q < 4; q++) Bs[nq + q][k] = v[q];
+        }
+        __syncthreads();
+        shortx4 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            uint64_t w = *reinterpret_cast<const uint64_t *>(&As[wr * 32 + i * 16 + (lane & 15)][4 * (lane >> 4)]);
+            memcpy(&a[i], &w, 8);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            uint64_t w = *reinterpret_cast<const uint64_t *>(&Bs[wc * 32 + j * 16 + (lane & 15)][4 * (lane >> 4)]);
+            memcpy(&b[j], &w, 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                if (row >= M || col >= N) continue;
+                const size_t o = size_t(row) * ldc + col;
+                float v = acc[i][j][r];
+                if (EPI == EPI_GELU) {
+                    v += bias[col];
+                    if (Cf) Cf[o] = v;
+                    Cb[o] = f2bf(gelu(v));
+                } else if (EPI == EPI_GELU_BWD) {
+                    v *= gelu_grad(Zin[o]);
+                    Cf[o] = v;
+                    Cb[o] = f2bf(v);
+                } else {
+                    if (Cf) Cf[o] = v;
+                    if (Cb) Cb[o] = f2bf(v);
+                }
+            }
+}
+
+// y[r] = H[r,:] . w + b ; with targets: dy = 2 (y - t) / B, loss partials, and
+// dZ[r,k] = dy w[k] gelu'(Z[r,k]) (FP32 and BF16).
+__global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ H, const float *__restrict__ w, float b, int rows,
+                                              int hid, float *__restrict__ y, const float *__restrict__ t, float inv_b,
+                                              const float *__restrict__ Z, float *__restrict__ dZf, bf16 *__restrict__ dZb,
+                                              float *__restrict__ dy, float *__restrict__ loss_part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = blockIdx.x * 4 + wave;
+    __shared__ float lsum[4];
+    float l2 = 0.f;
+    if (r < rows) {
+        float s = 0.f;
+        for (int k = lane; k < hid; k += 64) s += bf2f(H[size_t(r) * hid + k]) * w[k];
+        for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
+        s += b;
+        if (y) y[r] = s;
+        if (t) {
+            const float d = 2.f * (s - t[r]) * inv_b;
+            if (lane == 0) dy[r] = d;
+            l2 = (s - t[r]) * (s - t[r]);
+            for (int k = lane; k < hid; k += 64) {
+                const size_t o = size_t(r) * hid + k;
+                const float g = d * w[k] * gelu_grad(Z[o]);
+                dZf[o] = g;
+                dZb[o] = f2bf(g);
+            }
+        }
+    }
+    if (loss_part) {
+        if (lane == 0) lsum[wave] = l2;
+        __syncthreads();
+        if (threadIdx.x == 0) loss_part[blockIdx.x] = lsum[0] + lsum[1] + lsum[2] + lsum[3];
+    }
+}
+
+// Column sums of X (rows x cols, FP32) into out[cols]: grid (ceil(cols/64), row chunks of 256).
+__global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ X, int rows, int cols, float *__restrict__ part) {
+    __shared__ float sh[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * 256;
+    float s = 0.f;
+    if (c < cols)
+        for (int r = r0 + g; r < r0 + 256 && r < rows; r += 4) s += X[size_t(r) * cols + c];
+    sh[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && c < cols) part[size_t(blockIdx.y) * cols + c] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+}
+__global__ __launch_bounds__(256) void k_colsum_fold(const float *__restrict__ part, int chunks, int cols, float *__restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.f;
+    for (int k = 0; k < chunks; k++) s += part[size_t(k) * cols + c];
+    out[c] = s;
+}
+
+// dw[k] = sum_r H[r,k] dy[r]  (head weights), db = sum_r dy[r]
+__global__ __launch_bounds__(256) void k_head_grad(const bf16 *__restrict__ H, const float *__restrict__ dy, int rows, int hid,
+                                                   float *__restrict__ dw, float *__restrict__ db) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < hid) {
+        float s = 0.f;
+        for (int r = 0; r < rows; r++) s += bf2f(H[size_t(r) * hid + k]) * dy[r];
+        dw[k] = s;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        float s = 0.f;
+        for (int r = 0; r < rows; r++) s += dy[r];
+        *db = s;
+    }
+}
+
+// Adam on FP32 masters; refreshes the BF16 copy (nullable) of each parameter.
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                                              const float *__restrict__ g, bf16 *__restrict__ wb, size_t n, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = f2bf(wi);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, bf16 *__restrict__ y, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = f2bf(x[i]);
+}
+
+// gather rows idx[0..rows) of a BF16 matrix (cols wide) into a contiguous block
+__global__ __launch_bounds__(256) void k_gather_rows(const bf16 *__restrict__ X, const uint32_t *__restrict__ idx, int rows,
+                                                     int cols, bf16 *__restrict__ out, const float *__restrict__ t,
+                                                     float *__restrict__ tout) {
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    const size_t src = size_t(idx[r]) * cols;
+    for (int c = threadIdx.x; c < cols; c += blockDim.x) out[size_t(r) * cols + c] = X[src + c];
+    if (threadIdx.x == 0 && t) tout[r] = t[idx[r]];
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host --
+struct Mlp {
+    Ctx *ctx = nullptr;
+    int n_in = 0, k_in = 0, hid = 0;  // k_in: n_in padded to a multiple of 16
+    // parameters: W1 [k_in x hid], b1, W2 [hid x hid], b2, W3 [hid x hid], b3, w4 [hid], b4
+    size_t off[8] = {0}, len[8] = {0}, nparam = 0;
+    float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 masters, Adam moments, grads
+    bf16 *wb = nullptr;                                            // BF16 copies (W1, W2, W3)
+    int step = 0;
+    // activations for a batch of up to cap rows
+    size_t cap = 0;
+    bf16 *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *dZb[3] = {nullptr, nullptr, nullptr};
+    float *Z[3] = {nullptr, nullptr, nullptr}, *dZf[3] = {nullptr, nullptr, nullptr};
+    float *y = nullptr, *t = nullptr, *dy = nullptr, *part = nullptr, *lossp = nullptr;
+    // resident training set
+    bf16 *dataX = nullptr;
+    float *dataT = nullptr;
+    uint32_t *perm = nullptr;
+    size_t ndata = 0;
+    ~Mlp() {
+        void *ps[] = {w, m, v, g, wb, X, H[0], H[1], H[2], dZb[0], dZb[1], dZb[2], Z[0], Z[1], Z[2], dZf[0], dZf[1], dZf[2],
+                      y, t, dy, part, lossp, dataX, dataT, perm};
+        for (void *p : ps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+static Status mlp_alloc(Mlp &N, size_t rows) {
+    if (rows <= N.cap) return Status{};
+    void **ps[] = {(void **)&N.X, (void **)&N.H[0], (void **)&N.H[1], (void **)&N.H[2], (void **)&N.dZb[0], (void **)&N.dZb[1],
+                   (void **)&N.dZb[2], (void **)&N.Z[0], (void **)&N.Z[1], (void **)&N.Z[2], (void **)&N.dZf[0],
+                   (void **)&N.dZf[1], (void **)&N.dZf[2], (void **)&N.y, (void **)&N.t, (void **)&N.dy, (void **)&N.part,
+                   (void **)&N.lossp};
+    QI_HIP_TRY(hipDeviceSynchronize());
+    for (void **p : ps)
+        if (*p) { (void)hipFree(*p); *p = nullptr; }
+    const size_t H = N.hid;
+    QI_HIP_TRY(hipMalloc(&N.X, rows * N.k_in * sizeof(bf16)));
+    for (int l = 0; l < 3; l++) {
+        QI_HIP_TRY(hipMalloc(&N.H[l], rows * H * sizeof(bf16)));
+        QI_HIP_TRY(hipMalloc(&N.dZb[l], rows * H * sizeof(bf16)));
+        QI_HIP_TRY(hipMalloc(&N.Z[l], rows * H * sizeof(float)));
+        QI_HIP_TRY(hipMalloc(&N.dZf[l], rows * H * sizeof(float)));
+    }
+    QI_HIP_TRY(hipMalloc(&N.y, rows * sizeof(float)));
+    QI_HIP_TRY(hipMalloc(&N.t, rows * sizeof(float)));
+    QI_HIP_TRY(hipMalloc(&N.dy, rows * sizeof(float)));
+    QI_HIP_TRY(hipMalloc(&N.part, ((rows + 255) / 256) * std::max<size_t>(H, N.k_in) * sizeof(float)));
+    QI_HIP_TRY(hipMalloc(&N.lossp, ((rows + 3) / 4) * sizeof(float)));
+    N.cap = rows;
+    return Status{};
+}
+
+template <bool TA, bool TB, int EPI>
+static void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int Nn, int K, float *Cf, bf16 *Cb,
+                 int ldc, const float *bias = nullptr, const float *Zin = nullptr) {
+    dim3 grid((Nn + TN - 1) / TN, (M + TM - 1) / TM);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, A, lda, B, ldb, M, Nn, K, Cf, Cb, ldc, bias, Zin);
+}
+
+static void colsum(Mlp &N, hipStream_t st, const float *X, int rows, int cols, float *out) {
+    const int chunks = (rows + 255) / 256;
+    hipLaunchKernelGGL(k_colsum_part, dim3((cols + 63) / 64, chunks), dim3(256), 0, st, X, rows, cols, N.part);
+    hipLaunchKernelGGL(k_colsum_fold, dim3((cols + 255) / 256), dim3(256), 0, st, (const float *)N.part, chunks, cols, out);
+}
+
+// Forward over rows already staged in N.X; keeps Z (pre-activations) when training.
+static void forward(Mlp &N, hipStream_t st, int rows, bool keepZ) {
+    const bf16 *W1 = N.wb + N.off[0], *W2 = N.wb + N.off[2], *W3 = N.wb + N.off[4];
+    const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
+    gemm<false, false, EPI_GELU>(st, N.X, N.k_in, W1, N.hid, rows, N.hid, N.k_in, keepZ ? N.Z[0] : nullptr, N.H[0], N.hid, b1);
+    gemm<false, false, EPI_GELU>(st, N.H[0], N.hid, W2, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[1] : nullptr, N.H[1], N.hid, b2);
+    gemm<false, false, EPI_GELU>(st, N.H[1], N.hid, W3, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[2] : nullptr, N.H[2], N.hid, b3);
+}
+
+// Gradients of the mean squared error over `rows` staged rows (targets in N.t) into N.g.
+static void backward(Mlp &N, hipStream_t st, int rows, float b4) {
+    const int H = N.hid;
+    const float *w4 = N.w + N.off[6];
+    const int hb = (rows + 3) / 4;
+    hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, (const bf16 *)N.H[2], w4, b4, rows, H, N.y, (const float *)N.t,
+                       1.0f / rows, (const float *)N.Z[2], N.dZf[2], N.dZb[2], N.dy, N.lossp);
+    hipLaunchKernelGGL(k_head_grad, dim3((H + 255) / 256), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)N.dy, rows,
+                       H, N.g + N.off[6], N.g + N.off[7]);
+    // layer 3: dW3 = H2^T dZ3, db3 = colsum dZ3, dZ2 = (dZ3 W3^T) * gelu'(Z2)
+    const bf16 *W2 = N.wb + N.off[2], *W3 = N.wb + N.off[4];
+    gemm<true, false, EPI_STORE>(st, N.H[1], H, N.dZb[2], H, H, H, rows, N.g + N.off[4], nullptr, H);
+    colsum(N, st, N.dZf[2], rows, H, N.g + N.off[5]);
+    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[2], H, W3, H, rows, H, H, N.dZf[1], N.dZb[1], H, nullptr, N.Z[1]);
+    // layer 2
+    gemm<true, false, EPI_STORE>(st, N.H[0], H, N.dZb[1], H, H, H, rows, N.g + N.off[2], nullptr, H);
+    colsum(N, st, N.dZf[1], rows, H, N.g + N.off[3]);
+    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[1], H, W2, H, rows, H, H, N.dZf[0], N.dZb[0], H, nullptr, N.Z[0]);
+    // layer 1: dW1 = X^T dZ1, db1
+    gemm<true, false, EPI_STORE>(st, N.X, N.k_in, N.dZb[0], H, N.k_in, H, rows, N.g + N.off[0], nullptr, H);
+    colsum(N, st, N.dZf[0], rows, H, N.g + N.off[1]);
+}
+
+static Status stage_rows(Mlp &N, hipStream_t st, const float *x, const float *t, size_t rows) {
+    // host FP32 rows (n_in wide) -> device BF16 rows (k_in wide, zero padded)
+    std::vector<bf16> hx(rows * N.k_in, 0);
+    for (size_t r = 0; r < rows; r++)
+        for (int c = 0; c < N.n_in; c++) hx[r * N.k_in + c] = f2bf(x[r * N.n_in + c]);
+    QI_HIP_TRY(hipMemcpyAsync(N.X, hx.data(), hx.size() * sizeof(bf16), hipMemcpyHostToDevice, st));
+    if (t) QI_HIP_TRY(hipMemcpyAsync(N.t, t, rows * sizeof(float), hipMemcpyHostToDevice, st));
+    QI_HIP_TRY(hipStreamSynchronize(st));  // hx is a host temporary
+    return Status{};
+}
+
+static Status head_bias(Mlp &N, hipStream_t st, float *b4) {
+    QI_HIP_TRY(hipMemcpyAsync(b4, N.w + N.off[7], sizeof(float), hipMemcpyDeviceToHost, st));
+    QI_HIP_TRY(hipStreamSynchronize(st));
+    return Status{};
+}
+
+}  // namespace qi
+
+using namespace qi;
+
+struct qi_mlp_s {
+    qi::Mlp m;
+};
+
+#define QM_RET(st)                 \
+    do {                           \
+        Status s_ = (st);          \
+        fill_err(err, s_);         \
+        return (qi_status)s_.code; \
+    } while (0)
+#define QM_TRY(expr)                            \
+    do {                                        \
+        Status s2_ = (expr);                    \
+        if (!s2_.ok()) QM_RET(s2_);             \
+    } while (0)
+
+extern "C" {
+
+qi_status qi_mlp_create(qi_ctx_t ctx, uint32_t n_in, uint32_t hidden, uint64_t seed, qi_mlp_t *out, qi_error *err) {
+    if (!ctx || !out) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    if (n_in == 0 || hidden == 0 || hidden % 64) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "hidden width must be a multiple of 64"));
+    std::unique_ptr<qi_mlp_s> h(new qi_mlp_s());
+    Mlp &N = h->m;
+    N.ctx = &ctx->c;
+    N.n_in = (int)n_in;
+    N.k_in = (int)((n_in + 15) / 16 * 16);
+    N.hid = (int)hidden;
+    const size_t H = hidden;
+    N.len[0] = size_t(N.k_in) * H; N.len[1] = H; N.len[2] = H * H; N.len[3] = H; N.len[4] = H * H; N.len[5] = H;
+    N.len[6] = H; N.len[7] = 1;
+    for (int i = 0; i < 8; i++) { N.off[i] = N.nparam; N.nparam += N.len[i]; }
+    // He-style init (normal, std sqrt(2 / fan_in)), zero biases; padded input rows zero
+    std::vector<float> w(N.nparam, 0.f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    auto fill = [&](int p, int fan_in, int rows_real, int cols) {
+        const float s = std::sqrt(2.0f / fan_in);
+        for (int r = 0; r < rows_real; r++)
+            for (int c = 0; c < cols; c++) w[N.off[p] + size_t(r) * cols + c] = s * nd(rng);
+    };
+    fill(0, N.n_in, N.n_in, N.hid);
+    fill(2, N.hid, N.hid, N.hid);
+    fill(4, N.hid, N.hid, N.hid);
+    fill(6, N.hid, N.hid, 1);
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    hipError_t e;
+    if ((e = hipMalloc(&N.w, N.nparam * 4)) || (e = hipMalloc(&N.m, N.nparam * 4)) || (e = hipMalloc(&N.v, N.nparam * 4)) ||
+        (e = hipMalloc(&N.g, N.nparam * 4)) || (e = hipMalloc(&N.wb, N.nparam * 2)))
+        QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+    if ((e = hipMemcpy(N.w, w.data(), N.nparam * 4, hipMemcpyHostToDevice)) || (e = hipMemset(N.m, 0, N.nparam * 4)) ||
+        (e = hipMemset(N.v, 0, N.nparam * 4)) || (e = hipMemset(N.g, 0, N.nparam * 4)))
+        QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, st, (const float *)N.w, N.wb, N.nparam);
+    if ((e = hipStreamSynchronize(st))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    *out = h.release();
+    QM_RET(Status{});
+}
+
+qi_status qi_mlp_destroy(qi_mlp_t m) {
+    if (!m) return QI_OK;
+    {
+        std::lock_guard<std::mutex> lk(m->m.ctx->mu);
+        (void)hipStreamSynchronize(m->m.ctx->stream);
+        delete m;
+    }
+    return QI_OK;
+}
+
+qi_status qi_mlp_num_params(qi_mlp_t m, uint64_t *n, qi_error *err) {
+    if (!m || !n) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    *n = m->m.nparam;
+    QM_RET(Status{});
+}
+
+// Parameters in the layout W1 [k_in x H] (rows >= n_in are zero), b1, W2, b2, W3, b3, w4, b4.
+qi_status qi_mlp_get_params(qi_mlp_t m, float *out, qi_error *err) {
+    if (!m || !out) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipError_t e = hipMemcpy(out, N.w, N.nparam * 4, hipMemcpyDeviceToHost);
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    QM_RET(Status{});
+}
+
+qi_status qi_mlp_set_params(qi_mlp_t m, const float *in, qi_error *err) {
+    if (!m || !in) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipError_t e = hipMemcpy(N.w, in, N.nparam * 4, hipMemcpyHostToDevice);
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, N.ctx->stream, (const float *)N.w, N.wb, N.nparam);
+    if ((e = hipStreamSynchronize(N.ctx->stream))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    QM_RET(Status{});
+}
+
+// Mean-squared-error gradient over `rows` samples (the check against the CPU oracle).
+qi_status qi_mlp_gradients(qi_mlp_t m, const float *x, const float *t, size_t rows, float *grads, float *loss, qi_error *err) {
+    if (!m || !x || !t || !grads || rows == 0 || rows % 16)
+        QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "rows must be a positive multiple of 16"));
+    Mlp &N = m->m;
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    QM_TRY(mlp_alloc(N, rows));
+    QM_TRY(stage_rows(N, st, x, t, rows));
+    float b4;
+    QM_TRY(head_bias(N, st, &b4));
+    forward(N, st, (int)rows, true);
+    backward(N, st, (int)rows, b4);
+    std::vector<float> lp((rows + 3) / 4);
+    hipError_t e = hipMemcpyAsync(grads, N.g, N.nparam * 4, hipMemcpyDeviceToHost, st);
+    if (!e) e = hipMemcpyAsync(lp.data(), N.lossp, lp.size() * 4, hipMemcpyDeviceToHost, st);
+    if (!e) e = hipStreamSynchronize(st);
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if (loss) { double s = 0; for (float v : lp) s += v; *loss = (float)(s / rows); }
+    QM_RET(Status{});
+}
+
+// Uploads a training set (rows x n_in inputs, rows targets) to the device once.
+qi_status qi_mlp_set_data(qi_mlp_t m, const float *x, const float *t, size_t rows, qi_error *err) {
+    if (!m || !x || !t || rows == 0) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    if (N.dataX) { (void)hipFree(N.dataX); (void)hipFree(N.dataT); (void)hipFree(N.perm); N.dataX = nullptr; }
+    hipError_t e;
+    if ((e = hipMalloc(&N.dataX, rows * N.k_in * sizeof(bf16))) || (e = hipMalloc(&N.dataT, rows * 4)) ||
+        (e = hipMalloc(&N.perm, rows * 4)))
+        QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+    const size_t chunk = 65536;
+    std::vector<bf16> hx(chunk * N.k_in);
+    for (size_t r0 = 0; r0 < rows; r0 += chunk) {
+        size_t nr = std::min(chunk, rows - r0);
+        std::fill(hx.begin(), hx.end(), 0);
+        for (size_t r = 0; r < nr; r++)
+            for (int c = 0; c < N.n_in; c++) hx[r * N.k_in + c] = f2bf(x[(r0 + r) * N.n_in + c]);
+        if ((e = hipMemcpy(N.dataX + r0 * N.k_in, hx.data(), nr * N.k_in * sizeof(bf16), hipMemcpyHostToDevice)))
+            QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    }
+    if ((e = hipMemcpy(N.dataT, t, rows * 4, hipMemcpyHostToDevice))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    N.ndata = rows;
+    QM_RET(Status{});
+}
+
+// One epoch of Adam over the resident training set in shuffled mini-batches.
+qi_status qi_mlp_train_epoch(qi_mlp_t m, uint32_t batch, float lr, uint64_t shuffle_seed, float *mean_loss, qi_error *err) {
+    if (!m || batch == 0 || batch % 16) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batch must be a positive multiple of 16"));
+    Mlp &N = m->m;
+    if (!N.ndata) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "no training data"));
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    QM_TRY(mlp_alloc(N, batch));
+    std::vector<uint32_t> perm(N.ndata);
+    for (size_t i = 0; i < N.ndata; i++) perm[i] = (uint32_t)i;
+    std::mt19937_64 rng(shuffle_seed);
+    std::shuffle(perm.begin(), perm.end(), rng);
+    hipError_t e = hipMemcpy(N.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice);
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    const size_t nb = N.ndata / batch;  // full batches only
+    std::vector<float> lp((batch + 3) / 4);
+    double loss_sum = 0.0;
+    for (size_t b = 0; b < nb; b++) {
+        hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, st, (const bf16 *)N.dataX, (const uint32_t *)N.perm + b * batch,
+                           (int)batch, N.k_in, N.X, (const float *)N.dataT, N.t);
+        float b4;
+        QM_TRY(head_bias(N, st, &b4));
+        forward(N, st, (int)batch, true);
+        backward(N, st, (int)batch, b4);
+        N.step++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)N.step), c2 = 1.f - std::pow(b2, (float)N.step);
+        hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, N.w, N.m, N.v, (const float *)N.g, N.wb, N.nparam, lr, b1, b2,
+                           1e-8f, c1, c2);
+        if ((e = hipMemcpyAsync(lp.data(), N.lossp, lp.size() * 4, hipMemcpyDeviceToHost, st))) break;
+        if ((e = hipStreamSynchronize(st))) break;
+        double s = 0;
+        for (float v : lp) s += v;
+        loss_sum += s / batch;
+    }
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if (mean_loss) *mean_loss = nb ? (float)(loss_sum / nb) : 0.f;
+    QM_RET(Status{});
+}
+
+// Inference: y[r] for rows x n_in inputs (FP32 host arrays), in chunks.
+qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_error *err) {
+    if (!m || !x || !y) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    const size_t chunk = 65536;
+    QM_TRY(mlp_alloc(N, std::min(chunk, std::max<size_t>(rows, 1))));
+    float b4;
+    QM_TRY(head_bias(N, st, &b4));
+    hipError_t e = hipSuccess;
+    double dev_ms = 0.0;
+    for (size_t r0 = 0; r0 < rows; r0 += chunk) {
+        const size_t nr = std::min(chunk, rows - r0);
+        QM_TRY(stage_rows(N, st, x + r0 * N.n_in, nullptr, nr));
+        hipEvent_t a = N.ctx->ev0, b = N.ctx->ev1;
+        (void)hipEventRecord(a, st);
+        forward(N, st, (int)nr, false);
+        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]), b4,
+                           (int)nr, N.hid, N.y, (const float *)nullptr, 0.f, (const float *)nullptr, (float *)nullptr,
+                           (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+        (void)hipEventRecord(b, st);
+        if ((e = hipMemcpyAsync(y + r0, N.y, nr * 4, hipMemcpyDeviceToHost, st))) break;
+        if ((e = hipStreamSynchronize(st))) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        dev_ms += ms;
+    }
+    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    N.ctx->last_exec_ms = dev_ms;
+    QM_RET(Status{});
+}
+
+}  // extern "C"

@@ -129,6 +129,7 @@ SIGNATURES = {
     "qi_batch_destroy": [P],
     "qi_batch_add": [P, U32, C.c_void_p, C.c_size_t, C.POINTER(C.c_int32), C.c_void_p, C.c_size_t, ERR],
     "qi_batch_set_angles": [P, DP, C.c_size_t, ERR],
+    "qi_batch_set_term_coeffs": [P, DP, C.c_size_t, ERR],
     "qi_batch_run": [P, DP, C.c_int, ERR],
     "qi_batch_info": [P, U64P, U64P, U64P, DP, ERR],
     "qi_last_exec_ms": [P, DP, ERR],

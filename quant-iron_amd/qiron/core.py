@@ -918,6 +918,13 @@ class CircuitBatch:
         err = L.QiError()
         _check(self.lib.qi_batch_set_angles(self._h, L.dptr(a), a.size, C.byref(err)), err)
 
+    def set_term_coeffs(self, coeffs) -> None:
+        """Every circuit's term coefficients (complex), circuit by circuit in add order."""
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128))
+        err = L.QiError()
+        _check(self.lib.qi_batch_set_term_coeffs(self._h, c.view(np.float64).ctypes.data_as(L.DP), c.size,
+                                                 C.byref(err)), err)
+
     def run(self, use_graph: bool = True) -> np.ndarray:
         out = np.zeros(2 * max(self.count, 1), dtype=np.float64)
         err = L.QiError()

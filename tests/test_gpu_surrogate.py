@@ -1,0 +1,80 @@
+"""The learned surrogate: hand-written HIP forward/backward checked against the
+FP64 CPU oracle, training on simulator-generated energies, and what screening
+with the trained network gains over screening without it."""
+import math
+
+import numpy as np
+import pytest
+
+import mlp_oracle as M
+import oracle as O
+from qiron import surrogate as S
+from qiron import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_gradients_match_fp64_oracle(ctx):
+    n_in, H, rows = 195, 512, 64
+    net = S.MLP(n_in, H, seed=3)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, size=(rows, n_in)).astype(np.float32)
+    t = rng.normal(size=rows).astype(np.float32)
+    loss, g = net.gradients(x, t)
+    p = net.params().astype(np.float64)
+    # the matrix cores see BF16 weights and inputs: give the oracle the same operands
+    k_in = net.k_in
+    sizes = [k_in * H, H, H * H, H, H * H, H, H, 1]
+    offs = np.cumsum([0] + sizes)
+    pq = p.copy()
+    for blk in (0, 2, 4):
+        pq[offs[blk]:offs[blk + 1]] = M.bf16_round(p[offs[blk]:offs[blk + 1]])
+    lo, go = M.loss_and_grad(pq, M.bf16_round(x), t.astype(np.float64), k_in, H)
+    assert abs(loss - lo) <= 2e-2 * abs(lo) + 1e-6, (loss, lo)
+    names = ["W1", "b1", "W2", "b2", "W3", "b3", "w4", "b4"]
+    for blk, name in enumerate(names):
+        a, b = g[offs[blk]:offs[blk + 1]].astype(np.float64), go[offs[blk]:offs[blk + 1]]
+        if blk == 0:  # padded input rows carry no gradient signal
+            a, b = a.reshape(k_in, H)[:n_in], b.reshape(k_in, H)[:n_in]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        cos = float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+        print(f"{name}: rel L2 err {rel:.3e} cos {cos:.6f}")
+        assert rel < 5e-2 and cos > 0.998, name
+
+
+def test_ansatz_energies_match_oracle(ctx):
+    n, layers = 5, 2
+    sim = S.AnsatzEnergies(n, layers, slots=16, precision="c64")
+    rng = W.SplitMix64(11)
+    th, J = S.sample_params(rng, 20, sim.nang)
+    e = sim.energies(th, J)
+    for k in range(20):
+        gates = W.ansatz_circuit(n, layers, th[k])
+        ref = O.expectation(O.apply_gates(O.init(n, 0), gates), W.heisenberg_open_1d(n, *J[k]))
+        assert abs(e[k] - ref.real) < 2e-5, (k, e[k], ref)
+
+
+def test_training_on_simulated_energies_gains_over_unscreened_selection(ctx):
+    n, layers = 6, 2
+    sim = S.AnsatzEnergies(n, layers, slots=4096, precision="c64")
+    rng = W.SplitMix64(21)
+    th, J = S.sample_params(rng, 65536, sim.nang)
+    E = sim.energies(th, J)
+    mu, sd = float(E.mean()), float(E.std())
+    net = S.MLP(S.n_features(sim.nang), 256, seed=5)
+    net.set_data(S.features(th, J), ((E - mu) / sd).astype(np.float32))
+    losses = [net.train_epoch(256, 1e-3, seed=e) for e in range(12)]
+    print("training loss", [round(x, 4) for x in losses])
+    assert losses[-1] < 0.5 * losses[0]
+    # held-out fit
+    thv, Jv = S.sample_params(rng, 4096, sim.nang)
+    Ev = sim.energies(thv, Jv)
+    pv = net.predict(S.features(thv, Jv)) * sd + mu
+    r2 = 1 - np.mean((pv - Ev) ** 2) / np.var(Ev)
+    print(f"held-out R^2 {r2:.3f}")
+    assert r2 > 0.3
+    # screen fresh candidates: top 1% by the surrogate vs a random 1%
+    th2, J2 = S.sample_params(rng, 8192, sim.nang)
+    res = S.screen(net, sim, th2, J2, 0.01, mu, sd, np.random.default_rng(1))
+    print(res)
+    assert res.mean_exact_kept < res.mean_exact_random - 0.5 * sd
