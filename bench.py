@@ -3,7 +3,7 @@
 One step = Circuit::execute of the 1 880-gate circuit (per layer: a random gate
 from {H,X,Y,Z,RX,RY,RZ,P} on each of 30 qubits, 15 disjoint CNOT/CZ/SWAP, 2
 Toffolis) on a device-resident 2^30-amplitude state. The headline value is
-whole-job gates/s of the LDS-tiled executor with complex64 amplitudes;
+whole-job gates/s of the register-tiled executor with complex64 amplitudes;
 complex128 and the other executors are reported beside it.
 
 Prints ONE JSON line (rank 0). Progress goes to stderr.
@@ -79,6 +79,7 @@ def main():
     ap.add_argument("--only", default="", help="run only the headline executor (for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the state-vector engine here is single-GPU "
@@ -97,7 +98,7 @@ def main():
     G = int(gates.size)
     log(f"circuit: {G} gates on {n} qubits")
 
-    tiled = dict(mode=L.EXEC_TILED, use_graph=True)
+    tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)
     results = {}
     # ---- headline: tiled executor, complex64
     wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c64", gates, tiled, args.steps, args.warmup)
@@ -112,7 +113,7 @@ def main():
                                 pass_GBps=achieved, norm=norm)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_tile_pass<float,13>", "algorithmic_bytes_per_launch": 2 * S64}
+                "kernel": "k_reg_pass<float,13>", "algorithmic_bytes_per_launch": 2 * S64}
 
     if not args.only:
         # ---- single 30-qubit gate application (streaming kernel): the HBM-roofline target
@@ -173,7 +174,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"random circuit, {n} qubits, depth {depth} ({G} gates), complex64 amplitudes, "
-                               f"LDS-tiled executor", "qubits": n, "depth": depth, "gates": G, "seed": W.SEED_BASE + 2},
+                               f"register-tiled executor", "qubits": n, "depth": depth, "gates": G, "seed": W.SEED_BASE + 2},
         "roofline": roofline, "cpu_baseline": cpu, "device": info["name"], "extra": results,
     }
     print(json.dumps(out), flush=True)

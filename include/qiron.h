@@ -315,11 +315,20 @@ qi_status qi_mlp_set_params(qi_mlp_t m, const float *in, qi_error *err);
 qi_status qi_mlp_gradients(qi_mlp_t m, const float *x, const float *t, size_t rows, float *grads,
                            float *loss, qi_error *err);
 /* Uploads a training set once; qi_mlp_train_epoch runs Adam over shuffled
- * mini-batches of `batch` rows (batch % 16 == 0). */
+ * mini-batches of `batch` rows (batch % 16 == 0) with no host round trip
+ * inside the epoch, and returns the epoch's mean squared error. */
 qi_status qi_mlp_set_data(qi_mlp_t m, const float *x, const float *t, size_t rows, qi_error *err);
 qi_status qi_mlp_train_epoch(qi_mlp_t m, uint32_t batch, float lr, uint64_t shuffle_seed,
                              float *mean_loss, qi_error *err);
 qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_error *err);
+/* Predictions for `count` ansatz candidates generated on the device (n_in must
+ * be 2 nang + 3). Candidate c's angles are SplitMix64 draws
+ * draw0 + c*nang + j + 1 of the stream `seed` (theta_j = 2 pi u), its
+ * couplings the draws draw0 + count*nang + 3c + i + 1 (J_i = 0.5 + u); the
+ * inputs are (cos theta_j, sin theta_j, J_i - 1). The caller regenerates the
+ * parameters of the candidates it keeps from their indices. */
+qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang,
+                                float *y, qi_error *err);
 
 /* ---------------- timing hooks (for benchmarks) ---------------- */
 /* Device time of the last qi_apply_gates call's launch sequence, from HIP

@@ -26,9 +26,15 @@ HIP_FLAGS = [
 ]
 
 
+# Per-file flags. tiled.hip: common-code sinking across the switch over
+# register bits would merge element updates into stores through selected
+# addresses, which keeps the thread's tile in scratch memory.
+FILE_FLAGS = {"tiled.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{r.stderr[-6000:]}")

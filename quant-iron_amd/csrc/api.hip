@@ -379,13 +379,13 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
 static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates) {
     Ctx &C = *s.ctx;
     hipStream_t st = C.stream;
-    const TileOp *dops = (const TileOp *)C.plan.p;
+    const RegOp *dops = (const RegOp *)C.plan.p;
     const unsigned char *dmats = (const unsigned char *)C.plan.p + plan.mats_offset();
     for (const ExecStep &step : plan.steps) {
         Status e;
         switch (step.type) {
         case ExecStep::GATE: e = launch_gate(s, step.L, st); break;
-        case ExecStep::TILE: e = launch_tile_pass(s, step.tile, dops, st); break;
+        case ExecStep::REG: e = launch_reg_pass(s, step.reg, dops, st); break;
         case ExecStep::FUSED: e = launch_fused_block(s, step.fused, dmats + step.fused.mat_off, st); break;
         default: e = apply_validated(s, gates[step.host_gate]); break;
         }
@@ -418,7 +418,7 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
         if (!(e = ensure(C.plan, plan.blob_bytes())).ok()) return e;
         QI_HIP_TRY(hipStreamSynchronize(C.stream));  // the previous plan may still be in use
         std::vector<unsigned char> blob(plan.blob_bytes(), 0);
-        if (!plan.tile_ops.empty()) memcpy(blob.data(), plan.tile_ops.data(), plan.tile_ops.size() * sizeof(TileOp));
+        if (!plan.reg_ops.empty()) memcpy(blob.data(), plan.reg_ops.data(), plan.reg_ops.size() * sizeof(RegOp));
         if (!plan.mats.empty()) memcpy(blob.data() + plan.mats_offset(), plan.mats.data(), plan.mats.size());
         QI_HIP_TRY(hipMemcpy(C.plan.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     }

@@ -438,7 +438,7 @@ Status batch_plan(Batch &B) {
         State fake;
         fake.n = P.n; fake.prec = B.prec;
         ExecPlan plan;
-        Status e = plan_tiled(fake, lw, B.K, plan);
+        Status e = plan_lds_passes(fake, lw, B.K, plan);
         if (!e.ok()) return e;
         P.ops_base = (uint32_t)ops.size();
         for (TileOp op : plan.tile_ops) { op.pad = (uint32_t)lslot[op.pad]; ops.push_back(op); }

@@ -27,6 +27,48 @@ struct TilePass {
     uint8_t pos[48];     // qubit of each local bit, ascending
 };
 
+// ---- register-resident tile passes (single-state executor) ----
+// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread. A layout
+// assigns each of the K local index bits to a thread bit or a register bit.
+constexpr int kRegThreadBits = 8;
+
+enum RegOpKind : int32_t {
+    RO_X = 0, RO_H, RO_REAL, RO_GEN,   // 2x2 on a register-held target
+    RO_Z, RO_PHASE, RO_RZ,             // diagonal on any target
+    RO_TRANSPOSE,                      // new layout through LDS
+    RO_LDS,                            // gate on the LDS copy (controlled SWAP, Matchgate)
+    RO_NOP                             // closes each pass's slice (read ahead, never run)
+};
+
+struct RegOp {
+    int32_t kind;        // RegOpKind
+    int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
+    uint8_t rb;          // pair ops: register bit of the target
+    uint8_t dloc;        // diagonal ops: 0 register bit, 1 thread-held local bit, 2 qubit outside the tile
+    uint8_t dbit;        // that register bit / local bit / qubit
+    uint8_t lp0, lp1;    // RO_LDS: local bits of the targets
+    uint8_t pad8[3];
+    uint32_t rcm;        // controls held in registers (register-bit mask)
+    uint32_t tcm;        // controls held by threads (local-bit mask)
+    uint32_t lcm;        // RO_LDS: controls inside the tile (local-bit mask)
+    uint32_t src;        // source gate index
+    uint64_t gcm;        // controls outside the tile (qubit mask)
+    uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
+    uint8_t lay_t[8];    //               local bit of each thread bit
+    double c[8];         // coefficients (see the planner)
+};
+
+struct RegPass {
+    uint64_t qmask;      // qubits spanned by a tile
+    int32_t K, op0, nops, pad;
+    uint8_t lay_r[8];    // layout at load: local bit of each register bit
+    uint8_t lay_t[8];    //                 local bit of each thread bit
+    uint8_t qr_in[8];    // qubit of each register bit at load
+    uint8_t qt_in[8];    // qubit of each thread bit at load
+    uint8_t qr_out[8];   // at store (SWAPs relabel qubits, the final layout differs)
+    uint8_t qt_out[8];
+};
+
 // A dense 2^k x 2^k block applied to tiles of 2^(k + kc) amplitudes.
 struct FusedBlock {
     uint64_t bmask;      // block qubits
@@ -40,28 +82,36 @@ struct FusedBlock {
 };
 
 struct ExecStep {
-    enum Type { GATE = 0, TILE = 1, FUSED = 2, HOST = 3 } type = GATE;
+    enum Type { GATE = 0, TILE = 1, FUSED = 2, HOST = 3, REG = 4 } type = GATE;
     Lowered L;
     TilePass tile{};
+    RegPass reg{};
     FusedBlock fused{};
     size_t host_gate = 0;  // index into the caller's gate list (HOST)
     static ExecStep gate(const Lowered &l) { ExecStep s; s.type = GATE; s.L = l; return s; }
     static ExecStep tile_step(const TilePass &p) { ExecStep s; s.type = TILE; s.tile = p; return s; }
     static ExecStep fused_step(const FusedBlock &b) { ExecStep s; s.type = FUSED; s.fused = b; return s; }
+    static ExecStep reg_step(const RegPass &p) { ExecStep s; s.type = REG; s.reg = p; return s; }
     static ExecStep host(size_t g) { ExecStep s; s.type = HOST; s.host_gate = g; return s; }
 };
 
 struct ExecPlan {
     std::vector<ExecStep> steps;
-    std::vector<TileOp> tile_ops;          // uploaded first in the blob
-    std::vector<unsigned char> mats;       // fused matrices, after the tile ops
-    size_t mats_offset() const { return (tile_ops.size() * sizeof(TileOp) + 255) & ~size_t(255); }
+    std::vector<TileOp> tile_ops;          // LDS passes (batched engine; uploaded by it)
+    std::vector<RegOp> reg_ops;            // uploaded first in the blob
+    std::vector<unsigned char> mats;       // fused matrices, after the register-pass ops
+    size_t mats_offset() const { return (reg_ops.size() * sizeof(RegOp) + 255) & ~size_t(255); }
     size_t blob_bytes() const { return mats_offset() + mats.size(); }
 };
 
 int default_tile_qubits(const State &s);
+int tile_low_qubits(int prec);
+// Register-resident passes for one state (ExecStep::REG, plan.reg_ops).
 Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan);
-Status launch_tile_pass(State &s, const TilePass &p, const TileOp *dops, hipStream_t st);
+Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st);
+// LDS passes over the same pass selection (ExecStep::TILE, plan.tile_ops),
+// used by the batched engine.
+Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan);
 
 Status plan_fused(const State &s, const std::vector<Lowered> &gates, int kb, ExecPlan &plan);
 Status launch_fused_block(State &s, const FusedBlock &b, const void *dmat, hipStream_t st);

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int ld
 
 // y[r] = H[r,:] . w + b ; with targets: dy = 2 (y - t) / B, loss partials, and
 // dZ[r,k] = dy w[k] gelu'(Z[r,k]) (FP32 and BF16).
-__global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ H, const float *__restrict__ w, float b, int rows,
+__global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ H, const float *__restrict__ w, const float *__restrict__ bp, int rows,
                                               int hid, float *__restrict__ y, const float *__restrict__ t, float inv_b,
                                               const float *__restrict__ Z, float *__restrict__ dZf, bf16 *__restrict__ dZb,
                                               float *__restrict__ dy, float *__restrict__ loss_part) {
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ H, const 
         float s = 0.f;
         for (int k = lane; k < hid; k += 64) s += bf2f(H[size_t(r) * hid + k]) * w[k];
         for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m);
-        s += b;
+        s += *bp;
         if (y) y[r] = s;
         if (t) {
             const float d = 2.f * (s - t[r]) * inv_b;
@@ -252,6 +252,52 @@ __global__ __launch_bounds__(256) void k_gather_rows(const bf16 *__restrict__ X,
 
 }  // namespace
 
+// acc += sum of the loss partials of one batch (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_loss_accum(const float *__restrict__ part, int n, double *__restrict__ acc) {
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += sh[0];
+}
+
+// SplitMix64 draw k (1-based) of the stream seeded with `seed` (Steele, Lea &
+// Flood 2014), as a uniform in [0, 1) with 53 bits.
+__device__ __forceinline__ double sm64_uniform(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * 0x1.0p-53;
+}
+
+// Network inputs of ansatz candidates c0 .. c0 + rows drawn from a SplitMix64
+// stream: candidate c's angles are draws draw0 + c nang + j + 1 (theta = 2 pi u),
+// its couplings draws draw0 + count nang + 3 c + i + 1 (J = 0.5 + u). Row
+// layout: cos theta_j, sin theta_j, J_i - 1, zero padding to k_in.
+__global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t draw0, uint64_t count, uint64_t c0, int rows,
+                                                         int nang, int k_in, bf16 *__restrict__ X) {
+    const size_t total = size_t(rows) * k_in;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+        const uint64_t r = i / k_in, col = i % k_in, c = c0 + r;
+        float f = 0.f;
+        if (col < uint64_t(2 * nang)) {
+            const uint64_t j = col < uint64_t(nang) ? col : col - nang;
+            const double th = 2.0 * 3.14159265358979323846 * sm64_uniform(seed, draw0 + c * nang + j + 1);
+            f = (float)(col < uint64_t(nang) ? cos(th) : sin(th));
+        } else if (col < uint64_t(2 * nang + 3)) {
+            const uint64_t q = col - 2 * nang;
+            f = (float)(0.5 + sm64_uniform(seed, draw0 + count * nang + 3 * c + q + 1) - 1.0);
+        }
+        X[i] = f2bf(f);
+    }
+}
+
 // ------------------------------------------------------------------ host --
 struct Mlp {
     Ctx *ctx = nullptr;
@@ -266,6 +312,9 @@ struct Mlp {
     bf16 *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *dZb[3] = {nullptr, nullptr, nullptr};
     float *Z[3] = {nullptr, nullptr, nullptr}, *dZf[3] = {nullptr, nullptr, nullptr};
     float *y = nullptr, *t = nullptr, *dy = nullptr, *part = nullptr, *lossp = nullptr;
+    double *lacc = nullptr;                                        // epoch loss accumulator
+    float *yall = nullptr;                                         // predictions of a candidate screen
+    size_t ycap = 0;
     // resident training set
     bf16 *dataX = nullptr;
     float *dataT = nullptr;
@@ -273,7 +322,7 @@ struct Mlp {
     size_t ndata = 0;
     ~Mlp() {
         void *ps[] = {w, m, v, g, wb, X, H[0], H[1], H[2], dZb[0], dZb[1], dZb[2], Z[0], Z[1], Z[2], dZf[0], dZf[1], dZf[2],
-                      y, t, dy, part, lossp, dataX, dataT, perm};
+                      y, t, dy, part, lossp, dataX, dataT, perm, lacc, yall};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -301,6 +350,7 @@ static Status mlp_alloc(Mlp &N, size_t rows) {
     QI_HIP_TRY(hipMalloc(&N.dy, rows * sizeof(float)));
     QI_HIP_TRY(hipMalloc(&N.part, ((rows + 255) / 256) * std::max<size_t>(H, N.k_in) * sizeof(float)));
     QI_HIP_TRY(hipMalloc(&N.lossp, ((rows + 3) / 4) * sizeof(float)));
+    if (!N.lacc) QI_HIP_TRY(hipMalloc(&N.lacc, sizeof(double)));
     N.cap = rows;
     return Status{};
 }
@@ -328,11 +378,12 @@ static void forward(Mlp &N, hipStream_t st, int rows, bool keepZ) {
 }
 
 // Gradients of the mean squared error over `rows` staged rows (targets in N.t) into N.g.
-static void backward(Mlp &N, hipStream_t st, int rows, float b4) {
+static void backward(Mlp &N, hipStream_t st, int rows) {
     const int H = N.hid;
     const float *w4 = N.w + N.off[6];
     const int hb = (rows + 3) / 4;
-    hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, (const bf16 *)N.H[2], w4, b4, rows, H, N.y, (const float *)N.t,
+    hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, (const bf16 *)N.H[2], w4, (const float *)(N.w + N.off[7]), rows, H,
+                       N.y, (const float *)N.t,
                        1.0f / rows, (const float *)N.Z[2], N.dZf[2], N.dZb[2], N.dy, N.lossp);
     hipLaunchKernelGGL(k_head_grad, dim3((H + 255) / 256), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)N.dy, rows,
                        H, N.g + N.off[6], N.g + N.off[7]);
@@ -361,11 +412,6 @@ static Status stage_rows(Mlp &N, hipStream_t st, const float *x, const float *t,
     return Status{};
 }
 
-static Status head_bias(Mlp &N, hipStream_t st, float *b4) {
-    QI_HIP_TRY(hipMemcpyAsync(b4, N.w + N.off[7], sizeof(float), hipMemcpyDeviceToHost, st));
-    QI_HIP_TRY(hipStreamSynchronize(st));
-    return Status{};
-}
 
 }  // namespace qi
 
@@ -476,10 +522,8 @@ qi_status qi_mlp_gradients(qi_mlp_t m, const float *x, const float *t, size_t ro
     hipStream_t st = N.ctx->stream;
     QM_TRY(mlp_alloc(N, rows));
     QM_TRY(stage_rows(N, st, x, t, rows));
-    float b4;
-    QM_TRY(head_bias(N, st, &b4));
     forward(N, st, (int)rows, true);
-    backward(N, st, (int)rows, b4);
+    backward(N, st, (int)rows);
     std::vector<float> lp((rows + 3) / 4);
     hipError_t e = hipMemcpyAsync(grads, N.g, N.nparam * 4, hipMemcpyDeviceToHost, st);
     if (!e) e = hipMemcpyAsync(lp.data(), N.lossp, lp.size() * 4, hipMemcpyDeviceToHost, st);
@@ -529,28 +573,30 @@ qi_status qi_mlp_train_epoch(qi_mlp_t m, uint32_t batch, float lr, uint64_t shuf
     hipError_t e = hipMemcpy(N.perm, perm.data(), perm.size() * 4, hipMemcpyHostToDevice);
     if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
     const size_t nb = N.ndata / batch;  // full batches only
-    std::vector<float> lp((batch + 3) / 4);
-    double loss_sum = 0.0;
+    if ((e = hipMemsetAsync(N.lacc, 0, sizeof(double), st))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    (void)hipEventRecord(N.ctx->ev0, st);
+    // no host round trip inside the epoch: the head bias is read on the
+    // device and the loss accumulates on the device
     for (size_t b = 0; b < nb; b++) {
         hipLaunchKernelGGL(k_gather_rows, dim3(batch), dim3(256), 0, st, (const bf16 *)N.dataX, (const uint32_t *)N.perm + b * batch,
                            (int)batch, N.k_in, N.X, (const float *)N.dataT, N.t);
-        float b4;
-        QM_TRY(head_bias(N, st, &b4));
         forward(N, st, (int)batch, true);
-        backward(N, st, (int)batch, b4);
+        backward(N, st, (int)batch);
+        hipLaunchKernelGGL(k_loss_accum, dim3(1), dim3(256), 0, st, (const float *)N.lossp, (int)((batch + 3) / 4), N.lacc);
         N.step++;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - std::pow(b1, (float)N.step), c2 = 1.f - std::pow(b2, (float)N.step);
         hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, N.w, N.m, N.v, (const float *)N.g, N.wb, N.nparam, lr, b1, b2,
                            1e-8f, c1, c2);
-        if ((e = hipMemcpyAsync(lp.data(), N.lossp, lp.size() * 4, hipMemcpyDeviceToHost, st))) break;
-        if ((e = hipStreamSynchronize(st))) break;
-        double s = 0;
-        for (float v : lp) s += v;
-        loss_sum += s / batch;
     }
-    if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
-    if (mean_loss) *mean_loss = nb ? (float)(loss_sum / nb) : 0.f;
+    (void)hipEventRecord(N.ctx->ev1, st);
+    double lsum = 0.0;
+    if ((e = hipMemcpyAsync(&lsum, N.lacc, sizeof(double), hipMemcpyDeviceToHost, st))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    if ((e = hipStreamSynchronize(st))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, N.ctx->ev0, N.ctx->ev1);
+    N.ctx->last_exec_ms = ms;
+    if (mean_loss) *mean_loss = nb ? (float)(lsum / (double(nb) * batch)) : 0.f;
     QM_RET(Status{});
 }
 
@@ -562,8 +608,6 @@ qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_e
     hipStream_t st = N.ctx->stream;
     const size_t chunk = 65536;
     QM_TRY(mlp_alloc(N, std::min(chunk, std::max<size_t>(rows, 1))));
-    float b4;
-    QM_TRY(head_bias(N, st, &b4));
     hipError_t e = hipSuccess;
     double dev_ms = 0.0;
     for (size_t r0 = 0; r0 < rows; r0 += chunk) {
@@ -572,9 +616,9 @@ qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_e
         hipEvent_t a = N.ctx->ev0, b = N.ctx->ev1;
         (void)hipEventRecord(a, st);
         forward(N, st, (int)nr, false);
-        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]), b4,
-                           (int)nr, N.hid, N.y, (const float *)nullptr, 0.f, (const float *)nullptr, (float *)nullptr,
-                           (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
+                           (const float *)(N.w + N.off[7]), (int)nr, N.hid, N.y, (const float *)nullptr, 0.f,
+                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
         (void)hipEventRecord(b, st);
         if ((e = hipMemcpyAsync(y + r0, N.y, nr * 4, hipMemcpyDeviceToHost, st))) break;
         if ((e = hipStreamSynchronize(st))) break;
@@ -584,6 +628,46 @@ qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_e
     }
     if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
     N.ctx->last_exec_ms = dev_ms;
+    QM_RET(Status{});
+}
+
+// Predictions for `count` ansatz candidates whose parameters are drawn on the
+// device from a SplitMix64 stream (features as k_ansatz_features documents);
+// the host can regenerate any candidate's parameters from its index.
+qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang, float *y,
+                                qi_error *err) {
+    if (!m || !y || count == 0) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    if ((uint64_t)N.n_in != 2ull * nang + 3) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "network inputs != 2 nang + 3"));
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    const size_t chunk = 65536;
+    QM_TRY(mlp_alloc(N, std::min<size_t>(chunk, count)));
+    hipError_t e;
+    if (count > N.ycap) {
+        if (N.yall) (void)hipFree(N.yall);
+        N.yall = nullptr;
+        N.ycap = 0;
+        if ((e = hipMalloc(&N.yall, count * sizeof(float)))) QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+        N.ycap = count;
+    }
+    (void)hipEventRecord(N.ctx->ev0, st);
+    for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
+        const int nr = (int)std::min<uint64_t>(chunk, count - c0);
+        const size_t cells = size_t(nr) * N.k_in;
+        hipLaunchKernelGGL(k_ansatz_features, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 8192)), dim3(256), 0, st, seed,
+                           draw0, count, c0, nr, (int)nang, N.k_in, N.X);
+        forward(N, st, nr, false);
+        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
+                           (const float *)(N.w + N.off[7]), nr, N.hid, N.yall + c0, (const float *)nullptr, 0.f,
+                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+    }
+    (void)hipEventRecord(N.ctx->ev1, st);
+    if ((e = hipMemcpyAsync(y, N.yall, count * sizeof(float), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+        QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, N.ctx->ev0, N.ctx->ev1);
+    N.ctx->last_exec_ms = ms;
     QM_RET(Status{});
 }
 

@@ -1,20 +1,16 @@
-// tiled.hip — cache-blocked circuit execution: a "pass" loads tiles of 2^K
-// amplitudes spanned by a chosen qubit set Q into LDS, applies every gate of
-// the pass inside the tile, and writes the tile back, so one HBM round trip
-// carries many gates instead of one.
+// tiled.hip — register-resident execution of a tile pass (planned in
+// tile_plan.hip).
 //
-// A gate fits a pass when its non-diagonal qubits (pair targets, both SWAP /
-// Matchgate targets) lie in Q. Controls and diagonal targets outside Q are
-// constant across a tile (the tile fixes all other index bits), so they become
-// a per-tile predicate or phase. Q always contains the lowest qubits, which
-// makes every tile row a contiguous run of at least 128 bytes in HBM.
+// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in
+// registers. A layout assigns the K local index bits to 8 thread bits and
+// K-8 register bits. A 2x2 gate on a register-held target touches only the
+// thread's own registers: no LDS traffic and no barrier. Diagonal gates and
+// controls work on any bit. When the next gates need other register bits,
+// the tile is transposed to a new layout through LDS. Controlled SWAPs and
+// Matchgates run on the LDS copy of the tile.
 //
-// Scheduling: gates are taken in list order; a gate may move ahead of
-// earlier gates only when their qubit supports are disjoint (they commute),
-// so the pass sequence applies exactly the circuit's unitary. Within a tile
-// each gate uses the same per-element formulas as the streaming kernels.
-#include <algorithm>
-
+// Arithmetic: complex128 follows the reference's operand order (no
+// contraction); complex64 uses fused multiply-adds.
 #include "qi_internal.h"
 #include "exec_plan.h"
 
@@ -28,13 +24,18 @@ template <typename R> struct V2;
 template <> struct V2<float> { using T = float2; };
 template <> struct V2<double> { using T = double2; };
 
-template <typename T> __device__ __forceinline__ T cmul(T a, T b) {
-    T r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
-}
-template <typename T> __device__ __forceinline__ T cadd(T a, T b) { T r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-template <typename T> __device__ __forceinline__ T csub(T a, T b) { T r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-template <typename T, typename R> __device__ __forceinline__ T cscale(R f, T a) { T r; r.x = f * a.x; r.y = f * a.y; return r; }
 template <typename T> __device__ __forceinline__ T mk(decltype(T::x) re, decltype(T::x) im) { T r; r.x = re; r.y = im; return r; }
+
+// a * b for complex values
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return mk<float2>(__builtin_fmaf(-a.y, b.y, a.x * b.x), __builtin_fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return mk<double2>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ T cadd(T a, T b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __device__ __forceinline__ T csub(T a, T b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T, typename R> __device__ __forceinline__ T cscale(R f, T a) { return mk<T>(f * a.x, f * a.y); }
 
 __device__ __forceinline__ uint32_t ins0(uint32_t k, unsigned p) { return ((k >> p) << (p + 1)) | (k & ((1u << p) - 1)); }
 
@@ -47,199 +48,284 @@ __device__ __forceinline__ uint64_t insert_zeros64(uint64_t k, uint64_t fixed) {
     return k;
 }
 
-// Deposit the low bits of e onto the qubit positions listed in pos[].
-__device__ __forceinline__ uint64_t deposit_pos(uint32_t e, const uint8_t *pos, int k) {
-    uint64_t r = 0;
-#pragma unroll
-    for (int j = 0; j < k; j++) r |= uint64_t((e >> j) & 1) << pos[j];
-    return r;
+constexpr int kTT = 1 << kRegThreadBits;
+
+// The thread's share of the tile: E complex amplitudes held as one vector
+// value. Element writes are whole-vector updates of an SSA value, so no
+// code path can turn them into stores through selected addresses (which
+// would keep the tile in scratch memory).
+template <typename R, int E> struct Regs {
+    using T = typename V2<R>::T;
+    R d[2 * E];
+    __device__ __forceinline__ T get(int r) const { return mk<T>(d[2 * r], d[2 * r + 1]); }
+    __device__ __forceinline__ void set(int r, T x) { d[2 * r] = x.x; d[2 * r + 1] = x.y; }
+};
+
+// LDS slot of local index e: the bits above the low F fold into the low F
+// (XOR-linear, a bijection) so strided thread bits spread over the banks.
+template <int F> __device__ __forceinline__ uint32_t swz(uint32_t e) {
+    static_assert(F == 3 || F == 4, "fold width");
+    if constexpr (F == 4) return e ^ (((e >> 4) ^ (e >> 8) ^ (e >> 12)) & 15u);
+    else return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
 }
 
-constexpr int kTileThreads = 256;
+// --------------------------------------------------------------- 2x2 ops --
+template <int KIND, typename R> __device__ __forceinline__ void pair(float2 &a, float2 &b, const R *c) {
+    if constexpr (KIND == RO_X) {
+        const float2 t = a; a = b; b = t;
+    } else if constexpr (KIND == RO_H) {
+        const float2 s = cadd(a, b), d = csub(a, b);
+        a = cscale<float2>(c[0], s);
+        b = cscale<float2>(c[0], d);
+    } else if constexpr (KIND == RO_REAL) {  // [[c, -s], [s, c]]
+        const float2 na = mk<float2>(__builtin_fmaf(-c[1], b.x, c[0] * a.x), __builtin_fmaf(-c[1], b.y, c[0] * a.y));
+        const float2 nb = mk<float2>(__builtin_fmaf(c[1], a.x, c[0] * b.x), __builtin_fmaf(c[1], a.y, c[0] * b.y));
+        a = na; b = nb;
+    } else {  // general complex 2x2, row-major
+        float2 na, nb;
+        na.x = __builtin_fmaf(-c[3], b.y, __builtin_fmaf(c[2], b.x, __builtin_fmaf(-c[1], a.y, c[0] * a.x)));
+        na.y = __builtin_fmaf(c[3], b.x, __builtin_fmaf(c[2], b.y, __builtin_fmaf(c[1], a.x, c[0] * a.y)));
+        nb.x = __builtin_fmaf(-c[7], b.y, __builtin_fmaf(c[6], b.x, __builtin_fmaf(-c[5], a.y, c[4] * a.x)));
+        nb.y = __builtin_fmaf(c[7], b.x, __builtin_fmaf(c[6], b.y, __builtin_fmaf(c[5], a.x, c[4] * a.y)));
+        a = na; b = nb;
+    }
+}
+template <int KIND, typename R> __device__ __forceinline__ void pair(double2 &a, double2 &b, const R *c) {
+    if constexpr (KIND == RO_X) {
+        const double2 t = a; a = b; b = t;
+    } else if constexpr (KIND == RO_H) {
+        const double2 s = cadd(a, b), d = csub(a, b);
+        a = cscale<double2>(c[0], s);
+        b = cscale<double2>(c[0], d);
+    } else if constexpr (KIND == RO_REAL) {
+        const double2 na = csub(cscale<double2>(c[0], a), cscale<double2>(c[1], b));
+        const double2 nb = cadd(cscale<double2>(c[1], a), cscale<double2>(c[0], b));
+        a = na; b = nb;
+    } else {
+        const double2 na = cadd(cmul(mk<double2>(c[0], c[1]), a), cmul(mk<double2>(c[2], c[3]), b));
+        const double2 nb = cadd(cmul(mk<double2>(c[4], c[5]), a), cmul(mk<double2>(c[6], c[7]), b));
+        a = na; b = nb;
+    }
+}
 
-template <typename R, int K>
-__global__ __launch_bounds__(kTileThreads) void k_tile_pass(typename V2<R>::T *__restrict__ psi, TilePass pass,
-                                                           const TileOp *__restrict__ ops) {
+// Element updates carry no control flow: a controlled op computes every pair
+// and keeps the old values by a select, so the tile stays in the same
+// registers on every path through the op switch.
+template <typename T> __device__ __forceinline__ T sel(bool on, T x, T y) { return mk<T>(on ? x.x : y.x, on ? x.y : y.y); }
+
+template <int B, int KIND, bool CTRL, int E, typename T, typename R>
+__device__ __forceinline__ void pairs(Regs<R, E> &v, const R *c, uint32_t rcm, bool tok) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if (r & (1 << B)) continue;
+        const T a = v.get(r), b = v.get(r | (1 << B));
+        T na = a, nb = b;
+        pair<KIND, R>(na, nb, c);
+        if constexpr (CTRL) {
+            const bool on = tok && ((uint32_t(r) & rcm) == rcm);
+            na = sel(on, na, a);
+            nb = sel(on, nb, b);
+        }
+        v.set(r, na);
+        v.set(r | (1 << B), nb);
+    }
+}
+
+template <int KIND, bool CTRL, int RB, int E, typename T, typename R>
+__device__ __forceinline__ void pairs_at(Regs<R, E> &v, int rb, const R *c, uint32_t rcm, bool tok) {
+#define QI_RB(BB) case BB: if constexpr (BB < RB) pairs<BB, KIND, CTRL, E, T, R>(v, c, rcm, tok); break;
+    switch (rb) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+
+// ---------------------------------------------------------- diagonal ops --
+template <int KIND, typename T> __device__ __forceinline__ T dphase(T a, T p0, T p1, bool one) {
+    if constexpr (KIND == RO_Z) return sel(one, mk<T>(-a.x, -a.y), a);
+    else if constexpr (KIND == RO_PHASE) return sel(one, cmul(a, p1), a);
+    else return cmul(a, sel(one, p1, p0));
+}
+
+// B >= 0: target on register bit B; B < 0: target fixed per thread (`tone`)
+template <int B, int KIND, bool CTRL, int E, typename T, typename R>
+__device__ __forceinline__ void diag(Regs<R, E> &v, T p0, T p1, uint32_t rcm, bool tok, bool tone) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const bool one = B >= 0 ? (((r >> (B >= 0 ? B : 0)) & 1) != 0) : tone;
+        const T a = v.get(r);
+        T na = dphase<KIND, T>(a, p0, p1, one);
+        if constexpr (CTRL) na = sel(tok && ((uint32_t(r) & rcm) == rcm), na, a);
+        v.set(r, na);
+    }
+}
+template <int KIND, bool CTRL, int RB, int E, typename T, typename R>
+__device__ __forceinline__ void diag_at(Regs<R, E> &v, int dloc, int dbit, T p0, T p1, uint32_t rcm, bool tok, bool tone) {
+    if (dloc != 0) { diag<-1, KIND, CTRL, E, T, R>(v, p0, p1, rcm, tok, tone); return; }
+#define QI_RB(BB) case BB: if constexpr (BB < RB) diag<BB, KIND, CTRL, E, T, R>(v, p0, p1, rcm, tok, tone); break;
+    switch (dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+
+// ------------------------------------------------------------- LDS ops --
+// Controlled SWAP or Matchgate (CPU form, operator.rs:993-1004) on the LDS
+// copy; local index e lives at slot swz(e).
+template <typename R, int K, int F>
+__device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &op) {
     using T = typename V2<R>::T;
     constexpr int N = 1 << K;
-    constexpr int PER = N / kTileThreads;
-    __shared__ T tile[N];
+    const uint32_t lcm = op.lcm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
+    const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
+    const R c0 = R(op.c[0]), c1 = R(op.c[1]);
+    const T m23 = mk<T>(R(op.c[2]), R(op.c[3])), m45 = mk<T>(R(op.c[4]), R(op.c[5])), m67 = mk<T>(R(op.c[6]), R(op.c[7]));
+    for (uint32_t q = threadIdx.x; q < N / 4; q += kTT) {
+        const uint32_t l = ins0(ins0(q, lo), hi);
+        if (op.lkind == LK_SWAP) {
+            const uint32_t i = l | b0, j = l | b1;
+            if ((i & lcm) != lcm) continue;
+            const T a = tile[swz<F>(i)], b = tile[swz<F>(j)];
+            tile[swz<F>(i)] = b;
+            tile[swz<F>(j)] = a;
+        } else {
+            const uint32_t i01 = l | b0, i10 = l | b1, i11 = l | b0 | b1;
+            if ((i01 & lcm) == lcm) {
+                const T a01 = tile[swz<F>(i01)], a10 = tile[swz<F>(i10)];
+                tile[swz<F>(i01)] = csub(cscale<T>(c0, a01), cmul(m23, a10));
+                tile[swz<F>(i10)] = cadd(cscale<T>(c1, a01), cmul(m45, a10));
+            }
+            if ((i11 & lcm) == lcm) tile[swz<F>(i11)] = cmul(tile[swz<F>(i11)], m67);
+        }
+    }
+}
+
+// ------------------------------------------------------------ the pass --
+template <typename R, int K>
+__global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
+                                                  const RegOp *__restrict__ ops) {
+    using T = typename V2<R>::T;
+    constexpr int RB = K - kRegThreadBits, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
+    __shared__ T lds[1 << K];
     const uint32_t tid = threadIdx.x;
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
-    // element e = tid + it * 256: its global offset splits into a per-thread
-    // part (low 8 local bits) and a per-iteration part (uniform)
-    const uint64_t off_t = deposit_pos(tid, pass.pos, 8);
+
+    Regs<R, E> v;
+    uint32_t mr[RB];  // local-bit mask of each register bit (uniform)
+    uint32_t tl = 0;  // local index bits held by this thread
+    {
+        uint64_t at = 0;
 #pragma unroll
-    for (int it = 0; it < PER; it++) {
-        uint64_t off_i = deposit_pos(uint32_t(it) << 8, pass.pos, K);
-        tile[tid + it * kTileThreads] = psi[base | off_t | off_i];
-    }
-    __syncthreads();
-    for (int o = 0; o < pass.nops; o++) {
-        const TileOp op = ops[pass.op0 + o];
-        if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile
-        const uint32_t lcm = op.lcm;
-        switch (op.kind) {
-        case LK_Z: case LK_PHASE: case LK_RZ: {
-            // diagonal: phase of the target bit (local or fixed by the tile)
-            T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
-            for (int it = 0; it < PER; it++) {
-                uint32_t e = tid + it * kTileThreads;
-                if ((e & lcm) != lcm) continue;
-                bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
-                T a = tile[e];
-                if (op.kind == LK_Z) { if (one) { a.x = -a.x; a.y = -a.y; } }
-                else if (op.kind == LK_PHASE) { if (one) a = cmul(a, p1); }
-                else a = cmul(a, one ? p1 : p0);
-                tile[e] = a;
-            }
-        } break;
-        case LK_SWAP: case LK_MATCH: {
-            const uint32_t b0 = 1u << op.lp0, b1 = 1u << op.lp1;
-            const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
-            for (int it = 0; it < PER / 4; it++) {
-                uint32_t q = tid + it * kTileThreads;
-                uint32_t l = ins0(ins0(q, lo), hi);
-                if (op.kind == LK_SWAP) {
-                    uint32_t i = l | b0, j = l | b1;
-                    if ((i & lcm) != lcm) continue;
-                    T a = tile[i], b = tile[j];
-                    tile[i] = b; tile[j] = a;
-                } else {
-                    uint32_t i01 = l | b0, i10 = l | b1, i11 = l | b0 | b1;
-                    if ((i01 & lcm) == lcm) {
-                        T a01 = tile[i01], a10 = tile[i10];
-                        tile[i01] = csub(cscale<T>(R(op.c[0]), a01), cmul(mk<T>(R(op.c[2]), R(op.c[3])), a10));
-                        tile[i10] = cadd(cscale<T>(R(op.c[1]), a01), cmul(mk<T>(R(op.c[4]), R(op.c[5])), a10));
-                    }
-                    if ((i11 & lcm) == lcm) tile[i11] = cmul(tile[i11], mk<T>(R(op.c[6]), R(op.c[7])));
-                }
-            }
-            // Matchgate controls may sit on q2 (only overlap with q1 is rejected,
-            // operator.rs:214-273); those are tested per element above.
-        } break;
-        default: {  // pair gates on a local target
-            const uint32_t tb = 1u << op.lp0;
-            for (int it = 0; it < PER / 2; it++) {
-                uint32_t q = tid + it * kTileThreads;
-                uint32_t i = ins0(q, op.lp0), j = i | tb;
-                if ((i & lcm) != lcm) continue;
-                T a = tile[i], b = tile[j], na, nb;
-                switch (op.kind) {
-                case LK_H: na = cscale<T>(R(op.c[0]), cadd(a, b)); nb = cscale<T>(R(op.c[0]), csub(a, b)); break;
-                case LK_X: na = b; nb = a; break;
-                case LK_Y: na = cmul(mk<T>(R(op.c[0]), R(op.c[1])), b); nb = cmul(mk<T>(R(op.c[2]), R(op.c[3])), a); break;
-                case LK_RX:
-                    na = csub(cscale<T>(R(op.c[0]), a), cmul(mk<T>(R(op.c[1]), R(op.c[2])), b));
-                    nb = cadd(cmul(mk<T>(R(op.c[3]), R(op.c[4])), a), cscale<T>(R(op.c[0]), b));
-                    break;
-                case LK_RY:
-                    na = csub(cscale<T>(R(op.c[0]), a), cscale<T>(R(op.c[1]), b));
-                    nb = cadd(cscale<T>(R(op.c[1]), a), cscale<T>(R(op.c[0]), b));
-                    break;
-                default:  // LK_U2
-                    na = cadd(cmul(mk<T>(R(op.c[0]), R(op.c[1])), a), cmul(mk<T>(R(op.c[2]), R(op.c[3])), b));
-                    nb = cadd(cmul(mk<T>(R(op.c[4]), R(op.c[5])), a), cmul(mk<T>(R(op.c[6]), R(op.c[7])), b));
-                    break;
-                }
-                tile[i] = na; tile[j] = nb;
-            }
-        } break;
+        for (int i = 0; i < kRegThreadBits; i++) {
+            const uint32_t bit = (tid >> i) & 1u;
+            tl |= bit << pass.lay_t[i];
+            at |= uint64_t(bit) << pass.qt_in[i];
         }
-        __syncthreads();
-    }
+        uint64_t ar[RB];
 #pragma unroll
-    for (int it = 0; it < PER; it++) {
-        uint64_t off_i = deposit_pos(uint32_t(it) << 8, pass.pos, K);
-        psi[base | off_t | off_i] = tile[tid + it * kTileThreads];
+        for (int j = 0; j < RB; j++) { mr[j] = 1u << pass.lay_r[j]; ar[j] = uint64_t(1) << pass.qr_in[j]; }
+        const T *src = psi + (base | at);
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= ar[j];
+            v.set(r, src[o]);
+        }
+    }
+    bool dirty = false;  // LDS written since the last barrier
+    // the op table of a pass ends with a no-op, so op o + 1 can be fetched
+    // while op o runs
+    RegOp nxt = ops[pass.op0];
+    for (int o = 0; o < pass.nops; o++) {
+        const RegOp op = nxt;
+        nxt = ops[pass.op0 + o + 1];
+        const int kind = op.kind;
+        if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
+        const bool ctrl = (op.rcm | op.tcm) != 0;  // controls held in the tile (uniform)
+        const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
+        if (kind <= RO_GEN) {
+            R c[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) c[k] = R(op.c[k]);
+#define QI_PAIR(KD)                                                                     \
+    case KD:                                                                            \
+        if (ctrl) pairs_at<KD, true, RB, E, T, R>(v, op.rb, c, op.rcm, tok);            \
+        else pairs_at<KD, false, RB, E, T, R>(v, op.rb, c, op.rcm, tok);                \
+        break;
+            switch (kind) { QI_PAIR(RO_X) QI_PAIR(RO_H) QI_PAIR(RO_REAL) QI_PAIR(RO_GEN) default: break; }
+#undef QI_PAIR
+        } else if (kind <= RO_RZ) {
+            const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
+            const bool tone = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+#define QI_DIAG(KD)                                                                             \
+    case KD:                                                                                    \
+        if (ctrl) diag_at<KD, true, RB, E, T, R>(v, op.dloc, op.dbit, p0, p1, op.rcm, tok, tone);  \
+        else diag_at<KD, false, RB, E, T, R>(v, op.dloc, op.dbit, p0, p1, op.rcm, tok, tone);      \
+        break;
+            switch (kind) { QI_DIAG(RO_Z) QI_DIAG(RO_PHASE) QI_DIAG(RO_RZ) default: break; }
+#undef QI_DIAG
+        } else if (kind <= RO_LDS) {
+            // through LDS: a transpose to a new layout, or a gate on the LDS copy
+            if (dirty) __syncthreads();
+            {
+                const uint32_t st = swz<F>(tl);
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    uint32_t e = 0;
+#pragma unroll
+                    for (int j = 0; j < RB; j++) if ((r >> j) & 1) e |= mr[j];
+                    lds[st ^ swz<F>(e)] = v.get(r);
+                }
+            }
+            __syncthreads();
+            if (kind == RO_TRANSPOSE) {
+                tl = 0;
+#pragma unroll
+                for (int i = 0; i < kRegThreadBits; i++) tl |= ((tid >> i) & 1u) << op.lay_t[i];
+#pragma unroll
+                for (int j = 0; j < RB; j++) mr[j] = 1u << op.lay_r[j];
+            } else {
+                lds_gate<R, K, F>(lds, op);
+                __syncthreads();
+            }
+            {
+                const uint32_t st = swz<F>(tl);
+#pragma unroll
+                for (int r = 0; r < E; r++) {
+                    uint32_t e = 0;
+#pragma unroll
+                    for (int j = 0; j < RB; j++) if ((r >> j) & 1) e |= mr[j];
+                    v.set(r, lds[st ^ swz<F>(e)]);
+                }
+            }
+            dirty = true;
+        }
+    }
+    {
+        // store: thread and register bits hold the qubits of the final layout
+        uint64_t at = 0;
+#pragma unroll
+        for (int i = 0; i < kRegThreadBits; i++) at |= uint64_t((tid >> i) & 1u) << pass.qt_out[i];
+        uint64_t ar[RB];
+#pragma unroll
+        for (int j = 0; j < RB; j++) ar[j] = uint64_t(1) << pass.qr_out[j];
+        T *dst = psi + (base | at);
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= ar[j];
+            dst[o] = v.get(r);
+        }
     }
 }
 
-// Qubits a gate needs inside the tile, and all qubits it touches.
-void gate_sets(const Lowered &L, uint64_t &need, uint64_t &supp) {
-    uint64_t t0 = uint64_t(1) << L.t0, t1 = uint64_t(1) << L.t1;
-    switch (L.kind) {
-    case LK_Z: case LK_PHASE: case LK_RZ: need = 0; supp = t0 | L.ctrl_mask; break;
-    case LK_SWAP: case LK_MATCH: need = t0 | t1; supp = t0 | t1 | L.ctrl_mask; break;
-    default: need = t0; supp = t0 | L.ctrl_mask; break;
-    }
-}
-
-template <typename R, int K> void launch_pass(State &s, const TilePass &p, const TileOp *ops, hipStream_t st) {
+template <typename R, int K> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     using T = typename V2<R>::T;
-    unsigned tiles = (unsigned)(uint64_t(1) << (s.n - K));
-    hipLaunchKernelGGL((k_tile_pass<R, K>), dim3(tiles), dim3(kTileThreads), 0, st, (T *)s.d, p, ops);
+    const unsigned tiles = (unsigned)(uint64_t(1) << (s.n - K));
+    hipLaunchKernelGGL((k_reg_pass<R, K>), dim3(tiles), dim3(kTT), 0, st, (T *)s.d, p, ops);
 }
 
 }  // namespace
 
-int tile_low_qubits(int prec) { return prec == QI_C128 ? 3 : 4; }  // 128-byte rows
-
-int default_tile_qubits(const State &s) { return s.prec == QI_C128 ? 12 : 13; }
-
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan) {
-    if (K <= 0) K = default_tile_qubits(s);
-    const int kmax = s.prec == QI_C128 ? 12 : 13;
-    if (K < 10 || K > kmax) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width out of range");
-    if ((int)s.n < K) {  // state smaller than a tile: stream gate by gate
-        for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
-        return Status{};
-    }
-    const uint64_t all = (s.n >= 64) ? ~uint64_t(0) : ((uint64_t(1) << s.n) - 1);
-    const uint64_t low = (uint64_t(1) << tile_low_qubits(s.prec)) - 1;
-    std::vector<char> done(gates.size(), 0);
-    size_t first = 0;
-    while (first < gates.size()) {
-        uint64_t active = low, blocked = 0;
-        std::vector<size_t> members;
-        for (size_t g = first; g < gates.size(); g++) {
-            if (done[g]) continue;
-            uint64_t need, supp;
-            gate_sets(gates[g], need, supp);
-            if (supp & blocked) { blocked |= supp; }
-            else if (__builtin_popcountll(active | need) <= K) { active |= need; members.push_back(g); done[g] = 1; }
-            else blocked |= supp;
-            if ((blocked & all) == all) break;
-        }
-        // fill the tile with the lowest unused qubits (longer contiguous rows)
-        uint64_t q = active;
-        for (int b = 0; __builtin_popcountll(q) < K && b < (int)s.n; b++) q |= uint64_t(1) << b;
-        TilePass p{};
-        p.qmask = q;
-        p.K = K;
-        int j = 0;
-        uint8_t local[64];
-        for (int b = 0; b < (int)s.n; b++)
-            if ((q >> b) & 1) { local[b] = (uint8_t)j; p.pos[j++] = (uint8_t)b; }
-        p.op0 = (int)plan.tile_ops.size();
-        for (size_t g : members) {
-            const Lowered &L = gates[g];
-            TileOp op{};
-            op.kind = L.kind;
-            for (int c = 0; c < 8; c++) op.c[c] = L.c[c];
-            if (L.kind == LK_Z) { op.c[2] = -1.0; op.c[3] = 0.0; }
-            if (L.kind == LK_PHASE) { op.c[2] = L.c[0]; op.c[3] = L.c[1]; }
-            uint64_t tb0 = uint64_t(1) << L.t0;
-            if ((q & tb0) == 0) { op.outside = 1; op.gt = (uint8_t)L.t0; }
-            else op.lp0 = local[L.t0];
-            if (L.kind == LK_SWAP || L.kind == LK_MATCH) op.lp1 = local[L.t1];
-            uint64_t lc = L.ctrl_mask & q, gc = L.ctrl_mask & ~q;
-            op.gcm = gc;
-            uint32_t lcm = 0;
-            for (int b = 0; b < (int)s.n; b++)
-                if ((lc >> b) & 1) lcm |= 1u << local[b];
-            op.lcm = lcm;
-            op.pad = (uint32_t)g;  // source gate (the batched engine maps it to an angle slot)
-            plan.tile_ops.push_back(op);
-        }
-        p.nops = (int)members.size();
-        plan.steps.push_back(ExecStep::tile_step(p));
-        while (first < gates.size() && done[first]) first++;
-    }
-    return Status{};
-}
-
-Status launch_tile_pass(State &s, const TilePass &p, const TileOp *dops, hipStream_t st) {
+Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st) {
     if (s.prec == QI_C128) {
         switch (p.K) {
         case 10: launch_pass<double, 10>(s, p, dops, st); break;

@@ -33,6 +33,7 @@ L.SIGNATURES.update({
     "qi_mlp_set_data": [L.P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t, L.ERR],
     "qi_mlp_train_epoch": [L.P, C.c_uint32, C.c_float, C.c_uint64, C.POINTER(C.c_float), L.ERR],
     "qi_mlp_predict": [L.P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), L.ERR],
+    "qi_mlp_predict_ansatz": [L.P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(C.c_float), L.ERR],
 })
 
 
@@ -47,7 +48,8 @@ class MLP:
         self.ctx = ctx or default_context()
         self.lib = L.load()
         for name in ("qi_mlp_create", "qi_mlp_destroy", "qi_mlp_num_params", "qi_mlp_get_params", "qi_mlp_set_params",
-                     "qi_mlp_gradients", "qi_mlp_set_data", "qi_mlp_train_epoch", "qi_mlp_predict"):
+                     "qi_mlp_gradients", "qi_mlp_set_data", "qi_mlp_train_epoch", "qi_mlp_predict",
+                     "qi_mlp_predict_ansatz"):
             f = getattr(self.lib, name)
             f.argtypes = L.SIGNATURES[name]
             f.restype = C.c_int
@@ -105,6 +107,17 @@ class MLP:
         err = L.QiError()
         _check(self.lib.qi_mlp_predict(self._h, _fp(x), x.shape[0], _fp(y), C.byref(err)), err)
         return y
+
+    def predict_stream(self, seed: int, draw0: int, count: int, nang: int) -> np.ndarray:
+        """Predictions for `count` candidates drawn on the device from the
+        SplitMix64 stream (seed, draw0) as sample_params would draw them."""
+        y = np.empty(count, dtype=np.float32)
+        err = L.QiError()
+        _check(self.lib.qi_mlp_predict_ansatz(self._h, seed, draw0, count, nang, _fp(y), C.byref(err)), err)
+        return y
+
+    def last_ms(self) -> float:
+        return self.ctx.last_exec_ms()
 
 
 # ----------------------------------------------------------- ansatz energies --
@@ -169,6 +182,39 @@ def sample_params(rng: W.SplitMix64, count: int, nang: int):
     return th, J
 
 
+def params_at(seed: int, draw0: int, count: int, nang: int, idx: np.ndarray):
+    """The (theta, J) that sample_params draws for candidates `idx` of a block
+    of `count` starting after draw `draw0` of the stream `seed`."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    j = np.arange(nang, dtype=np.uint64)
+    th = 2.0 * math.pi * W.splitmix_at(seed, np.uint64(draw0) + idx[:, None] * np.uint64(nang) + j + np.uint64(1))
+    q = np.arange(3, dtype=np.uint64)
+    J = 0.5 + W.splitmix_at(seed, np.uint64(draw0 + count * nang) + idx[:, None] * np.uint64(3) + q + np.uint64(1))
+    return th.reshape(len(idx), nang), J.reshape(len(idx), 3)
+
+
+def ansatz_batch(specs, precision: str = "c64", ctx: Optional[Context] = None) -> CircuitBatch:
+    """One CircuitBatch over (n, L, thetas, J) specs: every (n, L) shape is one
+    shared program whose rotations take their angles from the batch's table."""
+    b = CircuitBatch(precision, ctx)
+    protos = {}
+    angles = []
+    slot = 0
+    for n, layers, th, J in specs:
+        if (n, layers) not in protos:
+            proto = W.ansatz_circuit(n, layers, np.zeros(2 * n * layers))
+            rot = np.where(np.isin(proto["kind"], [L.GATE_KINDS["RY"], L.GATE_KINDS["RZ"]]))[0]
+            protos[(n, layers)] = (proto, rot)
+        proto, rot = protos[(n, layers)]
+        sl = np.full(proto.size, -1, dtype=np.int32)
+        sl[rot] = slot + np.arange(rot.size)
+        b.add(n, proto, W.heisenberg_open_1d(n, *J), sl)
+        angles.append(np.asarray(th, dtype=np.float64))
+        slot += rot.size
+    b.set_angles(np.concatenate(angles))
+    return b
+
+
 @dataclass
 class ScreenResult:
     candidates: int
@@ -180,6 +226,31 @@ class ScreenResult:
     mean_exact_random: float
     surrogate_mae_kept: float
     max_abs_err_kept: float
+
+
+def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: int, keep_frac: float, t_mean: float,
+                  t_std: float, rng_np: np.random.Generator, random_baseline: bool = True) -> ScreenResult:
+    """Screen `count` candidates generated on the device: rank by the
+    surrogate, regenerate the top fraction's parameters on the host from their
+    indices and evaluate them exactly; optionally evaluate a random subset of
+    the same size (selection without the surrogate)."""
+    nang = sim.nang
+    t0 = time.perf_counter()
+    pred = mlp.predict_stream(seed, draw0, count, nang).astype(np.float64) * t_std + t_mean
+    k = max(1, int(round(keep_frac * count)))
+    top = np.argpartition(pred, k - 1)[:k]
+    t1 = time.perf_counter()
+    th, J = params_at(seed, draw0, count, nang, top)
+    exact = sim.energies(th, J)
+    t2 = time.perf_counter()
+    mean_rnd = float("nan")
+    if random_baseline:
+        rnd = rng_np.choice(count, size=k, replace=False)
+        thr, Jr = params_at(seed, draw0, count, nang, rnd)
+        mean_rnd = float(sim.energies(thr, Jr).mean())
+    err = np.abs(pred[top] - exact)
+    return ScreenResult(count, k, t1 - t0, t2 - t1, float(exact.min()), float(exact.mean()), mean_rnd,
+                        float(err.mean()), float(err.max()))
 
 
 def screen(mlp: MLP, sim: AnsatzEnergies, thetas: np.ndarray, J: np.ndarray, keep_frac: float, t_mean: float,

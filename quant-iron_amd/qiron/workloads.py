@@ -41,6 +41,16 @@ class SplitMix64:
         return float(self.uniforms(1)[0])
 
 
+def splitmix_at(seed: int, draws: np.ndarray) -> np.ndarray:
+    """Uniforms of the given draw numbers (1-based) of the stream `seed`."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + np.asarray(draws, dtype=np.uint64) * GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+
+
 K = L.GATE_KINDS
 
 
