@@ -69,6 +69,111 @@ def cpu_baseline(qiron_workloads, n_cpu: int, budget_s: float, threads: int):
     return done, dt
 
 
+def bench_energies(qiron, L, W, args):
+    """Batched ansatz energies (4096 circuits, 10..22 qubits, 4..8 layers, each
+    with its own open Heisenberg chain) in complex64 and complex128, against
+    the plain FP64 GPU path: one circuit at a time, per-gate kernels in
+    complex128, then the SumOp expectation."""
+    import numpy as np
+
+    from qiron import surrogate as S
+
+    specs = W.batched_ansatz(args.batch_circuits)
+    count = len(specs)
+    res, E = {}, {}
+    for prec in ("c64", "c128"):
+        t0 = time.perf_counter()
+        b = S.ansatz_batch(specs, prec)
+        e = b.run(use_graph=True)  # plans, captures the graph, runs once
+        setup = time.perf_counter() - t0
+        walls = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            e = b.run(use_graph=True)
+            walls.append(time.perf_counter() - t1)
+        info = b.info()
+        wall = min(walls)
+        res[f"batch_{prec}"] = dict(energies_per_s=count / wall, wall_ms=wall * 1e3, device_ms=info["last_ms"],
+                                    setup_s=setup, levels=info["levels"], lds_resident=info["lds_resident"])
+        log(f"batched {prec}: {count} energies in {wall * 1e3:.1f} ms ({count / wall:.0f}/s), device "
+            f"{info['last_ms']:.1f} ms, setup {setup:.1f} s")
+        E[prec] = e.real
+        del b
+    dE = float(np.max(np.abs(E["c64"] - E["c128"])))
+    res["max_abs_dE_fp32_vs_fp64"] = dE
+    log(f"max |E_fp32 - E_fp64| over {count} circuits: {dE:.2e}")
+    # the plain FP64 path on every k-th circuit
+    stride = max(1, count // args.plain_circuits)
+    t_plain, worst, n_plain = 0.0, 0.0, 0
+    for idx in range(0, count, stride):
+        n, layers, th, J = specs[idx]
+        gates = W.ansatz_circuit(n, layers, th)
+        terms = W.heisenberg_open_1d(n, *J)
+        t0 = time.perf_counter()
+        s = qiron.State.new_zero(n, "c128")
+        s.apply_gates_inplace(gates, mode=L.EXEC_PER_GATE)
+        e = qiron.expectation_terms(s, terms)
+        t_plain += time.perf_counter() - t0
+        del s
+        worst = max(worst, abs(e.real - E["c128"][idx]))
+        n_plain += 1
+    plain = n_plain / t_plain
+    res["plain_fp64"] = dict(energies_per_s=plain, circuits=n_plain, seconds=t_plain, max_abs_dE_vs_batched_fp64=worst)
+    res["batched_fp32_vs_plain_fp64"] = res["batch_c64"]["energies_per_s"] / plain
+    log(f"plain FP64 path: {n_plain} circuits (every {stride}th) in {t_plain:.2f} s = {plain:.1f}/s; "
+        f"max |dE| vs batched FP64 {worst:.1e}; batched FP32 is {res['batched_fp32_vs_plain_fp64']:.0f}x")
+    return res, plain
+
+
+def bench_surrogate(W, args, plain_eps):
+    """The learned screen on the 16-qubit, 6-layer ansatz (192 angles + 3
+    couplings): train on batched-simulator energies, rank candidates drawn on
+    the device, re-evaluate the top 1% exactly."""
+    import numpy as np
+
+    from qiron import surrogate as S
+
+    n, layers = 16, 6
+    sim = S.AnsatzEnergies(n, layers, slots=4096, precision="c64")
+    nang = sim.nang
+    rng = W.SplitMix64(W.SEED_BASE + 5)
+    t0 = time.perf_counter()
+    th, J = S.sample_params(rng, args.train_samples, nang)
+    E = sim.energies(th, J)
+    t_gen = time.perf_counter() - t0
+    mu, sd = float(E.mean()), float(E.std())
+    net = S.MLP(S.n_features(nang), 512, seed=7)
+    t0 = time.perf_counter()
+    net.set_data(S.features(th, J), ((E - mu) / sd).astype(np.float32))
+    del th
+    t_up = time.perf_counter() - t0
+    losses, t_train = [], 0.0
+    for ep in range(args.epochs):
+        t1 = time.perf_counter()
+        losses.append(net.train_epoch(1024, 1e-3, seed=ep))
+        t_train += time.perf_counter() - t1
+    thv, Jv = S.sample_params(rng, 4096, nang)
+    Ev = sim.energies(thv, Jv)
+    pv = net.predict(S.features(thv, Jv)).astype(np.float64) * sd + mu
+    r2 = float(1 - np.mean((pv - Ev) ** 2) / np.var(Ev))
+    # screening round: candidates drawn on the device from a fresh stream
+    seed, draw0, cand = W.SEED_BASE + 6, 0, args.candidates
+    res = S.screen_stream(net, sim, seed, draw0, cand, 0.01, mu, sd, np.random.default_rng(3))
+    t_round = res.surrogate_seconds + res.exact_seconds
+    screened = cand / t_round
+    out = dict(qubits=n, layers=layers, train_samples=args.train_samples, gen_seconds=t_gen,
+               gen_energies_per_s=args.train_samples / t_gen, upload_seconds=t_up, epochs=args.epochs,
+               train_seconds=t_train, loss=[round(x, 4) for x in losses], heldout_r2=r2, candidates=cand,
+               kept=res.kept, surrogate_seconds=res.surrogate_seconds, exact_seconds=res.exact_seconds,
+               screened_candidates_per_s=screened, screened_vs_plain_fp64=screened / plain_eps,
+               best_exact=res.best_exact, mean_exact_kept=res.mean_exact_kept, mean_exact_random=res.mean_exact_random,
+               energy_sd=sd, surrogate_mae_kept=res.surrogate_mae_kept, surrogate_max_err_kept=res.max_abs_err_kept)
+    log(f"surrogate: R^2 {r2:.3f}, losses {out['loss']}, {cand} candidates in {t_round:.2f} s "
+        f"({screened:.0f}/s, {screened / plain_eps:.0f}x plain FP64), kept mean {res.mean_exact_kept:.4f} vs "
+        f"random {res.mean_exact_random:.4f} (sd {sd:.4f})")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +185,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
+    ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--plain-circuits", type=int, default=64, help="circuits timed on the plain FP64 path")
+    ap.add_argument("--train-samples", type=int, default=262144)
+    ap.add_argument("--epochs", type=int, default=4)
+    ap.add_argument("--candidates", type=int, default=1 << 20)
+    ap.add_argument("--no-energies", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the state-vector engine here is single-GPU "
@@ -155,6 +266,11 @@ def main():
                 r.update(mfma_TFLOPs=tf, mfma_util=tf / peak)
             results[name] = r
             log(name, r)
+
+    if not args.only and not args.no_energies:
+        eres, plain_eps = bench_energies(qiron, L, W, args)
+        results["energies"] = eres
+        results["surrogate"] = bench_surrogate(W, args, plain_eps)
 
     cpu = None
     if not args.no_cpu and not args.only:

@@ -153,14 +153,15 @@ typedef struct qi_pauli_term {
 /* Executor options for qi_apply_gates. */
 typedef enum qi_exec_mode {
     QI_EXEC_PER_GATE = 0,  /* one streaming kernel per gate                  */
-    QI_EXEC_TILED = 1,     /* LDS-tiled passes: all gates on a qubit set per HBM pass */
+    QI_EXEC_TILED = 1,     /* tile passes: the gates on a 2^k-amplitude qubit set per HBM round trip,
+                              the tile held in registers (transposed through LDS) */
     QI_EXEC_FUSED_MFMA = 2,/* dense 2^k x 2^k blocks applied as complex GEMM on MFMA */
     QI_EXEC_AUTO = 3       /* tiled unless the state is too small for tiles   */
 } qi_exec_mode;
 
 typedef struct qi_exec_opts {
     int32_t mode;        /* qi_exec_mode                                    */
-    int32_t tile_qubits; /* LDS tile width k for QI_EXEC_TILED (0 = default) */
+    int32_t tile_qubits; /* tile width k for QI_EXEC_TILED: 10..13 (c64), 10..12 (c128); 0 = 13 / 12 */
     int32_t block_qubits;/* dense block width k (<= 6) for QI_EXEC_FUSED_MFMA */
     int32_t use_graph;   /* capture the launch sequence in a hipGraph         */
 } qi_exec_opts;

@@ -26,10 +26,12 @@ HIP_FLAGS = [
 ]
 
 
-# Per-file flags. tiled.hip: common-code sinking across the switch over
-# register bits would merge element updates into stores through selected
-# addresses, which keeps the thread's tile in scratch memory.
-FILE_FLAGS = {"tiled.hip": ["-mllvm", "-simplifycfg-sink-common=false"]}
+# Per-file flags. tiled.hip keeps a thread's share of the tile in registers
+# across a loop over op kinds: common-code sinking would merge element
+# updates into stores through selected addresses (the tile would live in
+# scratch memory), and structurizing the uniform op switch would give each
+# path its own copy of the tile registers (whole-tile moves at every join).
+FILE_FLAGS = {"tiled.hip": ["-mllvm", "-simplifycfg-sink-common=false", "-mllvm", "-structurizecfg-skip-uniform-regions"]}
 
 
 def _compile(src: Path) -> Path:

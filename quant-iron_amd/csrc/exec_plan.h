@@ -28,30 +28,31 @@ struct TilePass {
 };
 
 // ---- register-resident tile passes (single-state executor) ----
-// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread. A layout
-// assigns each of the K local index bits to a thread bit or a register bit.
+// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in registers.
+// A layout assigns each of the K local index bits to a thread bit (lane bits
+// 0..5, wave bits 6..7) or a register bit. 2x2 gates act on lane-held targets
+// (the partner amplitude comes from another lane of the same wave).
 constexpr int kRegThreadBits = 8;
+constexpr int kLaneBits = 6;
 
 enum RegOpKind : int32_t {
-    RO_X = 0, RO_H, RO_REAL, RO_GEN,   // 2x2 on a register-held target
+    RO_X = 0, RO_H, RO_REAL, RO_GEN,   // 2x2 on a lane-held target
     RO_Z, RO_PHASE, RO_RZ,             // diagonal on any target
     RO_TRANSPOSE,                      // new layout through LDS
     RO_LDS,                            // gate on the LDS copy (controlled SWAP, Matchgate)
     RO_NOP                             // closes each pass's slice (read ahead, never run)
 };
 
-struct RegOp {
+struct RegOp {            // every field dword-sized: the kernel reads ops with scalar loads
     int32_t kind;        // RegOpKind
     int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
-    uint8_t rb;          // pair ops: register bit of the target
-    uint8_t dloc;        // diagonal ops: 0 register bit, 1 thread-held local bit, 2 qubit outside the tile
-    uint8_t dbit;        // that register bit / local bit / qubit
-    uint8_t lp0, lp1;    // RO_LDS: local bits of the targets
-    uint8_t pad8[3];
-    uint32_t rcm;        // controls held in registers (register-bit mask)
-    uint32_t tcm;        // controls held by threads (local-bit mask)
-    uint32_t lcm;        // RO_LDS: controls inside the tile (local-bit mask)
+    int32_t lane;        // pair ops: lane bit (0..5) holding the target
+    int32_t dloc;        // diagonal ops: 0 target inside the tile (dbit = its local bit), 2 outside (dbit = qubit)
+    int32_t dbit;
+    int32_t lp0, lp1;    // RO_LDS: local bits of the targets
+    uint32_t cm;         // controls inside the tile (local-bit mask)
     uint32_t src;        // source gate index
+    uint32_t pad[3];
     uint64_t gcm;        // controls outside the tile (qubit mask)
     uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
     uint8_t lay_t[8];    //               local bit of each thread bit

@@ -82,12 +82,12 @@ struct RegPlanner {
     int K = 0, RB = 0, L = 0;
     int lbit[64];       // qubit -> local bit (-1: outside the tile)
     int qof[16];        // local bit -> qubit
-    uint8_t cr[8], ct[8];
+    uint8_t cr[8], ct[8];  // layout: local bit of each register / thread bit
     std::vector<RegOp> out;
 
-    uint32_t reg_set() const {
+    uint32_t lane_set() const {
         uint32_t m = 0;
-        for (int j = 0; j < RB; j++) m |= 1u << cr[j];
+        for (int i = 0; i < kLaneBits; i++) m |= 1u << ct[i];
         return m;
     }
     uint32_t io_set() const {
@@ -95,9 +95,9 @@ struct RegPlanner {
         for (int k = 0; k < L; k++) m |= 1u << lbit[k];
         return m;
     }
-    int reg_index(int lb) const {
-        for (int j = 0; j < RB; j++)
-            if (cr[j] == lb) return j;
+    int lane_index(int lb) const {
+        for (int i = 0; i < kLaneBits; i++)
+            if (ct[i] == lb) return i;
         return -1;
     }
     bool io_ok() const {
@@ -105,29 +105,33 @@ struct RegPlanner {
             if (ct[k] != lbit[k]) return false;
         return true;
     }
-    // Layout with register bits `rset`; the local bits of the row qubits
-    // become thread bits 0..L-1 when none of them is a register bit.
-    void make(uint32_t rset, uint8_t *r, uint8_t *t) const {
-        int j = 0, i = 0;
-        for (int b = 0; b < K; b++)
-            if ((rset >> b) & 1) r[j++] = (uint8_t)b;
-        for (; j < 8; j++) r[j] = 0;
-        uint32_t used = rset;
-        if ((rset & io_set()) == 0)
+    // Lane set of kLaneBits local bits containing `need`: the row qubits'
+    // bits join when they fit (so loads and stores need no transpose), then
+    // the lowest other bits.
+    uint32_t fill_lanes(uint32_t need) const {
+        uint32_t s = need;
+        if (__builtin_popcount(s | io_set()) <= kLaneBits) s |= io_set();
+        for (int b = 0; b < K && __builtin_popcount(s) < kLaneBits; b++) s |= 1u << b;
+        return s;
+    }
+    // Layout with lane bits `lanes` (row qubits first when all are lanes),
+    // then the two wave bits and the register bits in ascending order.
+    void make(uint32_t lanes, uint8_t *r, uint8_t *t) const {
+        int i = 0, j = 0;
+        uint32_t used = 0;
+        if ((lanes & io_set()) == io_set())
             for (int k = 0; k < L; k++) { t[i++] = (uint8_t)lbit[k]; used |= 1u << lbit[k]; }
         for (int b = 0; b < K; b++)
-            if (!((used >> b) & 1)) t[i++] = (uint8_t)b;
+            if (((lanes >> b) & 1) && !((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
+        for (int b = 0; b < K && i < kRegThreadBits; b++)
+            if (!((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
+        for (int b = 0; b < K; b++)
+            if (!((used >> b) & 1)) r[j++] = (uint8_t)b;
+        for (; j < 8; j++) r[j] = 0;
     }
-    // fill a register set to RB bits, avoiding the row qubits' bits
-    uint32_t fill(uint32_t rset, uint32_t avoid) const {
-        for (int b = K - 1; b >= 0 && __builtin_popcount(rset) < RB; b--)
-            if (!((avoid >> b) & 1)) rset |= 1u << b;
-        for (int b = K - 1; b >= 0 && __builtin_popcount(rset) < RB; b--) rset |= 1u << b;
-        return rset;
-    }
-    void transpose_to(uint32_t rset) {
+    void transpose_to(uint32_t lanes) {
         uint8_t r[8], t[8];
-        make(rset, r, t);
+        make(lanes, r, t);
         if (!memcmp(r, cr, 8) && !memcmp(t, ct, 8)) return;
         RegOp op{};
         op.kind = RO_TRANSPOSE;
@@ -138,13 +142,10 @@ struct RegPlanner {
         memcpy(ct, t, 8);
     }
     void controls(uint64_t cm, RegOp &op) const {
-        while (cm) {
-            int q = __builtin_ctzll(cm);
-            cm &= cm - 1;
-            int lb = lbit[q];
-            if (lb < 0) op.gcm |= uint64_t(1) << q;
-            else if (reg_index(lb) >= 0) op.rcm |= 1u << reg_index(lb);
-            else op.tcm |= 1u << lb;
+        for (; cm; cm &= cm - 1) {
+            const int q = __builtin_ctzll(cm);
+            if (lbit[q] < 0) op.gcm |= uint64_t(1) << q;
+            else op.cm |= 1u << lbit[q];
         }
     }
     Status emit(const Lowered &G, uint32_t src) {
@@ -157,9 +158,8 @@ struct RegPlanner {
             else if (G.kind == LK_PHASE) { op.kind = RO_PHASE; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; }
             else { op.kind = RO_RZ; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
             const int lb = lbit[G.t0];
-            if (lb < 0) { op.dloc = 2; op.dbit = (uint8_t)G.t0; }
-            else if (reg_index(lb) >= 0) { op.dloc = 0; op.dbit = (uint8_t)reg_index(lb); }
-            else { op.dloc = 1; op.dbit = (uint8_t)lb; }
+            if (lb < 0) { op.dloc = 2; op.dbit = (int)G.t0; }
+            else { op.dloc = 0; op.dbit = lb; }
             controls(G.ctrl_mask, op);
         } break;
         case LK_SWAP: case LK_MATCH: {
@@ -173,18 +173,14 @@ struct RegPlanner {
             op.kind = RO_LDS;
             op.lkind = G.kind;
             for (int k = 0; k < 8; k++) op.c[k] = G.c[k];
-            op.lp0 = (uint8_t)lbit[G.t0];
-            op.lp1 = (uint8_t)lbit[G.t1];
-            for (uint64_t cm = G.ctrl_mask; cm; cm &= cm - 1) {
-                const int q = __builtin_ctzll(cm);
-                if (lbit[q] >= 0) op.lcm |= 1u << lbit[q];
-                else op.gcm |= uint64_t(1) << q;
-            }
+            op.lp0 = lbit[G.t0];
+            op.lp1 = lbit[G.t1];
+            controls(G.ctrl_mask, op);
         } break;
         default: {
-            const int j = reg_index(lbit[G.t0]);
-            if (j < 0) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: target not in registers");
-            op.rb = (uint8_t)j;
+            const int i = lane_index(lbit[G.t0]);
+            if (i < 0) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: target not on a lane");
+            op.lane = i;
             switch (G.kind) {
             case LK_X: op.kind = RO_X; break;
             case LK_H: op.kind = RO_H; op.c[0] = G.c[0]; break;
@@ -213,7 +209,7 @@ struct RegPlanner {
 
 struct Phase {
     std::vector<size_t> ops;  // indices into the pass members, in execution order
-    uint32_t need = 0;        // local bits that must be register bits
+    uint32_t need = 0;        // local bits that must be lane bits
 };
 
 }  // namespace
@@ -226,7 +222,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
         return Status{};
     }
-    const int low = tile_low_qubits(s.prec), RB = K - kRegThreadBits;
+    const int low = tile_low_qubits(s.prec), RB = K - kRegThreadBits, LANES = kLaneBits;
     std::vector<GateSets> gs(gates.size());
     for (size_t g = 0; g < gates.size(); g++) gs[g] = gate_sets(gates[g]);
     std::vector<char> done(gates.size(), 0);
@@ -242,7 +238,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         rp.K = K;
         for (int b = 0, j = 0; b < (int)s.n; b++)
             if ((q >> b) & 1) { P.lbit[b] = j; P.qof[j] = b; j++; }
-        // ---- phases: ops whose register needs fit RB bits, commuting-greedy
+        // ---- phases: ops whose targets fit the lane bits, commuting-greedy
         std::vector<Phase> phases;
         {
             int lb[64];
@@ -258,7 +254,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
                     const GateSets &g = gs[members[i]];
                     if (blk.blocks(g)) { blk.add(g); continue; }
                     const uint32_t need = is_pair(G) ? 1u << lb[G.t0] : 0u;
-                    if (__builtin_popcount(ph.need | need) > RB) { blk.add(g); continue; }
+                    if (__builtin_popcount(ph.need | need) > LANES) { blk.add(g); continue; }
                     ph.need |= need;
                     ph.ops.push_back(i);
                     pd[i] = 1;
@@ -268,40 +264,38 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
                 phases.push_back(std::move(ph));
             }
         }
-        // ---- layout groups: consecutive phases whose needs fit one register set
+        // ---- layout groups: consecutive phases whose targets fit one lane set
         struct Group { size_t p0, p1; uint32_t need; };
         std::vector<Group> groups;
         for (size_t i = 0; i < phases.size();) {
             uint32_t need = phases[i].need;
             size_t j = i + 1;
-            while (j < phases.size() && __builtin_popcount(need | phases[j].need) <= RB) need |= phases[j++].need;
+            while (j < phases.size() && __builtin_popcount(need | phases[j].need) <= LANES) need |= phases[j++].need;
             groups.push_back({i, j, need});
             i = j;
         }
-        // ---- load layout: the first group's, when it keeps the row qubits on threads
+        // ---- load layout: the first group's, when the row qubits fit beside its targets
         {
-            uint32_t rs = groups.empty() ? 0u : groups[0].need;
-            if (rs & P.io_set()) rs = 0;
-            rs = P.fill(rs, P.io_set());
-            P.make(rs, P.cr, P.ct);
+            const uint32_t first_need = groups.empty() ? 0u : groups[0].need;
+            const uint32_t lanes = __builtin_popcount(first_need | P.io_set()) <= LANES ? P.fill_lanes(first_need)
+                                                                                      : P.fill_lanes(0);
+            P.make(lanes, P.cr, P.ct);
             memcpy(rp.lay_r, P.cr, 8);
             memcpy(rp.lay_t, P.ct, 8);
             for (int j = 0; j < RB; j++) rp.qr_in[j] = (uint8_t)P.qof[P.cr[j]];
             for (int i = 0; i < kRegThreadBits; i++) rp.qt_in[i] = (uint8_t)P.qof[P.ct[i]];
         }
         for (const Group &gr : groups) {
-            if ((gr.need & P.reg_set()) != gr.need) P.transpose_to(P.fill(gr.need, P.io_set()));
+            if ((gr.need & P.lane_set()) != gr.need) P.transpose_to(P.fill_lanes(gr.need));
             for (size_t p = gr.p0; p < gr.p1; p++)
                 for (size_t i : phases[p].ops) {
                     Status e = P.emit(gates[members[i]], (uint32_t)members[i]);
                     if (!e.ok()) return e;
                 }
         }
-        // ---- store: row qubits back on thread bits 0..L-1
+        // ---- store: row qubits back on lanes 0..L-1
         if (!P.io_ok()) {
-            uint32_t rs = P.reg_set();
-            if (rs & P.io_set()) rs = P.fill(0, P.io_set());
-            P.transpose_to(rs);
+            P.transpose_to(P.fill_lanes(0));
             if (!P.io_ok()) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: store layout");
         }
         for (int j = 0; j < RB; j++) rp.qr_out[j] = (uint8_t)P.qof[P.cr[j]];

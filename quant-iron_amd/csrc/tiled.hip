@@ -2,12 +2,15 @@
 // tile_plan.hip).
 //
 // 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in
-// registers. A layout assigns the K local index bits to 8 thread bits and
-// K-8 register bits. A 2x2 gate on a register-held target touches only the
-// thread's own registers: no LDS traffic and no barrier. Diagonal gates and
-// controls work on any bit. When the next gates need other register bits,
-// the tile is transposed to a new layout through LDS. Controlled SWAPs and
-// Matchgates run on the LDS copy of the tile.
+// registers. A layout assigns the K local index bits to 8 thread bits (6 lane
+// bits, 2 wave bits) and K-8 register bits. A 2x2 gate acts on a lane-held
+// target: every lane fetches its partner's amplitudes from lane ^ 2^i with
+// ds_bpermute (the LDS crossbar, no LDS memory, no barrier) and computes its
+// own half of the pair, so the code of an op is the same for every target
+// and every register. Diagonal gates and controls test the local index
+// (thread part | register part) of each amplitude. When the next gates need
+// other target bits on lanes, the tile is transposed to a new layout through
+// LDS. Controlled SWAPs and Matchgates run on the LDS copy of the tile.
 //
 // Arithmetic: complex128 follows the reference's operand order (no
 // contraction); complex64 uses fused multiply-adds.
@@ -69,73 +72,66 @@ template <int F> __device__ __forceinline__ uint32_t swz(uint32_t e) {
     else return e ^ (((e >> 3) ^ (e >> 6) ^ (e >> 9) ^ (e >> 12)) & 7u);
 }
 
-// --------------------------------------------------------------- 2x2 ops --
-template <int KIND, typename R> __device__ __forceinline__ void pair(float2 &a, float2 &b, const R *c) {
-    if constexpr (KIND == RO_X) {
-        const float2 t = a; a = b; b = t;
-    } else if constexpr (KIND == RO_H) {
-        const float2 s = cadd(a, b), d = csub(a, b);
-        a = cscale<float2>(c[0], s);
-        b = cscale<float2>(c[0], d);
-    } else if constexpr (KIND == RO_REAL) {  // [[c, -s], [s, c]]
-        const float2 na = mk<float2>(__builtin_fmaf(-c[1], b.x, c[0] * a.x), __builtin_fmaf(-c[1], b.y, c[0] * a.y));
-        const float2 nb = mk<float2>(__builtin_fmaf(c[1], a.x, c[0] * b.x), __builtin_fmaf(c[1], a.y, c[0] * b.y));
-        a = na; b = nb;
-    } else {  // general complex 2x2, row-major
-        float2 na, nb;
-        na.x = __builtin_fmaf(-c[3], b.y, __builtin_fmaf(c[2], b.x, __builtin_fmaf(-c[1], a.y, c[0] * a.x)));
-        na.y = __builtin_fmaf(c[3], b.x, __builtin_fmaf(c[2], b.y, __builtin_fmaf(c[1], a.x, c[0] * a.y)));
-        nb.x = __builtin_fmaf(-c[7], b.y, __builtin_fmaf(c[6], b.x, __builtin_fmaf(-c[5], a.y, c[4] * a.x)));
-        nb.y = __builtin_fmaf(c[7], b.x, __builtin_fmaf(c[6], b.y, __builtin_fmaf(c[5], a.x, c[4] * a.y)));
-        a = na; b = nb;
-    }
-}
-template <int KIND, typename R> __device__ __forceinline__ void pair(double2 &a, double2 &b, const R *c) {
-    if constexpr (KIND == RO_X) {
-        const double2 t = a; a = b; b = t;
-    } else if constexpr (KIND == RO_H) {
-        const double2 s = cadd(a, b), d = csub(a, b);
-        a = cscale<double2>(c[0], s);
-        b = cscale<double2>(c[0], d);
-    } else if constexpr (KIND == RO_REAL) {
-        const double2 na = csub(cscale<double2>(c[0], a), cscale<double2>(c[1], b));
-        const double2 nb = cadd(cscale<double2>(c[1], a), cscale<double2>(c[0], b));
-        a = na; b = nb;
-    } else {
-        const double2 na = cadd(cmul(mk<double2>(c[0], c[1]), a), cmul(mk<double2>(c[2], c[3]), b));
-        const double2 nb = cadd(cmul(mk<double2>(c[4], c[5]), a), cmul(mk<double2>(c[6], c[7]), b));
-        a = na; b = nb;
-    }
-}
-
-// Element updates carry no control flow: a controlled op computes every pair
-// and keeps the old values by a select, so the tile stays in the same
-// registers on every path through the op switch.
 template <typename T> __device__ __forceinline__ T sel(bool on, T x, T y) { return mk<T>(on ? x.x : y.x, on ? x.y : y.y); }
 
-template <int B, int KIND, bool CTRL, int E, typename T, typename R>
-__device__ __forceinline__ void pairs(Regs<R, E> &v, const R *c, uint32_t rcm, bool tok) {
-#pragma unroll
-    for (int r = 0; r < E; r++) {
-        if (r & (1 << B)) continue;
-        const T a = v.get(r), b = v.get(r | (1 << B));
-        T na = a, nb = b;
-        pair<KIND, R>(na, nb, c);
-        if constexpr (CTRL) {
-            const bool on = tok && ((uint32_t(r) & rcm) == rcm);
-            na = sel(on, na, a);
-            nb = sel(on, nb, b);
+// The amplitude of lane `addr / 4` (ds_bpermute: the LDS crossbar, no LDS memory).
+__device__ __forceinline__ float xlane(int addr, float x) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+__device__ __forceinline__ double xlane(int addr, double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <typename T> __device__ __forceinline__ T xlane2(int addr, T x) { return mk<T>(xlane(addr, x.x), xlane(addr, x.y)); }
+
+// ------------------------------------------------------------- 2x2 ops --
+// Lane half `bit` of the pair (a, b) on its target: bit 0 holds a and
+// computes a', bit 1 holds b and computes b'; s = own, p = partner.
+//   X:    a' = b,               b' = a
+//   H:    a' = h (a + b),       b' = h (a - b)          -> h (sgn s + p)
+//   REAL: a' = c a - s b,       b' = s a + c b          -> c s + (+-s) p
+//   GEN:  a' = m00 a + m01 b,   b' = m10 a + m11 b      -> cs s + cp p
+// The complex128 forms keep the reference's products and only reorder
+// commutative sums, which IEEE arithmetic evaluates exactly alike.
+template <int KIND, typename T, typename R>
+__device__ __forceinline__ T pair_lane(T s, T p, const R *k) {
+    if constexpr (KIND == RO_X) {
+        return p;
+    } else if constexpr (KIND == RO_H) {  // k0 = sgn, k1 = h
+        if constexpr (sizeof(R) == 4) return mk<T>(k[1] * __builtin_fmaf(k[0], s.x, p.x), k[1] * __builtin_fmaf(k[0], s.y, p.y));
+        else return mk<T>(k[1] * (k[0] * s.x + p.x), k[1] * (k[0] * s.y + p.y));
+    } else if constexpr (KIND == RO_REAL) {  // k0 = c, k1 = +-s
+        if constexpr (sizeof(R) == 4)
+            return mk<T>(__builtin_fmaf(k[1], p.x, k[0] * s.x), __builtin_fmaf(k[1], p.y, k[0] * s.y));
+        else return mk<T>(k[0] * s.x + k[1] * p.x, k[0] * s.y + k[1] * p.y);
+    } else {  // k0,k1 = cs, k2,k3 = cp
+        if constexpr (sizeof(R) == 4) {
+            float x = __builtin_fmaf(-k[1], s.y, k[0] * s.x);
+            x = __builtin_fmaf(k[2], p.x, x);
+            x = __builtin_fmaf(-k[3], p.y, x);
+            float y = __builtin_fmaf(k[1], s.x, k[0] * s.y);
+            y = __builtin_fmaf(k[2], p.y, y);
+            y = __builtin_fmaf(k[3], p.x, y);
+            return mk<T>(x, y);
+        } else {
+            return mk<T>((k[0] * s.x - k[1] * s.y) + (k[2] * p.x - k[3] * p.y),
+                         (k[0] * s.y + k[1] * s.x) + (k[2] * p.y + k[3] * p.x));
         }
-        v.set(r, na);
-        v.set(r | (1 << B), nb);
     }
 }
 
-template <int KIND, bool CTRL, int RB, int E, typename T, typename R>
-__device__ __forceinline__ void pairs_at(Regs<R, E> &v, int rb, const R *c, uint32_t rcm, bool tok) {
-#define QI_RB(BB) case BB: if constexpr (BB < RB) pairs<BB, KIND, CTRL, E, T, R>(v, c, rcm, tok); break;
-    switch (rb) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
-#undef QI_RB
+template <int KIND, bool CTRL, int E, typename T, typename R>
+__device__ __forceinline__ void pairs(Regs<R, E> &v, int addr, const R *k, uint32_t tl, const uint32_t *rl, uint32_t cm) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        const T s = v.get(r);
+        const T p = xlane2<T>(addr, s);
+        T n = pair_lane<KIND, T, R>(s, p, k);
+        if constexpr (CTRL) n = sel(((tl | rl[r]) & cm) == cm, n, s);
+        v.set(r, n);
+    }
 }
 
 // ---------------------------------------------------------- diagonal ops --
@@ -145,24 +141,18 @@ template <int KIND, typename T> __device__ __forceinline__ T dphase(T a, T p0, T
     else return cmul(a, sel(one, p1, p0));
 }
 
-// B >= 0: target on register bit B; B < 0: target fixed per thread (`tone`)
-template <int B, int KIND, bool CTRL, int E, typename T, typename R>
-__device__ __forceinline__ void diag(Regs<R, E> &v, T p0, T p1, uint32_t rcm, bool tok, bool tone) {
+// one(r) = target bit of amplitude r: thread part `tb` | register part of rl[r]
+template <int KIND, bool CTRL, int E, typename T, typename R>
+__device__ __forceinline__ void diag(Regs<R, E> &v, T p0, T p1, uint32_t tb, uint32_t dmask, uint32_t tl, const uint32_t *rl,
+                                     uint32_t cm) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        const bool one = B >= 0 ? (((r >> (B >= 0 ? B : 0)) & 1) != 0) : tone;
+        const bool one = (tb | (rl[r] & dmask)) != 0;
         const T a = v.get(r);
-        T na = dphase<KIND, T>(a, p0, p1, one);
-        if constexpr (CTRL) na = sel(tok && ((uint32_t(r) & rcm) == rcm), na, a);
-        v.set(r, na);
+        T n = dphase<KIND, T>(a, p0, p1, one);
+        if constexpr (CTRL) n = sel(((tl | rl[r]) & cm) == cm, n, a);
+        v.set(r, n);
     }
-}
-template <int KIND, bool CTRL, int RB, int E, typename T, typename R>
-__device__ __forceinline__ void diag_at(Regs<R, E> &v, int dloc, int dbit, T p0, T p1, uint32_t rcm, bool tok, bool tone) {
-    if (dloc != 0) { diag<-1, KIND, CTRL, E, T, R>(v, p0, p1, rcm, tok, tone); return; }
-#define QI_RB(BB) case BB: if constexpr (BB < RB) diag<BB, KIND, CTRL, E, T, R>(v, p0, p1, rcm, tok, tone); break;
-    switch (dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
-#undef QI_RB
 }
 
 // ------------------------------------------------------------- LDS ops --
@@ -172,7 +162,7 @@ template <typename R, int K, int F>
 __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &op) {
     using T = typename V2<R>::T;
     constexpr int N = 1 << K;
-    const uint32_t lcm = op.lcm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
+    const uint32_t lcm = op.cm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
     const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
     const R c0 = R(op.c[0]), c1 = R(op.c[1]);
     const T m23 = mk<T>(R(op.c[2]), R(op.c[3])), m45 = mk<T>(R(op.c[4]), R(op.c[5])), m67 = mk<T>(R(op.c[6]), R(op.c[7]));
@@ -203,12 +193,12 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
     using T = typename V2<R>::T;
     constexpr int RB = K - kRegThreadBits, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
     __shared__ T lds[1 << K];
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
 
     Regs<R, E> v;
-    uint32_t mr[RB];  // local-bit mask of each register bit (uniform)
     uint32_t tl = 0;  // local index bits held by this thread
+    uint32_t rl[E];   // local index bits of register r (uniform)
     {
         uint64_t at = 0;
 #pragma unroll
@@ -218,85 +208,112 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
             at |= uint64_t(bit) << pass.qt_in[i];
         }
         uint64_t ar[RB];
+        uint32_t mr[RB];
 #pragma unroll
         for (int j = 0; j < RB; j++) { mr[j] = 1u << pass.lay_r[j]; ar[j] = uint64_t(1) << pass.qr_in[j]; }
         const T *src = psi + (base | at);
 #pragma unroll
         for (int r = 0; r < E; r++) {
             uint64_t o = 0;
+            uint32_t e = 0;
 #pragma unroll
-            for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= ar[j];
+            for (int j = 0; j < RB; j++)
+                if ((r >> j) & 1) { o |= ar[j]; e |= mr[j]; }
+            rl[r] = e;
             v.set(r, src[o]);
         }
     }
     bool dirty = false;  // LDS written since the last barrier
-    // the op table of a pass ends with a no-op, so op o + 1 can be fetched
-    // while op o runs
+    // The op table of a pass ends with a no-op, so op o + 1 can be fetched
+    // while op o runs. In-register ops run in an inner loop; LDS ops
+    // (transposes, LDS gates) sit outside it.
     RegOp nxt = ops[pass.op0];
-    for (int o = 0; o < pass.nops; o++) {
+    int o = 0;
+    for (;;) {
+        for (; o < pass.nops; o++) {
+            const RegOp op = nxt;
+            const int kind = op.kind;
+            if (kind >= RO_TRANSPOSE) break;
+            nxt = ops[pass.op0 + o + 1];
+            if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
+            const bool ctrl = op.cm != 0;             // controls inside the tile (uniform)
+            if (kind <= RO_GEN) {
+                const int addr = int((lane ^ (1u << op.lane)) << 2);
+                const bool bit = ((lane >> op.lane) & 1u) != 0;
+                R k[4];
+                if (kind == RO_H) {
+                    k[0] = bit ? R(-1) : R(1);
+                    k[1] = R(op.c[0]);
+                } else if (kind == RO_REAL) {
+                    k[0] = R(op.c[0]);
+                    k[1] = bit ? R(op.c[1]) : -R(op.c[1]);
+                } else {
+                    k[0] = R(bit ? op.c[6] : op.c[0]);
+                    k[1] = R(bit ? op.c[7] : op.c[1]);
+                    k[2] = R(bit ? op.c[4] : op.c[2]);
+                    k[3] = R(bit ? op.c[5] : op.c[3]);
+                }
+#define QI_PAIR(KD)                                                          \
+    case KD:                                                                 \
+        if (ctrl) pairs<KD, true, E, T, R>(v, addr, k, tl, rl, op.cm);       \
+        else pairs<KD, false, E, T, R>(v, addr, k, tl, rl, op.cm);           \
+        break;
+                switch (kind) { QI_PAIR(RO_X) QI_PAIR(RO_H) QI_PAIR(RO_REAL) QI_PAIR(RO_GEN) default: break; }
+#undef QI_PAIR
+            } else {
+                const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
+                uint32_t tb, dmask;
+                if (op.dloc == 0) { dmask = 1u << op.dbit; tb = tl & dmask; }
+                else { dmask = 0; tb = uint32_t((base >> op.dbit) & 1u); }
+#define QI_DIAG(KD)                                                                  \
+    case KD:                                                                         \
+        if (ctrl) diag<KD, true, E, T, R>(v, p0, p1, tb, dmask, tl, rl, op.cm);      \
+        else diag<KD, false, E, T, R>(v, p0, p1, tb, dmask, tl, rl, op.cm);          \
+        break;
+                switch (kind) { QI_DIAG(RO_Z) QI_DIAG(RO_PHASE) QI_DIAG(RO_RZ) default: break; }
+#undef QI_DIAG
+            }
+        }
+        if (o >= pass.nops) break;
+        // through LDS: a transpose to a new layout, or a gate on the LDS copy
         const RegOp op = nxt;
         nxt = ops[pass.op0 + o + 1];
-        const int kind = op.kind;
-        if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-        const bool ctrl = (op.rcm | op.tcm) != 0;  // controls held in the tile (uniform)
-        const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
-        if (kind <= RO_GEN) {
-            R c[8];
+        o++;
+        if ((base & op.gcm) != op.gcm) continue;
+        if (dirty) __syncthreads();
+        char *ldsb = reinterpret_cast<char *>(lds);
+        {
+            const uint32_t stb = swz<F>(tl) * uint32_t(sizeof(T));  // XOR-linear: slot(tl | e) = slot(tl) ^ slot(e)
 #pragma unroll
-            for (int k = 0; k < 8; k++) c[k] = R(op.c[k]);
-#define QI_PAIR(KD)                                                                     \
-    case KD:                                                                            \
-        if (ctrl) pairs_at<KD, true, RB, E, T, R>(v, op.rb, c, op.rcm, tok);            \
-        else pairs_at<KD, false, RB, E, T, R>(v, op.rb, c, op.rcm, tok);                \
-        break;
-            switch (kind) { QI_PAIR(RO_X) QI_PAIR(RO_H) QI_PAIR(RO_REAL) QI_PAIR(RO_GEN) default: break; }
-#undef QI_PAIR
-        } else if (kind <= RO_RZ) {
-            const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
-            const bool tone = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
-#define QI_DIAG(KD)                                                                             \
-    case KD:                                                                                    \
-        if (ctrl) diag_at<KD, true, RB, E, T, R>(v, op.dloc, op.dbit, p0, p1, op.rcm, tok, tone);  \
-        else diag_at<KD, false, RB, E, T, R>(v, op.dloc, op.dbit, p0, p1, op.rcm, tok, tone);      \
-        break;
-            switch (kind) { QI_DIAG(RO_Z) QI_DIAG(RO_PHASE) QI_DIAG(RO_RZ) default: break; }
-#undef QI_DIAG
-        } else if (kind <= RO_LDS) {
-            // through LDS: a transpose to a new layout, or a gate on the LDS copy
-            if (dirty) __syncthreads();
-            {
-                const uint32_t st = swz<F>(tl);
-#pragma unroll
-                for (int r = 0; r < E; r++) {
-                    uint32_t e = 0;
-#pragma unroll
-                    for (int j = 0; j < RB; j++) if ((r >> j) & 1) e |= mr[j];
-                    lds[st ^ swz<F>(e)] = v.get(r);
-                }
-            }
-            __syncthreads();
-            if (kind == RO_TRANSPOSE) {
-                tl = 0;
-#pragma unroll
-                for (int i = 0; i < kRegThreadBits; i++) tl |= ((tid >> i) & 1u) << op.lay_t[i];
-#pragma unroll
-                for (int j = 0; j < RB; j++) mr[j] = 1u << op.lay_r[j];
-            } else {
-                lds_gate<R, K, F>(lds, op);
-                __syncthreads();
-            }
-            {
-                const uint32_t st = swz<F>(tl);
-#pragma unroll
-                for (int r = 0; r < E; r++) {
-                    uint32_t e = 0;
-#pragma unroll
-                    for (int j = 0; j < RB; j++) if ((r >> j) & 1) e |= mr[j];
-                    v.set(r, lds[st ^ swz<F>(e)]);
-                }
-            }
-            dirty = true;
+            for (int r = 0; r < E; r++)
+                *reinterpret_cast<T *>(ldsb + (stb ^ (swz<F>(rl[r]) * uint32_t(sizeof(T))))) = v.get(r);
         }
+        __syncthreads();
+        if (op.kind == RO_TRANSPOSE) {
+            tl = 0;
+#pragma unroll
+            for (int i = 0; i < kRegThreadBits; i++) tl |= ((tid >> i) & 1u) << op.lay_t[i];
+            uint32_t mr[RB];
+#pragma unroll
+            for (int j = 0; j < RB; j++) mr[j] = 1u << op.lay_r[j];
+#pragma unroll
+            for (int r = 0; r < E; r++) {
+                uint32_t e = 0;
+#pragma unroll
+                for (int j = 0; j < RB; j++) if ((r >> j) & 1) e |= mr[j];
+                rl[r] = e;
+            }
+        } else {
+            lds_gate<R, K, F>(lds, op);
+            __syncthreads();
+        }
+        {
+            const uint32_t stb = swz<F>(tl) * uint32_t(sizeof(T));
+#pragma unroll
+            for (int r = 0; r < E; r++)
+                v.set(r, *reinterpret_cast<const T *>(ldsb + (stb ^ (swz<F>(rl[r]) * uint32_t(sizeof(T))))));
+        }
+        dirty = true;
     }
     {
         // store: thread and register bits hold the qubits of the final layout

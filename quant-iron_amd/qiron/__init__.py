@@ -22,6 +22,7 @@ from .core import (  # noqa: F401
     SumOp,
     Subroutine,
     default_context,
+    expectation_terms,
     first_order_trotter_step,
     gate_array,
     ry_phase_dag_matrix,

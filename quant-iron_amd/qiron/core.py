@@ -748,6 +748,16 @@ class SumOp:
         return complex(re.value, im.value)
 
 
+def expectation_terms(state: State, terms: np.ndarray) -> complex:
+    """<psi| sum_k c_k P_k |psi> for a qi_pauli_term array (SumOp::expectation_value)."""
+    arr = np.ascontiguousarray(terms)
+    re, im = C.c_double(), C.c_double()
+    err = L.QiError()
+    _check(state.lib.qi_sumop_expectation(state._h, arr.ctypes.data_as(C.c_void_p), arr.size, C.byref(re), C.byref(im),
+                                           C.byref(err)), err)
+    return complex(re.value, im.value)
+
+
 # --------------------------------------------------------- time evolution --
 FIRST, SECOND = 1, 2
 
