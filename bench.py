@@ -69,6 +69,108 @@ def cpu_baseline(qiron_workloads, n_cpu: int, budget_s: float, threads: int):
     return done, dt
 
 
+def bench_config1(qiron, L, W, args):
+    """20-qubit complex128: QFT on |+..+>, 5 rotation/CNOT-ladder layers, the
+    energy of an open Heisenberg chain with fields; GPU against the oracle
+    (amplitudes and energy) and the reference-structured CPU path."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    gates, terms = W.config1()
+    n = 20
+    s = qiron.State.new_plus(n, "c128")
+    s.apply_gates_inplace(gates)  # warm-up (plans, loads code)
+    s = qiron.State.new_plus(n, "c128")
+    s.ctx.synchronize()
+    t0 = time.perf_counter()
+    s.apply_gates_inplace(gates)
+    e = qiron.expectation_terms(s, terms)
+    s.ctx.synchronize()
+    t_gpu = time.perf_counter() - t0
+    psi = s.state_vector
+    t0 = time.perf_counter()
+    ref = O.apply_gates(O.init(n, 2), gates)
+    t_oracle = time.perf_counter() - t0
+    e_ref = O.expectation(ref, terms)
+    out = dict(gates=int(gates.size), gpu_seconds=t_gpu, energy=e.real, energy_oracle=e_ref.real,
+               max_abs_amp_diff=float(np.max(np.abs(psi - ref))), abs_dE=abs(e.real - e_ref.real),
+               oracle_seconds=t_oracle)
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        rs = O.reference_structured_apply(O.init(n, 2), gates, threads)
+        out["reference_structured_seconds"] = time.perf_counter() - t0
+        out["reference_structured_threads"] = threads
+        out["reference_structured_max_diff"] = float(np.max(np.abs(rs - ref)))
+    log("config 1 (20q c128):", out)
+    return out
+
+
+def bench_trotter(qiron, W, args):
+    """5x5 open 2-D Heisenberg, 200 first-order steps of dt = 0.02 from the Neel
+    state, complex64 beside complex128: energy drift and fidelity every 20 steps."""
+    import numpy as np
+
+    terms, xs = W.trotter_2d()
+    n = 25
+    states = {}
+    for prec in ("c64", "c128"):
+        st = qiron.State.new_zero(n, prec)
+        for q in xs:
+            st.apply_gate_inplace(qiron.Gate.x_gate(q))
+        states[prec] = st
+    e0 = {p: qiron.expectation_terms(st, terms).real for p, st in states.items()}
+    rows, t_c64 = [], 0.0
+    for chunk in range(10):
+        for prec, st in states.items():
+            st.ctx.synchronize()
+            t0 = time.perf_counter()
+            qiron.trotter_evolve_terms_inplace(st, terms, 0.02, 20)
+            st.ctx.synchronize()
+            if prec == "c64":
+                t_c64 += time.perf_counter() - t0
+        a, b = states["c64"].state_vector, states["c128"].state_vector
+        fid = float(abs(np.vdot(b, a)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real))
+        e = {p: qiron.expectation_terms(st, terms).real for p, st in states.items()}
+        rows.append(dict(step=20 * (chunk + 1), fidelity_c64_vs_c128=fid, drift_c64=e["c64"] - e0["c64"],
+                         drift_c128=e["c128"] - e0["c128"], dE_c64_vs_c128=e["c64"] - e["c128"]))
+    apps = 200 * terms.size
+    S = (1 << n) * 8
+    out = dict(terms=int(terms.size), steps=200, exp_applications=apps, c64_seconds=t_c64,
+               c64_exp_per_s=apps / t_c64, c64_effective_GBps=2 * S * apps / t_c64 / 1e9, checkpoints=rows)
+    log(f"trotter 25q: {apps / t_c64:.0f} exp(P)/s (c64), final fidelity {rows[-1]['fidelity_c64_vs_c128']:.9f}, "
+        f"drift c64 {rows[-1]['drift_c64']:.2e}")
+    return out
+
+
+def bench_ceiling(qiron, L, W):
+    """Memory ceiling: one layer of the random circuit at 34 qubits complex64 and
+    33 qubits complex128 (128 GiB states), tiled executor."""
+    out = {}
+    for n, prec, ab in ((34, "c64", 8), (33, "c128", 16)):
+        gates = W.random_circuit(n=n, depth=1, seed=W.SEED_BASE + 2)
+        try:
+            s = qiron.State.new_plus(n, prec)
+            s.apply_gates_inplace(gates, mode=L.EXEC_TILED)
+            s.ctx.synchronize()
+            t0 = time.perf_counter()
+            s.apply_gates_inplace(gates, mode=L.EXEC_TILED)
+            s.ctx.synchronize()
+            dt = time.perf_counter() - t0
+            passes = s.ctx.last_exec_steps()
+            norm = s.norm_sqr()
+            del s
+            S = (1 << n) * ab
+            out[f"{n}q_{prec}"] = dict(gates=int(gates.size), seconds=dt, gates_per_s=gates.size / dt, passes=passes,
+                                       pass_GBps=2 * S * passes / dt / 1e9, norm=norm, state_GiB=S / 2 ** 30)
+        except Exception as ex:  # report, do not hide
+            out[f"{n}q_{prec}"] = dict(error=str(ex))
+        log(f"ceiling {n}q {prec}:", out[f"{n}q_{prec}"])
+    return out
+
+
 def bench_energies(qiron, L, W, args):
     """Batched ansatz energies (4096 circuits, 10..22 qubits, 4..8 layers, each
     with its own open Heisenberg chain) in complex64 and complex128, against
@@ -84,7 +186,7 @@ def bench_energies(qiron, L, W, args):
     for prec in ("c64", "c128"):
         t0 = time.perf_counter()
         b = S.ansatz_batch(specs, prec)
-        e = b.run(use_graph=True)  # plans, captures the graph, runs once
+        k, ks, tuned_ms = b.autotune(trials=2)  # plans, captures and times every schedule
         setup = time.perf_counter() - t0
         walls = []
         for _ in range(args.steps):
@@ -94,9 +196,10 @@ def bench_energies(qiron, L, W, args):
         info = b.info()
         wall = min(walls)
         res[f"batch_{prec}"] = dict(energies_per_s=count / wall, wall_ms=wall * 1e3, device_ms=info["last_ms"],
-                                    setup_s=setup, levels=info["levels"], lds_resident=info["lds_resident"])
+                                    setup_s=setup, levels=info["levels"], lds_resident=info["lds_resident"],
+                                    schedule={"tile_qubits": k, "lds_resident_max_qubits": ks, "tuned_ms": tuned_ms})
         log(f"batched {prec}: {count} energies in {wall * 1e3:.1f} ms ({count / wall:.0f}/s), device "
-            f"{info['last_ms']:.1f} ms, setup {setup:.1f} s")
+            f"{info['last_ms']:.1f} ms, autotuned schedule k={k} ks={ks}, setup {setup:.1f} s")
         E[prec] = e.real
         del b
     dE = float(np.max(np.abs(E["c64"] - E["c128"])))
@@ -191,6 +294,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--candidates", type=int, default=1 << 20)
     ap.add_argument("--no-energies", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip configs 1 and 4 and the memory-ceiling runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the state-vector engine here is single-GPU "
@@ -271,6 +375,10 @@ def main():
         eres, plain_eps = bench_energies(qiron, L, W, args)
         results["energies"] = eres
         results["surrogate"] = bench_surrogate(W, args, plain_eps)
+    if not args.only and not args.no_extras:
+        results["config1"] = bench_config1(qiron, L, W, args)
+        results["trotter"] = bench_trotter(qiron, W, args)
+        results["ceiling"] = bench_ceiling(qiron, L, W)
 
     cpu = None
     if not args.no_cpu and not args.only:

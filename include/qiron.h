@@ -297,6 +297,16 @@ qi_status qi_batch_set_term_coeffs(qi_batch_t b, const double *coeffs, size_t nt
 qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *err);
 qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uint64_t *nlevels,
                         double *last_ms, qi_error *err);
+/* Schedule of the batch: circuits with n <= ks qubits run LDS-resident (one
+ * workgroup each); larger ones run tiled passes of 2^k amplitudes.
+ * 10 <= k <= ks <= 13 (complex64) / 12 (complex128). Default k = ks = 13 / 12. */
+qi_status qi_batch_set_schedule(qi_batch_t b, uint32_t k, uint32_t ks, qi_error *err);
+qi_status qi_batch_get_schedule(qi_batch_t b, uint32_t *k, uint32_t *ks, qi_error *err);
+/* Autotuner: plans, captures and times every schedule on this batch (best of
+ * `trials` replays after a warm-up, HIP-event device time), keeps the fastest
+ * and leaves the batch planned with it. No reference counterpart. */
+qi_status qi_batch_autotune(qi_batch_t b, uint32_t trials, uint32_t *k, uint32_t *ks, double *best_ms,
+                            qi_error *err);
 
 /* ---------------- energy surrogate (MLP) ---------------- */
 /* An MLP n_in -> H -> H -> H -> 1 with GELU, BF16 weights on the matrix

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <unordered_map>
 
 #include "qi_internal.h"
 #include "exec_plan.h"
@@ -367,7 +368,8 @@ struct Batch {
     std::vector<std::pair<size_t, size_t>> elevels;
     std::vector<uint32_t> elevel_blocks;
     size_t nsmall = 0, nfold = 0;
-    int K = 13, KS = 13;
+    int K = 0, KS = 0;        // tile width of tiled circuits, largest LDS-resident circuit (0: defaults)
+    std::unordered_multimap<uint64_t, uint32_t> prog_index;  // shape hash -> program
     bool planned = false;
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
@@ -402,8 +404,9 @@ template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &
 // out per-circuit tasks, expectation passes and the energy fold.
 Status batch_plan(Batch &B) {
     const bool c128 = B.prec == QI_C128;
-    B.K = c128 ? 12 : 13;
-    B.KS = B.K;
+    if (!B.K) B.K = c128 ? 12 : 13;
+    if (!B.KS) B.KS = B.K;
+    if (B.KS < B.K) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "LDS-resident limit below the tile width");
     const int L = c128 ? 3 : 4;
     std::vector<TileOp> ops;
     for (auto &P : B.progs) {
@@ -507,6 +510,7 @@ Status batch_plan(Batch &B) {
                 if (__builtin_popcountll(q | x) <= B.K) { q |= x; take.push_back(k); }
                 else left.push_back(k);
             }
+            if (take.empty()) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "Pauli term wider than the tile");
             for (int b = 0; __builtin_popcountll(q) < B.K; b++) q |= uint64_t(1) << b;
             BigTask bt{};
             bt.off = off; bt.qmask = q;
@@ -608,8 +612,20 @@ static Status batch_launch(Batch &B, hipStream_t st) {
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
                                (const double *)B.d_angles.p, (double *)B.d_energy.p);
     }
-    if (B.prec == QI_C128) launch_levels<double, 12>(B, st);
-    else launch_levels<float, 13>(B, st);
+    if (B.prec == QI_C128) {
+        switch (B.K) {
+        case 10: launch_levels<double, 10>(B, st); break;
+        case 11: launch_levels<double, 11>(B, st); break;
+        default: launch_levels<double, 12>(B, st); break;
+        }
+    } else {
+        switch (B.K) {
+        case 10: launch_levels<float, 10>(B, st); break;
+        case 11: launch_levels<float, 11>(B, st); break;
+        case 12: launch_levels<float, 12>(B, st); break;
+        default: launch_levels<float, 13>(B, st); break;
+        }
+    }
     QI_HIP_TRY(hipEventRecord(B.join, B.side));
     QI_HIP_TRY(hipStreamWaitEvent(st, B.join, 0));
     QI_HIP_TRY(hipGetLastError());
@@ -707,13 +723,22 @@ qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ng
             QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "Pauli term too wide for batched evaluation"));
     }
     std::lock_guard<std::mutex> lk(B.ctx->mu);
-    // reuse the previous program when the circuit has the same shape
+    // circuits of the same shape (gates and relative angle slots) share one program
+    uint64_t h = 1469598103934665603ull ^ n;
+    auto mix = [&h](const void *p, size_t len) {
+        const unsigned char *c = (const unsigned char *)p;
+        for (size_t i = 0; i < len; i++) { h ^= c[i]; h *= 1099511628211ull; }
+    };
+    if (ngates) mix(gates, ngates * sizeof(qi_gate));
+    if (ngates) mix(rel.data(), ngates * sizeof(int32_t));
     uint32_t prog = (uint32_t)B.progs.size();
-    if (!B.progs.empty()) {
-        const auto &P = B.progs.back();
+    for (auto it = B.prog_index.equal_range(h); it.first != it.second; ++it.first) {
+        const auto &P = B.progs[it.first->second];
         if (P.n == n && P.gates.size() == ngates && P.rel_slot == rel &&
-            (ngates == 0 || memcmp(P.gates.data(), gates, ngates * sizeof(qi_gate)) == 0))
-            prog = (uint32_t)B.progs.size() - 1;
+            (ngates == 0 || memcmp(P.gates.data(), gates, ngates * sizeof(qi_gate)) == 0)) {
+            prog = it.first->second;
+            break;
+        }
     }
     if (prog == B.progs.size()) {
         Batch::Program P;
@@ -730,6 +755,7 @@ qi_status qi_batch_add(qi_batch_t b, uint32_t n, const qi_gate *gates, size_t ng
         P.gates.assign(gates, gates + ngates);
         P.rel_slot = rel;
         B.progs.push_back(std::move(P));
+        B.prog_index.emplace(h, prog);
     }
     Batch::Circ C;
     C.prog = prog;
@@ -794,6 +820,77 @@ qi_status qi_batch_run(qi_batch_t b, double *energies, int use_graph, qi_error *
             QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slot beyond the angle table"));
     }
     QB_RET(batch_run(B, energies, use_graph != 0));
+}
+
+// Sets the schedule: tile width K of the tiled circuits (10..13 for complex64,
+// 10..12 for complex128) and the largest circuit run LDS-resident (K <= ks <= 13 / 12).
+qi_status qi_batch_set_schedule(qi_batch_t b, uint32_t k, uint32_t ks, qi_error *err) {
+    if (!b) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    const uint32_t kmax = B.prec == QI_C128 ? 12 : 13;
+    if (k < 10 || k > kmax || ks < k || ks > kmax) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "schedule out of range"));
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    B.K = (int)k;
+    B.KS = (int)ks;
+    B.planned = false;
+    QB_RET(Status{});
+}
+
+qi_status qi_batch_get_schedule(qi_batch_t b, uint32_t *k, uint32_t *ks, qi_error *err) {
+    if (!b) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    const int kd = B.prec == QI_C128 ? 12 : 13;
+    if (k) *k = (uint32_t)(B.K ? B.K : kd);
+    if (ks) *ks = (uint32_t)(B.KS ? B.KS : (B.K ? B.K : kd));
+    QB_RET(Status{});
+}
+
+// Autotuner: plans, captures and times every schedule (K, ks) on this batch
+// (best of `trials` graph replays after one warm-up run each, device time
+// from HIP events) and keeps the fastest; schedules the batch cannot use
+// (a Pauli term wider than the tile) are skipped. The batch is left planned
+// with the winner.
+qi_status qi_batch_autotune(qi_batch_t b, uint32_t trials, uint32_t *k_out, uint32_t *ks_out, double *best_ms,
+                            qi_error *err) {
+    if (!b) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Batch &B = b->b;
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    if (B.circs.empty()) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "empty batch"));
+    for (auto &C : B.circs) {
+        int32_t top = -1;
+        for (int32_t r : B.progs[C.prog].rel_slot) top = std::max(top, r);
+        if (top >= 0 && size_t(C.slot_base) + size_t(top) >= B.angles.size())
+            QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "angle slot beyond the angle table"));
+    }
+    const int kmax = B.prec == QI_C128 ? 12 : 13;
+    std::vector<double> en(2 * B.circs.size());
+    double best = 1e300;
+    int bk = 0, bks = 0;
+    for (int k = 10; k <= kmax; k++)
+        for (int ks = k; ks <= kmax; ks++) {
+            B.K = k;
+            B.KS = ks;
+            B.planned = false;
+            Status e = batch_run(B, en.data(), true);  // plan, capture, warm up
+            if (e.code == QI_ERR_INVALID_ARGUMENT) continue;
+            if (!e.ok()) QB_RET(e);
+            double t = 1e300;
+            for (uint32_t r = 0; r < std::max<uint32_t>(trials, 1); r++) {
+                if (!(e = batch_run(B, en.data(), true)).ok()) QB_RET(e);
+                t = std::min(t, B.last_ms);
+            }
+            if (t < best) { best = t; bk = k; bks = ks; }
+        }
+    if (!bk) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "no schedule fits the batch"));
+    B.K = bk;
+    B.KS = bks;
+    B.planned = false;
+    Status e = batch_run(B, en.data(), true);
+    if (!e.ok()) QB_RET(e);
+    if (k_out) *k_out = (uint32_t)bk;
+    if (ks_out) *ks_out = (uint32_t)bks;
+    if (best_ms) *best_ms = best;
+    QB_RET(Status{});
 }
 
 qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uint64_t *nlevels, double *last_ms,

@@ -61,7 +61,8 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
 
 struct RegPass {
     uint64_t qmask;      // qubits spanned by a tile
-    int32_t K, op0, nops, pad;
+    int32_t K, op0, nops;
+    int32_t sync_store;  // 1: amplitudes are stored to other addresses than they were loaded from
     uint8_t lay_r[8];    // layout at load: local bit of each register bit
     uint8_t lay_t[8];    //                 local bit of each thread bit
     uint8_t qr_in[8];    // qubit of each register bit at load

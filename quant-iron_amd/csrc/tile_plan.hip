@@ -300,6 +300,9 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         }
         for (int j = 0; j < RB; j++) rp.qr_out[j] = (uint8_t)P.qof[P.cr[j]];
         for (int i = 0; i < kRegThreadBits; i++) rp.qt_out[i] = (uint8_t)P.qof[P.ct[i]];
+        // SWAP relabels or transposes may send an amplitude to an address another
+        // thread loads: the kernel then waits for every load before storing
+        rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 8) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
         plan.reg_ops.insert(plan.reg_ops.end(), P.out.begin(), P.out.end());

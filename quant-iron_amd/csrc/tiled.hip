@@ -315,6 +315,14 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
         }
         dirty = true;
     }
+    // In-place permutation (SWAP relabels, a final layout that differs from the
+    // load layout): an amplitude may be stored where another wave has not
+    // loaded yet, so every wave's loads complete (s_waitcnt 0: the barrier alone
+    // does not wait for loads in flight) before any wave stores.
+    if (pass.sync_store) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+    }
     {
         // store: thread and register bits hold the qubits of the final layout
         uint64_t at = 0;

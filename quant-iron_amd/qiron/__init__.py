@@ -29,4 +29,5 @@ from .core import (  # noqa: F401
     ry_phase_matrix,
     second_order_trotter_step,
     trotter_evolve_state,
+    trotter_evolve_terms_inplace,
 )

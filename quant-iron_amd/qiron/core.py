@@ -772,6 +772,15 @@ def trotter_evolve_state(hamiltonian: SumOp, state: State, dt: float, num_steps:
     return out
 
 
+def trotter_evolve_terms_inplace(state: State, terms: np.ndarray, dt: float, num_steps: int, order: int = FIRST) -> State:
+    """trotter_evolve_state on a qi_pauli_term array, in place on `state`."""
+    arr = np.ascontiguousarray(terms)
+    err = L.QiError()
+    _check(state.lib.qi_trotter_evolve(state._h, arr.ctypes.data_as(C.c_void_p), arr.size, float(dt), int(num_steps),
+                                       int(order), C.byref(err)), err)
+    return state
+
+
 def first_order_trotter_step(hamiltonian: SumOp, state: State, dt: float) -> State:
     return trotter_evolve_state(hamiltonian, state, dt, 1, FIRST)
 
@@ -940,6 +949,24 @@ class CircuitBatch:
         err = L.QiError()
         _check(self.lib.qi_batch_run(self._h, L.dptr(out), 1 if use_graph else 0, C.byref(err)), err)
         return out[0::2][: self.count] + 1j * out[1::2][: self.count]
+
+    def set_schedule(self, k: int, ks: int) -> None:
+        """Tile width k of the tiled circuits and the largest LDS-resident circuit ks."""
+        err = L.QiError()
+        _check(self.lib.qi_batch_set_schedule(self._h, k, ks, C.byref(err)), err)
+
+    def schedule(self):
+        k, ks = C.c_uint32(), C.c_uint32()
+        err = L.QiError()
+        _check(self.lib.qi_batch_get_schedule(self._h, C.byref(k), C.byref(ks), C.byref(err)), err)
+        return k.value, ks.value
+
+    def autotune(self, trials: int = 2):
+        """Times every schedule on this batch and keeps the fastest: (k, ks, ms)."""
+        k, ks, ms = C.c_uint32(), C.c_uint32(), C.c_double()
+        err = L.QiError()
+        _check(self.lib.qi_batch_autotune(self._h, trials, C.byref(k), C.byref(ks), C.byref(ms), C.byref(err)), err)
+        return k.value, ks.value, ms.value
 
     def info(self):
         nc, ns, nl = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -74,3 +74,32 @@ def test_batch_rejects_invalid_gates(ctx):
     with pytest.raises(qiron.QuantIronError) as ei:
         b.add(3, g.array(), W.heisenberg_open_1d(3, 1, 1, 1))
     assert ei.value.variant == "OverlappingControlAndTargetQubits"
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_every_schedule_and_the_autotuned_one_match_the_oracle(ctx, prec):
+    specs = W.batched_ansatz(count=24, seed=78, nmin=9, nmax=16, lmin=1, lmax=3)
+    b = qiron.CircuitBatch(prec)
+    _, refs = build(b, specs)
+    ref = oracle_energies(refs)
+    tol = 1e-11 if prec == "c128" else 2e-5
+    kmax = 12 if prec == "c128" else 13
+    for k in range(10, kmax + 1):
+        for ks in (k, kmax):
+            b.set_schedule(k, ks)
+            e = b.run(use_graph=True)
+            assert np.max(np.abs(e - ref)) <= tol, (k, ks, np.max(np.abs(e - ref)))
+            assert b.schedule() == (k, ks)
+    k, ks, ms = b.autotune(trials=1)
+    assert 10 <= k <= ks <= kmax and ms > 0
+    assert b.schedule() == (k, ks)
+    e = b.run(use_graph=True)
+    assert np.max(np.abs(e - ref)) <= tol
+
+
+def test_schedule_out_of_range_is_rejected(ctx):
+    b = qiron.CircuitBatch("c128")
+    with pytest.raises(qiron.QuantIronError):
+        b.set_schedule(13, 13)
+    with pytest.raises(qiron.QuantIronError):
+        b.set_schedule(12, 11)
