@@ -30,33 +30,41 @@ struct TilePass {
 // ---- register-resident tile passes (single-state executor) ----
 // 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in registers.
 // A layout assigns each of the K local index bits to a thread bit (lane bits
-// 0..5, wave bits 6..7) or a register bit. 2x2 gates act on lane-held targets
-// (the partner amplitude comes from another lane of the same wave).
+// 0..5, wave bits 6..7) or a register bit. A 2x2 gate acts on a target held
+// in registers (pairs inside a thread) or on lanes (the partner comes from
+// another lane of the wave); wave bits are never targets.
 constexpr int kRegThreadBits = 8;
 constexpr int kLaneBits = 6;
+constexpr int kLaneTarget = 8;   // RegOp::tgt >= kLaneTarget: lane bit tgt - kLaneTarget
 
 enum RegOpKind : int32_t {
-    RO_X = 0, RO_H, RO_REAL, RO_GEN,   // 2x2 on a lane-held target
-    RO_Z, RO_PHASE, RO_RZ,             // diagonal on any target
-    RO_TRANSPOSE,                      // new layout through LDS
-    RO_LDS,                            // gate on the LDS copy (controlled SWAP, Matchgate)
-    RO_NOP                             // closes each pass's slice (read ahead, never run)
+    RO_X = 0,          // swap the pair
+    RO_RR,             // real 2x2 [[c0, c1], [c2, c3]] (H, RY)
+    RO_GEN,            // complex 2x2, row-major (c0,c1) (c2,c3) / (c4,c5) (c6,c7) (Y, RX, U2)
+    RO_Z,              // -1 where the target bit is 1
+    RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (S, T, P, RZ)
+    RO_TRANSPOSE,      // new layout through LDS
+    RO_LDS,            // gate on the LDS copy (controlled SWAP, Matchgate)
+    RO_NOP             // closes each pass's slice (read ahead, never run)
 };
 
 struct RegOp {            // every field dword-sized: the kernel reads ops with scalar loads
     int32_t kind;        // RegOpKind
     int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
-    int32_t lane;        // pair ops: lane bit (0..5) holding the target
-    int32_t dloc;        // diagonal ops: 0 target inside the tile (dbit = its local bit), 2 outside (dbit = qubit)
+    int32_t tgt;         // pair ops: register bit, or kLaneTarget + lane bit
+    int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = local bit),
+                         //               2 outside the tile (dbit = qubit)
     int32_t dbit;
+    int32_t p0one;       // RO_DG: p0 == 1
     int32_t lp0, lp1;    // RO_LDS: local bits of the targets
-    uint32_t cm;         // controls inside the tile (local-bit mask)
+    uint32_t rcm;        // controls on register bits (register-bit mask)
+    uint32_t tcm;        // controls on thread bits (local-bit mask)
+    uint32_t lcm;        // RO_LDS: controls inside the tile (local-bit mask)
     uint32_t src;        // source gate index
-    uint32_t pad[3];
     uint64_t gcm;        // controls outside the tile (qubit mask)
     uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
     uint8_t lay_t[8];    //               local bit of each thread bit
-    double c[8];         // coefficients (see the planner)
+    double c[8];         // coefficients
 };
 
 struct RegPass {

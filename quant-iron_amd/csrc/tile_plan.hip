@@ -18,6 +18,7 @@
 // state (plan_tiled, executed by k_reg_pass) and LDS passes for the batched
 // engine (plan_lds_passes).
 #include <algorithm>
+#include <array>
 
 #include "qi_internal.h"
 #include "exec_plan.h"
@@ -78,6 +79,10 @@ bool is_relabel(const Lowered &L) { return L.kind == LK_SWAP && L.ctrl_mask == 0
 bool is_pair(const Lowered &L) { return L.kind <= LK_U2; }
 
 // ---------------------------------------------------------- register passes --
+// Targets of 2x2 gates must sit on register bits or lane bits; the two wave
+// bits hold local bits no gate of the current group targets. Register
+// targets are the cheapest (pairs inside a thread), so the most used targets
+// and controls of a group go to registers.
 struct RegPlanner {
     int K = 0, RB = 0, L = 0;
     int lbit[64];       // qubit -> local bit (-1: outside the tile)
@@ -85,15 +90,16 @@ struct RegPlanner {
     uint8_t cr[8], ct[8];  // layout: local bit of each register / thread bit
     std::vector<RegOp> out;
 
-    uint32_t lane_set() const {
-        uint32_t m = 0;
-        for (int i = 0; i < kLaneBits; i++) m |= 1u << ct[i];
-        return m;
-    }
     uint32_t io_set() const {
         uint32_t m = 0;
         for (int k = 0; k < L; k++) m |= 1u << lbit[k];
         return m;
+    }
+    uint32_t wave_set() const { return (1u << ct[6]) | (1u << ct[7]); }
+    int reg_index(int lb) const {
+        for (int j = 0; j < RB; j++)
+            if (cr[j] == lb) return j;
+        return -1;
     }
     int lane_index(int lb) const {
         for (int i = 0; i < kLaneBits; i++)
@@ -105,47 +111,67 @@ struct RegPlanner {
             if (ct[k] != lbit[k]) return false;
         return true;
     }
-    // Lane set of kLaneBits local bits containing `need`: the row qubits'
-    // bits join when they fit (so loads and stores need no transpose), then
-    // the lowest other bits.
-    uint32_t fill_lanes(uint32_t need) const {
-        uint32_t s = need;
-        if (__builtin_popcount(s | io_set()) <= kLaneBits) s |= io_set();
-        for (int b = 0; b < K && __builtin_popcount(s) < kLaneBits; b++) s |= 1u << b;
-        return s;
-    }
-    // Layout with lane bits `lanes` (row qubits first when all are lanes),
-    // then the two wave bits and the register bits in ascending order.
-    void make(uint32_t lanes, uint8_t *r, uint8_t *t) const {
-        int i = 0, j = 0;
-        uint32_t used = 0;
-        if ((lanes & io_set()) == io_set())
-            for (int k = 0; k < L; k++) { t[i++] = (uint8_t)lbit[k]; used |= 1u << lbit[k]; }
-        for (int b = 0; b < K; b++)
-            if (((lanes >> b) & 1) && !((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
-        for (int b = 0; b < K && i < kRegThreadBits; b++)
-            if (!((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
-        for (int b = 0; b < K; b++)
-            if (!((used >> b) & 1)) r[j++] = (uint8_t)b;
+    // Layout for a group: row qubits on lanes 0..L-1; wave bits from the bits
+    // the group does not target (latest next use first); register bits by
+    // score (pair targets and controls of the group); the rest on lanes.
+    void make(uint32_t need, const int *score, const int *next_use, uint8_t *r, uint8_t *t) const {
+        const uint32_t all = (1u << K) - 1, io = io_set();
+        int i = 0;
+        for (int k = 0; k < L; k++) t[i++] = (uint8_t)lbit[k];
+        uint32_t used = io;
+        // waves
+        int w[2], nw = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            int best = -1;
+            for (int b = 0; b < K; b++) {
+                if (((used >> b) & 1) || ((need >> b) & 1)) continue;
+                if (nw && w[0] == b) continue;
+                if (best < 0 || next_use[b] > next_use[best] || (next_use[b] == next_use[best] && b > best)) best = b;
+            }
+            if (best < 0)  // the group targets every non-row bit: take any (planner bounds prevent this)
+                for (int b = K - 1; b >= 0 && best < 0; b--)
+                    if (!((used >> b) & 1) && !(nw && w[0] == b)) best = b;
+            w[nw++] = best;
+        }
+        used |= (1u << w[0]) | (1u << w[1]);
+        // registers: highest score first
+        int j = 0;
+        while (j < RB) {
+            int best = -1;
+            for (int b = 0; b < K; b++) {
+                if ((used >> b) & 1) continue;
+                if (best < 0 || score[b] > score[best]) best = b;
+            }
+            r[j++] = (uint8_t)best;
+            used |= 1u << best;
+        }
         for (; j < 8; j++) r[j] = 0;
+        // remaining lanes
+        for (int b = 0; b < K; b++)
+            if (!((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
+        t[6] = (uint8_t)w[0];
+        t[7] = (uint8_t)w[1];
+        (void)all;
     }
-    void transpose_to(uint32_t lanes) {
-        uint8_t r[8], t[8];
-        make(lanes, r, t);
+    void set_layout(const uint8_t *r, const uint8_t *t, bool emit_op) {
         if (!memcmp(r, cr, 8) && !memcmp(t, ct, 8)) return;
-        RegOp op{};
-        op.kind = RO_TRANSPOSE;
-        memcpy(op.lay_r, r, 8);
-        memcpy(op.lay_t, t, 8);
-        out.push_back(op);
+        if (emit_op) {
+            RegOp op{};
+            op.kind = RO_TRANSPOSE;
+            memcpy(op.lay_r, r, 8);
+            memcpy(op.lay_t, t, 8);
+            out.push_back(op);
+        }
         memcpy(cr, r, 8);
         memcpy(ct, t, 8);
     }
     void controls(uint64_t cm, RegOp &op) const {
         for (; cm; cm &= cm - 1) {
             const int q = __builtin_ctzll(cm);
-            if (lbit[q] < 0) op.gcm |= uint64_t(1) << q;
-            else op.cm |= 1u << lbit[q];
+            const int lb = lbit[q];
+            if (lb < 0) op.gcm |= uint64_t(1) << q;
+            else if (reg_index(lb) >= 0) op.rcm |= 1u << reg_index(lb);
+            else op.tcm |= 1u << lb;
         }
     }
     Status emit(const Lowered &G, uint32_t src) {
@@ -153,13 +179,13 @@ struct RegPlanner {
         op.src = src;
         switch (G.kind) {
         case LK_Z: case LK_PHASE: case LK_RZ: {
-            // p0 = (c0, c1) on |0>, p1 = (c2, c3) on |1>
-            if (G.kind == LK_Z) { op.kind = RO_Z; op.c[0] = 1; op.c[2] = -1; }
-            else if (G.kind == LK_PHASE) { op.kind = RO_PHASE; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; }
-            else { op.kind = RO_RZ; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
+            if (G.kind == LK_Z) op.kind = RO_Z;
+            else if (G.kind == LK_PHASE) { op.kind = RO_DG; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; op.p0one = 1; }
+            else { op.kind = RO_DG; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
             const int lb = lbit[G.t0];
             if (lb < 0) { op.dloc = 2; op.dbit = (int)G.t0; }
-            else { op.dloc = 0; op.dbit = lb; }
+            else if (reg_index(lb) >= 0) { op.dloc = 0; op.dbit = reg_index(lb); }
+            else { op.dloc = 1; op.dbit = lb; }
             controls(G.ctrl_mask, op);
         } break;
         case LK_SWAP: case LK_MATCH: {
@@ -175,16 +201,24 @@ struct RegPlanner {
             for (int k = 0; k < 8; k++) op.c[k] = G.c[k];
             op.lp0 = lbit[G.t0];
             op.lp1 = lbit[G.t1];
-            controls(G.ctrl_mask, op);
+            for (uint64_t cm = G.ctrl_mask; cm; cm &= cm - 1) {
+                const int q = __builtin_ctzll(cm);
+                if (lbit[q] >= 0) op.lcm |= 1u << lbit[q];
+                else op.gcm |= uint64_t(1) << q;
+            }
         } break;
         default: {
-            const int i = lane_index(lbit[G.t0]);
-            if (i < 0) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: target not on a lane");
-            op.lane = i;
+            const int lb = lbit[G.t0];
+            const int j = reg_index(lb), i = lane_index(lb);
+            if (j >= 0) op.tgt = j;
+            else if (i >= 0) op.tgt = kLaneTarget + i;
+            else return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: target on a wave bit");
             switch (G.kind) {
             case LK_X: op.kind = RO_X; break;
-            case LK_H: op.kind = RO_H; op.c[0] = G.c[0]; break;
-            case LK_RY: op.kind = RO_REAL; op.c[0] = G.c[0]; op.c[1] = G.c[1]; break;
+            case LK_H:  // [[h, h], [h, -h]]
+                op.kind = RO_RR; op.c[0] = G.c[0]; op.c[1] = G.c[0]; op.c[2] = G.c[0]; op.c[3] = -G.c[0]; break;
+            case LK_RY:  // [[c, -s], [s, c]]
+                op.kind = RO_RR; op.c[0] = G.c[0]; op.c[1] = -G.c[1]; op.c[2] = G.c[1]; op.c[3] = G.c[0]; break;
             case LK_Y:  // [[0, (c0,c1)], [(c2,c3), 0]]
                 op.kind = RO_GEN;
                 op.c[2] = G.c[0]; op.c[3] = G.c[1]; op.c[4] = G.c[2]; op.c[5] = G.c[3];
@@ -209,7 +243,7 @@ struct RegPlanner {
 
 struct Phase {
     std::vector<size_t> ops;  // indices into the pass members, in execution order
-    uint32_t need = 0;        // local bits that must be lane bits
+    uint32_t need = 0;        // local bits the group's 2x2 gates target
 };
 
 }  // namespace
@@ -222,7 +256,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
         return Status{};
     }
-    const int low = tile_low_qubits(s.prec), RB = K - kRegThreadBits, LANES = kLaneBits;
+    const int low = tile_low_qubits(s.prec), RB = K - kRegThreadBits;
     std::vector<GateSets> gs(gates.size());
     for (size_t g = 0; g < gates.size(); g++) gs[g] = gate_sets(gates[g]);
     std::vector<char> done(gates.size(), 0);
@@ -238,8 +272,11 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         rp.K = K;
         for (int b = 0, j = 0; b < (int)s.n; b++)
             if ((q >> b) & 1) { P.lbit[b] = j; P.qof[j] = b; j++; }
-        // ---- phases: ops whose targets fit the lane bits, commuting-greedy
+        // ---- phases: ops whose targets fit the register and lane bits (all but
+        // the two wave bits), commuting-greedy; relabels tracked on the way
+        const int CAP = K - 2;
         std::vector<Phase> phases;
+        std::vector<std::vector<int>> phase_lb;  // labels (qubit -> local bit) at each phase start
         {
             int lb[64];
             memcpy(lb, P.lbit, sizeof lb);
@@ -247,14 +284,19 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
             size_t left = members.size();
             while (left) {
                 Phase ph;
+                phase_lb.emplace_back(lb, lb + 64);
                 Blocked blk;
+                uint32_t io_u = 0;  // row-qubit bits over the phase (relabels may move them)
                 for (size_t i = 0; i < members.size(); i++) {
                     if (pd[i]) continue;
                     const Lowered &G = gates[members[i]];
                     const GateSets &g = gs[members[i]];
                     if (blk.blocks(g)) { blk.add(g); continue; }
                     const uint32_t need = is_pair(G) ? 1u << lb[G.t0] : 0u;
-                    if (__builtin_popcount(ph.need | need) > LANES) { blk.add(g); continue; }
+                    uint32_t io = io_u;  // the row qubits stay on lanes: they count against the capacity
+                    for (int k = 0; k < low; k++) io |= 1u << lb[k];
+                    if (__builtin_popcount(ph.need | need | io) > CAP) { blk.add(g); continue; }
+                    io_u = io;
                     ph.need |= need;
                     ph.ops.push_back(i);
                     pd[i] = 1;
@@ -264,38 +306,62 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
                 phases.push_back(std::move(ph));
             }
         }
-        // ---- layout groups: consecutive phases whose targets fit one lane set
-        struct Group { size_t p0, p1; uint32_t need; };
-        std::vector<Group> groups;
-        for (size_t i = 0; i < phases.size();) {
-            uint32_t need = phases[i].need;
-            size_t j = i + 1;
-            while (j < phases.size() && __builtin_popcount(need | phases[j].need) <= LANES) need |= phases[j++].need;
-            groups.push_back({i, j, need});
-            i = j;
+        // ---- per phase: scores for register placement (targets count most, then controls)
+        std::vector<std::array<int, 16>> score(phases.size());
+        for (size_t p = 0; p < phases.size(); p++) {
+            score[p].fill(0);
+            int lb[64];
+            std::copy(phase_lb[p].begin(), phase_lb[p].end(), lb);
+            for (size_t i : phases[p].ops) {
+                const Lowered &G = gates[members[i]];
+                if (is_pair(G)) score[p][lb[G.t0]] += 4;
+                for (uint64_t cm = G.ctrl_mask; cm; cm &= cm - 1) {
+                    const int q = __builtin_ctzll(cm);
+                    if (lb[q] >= 0) score[p][lb[q]] += 1;
+                }
+                if (G.kind == LK_Z || G.kind == LK_PHASE || G.kind == LK_RZ)
+                    if (lb[G.t0] >= 0) score[p][lb[G.t0]] += 1;
+                if (is_relabel(G)) std::swap(lb[G.t0], lb[G.t1]);
+            }
         }
-        // ---- load layout: the first group's, when the row qubits fit beside its targets
+        // next use of each local bit as a target, from each phase on
+        auto next_use = [&](size_t from, int *nu) {
+            for (int b = 0; b < 16; b++) nu[b] = 1 << 30;
+            for (size_t p = phases.size(); p-- > from;)
+                for (int b = 0; b < K; b++)
+                    if ((phases[p].need >> b) & 1) nu[b] = (int)p;
+        };
+        // ---- layouts: a transpose only when a target sits on a wave bit
+        int nu[16];
+        next_use(0, nu);
         {
-            const uint32_t first_need = groups.empty() ? 0u : groups[0].need;
-            const uint32_t lanes = __builtin_popcount(first_need | P.io_set()) <= LANES ? P.fill_lanes(first_need)
-                                                                                      : P.fill_lanes(0);
-            P.make(lanes, P.cr, P.ct);
+            uint8_t r[8], t[8];
+            P.make(phases.empty() ? 0u : phases[0].need, phases.empty() ? nu : score[0].data(), nu, r, t);
+            P.set_layout(r, t, false);
             memcpy(rp.lay_r, P.cr, 8);
             memcpy(rp.lay_t, P.ct, 8);
             for (int j = 0; j < RB; j++) rp.qr_in[j] = (uint8_t)P.qof[P.cr[j]];
             for (int i = 0; i < kRegThreadBits; i++) rp.qt_in[i] = (uint8_t)P.qof[P.ct[i]];
         }
-        for (const Group &gr : groups) {
-            if ((gr.need & P.lane_set()) != gr.need) P.transpose_to(P.fill_lanes(gr.need));
-            for (size_t p = gr.p0; p < gr.p1; p++)
-                for (size_t i : phases[p].ops) {
-                    Status e = P.emit(gates[members[i]], (uint32_t)members[i]);
-                    if (!e.ok()) return e;
-                }
+        for (size_t p = 0; p < phases.size(); p++) {
+            if (phases[p].need & P.wave_set()) {
+                next_use(p, nu);
+                uint8_t r[8], t[8];
+                P.make(phases[p].need, score[p].data(), nu, r, t);
+                P.set_layout(r, t, true);
+            }
+            for (size_t i : phases[p].ops) {
+                Status e = P.emit(gates[members[i]], (uint32_t)members[i]);
+                if (!e.ok()) return e;
+            }
         }
-        // ---- store: row qubits back on lanes 0..L-1
+        // ---- store: row qubits back on lanes 0..L-1 (relabels may have moved them)
         if (!P.io_ok()) {
-            P.transpose_to(P.fill_lanes(0));
+            int zero[16] = {0}, far[16];
+            for (int b = 0; b < 16; b++) far[b] = 1 << 30;
+            uint8_t r[8], t[8];
+            P.make(0, zero, far, r, t);
+            P.set_layout(r, t, true);
             if (!P.io_ok()) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: store layout");
         }
         for (int j = 0; j < RB; j++) rp.qr_out[j] = (uint8_t)P.qof[P.cr[j]];

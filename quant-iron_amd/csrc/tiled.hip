@@ -3,16 +3,16 @@
 //
 // 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in
 // registers. A layout assigns the K local index bits to 8 thread bits (6 lane
-// bits, 2 wave bits) and K-8 register bits. A 2x2 gate acts on a lane-held
-// target: every lane fetches its partner's amplitudes from lane ^ 2^i with
-// ds_bpermute (the LDS crossbar, no LDS memory, no barrier) and computes its
-// own half of the pair, so the code of an op is the same for every target
-// and every register. Diagonal gates and controls test the local index
-// (thread part | register part) of each amplitude. When the next gates need
-// other target bits on lanes, the tile is transposed to a new layout through
-// LDS. Controlled SWAPs and Matchgates run on the LDS copy of the tile.
+// bits, 2 wave bits) and K-8 register bits. A 2x2 gate acts on a target held
+// in registers (the pair lives in one thread) or on lanes (each lane fetches
+// its partner with ds_bpermute, the LDS crossbar without LDS memory, and
+// computes its half of the pair). Controls and diagonal targets on register
+// bits are compile-time facts of each register, so untouched registers cost
+// nothing; on thread bits they are a per-thread select. When a target sits
+// on a wave bit, the tile is transposed to a new layout through LDS.
+// Controlled SWAPs and Matchgates run on the LDS copy of the tile.
 //
-// Arithmetic: complex128 follows the reference's operand order (no
+// Arithmetic: complex128 keeps each product of the reference's formulas (no
 // contraction); complex64 uses fused multiply-adds.
 #include "qi_internal.h"
 #include "exec_plan.h"
@@ -86,73 +86,139 @@ __device__ __forceinline__ double xlane(int addr, double x) {
 }
 template <typename T> __device__ __forceinline__ T xlane2(int addr, T x) { return mk<T>(xlane(addr, x.x), xlane(addr, x.y)); }
 
-// ------------------------------------------------------------- 2x2 ops --
-// Lane half `bit` of the pair (a, b) on its target: bit 0 holds a and
-// computes a', bit 1 holds b and computes b'; s = own, p = partner.
-//   X:    a' = b,               b' = a
-//   H:    a' = h (a + b),       b' = h (a - b)          -> h (sgn s + p)
-//   REAL: a' = c a - s b,       b' = s a + c b          -> c s + (+-s) p
-//   GEN:  a' = m00 a + m01 b,   b' = m10 a + m11 b      -> cs s + cp p
-// The complex128 forms keep the reference's products and only reorder
-// commutative sums, which IEEE arithmetic evaluates exactly alike.
-template <int KIND, typename T, typename R>
-__device__ __forceinline__ T pair_lane(T s, T p, const R *k) {
-    if constexpr (KIND == RO_X) {
-        return p;
-    } else if constexpr (KIND == RO_H) {  // k0 = sgn, k1 = h
-        if constexpr (sizeof(R) == 4) return mk<T>(k[1] * __builtin_fmaf(k[0], s.x, p.x), k[1] * __builtin_fmaf(k[0], s.y, p.y));
-        else return mk<T>(k[1] * (k[0] * s.x + p.x), k[1] * (k[0] * s.y + p.y));
-    } else if constexpr (KIND == RO_REAL) {  // k0 = c, k1 = +-s
-        if constexpr (sizeof(R) == 4)
-            return mk<T>(__builtin_fmaf(k[1], p.x, k[0] * s.x), __builtin_fmaf(k[1], p.y, k[0] * s.y));
-        else return mk<T>(k[0] * s.x + k[1] * p.x, k[0] * s.y + k[1] * p.y);
-    } else {  // k0,k1 = cs, k2,k3 = cp
-        if constexpr (sizeof(R) == 4) {
-            float x = __builtin_fmaf(-k[1], s.y, k[0] * s.x);
-            x = __builtin_fmaf(k[2], p.x, x);
-            x = __builtin_fmaf(-k[3], p.y, x);
-            float y = __builtin_fmaf(k[1], s.x, k[0] * s.y);
-            y = __builtin_fmaf(k[2], p.y, y);
-            y = __builtin_fmaf(k[3], p.x, y);
-            return mk<T>(x, y);
-        } else {
-            return mk<T>((k[0] * s.x - k[1] * s.y) + (k[2] * p.x - k[3] * p.y),
-                         (k[0] * s.y + k[1] * s.x) + (k[2] * p.y + k[3] * p.x));
-        }
+// y = m x for real m
+template <typename T, typename R> __device__ __forceinline__ T rmul_add(R m0, T a, R m1, T b) {
+    if constexpr (sizeof(R) == 4) return mk<T>(__builtin_fmaf(m1, b.x, m0 * a.x), __builtin_fmaf(m1, b.y, m0 * a.y));
+    else return mk<T>(m0 * a.x + m1 * b.x, m0 * a.y + m1 * b.y);
+}
+// m0 a + m1 b for complex m0 = (k0, k1), m1 = (k2, k3)
+template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R *k, T a, T b) {
+    if constexpr (sizeof(R) == 4) {
+        float x = __builtin_fmaf(-k[1], a.y, k[0] * a.x);
+        x = __builtin_fmaf(k[2], b.x, x);
+        x = __builtin_fmaf(-k[3], b.y, x);
+        float y = __builtin_fmaf(k[1], a.x, k[0] * a.y);
+        y = __builtin_fmaf(k[2], b.y, y);
+        y = __builtin_fmaf(k[3], b.x, y);
+        return mk<T>(x, y);
+    } else {
+        return mk<T>((k[0] * a.x - k[1] * a.y) + (k[2] * b.x - k[3] * b.y),
+                     (k[0] * a.y + k[1] * a.x) + (k[2] * b.y + k[3] * b.x));
     }
 }
 
-template <int KIND, bool CTRL, int E, typename T, typename R>
-__device__ __forceinline__ void pairs(Regs<R, E> &v, int addr, const R *k, uint32_t tl, const uint32_t *rl, uint32_t cm) {
+// ------------------------------------------------------------- 2x2 ops --
+// Pair (a, b) = registers (r, r | 2^B) of one thread. Registers whose
+// register-held controls are not all 1 are skipped (uniform, compile-time r);
+// thread-held controls select per lane (TC).
+template <int B, int KIND, bool TC, int E, typename T, typename R>
+__device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rcm, bool tok) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
+        if (r & (1 << B)) continue;
+        if ((uint32_t(r) & rcm) != rcm) continue;
+        const T a = v.get(r), b = v.get(r | (1 << B));
+        T na, nb;
+        if constexpr (KIND == RO_X) { na = b; nb = a; }
+        else if constexpr (KIND == RO_RR) { na = rmul_add<T, R>(k[0], a, k[1], b); nb = rmul_add<T, R>(k[2], a, k[3], b); }
+        else { na = cmul_add<T, R>(k, a, b); nb = cmul_add<T, R>(k + 4, a, b); }
+        if constexpr (TC) { na = sel(tok, na, a); nb = sel(tok, nb, b); }
+        v.set(r, na);
+        v.set(r | (1 << B), nb);
+    }
+}
+
+// Lane pairs: s = own amplitude, p = partner's; k holds this lane's row of
+// the matrix (own coefficient first).
+template <int KIND, bool TC, int E, typename T, typename R>
+__device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, uint32_t rcm, bool tok) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if ((uint32_t(r) & rcm) != rcm) continue;
         const T s = v.get(r);
         const T p = xlane2<T>(addr, s);
-        T n = pair_lane<KIND, T, R>(s, p, k);
-        if constexpr (CTRL) n = sel(((tl | rl[r]) & cm) == cm, n, s);
+        T n;
+        if constexpr (KIND == RO_X) n = p;
+        else if constexpr (KIND == RO_RR) n = rmul_add<T, R>(k[0], s, k[1], p);
+        else n = cmul_add<T, R>(k, s, p);
+        if constexpr (TC) n = sel(tok, n, s);
         v.set(r, n);
     }
+}
+
+template <int KIND, bool TC, int RB, int E, typename T, typename R>
+__device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t lane, uint32_t rcm, bool tok) {
+    if (op.tgt >= kLaneTarget) {
+        const int i = op.tgt - kLaneTarget;
+        const int addr = int((lane ^ (1u << i)) << 2);
+        const bool bit = ((lane >> i) & 1u) != 0;
+        R k[4];
+        if constexpr (KIND == RO_RR) {  // row 0: (m00, m01); row 1: (m11, m10)
+            k[0] = R(bit ? op.c[3] : op.c[0]);
+            k[1] = R(bit ? op.c[2] : op.c[1]);
+        } else if constexpr (KIND == RO_GEN) {
+            k[0] = R(bit ? op.c[6] : op.c[0]);
+            k[1] = R(bit ? op.c[7] : op.c[1]);
+            k[2] = R(bit ? op.c[4] : op.c[2]);
+            k[3] = R(bit ? op.c[5] : op.c[3]);
+        }
+        pairs_lane<KIND, TC, E, T, R>(v, addr, k, rcm, tok);
+        return;
+    }
+    R k[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) k[q] = R(op.c[q]);
+#define QI_RB(BB) case BB: if constexpr (BB < RB) pairs_reg<BB, KIND, TC, E, T, R>(v, k, rcm, tok); break;
+    switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
 }
 
 // ---------------------------------------------------------- diagonal ops --
-template <int KIND, typename T> __device__ __forceinline__ T dphase(T a, T p0, T p1, bool one) {
-    if constexpr (KIND == RO_Z) return sel(one, mk<T>(-a.x, -a.y), a);
-    else if constexpr (KIND == RO_PHASE) return sel(one, cmul(a, p1), a);
-    else return cmul(a, sel(one, p1, p0));
-}
-
-// one(r) = target bit of amplitude r: thread part `tb` | register part of rl[r]
-template <int KIND, bool CTRL, int E, typename T, typename R>
-__device__ __forceinline__ void diag(Regs<R, E> &v, T p0, T p1, uint32_t tb, uint32_t dmask, uint32_t tl, const uint32_t *rl,
-                                     uint32_t cm) {
+// Target on register bit B: each register's target value is known.
+template <int B, int KIND, bool TC, int E, typename T, typename R>
+__device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, bool p0one, uint32_t rcm, bool tok) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        const bool one = (tb | (rl[r] & dmask)) != 0;
+        if ((uint32_t(r) & rcm) != rcm) continue;
+        const bool one = ((r >> B) & 1) != 0;
+        if (!one && (KIND == RO_Z || p0one)) continue;
         const T a = v.get(r);
-        T n = dphase<KIND, T>(a, p0, p1, one);
-        if constexpr (CTRL) n = sel(((tl | rl[r]) & cm) == cm, n, a);
+        T n;
+        if constexpr (KIND == RO_Z) n = mk<T>(-a.x, -a.y);
+        else n = cmul(a, one ? p1 : p0);
+        if constexpr (TC) n = sel(tok, n, a);
         v.set(r, n);
     }
+}
+// Target fixed for the whole thread (thread bit or outside the tile): f is
+// this thread's factor.
+template <int KIND, bool TC, int E, typename T, typename R>
+__device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t rcm, bool tok) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if ((uint32_t(r) & rcm) != rcm) continue;
+        const T a = v.get(r);
+        T n;
+        if constexpr (KIND == RO_Z) n = sel(one && tok, mk<T>(-a.x, -a.y), a);
+        else {
+            n = cmul(a, f);
+            if constexpr (TC) n = sel(tok, n, a);
+        }
+        v.set(r, n);
+    }
+}
+
+template <int KIND, bool TC, int RB, int E, typename T, typename R>
+__device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t tl, uint64_t base, uint32_t rcm, bool tok) {
+    const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
+    if (op.dloc == 0) {
+#define QI_RB(BB) case BB: if constexpr (BB < RB) diag_reg<BB, KIND, TC, E, T, R>(v, p0, p1, op.p0one != 0, rcm, tok); break;
+        switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+        return;
+    }
+    const bool one = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+    diag_thr<KIND, TC, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
 // ------------------------------------------------------------- LDS ops --
@@ -162,7 +228,7 @@ template <typename R, int K, int F>
 __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &op) {
     using T = typename V2<R>::T;
     constexpr int N = 1 << K;
-    const uint32_t lcm = op.cm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
+    const uint32_t lcm = op.lcm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
     const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
     const R c0 = R(op.c[0]), c1 = R(op.c[1]);
     const T m23 = mk<T>(R(op.c[2]), R(op.c[3])), m45 = mk<T>(R(op.c[4]), R(op.c[5])), m67 = mk<T>(R(op.c[6]), R(op.c[7]));
@@ -236,43 +302,23 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
             if (kind >= RO_TRANSPOSE) break;
             nxt = ops[pass.op0 + o + 1];
             if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-            const bool ctrl = op.cm != 0;             // controls inside the tile (uniform)
-            if (kind <= RO_GEN) {
-                const int addr = int((lane ^ (1u << op.lane)) << 2);
-                const bool bit = ((lane >> op.lane) & 1u) != 0;
-                R k[4];
-                if (kind == RO_H) {
-                    k[0] = bit ? R(-1) : R(1);
-                    k[1] = R(op.c[0]);
-                } else if (kind == RO_REAL) {
-                    k[0] = R(op.c[0]);
-                    k[1] = bit ? R(op.c[1]) : -R(op.c[1]);
-                } else {
-                    k[0] = R(bit ? op.c[6] : op.c[0]);
-                    k[1] = R(bit ? op.c[7] : op.c[1]);
-                    k[2] = R(bit ? op.c[4] : op.c[2]);
-                    k[3] = R(bit ? op.c[5] : op.c[3]);
-                }
-#define QI_PAIR(KD)                                                          \
-    case KD:                                                                 \
-        if (ctrl) pairs<KD, true, E, T, R>(v, addr, k, tl, rl, op.cm);       \
-        else pairs<KD, false, E, T, R>(v, addr, k, tl, rl, op.cm);           \
+            const bool tc = op.tcm != 0;              // thread-held controls (uniform)
+            const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
+#define QI_OP(KD, FN)                                                                     \
+    case KD:                                                                              \
+        if (tc) FN<KD, true, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                      \
+        else FN<KD, false, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                        \
         break;
-                switch (kind) { QI_PAIR(RO_X) QI_PAIR(RO_H) QI_PAIR(RO_REAL) QI_PAIR(RO_GEN) default: break; }
-#undef QI_PAIR
-            } else {
-                const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
-                uint32_t tb, dmask;
-                if (op.dloc == 0) { dmask = 1u << op.dbit; tb = tl & dmask; }
-                else { dmask = 0; tb = uint32_t((base >> op.dbit) & 1u); }
-#define QI_DIAG(KD)                                                                  \
-    case KD:                                                                         \
-        if (ctrl) diag<KD, true, E, T, R>(v, p0, p1, tb, dmask, tl, rl, op.cm);      \
-        else diag<KD, false, E, T, R>(v, p0, p1, tb, dmask, tl, rl, op.cm);          \
-        break;
-                switch (kind) { QI_DIAG(RO_Z) QI_DIAG(RO_PHASE) QI_DIAG(RO_RZ) default: break; }
-#undef QI_DIAG
+#define ARGS lane
+            switch (kind) {
+                QI_OP(RO_X, pair_op) QI_OP(RO_RR, pair_op) QI_OP(RO_GEN, pair_op)
+#undef ARGS
+#define ARGS tl, base
+                QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op)
+#undef ARGS
+            default: break;
             }
+#undef QI_OP
         }
         if (o >= pass.nops) break;
         // through LDS: a transpose to a new layout, or a gate on the LDS copy
