@@ -314,6 +314,9 @@ def main():
     log(f"circuit: {G} gates on {n} qubits")
 
     tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)
+    # the library's complex64 defaults: 4096-amplitude tiles, 256 threads (512 from 8192 amplitudes)
+    tile_k = args.tile_qubits or 12
+    tile_tb = 9 if tile_k >= 13 else 8
     results = {}
     # ---- headline: tiled executor, complex64
     wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c64", gates, tiled, args.steps, args.warmup)
@@ -328,7 +331,7 @@ def main():
                                 pass_GBps=achieved, norm=norm)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_reg_pass<float,13>", "algorithmic_bytes_per_launch": 2 * S64}
+                "kernel": f"k_reg_pass<float, {tile_k}, {tile_tb}>", "algorithmic_bytes_per_launch": 2 * S64}
 
     if not args.only:
         # ---- single 30-qubit gate application (streaming kernel): the HBM-roofline target

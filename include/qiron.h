@@ -161,9 +161,10 @@ typedef enum qi_exec_mode {
 
 typedef struct qi_exec_opts {
     int32_t mode;        /* qi_exec_mode                                    */
-    int32_t tile_qubits; /* tile width k for QI_EXEC_TILED: 10..13 (c64), 10..12 (c128); 0 = 13 / 12 */
+    int32_t tile_qubits; /* tile width k for QI_EXEC_TILED: 10..13 (c64), 10..12 (c128); 0 = 12 (c64) / 11 (c128) */
     int32_t block_qubits;/* dense block width k (<= 6) for QI_EXEC_FUSED_MFMA */
     int32_t use_graph;   /* capture the launch sequence in a hipGraph         */
+    int32_t tile_threads_log2; /* QI_EXEC_TILED: 8 (256 threads) or 9 (512) per tile; 0 = default */
 } qi_exec_opts;
 
 /* ---------------- context ---------------- */

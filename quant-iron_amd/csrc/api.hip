@@ -367,7 +367,7 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
             j++;
         }
         Status e;
-        if (mode == QI_EXEC_TILED) e = plan_tiled(s, seg, o.tile_qubits, plan);
+        if (mode == QI_EXEC_TILED) e = plan_tiled(s, seg, o.tile_qubits, o.tile_threads_log2, plan);
         else if (mode == QI_EXEC_FUSED_MFMA) e = plan_fused(s, seg, o.block_qubits, plan);
         else for (const Lowered &L : seg) plan.steps.push_back(ExecStep::gate(L));
         if (!e.ok()) return e;

@@ -28,12 +28,13 @@ struct TilePass {
 };
 
 // ---- register-resident tile passes (single-state executor) ----
-// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in registers.
-// A layout assigns each of the K local index bits to a thread bit (lane bits
-// 0..5, wave bits 6..7) or a register bit. A 2x2 gate acts on a target held
-// in registers (pairs inside a thread) or on lanes (the partner comes from
-// another lane of the wave); wave bits are never targets.
-constexpr int kRegThreadBits = 8;
+// 2^TB threads (TB = 8 or 9) hold a tile of 2^K amplitudes, 2^(K-TB) per
+// thread in registers. A layout assigns each of the K local index bits to a
+// thread bit (lane bits 0..5, wave bits 6..TB-1) or a register bit. A 2x2
+// gate acts on a target held in registers (pairs inside a thread) or on lanes
+// (the partner comes from another lane of the wave); wave bits are never
+// targets.
+constexpr int kMaxThreadBits = 10;
 constexpr int kLaneBits = 6;
 constexpr int kLaneTarget = 8;   // RegOp::tgt >= kLaneTarget: lane bit tgt - kLaneTarget
 
@@ -42,7 +43,8 @@ enum RegOpKind : int32_t {
     RO_RR,             // real 2x2 [[c0, c1], [c2, c3]] (H, RY)
     RO_GEN,            // complex 2x2, row-major (c0,c1) (c2,c3) / (c4,c5) (c6,c7) (Y, RX, U2)
     RO_Z,              // -1 where the target bit is 1
-    RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (S, T, P, RZ)
+    RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (RZ)
+    RO_P1,             // p1 = (c2, c3) where the target bit is 1, nothing where it is 0 (S, T, P)
     RO_TRANSPOSE,      // new layout through LDS
     RO_LDS,            // gate on the LDS copy (controlled SWAP, Matchgate)
     RO_NOP             // closes each pass's slice (read ahead, never run)
@@ -55,7 +57,7 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = local bit),
                          //               2 outside the tile (dbit = qubit)
     int32_t dbit;
-    int32_t p0one;       // RO_DG: p0 == 1
+    int32_t pad0;
     int32_t lp0, lp1;    // RO_LDS: local bits of the targets
     uint32_t rcm;        // controls on register bits (register-bit mask)
     uint32_t tcm;        // controls on thread bits (local-bit mask)
@@ -63,7 +65,8 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     uint32_t src;        // source gate index
     uint64_t gcm;        // controls outside the tile (qubit mask)
     uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
-    uint8_t lay_t[8];    //               local bit of each thread bit
+    uint8_t lay_t[12];   //               local bit of each thread bit
+    uint32_t pad1;
     double c[8];         // coefficients
 };
 
@@ -71,12 +74,14 @@ struct RegPass {
     uint64_t qmask;      // qubits spanned by a tile
     int32_t K, op0, nops;
     int32_t sync_store;  // 1: amplitudes are stored to other addresses than they were loaded from
+    int32_t tb;          // thread bits (2^tb threads per tile)
+    int32_t pad;
     uint8_t lay_r[8];    // layout at load: local bit of each register bit
-    uint8_t lay_t[8];    //                 local bit of each thread bit
+    uint8_t lay_t[12];   //                 local bit of each thread bit
     uint8_t qr_in[8];    // qubit of each register bit at load
-    uint8_t qt_in[8];    // qubit of each thread bit at load
+    uint8_t qt_in[12];   // qubit of each thread bit at load
     uint8_t qr_out[8];   // at store (SWAPs relabel qubits, the final layout differs)
-    uint8_t qt_out[8];
+    uint8_t qt_out[12];
 };
 
 // A dense 2^k x 2^k block applied to tiles of 2^(k + kc) amplitudes.
@@ -115,9 +120,10 @@ struct ExecPlan {
 };
 
 int default_tile_qubits(const State &s);
+int default_tile_threads_log2(int prec, int K);
 int tile_low_qubits(int prec);
 // Register-resident passes for one state (ExecStep::REG, plan.reg_ops).
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan);
+Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan);
 Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st);
 // LDS passes over the same pass selection (ExecStep::TILE, plan.tile_ops),
 // used by the batched engine.

@@ -4,9 +4,12 @@
 // Each gate kernel enumerates exactly the amplitudes the gate changes: the
 // index space with the target and control bits removed is walked linearly and
 // the fixed bits are re-inserted (controls = 1, target = 0 or 1), so a CNOT
-// reads and writes half the state and a Toffoli a quarter, as the minimal-touch
-// byte count of each gate requires. Consecutive threads own consecutive free
-// indices, so every wave's loads are contiguous runs.
+// reads and writes half the state and a Toffoli a quarter. Memory moves in
+// 16-byte units (two complex64 amplitudes, or one complex128) with
+// non-temporal loads and stores; consecutive threads own consecutive units, so
+// every wave's loads are contiguous runs. A complex64 gate with a control on
+// qubit 0 rewrites the untouched even amplitudes of its units as read: HBM
+// moves whole 64-byte lines, so this costs no extra traffic.
 //
 // Arithmetic: the per-element formulas keep the reference's operand order
 // (operator.rs, e.g. Hadamard :396-412, RotateX :1756-1757, RotateY :1897-1898,
@@ -62,13 +65,131 @@ template <typename R> struct GateK {
     R c[10];
 };
 
-// Pair kernels: H, X, Y, RX, RY, U2 and RZ (two phases).
-template <typename R, int KIND>
-__global__ __launch_bounds__(kThreads) void k_pair(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
-    using T = typename V2<R>::T;
+// Per-pair and per-element updates shared by every layout below.
+template <typename R, int KIND, typename T>
+__device__ __forceinline__ void pair_update(const GateK<R> &g, T a, T b, T &na, T &nb) {
+    if constexpr (KIND == LK_H) {
+        R s = g.c[0];
+        na = cscale<T>(s, cadd(a, b));
+        nb = cscale<T>(s, csub(a, b));
+    } else if constexpr (KIND == LK_X) {
+        na = b; nb = a;
+    } else if constexpr (KIND == LK_Y) {
+        na = cmul(mk<T>(g.c[0], g.c[1]), b);   // (-i) * b
+        nb = cmul(mk<T>(g.c[2], g.c[3]), a);   // ( i) * a
+    } else if constexpr (KIND == LK_RX) {
+        na = csub(cscale<T>(g.c[0], a), cmul(mk<T>(g.c[1], g.c[2]), b));
+        nb = cadd(cmul(mk<T>(g.c[3], g.c[4]), a), cscale<T>(g.c[0], b));
+    } else if constexpr (KIND == LK_RY) {
+        na = csub(cscale<T>(g.c[0], a), cscale<T>(g.c[1], b));
+        nb = cadd(cscale<T>(g.c[1], a), cscale<T>(g.c[0], b));
+    } else if constexpr (KIND == LK_U2) {
+        na = cadd(cmul(mk<T>(g.c[0], g.c[1]), a), cmul(mk<T>(g.c[2], g.c[3]), b));
+        nb = cadd(cmul(mk<T>(g.c[4], g.c[5]), a), cmul(mk<T>(g.c[6], g.c[7]), b));
+    } else {  // LK_RZ
+        na = cmul(a, mk<T>(g.c[0], g.c[1]));
+        nb = cmul(b, mk<T>(g.c[2], g.c[3]));
+    }
+}
+
+template <typename R, int KIND, typename T>
+__device__ __forceinline__ T diag_update(const GateK<R> &g, T a) {
+    if constexpr (KIND == LK_Z) { T r; r.x = -a.x; r.y = -a.y; return r; }
+    else return cmul(a, mk<T>(g.c[0], g.c[1]));
+}
+
+// 16-byte units moved with non-temporal loads and stores: the state is
+// streamed once per gate and never re-read from cache, so the accesses skip
+// cache retention.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <typename V> __device__ __forceinline__ V ldnt(const V *p) { return __builtin_nontemporal_load(p); }
+template <typename V> __device__ __forceinline__ void stnt(V v, V *p) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ float2 lo(f32x4 v) { return make_float2(v.x, v.y); }
+__device__ __forceinline__ float2 hi(f32x4 v) { return make_float2(v.z, v.w); }
+__device__ __forceinline__ f32x4 pack(float2 a, float2 b) { f32x4 r; r.x = a.x; r.y = a.y; r.z = b.x; r.w = b.y; return r; }
+__device__ __forceinline__ double2 amp(f64x2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ f64x2 unamp(double2 a) { f64x2 r; r.x = a.x; r.y = a.y; return r; }
+
+// complex64 pair gates on units of two neighbouring amplitudes (index bit 0
+// inside the unit; `g` is expressed in unit indices).
+//   MODE 0: bit 0 is free: both halves pair with the partner unit's halves.
+//   MODE 1: bit 0 is a control: only the odd halves change; the even halves
+//           are written back as read (each unit has exactly one owner).
+//   MODE 2: bit 0 is the target: every unit holds one pair; two units per
+//           thread, a block apart, so each load instruction stays contiguous.
+template <int KIND, int MODE>
+__global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, GateK<float> g) {
+    if constexpr (MODE == 2) {
+        const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+        const bool h0 = k0 < g.items, h1 = k1 < g.items;
+        uint64_t j0 = 0, j1 = 0;
+        f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+        if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
+        if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+        float2 na, nb;
+        if (h0) { pair_update<float, KIND>(g, lo(u0), hi(u0), na, nb); stnt(pack(na, nb), psi + j0); }
+        if (h1) { pair_update<float, KIND>(g, lo(u1), hi(u1), na, nb); stnt(pack(na, nb), psi + j1); }
+    } else {
+        const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+        if (k >= g.items) return;
+        const uint64_t ja = insert_zeros(k, g.fixed) | g.set, jb = ja ^ g.pmask;
+        const f32x4 x = ldnt(psi + ja), y = ldnt(psi + jb);
+        float2 a0 = lo(x), b0 = lo(y), a1, b1;
+        pair_update<float, KIND>(g, hi(x), hi(y), a1, b1);
+        if constexpr (MODE == 0) pair_update<float, KIND>(g, lo(x), lo(y), a0, b0);
+        stnt(pack(a0, a1), psi + ja);
+        stnt(pack(b0, b1), psi + jb);
+    }
+}
+
+// complex64 diagonal gates (Z, phases) on two-amplitude units. MODE 0: bit 0
+// free (both halves change); MODE 1: bit 0 is set by the gate (a control or
+// the target), so only the odd halves change.
+template <int KIND, int MODE>
+__global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, GateK<float> g) {
+    const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const bool h0 = k0 < g.items, h1 = k1 < g.items;
+    uint64_t j0 = 0, j1 = 0;
+    f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
+    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+    if (h0) stnt(pack(MODE == 0 ? diag_update<float, KIND>(g, lo(u0)) : lo(u0), diag_update<float, KIND>(g, hi(u0))), psi + j0);
+    if (h1) stnt(pack(MODE == 0 ? diag_update<float, KIND>(g, lo(u1)) : lo(u1), diag_update<float, KIND>(g, hi(u1))), psi + j1);
+}
+
+// complex128: one amplitude is already a 16-byte unit; one pair per thread.
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi, GateK<double> g) {
+    const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (k >= g.items) return;
+    const uint64_t ia = insert_zeros(k, g.fixed) | g.set, ib = ia ^ g.pmask;
+    const double2 a = amp(ldnt(psi + ia)), b = amp(ldnt(psi + ib));
+    double2 na, nb;
+    pair_update<double, KIND>(g, a, b, na, nb);
+    stnt(unamp(na), psi + ia);
+    stnt(unamp(nb), psi + ib);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_diag_c128(f64x2 *__restrict__ psi, GateK<double> g) {
+    const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const bool h0 = k0 < g.items, h1 = k1 < g.items;
+    uint64_t j0 = 0, j1 = 0;
+    f64x2 u0 = {0, 0}, u1 = {0, 0};
+    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
+    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+    if (h0) stnt(unamp(diag_update<double, KIND>(g, amp(u0))), psi + j0);
+    if (h1) stnt(unamp(diag_update<double, KIND>(g, amp(u1))), psi + j1);
+}
+
+// complex64 SWAP with qubit 0 as one of its targets: the exchanged
+// amplitudes sit in different halves of their units, so this one walks
+// single amplitudes (8-byte accesses, several pairs per thread in flight).
+__global__ __launch_bounds__(kThreads) void k_swap_bit0(float2 *__restrict__ psi, GateK<float> g) {
     const uint64_t first = (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
     uint64_t i0[kItems];
-    T a[kItems], b[kItems];
+    float2 a[kItems], b[kItems];
 #pragma unroll
     for (int u = 0; u < kItems; u++) {
         uint64_t k = first + uint64_t(u) * kThreads;
@@ -78,54 +199,8 @@ __global__ __launch_bounds__(kThreads) void k_pair(typename V2<R>::T *__restrict
 #pragma unroll
     for (int u = 0; u < kItems; u++) {
         if (i0[u] == ~uint64_t(0)) continue;
-        T na, nb;
-        if constexpr (KIND == LK_H) {
-            R s = g.c[0];
-            na = cscale<T>(s, cadd(a[u], b[u]));
-            nb = cscale<T>(s, csub(a[u], b[u]));
-        } else if constexpr (KIND == LK_X) {
-            na = b[u]; nb = a[u];
-        } else if constexpr (KIND == LK_Y) {
-            na = cmul(mk<T>(g.c[0], g.c[1]), b[u]);   // (-i) * b
-            nb = cmul(mk<T>(g.c[2], g.c[3]), a[u]);   // ( i) * a
-        } else if constexpr (KIND == LK_RX) {
-            na = csub(cscale<T>(g.c[0], a[u]), cmul(mk<T>(g.c[1], g.c[2]), b[u]));
-            nb = cadd(cmul(mk<T>(g.c[3], g.c[4]), a[u]), cscale<T>(g.c[0], b[u]));
-        } else if constexpr (KIND == LK_RY) {
-            na = csub(cscale<T>(g.c[0], a[u]), cscale<T>(g.c[1], b[u]));
-            nb = cadd(cscale<T>(g.c[1], a[u]), cscale<T>(g.c[0], b[u]));
-        } else if constexpr (KIND == LK_U2) {
-            na = cadd(cmul(mk<T>(g.c[0], g.c[1]), a[u]), cmul(mk<T>(g.c[2], g.c[3]), b[u]));
-            nb = cadd(cmul(mk<T>(g.c[4], g.c[5]), a[u]), cmul(mk<T>(g.c[6], g.c[7]), b[u]));
-        } else {  // LK_RZ
-            na = cmul(a[u], mk<T>(g.c[0], g.c[1]));
-            nb = cmul(b[u], mk<T>(g.c[2], g.c[3]));
-        }
-        psi[i0[u]] = na;
-        psi[i0[u] ^ g.pmask] = nb;
-    }
-}
-
-// Target-1 diagonal kernels: Z (negate) and S/T/Sdag/Tdag/P (phase).
-template <typename R, int KIND>
-__global__ __launch_bounds__(kThreads) void k_diag(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
-    using T = typename V2<R>::T;
-    const uint64_t first = (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
-    uint64_t idx[kItems];
-    T a[kItems];
-#pragma unroll
-    for (int u = 0; u < kItems; u++) {
-        uint64_t k = first + uint64_t(u) * kThreads;
-        idx[u] = k < g.items ? (insert_zeros(k, g.fixed) | g.set) : ~uint64_t(0);
-        if (k < g.items) a[u] = psi[idx[u]];
-    }
-#pragma unroll
-    for (int u = 0; u < kItems; u++) {
-        if (idx[u] == ~uint64_t(0)) continue;
-        T r;
-        if constexpr (KIND == LK_Z) { r.x = -a[u].x; r.y = -a[u].y; }
-        else r = cmul(a[u], mk<T>(g.c[0], g.c[1]));
-        psi[idx[u]] = r;
+        psi[i0[u]] = b[u];
+        psi[i0[u] ^ g.pmask] = a[u];
     }
 }
 
@@ -164,29 +239,64 @@ template <typename R> GateK<R> make_args(const Lowered &L, uint32_t n) {
     return g;
 }
 
-template <typename R> Status launch_t(State &s, const Lowered &L, hipStream_t st) {
-    using T = typename V2<R>::T;
-    T *psi = reinterpret_cast<T *>(s.d);
-    GateK<R> g = make_args<R>(L, s.n);
-    const uint64_t per_block = uint64_t(kThreads) * kItems;
-    dim3 grid((unsigned)((g.items + per_block - 1) / per_block)), block(kThreads);
-    switch (L.kind) {
-    case LK_H: hipLaunchKernelGGL((k_pair<R, LK_H>), grid, block, 0, st, psi, g); break;
-    case LK_X: hipLaunchKernelGGL((k_pair<R, LK_X>), grid, block, 0, st, psi, g); break;
-    case LK_Y: hipLaunchKernelGGL((k_pair<R, LK_Y>), grid, block, 0, st, psi, g); break;
-    case LK_RX: hipLaunchKernelGGL((k_pair<R, LK_RX>), grid, block, 0, st, psi, g); break;
-    case LK_RY: hipLaunchKernelGGL((k_pair<R, LK_RY>), grid, block, 0, st, psi, g); break;
-    case LK_U2: hipLaunchKernelGGL((k_pair<R, LK_U2>), grid, block, 0, st, psi, g); break;
-    case LK_RZ: hipLaunchKernelGGL((k_pair<R, LK_RZ>), grid, block, 0, st, psi, g); break;
-    case LK_Z: hipLaunchKernelGGL((k_diag<R, LK_Z>), grid, block, 0, st, psi, g); break;
-    case LK_PHASE: hipLaunchKernelGGL((k_diag<R, LK_PHASE>), grid, block, 0, st, psi, g); break;
-    case LK_SWAP:  // exchange base|tb with base|tb2: the X pair kernel with partner i ^ (tb|tb2)
-        hipLaunchKernelGGL((k_pair<R, LK_X>), grid, block, 0, st, psi, g); break;
-    case LK_MATCH: {
-        dim3 mg((unsigned)((g.items + kThreads - 1) / kThreads));
-        hipLaunchKernelGGL((k_match<R>), mg, block, 0, st, psi, g);
-    } break;
-    default: return Status{};
+inline unsigned grid_for(uint64_t items, uint64_t per_block) { return (unsigned)((items + per_block - 1) / per_block); }
+
+#define QI_PAIR_KINDS(M) M(LK_H) M(LK_X) M(LK_Y) M(LK_RX) M(LK_RY) M(LK_U2) M(LK_RZ)
+
+Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
+    GateK<float> g = make_args<float>(L, s.n);
+    const dim3 block(kThreads);
+    if (L.kind == LK_MATCH) {
+        hipLaunchKernelGGL((k_match<float>), dim3(grid_for(g.items, kThreads)), block, 0, st, (float2 *)s.d, g);
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
+    if (L.kind == LK_SWAP && (L.t0 == 0 || L.t1 == 0)) {
+        hipLaunchKernelGGL(k_swap_bit0, dim3(grid_for(g.items, uint64_t(kThreads) * kItems)), block, 0, st,
+                           (float2 *)s.d, g);
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
+    // Re-express the index sets in two-amplitude units.
+    const bool diag = L.kind == LK_Z || L.kind == LK_PHASE;
+    const int mode = diag ? ((g.set & 1) ? 1 : 0) : (g.pmask & 1) ? 2 : (g.set & 1) ? 1 : 0;
+    GateK<float> u = g;
+    u.fixed = g.fixed >> 1;
+    u.set = g.set >> 1;
+    u.pmask = g.pmask >> 1;
+    u.items = uint64_t(1) << (s.n - 1 - __builtin_popcountll(u.fixed));
+    f32x4 *psi = reinterpret_cast<f32x4 *>(s.d);
+    const dim3 g1(grid_for(u.items, kThreads)), g2(grid_for(u.items, 2 * kThreads));
+    const int kind = L.kind == LK_SWAP ? LK_X : L.kind;  // SWAP: exchange base|tb with base|tb2
+    switch (kind * 4 + mode) {
+#define QI_PAIR(K) \
+    case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), g1, block, 0, st, psi, u); break; \
+    case K * 4 + 1: hipLaunchKernelGGL((k_pair_c64<K, 1>), g1, block, 0, st, psi, u); break; \
+    case K * 4 + 2: hipLaunchKernelGGL((k_pair_c64<K, 2>), g2, block, 0, st, psi, u); break;
+    QI_PAIR_KINDS(QI_PAIR)
+#undef QI_PAIR
+    case LK_Z * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_Z, 0>), g2, block, 0, st, psi, u); break;
+    case LK_Z * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_Z, 1>), g2, block, 0, st, psi, u); break;
+    case LK_PHASE * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 0>), g2, block, 0, st, psi, u); break;
+    case LK_PHASE * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 1>), g2, block, 0, st, psi, u); break;
+    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+    }
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
+    GateK<double> g = make_args<double>(L, s.n);
+    const dim3 block(kThreads), g1(grid_for(g.items, kThreads)), g2(grid_for(g.items, 2 * kThreads));
+    f64x2 *psi = reinterpret_cast<f64x2 *>(s.d);
+    switch (L.kind == LK_SWAP ? LK_X : L.kind) {
+#define QI_PAIR(K) case K: hipLaunchKernelGGL((k_pair_c128<K>), g1, block, 0, st, psi, g); break;
+    QI_PAIR_KINDS(QI_PAIR)
+#undef QI_PAIR
+    case LK_Z: hipLaunchKernelGGL((k_diag_c128<LK_Z>), g2, block, 0, st, psi, g); break;
+    case LK_PHASE: hipLaunchKernelGGL((k_diag_c128<LK_PHASE>), g2, block, 0, st, psi, g); break;
+    case LK_MATCH: hipLaunchKernelGGL((k_match<double>), g1, block, 0, st, (double2 *)s.d, g); break;
+    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
     }
     QI_HIP_TRY(hipGetLastError());
     return Status{};
@@ -294,7 +404,7 @@ Status lower_gate(uint32_t n, const qi_gate &g, Lowered &L) {
 
 Status launch_gate(State &s, const Lowered &L, hipStream_t st) {
     if (L.kind == LK_NOP) return Status{};
-    return s.prec == QI_C128 ? launch_t<double>(s, L, st) : launch_t<float>(s, L, st);
+    return s.prec == QI_C128 ? launch_c128(s, L, st) : launch_c64(s, L, st);
 }
 
 Status launch_init(State &s, int kind, uint64_t basis, hipStream_t st) {

@@ -27,7 +27,12 @@ namespace qi {
 
 int tile_low_qubits(int prec) { return prec == QI_C128 ? 3 : 4; }  // 128-byte rows
 
-int default_tile_qubits(const State &s) { return s.prec == QI_C128 ? 12 : 13; }
+int default_tile_qubits(const State &s) { return s.prec == QI_C128 ? 11 : 12; }
+
+// 512 threads for the 64 KiB tiles (twice the waves per CU at the same LDS
+// use), 256 below.  The 30-qubit random circuit runs fastest with 32 KiB
+// tiles on 256 threads: 4096 complex64 or 2048 complex128 amplitudes.
+int default_tile_threads_log2(int prec, int K) { return K >= 12 + (prec == QI_C128 ? 0 : 1) ? 9 : 8; }
 
 namespace {
 
@@ -84,10 +89,10 @@ bool is_pair(const Lowered &L) { return L.kind <= LK_U2; }
 // targets are the cheapest (pairs inside a thread), so the most used targets
 // and controls of a group go to registers.
 struct RegPlanner {
-    int K = 0, RB = 0, L = 0;
+    int K = 0, RB = 0, L = 0, TB = 8;
     int lbit[64];       // qubit -> local bit (-1: outside the tile)
     int qof[16];        // local bit -> qubit
-    uint8_t cr[8], ct[8];  // layout: local bit of each register / thread bit
+    uint8_t cr[8], ct[12];  // layout: local bit of each register / thread bit
     std::vector<RegOp> out;
 
     uint32_t io_set() const {
@@ -95,7 +100,11 @@ struct RegPlanner {
         for (int k = 0; k < L; k++) m |= 1u << lbit[k];
         return m;
     }
-    uint32_t wave_set() const { return (1u << ct[6]) | (1u << ct[7]); }
+    uint32_t wave_set() const {
+        uint32_t m = 0;
+        for (int i = kLaneBits; i < TB; i++) m |= 1u << ct[i];
+        return m;
+    }
     int reg_index(int lb) const {
         for (int j = 0; j < RB; j++)
             if (cr[j] == lb) return j;
@@ -120,20 +129,20 @@ struct RegPlanner {
         for (int k = 0; k < L; k++) t[i++] = (uint8_t)lbit[k];
         uint32_t used = io;
         // waves
-        int w[2], nw = 0;
-        for (int pass = 0; pass < 2; pass++) {
+        const int NW = TB - kLaneBits;
+        int w[4], nw = 0;
+        for (int pass = 0; pass < NW; pass++) {
             int best = -1;
             for (int b = 0; b < K; b++) {
                 if (((used >> b) & 1) || ((need >> b) & 1)) continue;
-                if (nw && w[0] == b) continue;
                 if (best < 0 || next_use[b] > next_use[best] || (next_use[b] == next_use[best] && b > best)) best = b;
             }
             if (best < 0)  // the group targets every non-row bit: take any (planner bounds prevent this)
                 for (int b = K - 1; b >= 0 && best < 0; b--)
-                    if (!((used >> b) & 1) && !(nw && w[0] == b)) best = b;
+                    if (!((used >> b) & 1)) best = b;
             w[nw++] = best;
+            used |= 1u << best;
         }
-        used |= (1u << w[0]) | (1u << w[1]);
         // registers: highest score first
         int j = 0;
         while (j < RB) {
@@ -149,21 +158,21 @@ struct RegPlanner {
         // remaining lanes
         for (int b = 0; b < K; b++)
             if (!((used >> b) & 1)) { t[i++] = (uint8_t)b; used |= 1u << b; }
-        t[6] = (uint8_t)w[0];
-        t[7] = (uint8_t)w[1];
+        for (int k = 0; k < NW; k++) t[kLaneBits + k] = (uint8_t)w[k];
+        for (int k = TB; k < 12; k++) t[k] = 0;
         (void)all;
     }
     void set_layout(const uint8_t *r, const uint8_t *t, bool emit_op) {
-        if (!memcmp(r, cr, 8) && !memcmp(t, ct, 8)) return;
+        if (!memcmp(r, cr, 8) && !memcmp(t, ct, 12)) return;
         if (emit_op) {
             RegOp op{};
             op.kind = RO_TRANSPOSE;
             memcpy(op.lay_r, r, 8);
-            memcpy(op.lay_t, t, 8);
+            memcpy(op.lay_t, t, 12);
             out.push_back(op);
         }
         memcpy(cr, r, 8);
-        memcpy(ct, t, 8);
+        memcpy(ct, t, 12);
     }
     void controls(uint64_t cm, RegOp &op) const {
         for (; cm; cm &= cm - 1) {
@@ -180,7 +189,7 @@ struct RegPlanner {
         switch (G.kind) {
         case LK_Z: case LK_PHASE: case LK_RZ: {
             if (G.kind == LK_Z) op.kind = RO_Z;
-            else if (G.kind == LK_PHASE) { op.kind = RO_DG; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; op.p0one = 1; }
+            else if (G.kind == LK_PHASE) { op.kind = RO_P1; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; }
             else { op.kind = RO_DG; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
             const int lb = lbit[G.t0];
             if (lb < 0) { op.dloc = 2; op.dbit = (int)G.t0; }
@@ -248,15 +257,17 @@ struct Phase {
 
 }  // namespace
 
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan) {
+Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
     if (K <= 0) K = default_tile_qubits(s);
+    if (TB <= 0) TB = default_tile_threads_log2(s.prec, K);
+    if (TB < 8 || TB > 9) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile threads out of range");
     const int kmax = s.prec == QI_C128 ? 12 : 13;
     if (K < 10 || K > kmax) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width out of range");
     if ((int)s.n < K) {  // state smaller than a tile: stream gate by gate
         for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
         return Status{};
     }
-    const int low = tile_low_qubits(s.prec), RB = K - kRegThreadBits;
+    const int low = tile_low_qubits(s.prec), RB = K - TB;
     std::vector<GateSets> gs(gates.size());
     for (size_t g = 0; g < gates.size(); g++) gs[g] = gate_sets(gates[g]);
     std::vector<char> done(gates.size(), 0);
@@ -265,16 +276,17 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         std::vector<size_t> members;
         const uint64_t q = select_pass(gs, done, first, K, (int)s.n, low, members);
         RegPlanner P;
-        P.K = K; P.RB = RB; P.L = low;
+        P.K = K; P.RB = RB; P.L = low; P.TB = TB;
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
         RegPass rp{};
         rp.qmask = q;
         rp.K = K;
+        rp.tb = TB;
         for (int b = 0, j = 0; b < (int)s.n; b++)
             if ((q >> b) & 1) { P.lbit[b] = j; P.qof[j] = b; j++; }
         // ---- phases: ops whose targets fit the register and lane bits (all but
         // the two wave bits), commuting-greedy; relabels tracked on the way
-        const int CAP = K - 2;
+        const int CAP = K - (TB - kLaneBits);
         std::vector<Phase> phases;
         std::vector<std::vector<int>> phase_lb;  // labels (qubit -> local bit) at each phase start
         {
@@ -335,18 +347,18 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         int nu[16];
         next_use(0, nu);
         {
-            uint8_t r[8], t[8];
+            uint8_t r[8], t[12];
             P.make(phases.empty() ? 0u : phases[0].need, phases.empty() ? nu : score[0].data(), nu, r, t);
             P.set_layout(r, t, false);
             memcpy(rp.lay_r, P.cr, 8);
-            memcpy(rp.lay_t, P.ct, 8);
+            memcpy(rp.lay_t, P.ct, 12);
             for (int j = 0; j < RB; j++) rp.qr_in[j] = (uint8_t)P.qof[P.cr[j]];
-            for (int i = 0; i < kRegThreadBits; i++) rp.qt_in[i] = (uint8_t)P.qof[P.ct[i]];
+            for (int i = 0; i < TB; i++) rp.qt_in[i] = (uint8_t)P.qof[P.ct[i]];
         }
         for (size_t p = 0; p < phases.size(); p++) {
             if (phases[p].need & P.wave_set()) {
                 next_use(p, nu);
-                uint8_t r[8], t[8];
+                uint8_t r[8], t[12];
                 P.make(phases[p].need, score[p].data(), nu, r, t);
                 P.set_layout(r, t, true);
             }
@@ -359,16 +371,16 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, Exec
         if (!P.io_ok()) {
             int zero[16] = {0}, far[16];
             for (int b = 0; b < 16; b++) far[b] = 1 << 30;
-            uint8_t r[8], t[8];
+            uint8_t r[8], t[12];
             P.make(0, zero, far, r, t);
             P.set_layout(r, t, true);
             if (!P.io_ok()) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: store layout");
         }
         for (int j = 0; j < RB; j++) rp.qr_out[j] = (uint8_t)P.qof[P.cr[j]];
-        for (int i = 0; i < kRegThreadBits; i++) rp.qt_out[i] = (uint8_t)P.qof[P.ct[i]];
+        for (int i = 0; i < TB; i++) rp.qt_out[i] = (uint8_t)P.qof[P.ct[i]];
         // SWAP relabels or transposes may send an amplitude to an address another
         // thread loads: the kernel then waits for every load before storing
-        rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 8) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
+        rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
         plan.reg_ops.insert(plan.reg_ops.end(), P.out.begin(), P.out.end());

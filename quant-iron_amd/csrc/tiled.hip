@@ -51,7 +51,6 @@ __device__ __forceinline__ uint64_t insert_zeros64(uint64_t k, uint64_t fixed) {
     return k;
 }
 
-constexpr int kTT = 1 << kRegThreadBits;
 
 // The thread's share of the tile: E complex amplitudes held as one vector
 // value. Element writes are whole-vector updates of an SSA value, so no
@@ -108,45 +107,50 @@ template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R 
 }
 
 // ------------------------------------------------------------- 2x2 ops --
-// Pair (a, b) = registers (r, r | 2^B) of one thread. Registers whose
-// register-held controls are not all 1 are skipped (uniform, compile-time r);
-// thread-held controls select per lane (TC).
-template <int B, int KIND, bool TC, int E, typename T, typename R>
+// Pair (a, b) = registers (r, r | 2^B) of one thread. With controls in the
+// tile (CT) every register computes and keeps the old value unless its
+// register-held controls (uniform per register) and this thread's
+// thread-held controls are all 1; ops without controls compute straight.
+template <int B, int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rcm, bool tok) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
         if (r & (1 << B)) continue;
-        if ((uint32_t(r) & rcm) != rcm) continue;
         const T a = v.get(r), b = v.get(r | (1 << B));
         T na, nb;
         if constexpr (KIND == RO_X) { na = b; nb = a; }
         else if constexpr (KIND == RO_RR) { na = rmul_add<T, R>(k[0], a, k[1], b); nb = rmul_add<T, R>(k[2], a, k[3], b); }
         else { na = cmul_add<T, R>(k, a, b); nb = cmul_add<T, R>(k + 4, a, b); }
-        if constexpr (TC) { na = sel(tok, na, a); nb = sel(tok, nb, b); }
+        if constexpr (CT) {
+            const bool on = tok && ((uint32_t(r) & rcm) == rcm);
+            na = sel(on, na, a);
+            nb = sel(on, nb, b);
+        }
         v.set(r, na);
         v.set(r | (1 << B), nb);
+        // one pair at a time: each result can take its input's register
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // Lane pairs: s = own amplitude, p = partner's; k holds this lane's row of
 // the matrix (own coefficient first).
-template <int KIND, bool TC, int E, typename T, typename R>
+template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, uint32_t rcm, bool tok) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        if ((uint32_t(r) & rcm) != rcm) continue;
         const T s = v.get(r);
         const T p = xlane2<T>(addr, s);
         T n;
         if constexpr (KIND == RO_X) n = p;
         else if constexpr (KIND == RO_RR) n = rmul_add<T, R>(k[0], s, k[1], p);
         else n = cmul_add<T, R>(k, s, p);
-        if constexpr (TC) n = sel(tok, n, s);
+        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, s);
         v.set(r, n);
     }
 }
 
-template <int KIND, bool TC, int RB, int E, typename T, typename R>
+template <int KIND, bool CT, int RB, int E, typename T, typename R>
 __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t lane, uint32_t rcm, bool tok) {
     if (op.tgt >= kLaneTarget) {
         const int i = op.tgt - kLaneTarget;
@@ -162,69 +166,79 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
             k[2] = R(bit ? op.c[4] : op.c[2]);
             k[3] = R(bit ? op.c[5] : op.c[3]);
         }
-        pairs_lane<KIND, TC, E, T, R>(v, addr, k, rcm, tok);
+        pairs_lane<KIND, CT, E, T, R>(v, addr, k, rcm, tok);
         return;
     }
     R k[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) k[q] = R(op.c[q]);
-#define QI_RB(BB) case BB: if constexpr (BB < RB) pairs_reg<BB, KIND, TC, E, T, R>(v, k, rcm, tok); break;
+#define QI_RB(BB) case BB: if constexpr (BB < RB) pairs_reg<BB, KIND, CT, E, T, R>(v, k, rcm, tok); break;
     switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
 }
 
 // ---------------------------------------------------------- diagonal ops --
-// Target on register bit B: each register's target value is known.
-template <int B, int KIND, bool TC, int E, typename T, typename R>
-__device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, bool p0one, uint32_t rcm, bool tok) {
+// Target on register bit B: each register's target value is known at
+// compile time (RO_Z and RO_P1 leave the target-0 registers alone).
+template <int B, int KIND, bool CT, int E, typename T, typename R>
+__device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, uint32_t rcm, bool tok) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        if ((uint32_t(r) & rcm) != rcm) continue;
         const bool one = ((r >> B) & 1) != 0;
-        if (!one && (KIND == RO_Z || p0one)) continue;
+        if (!one && KIND != RO_DG) continue;
         const T a = v.get(r);
         T n;
         if constexpr (KIND == RO_Z) n = mk<T>(-a.x, -a.y);
         else n = cmul(a, one ? p1 : p0);
-        if constexpr (TC) n = sel(tok, n, a);
+        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, a);
         v.set(r, n);
     }
 }
-// Target fixed for the whole thread (thread bit or outside the tile): f is
-// this thread's factor.
-template <int KIND, bool TC, int E, typename T, typename R>
+// Target fixed for the whole thread (thread bit or outside the tile).
+template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t rcm, bool tok) {
+    if constexpr (KIND == RO_Z && !CT) {
+        // sign flip as an XOR of the sign bits
+        const uint32_t m = one ? 0x80000000u : 0u;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const T a = v.get(r);
+            if constexpr (sizeof(R) == 4)
+                v.set(r, mk<T>(__uint_as_float(__float_as_uint(a.x) ^ m), __uint_as_float(__float_as_uint(a.y) ^ m)));
+            else
+                v.set(r, mk<T>(__longlong_as_double(__double_as_longlong(a.x) ^ (long long)(uint64_t(m) << 32)),
+                               __longlong_as_double(__double_as_longlong(a.y) ^ (long long)(uint64_t(m) << 32))));
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        if ((uint32_t(r) & rcm) != rcm) continue;
         const T a = v.get(r);
         T n;
-        if constexpr (KIND == RO_Z) n = sel(one && tok, mk<T>(-a.x, -a.y), a);
-        else {
-            n = cmul(a, f);
-            if constexpr (TC) n = sel(tok, n, a);
-        }
+        if constexpr (KIND == RO_Z) n = sel(one, mk<T>(-a.x, -a.y), a);
+        else n = cmul(a, f);
+        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, a);
         v.set(r, n);
     }
 }
 
-template <int KIND, bool TC, int RB, int E, typename T, typename R>
+template <int KIND, bool CT, int RB, int E, typename T, typename R>
 __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t tl, uint64_t base, uint32_t rcm, bool tok) {
     const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
     if (op.dloc == 0) {
-#define QI_RB(BB) case BB: if constexpr (BB < RB) diag_reg<BB, KIND, TC, E, T, R>(v, p0, p1, op.p0one != 0, rcm, tok); break;
+#define QI_RB(BB) case BB: if constexpr (BB < RB) diag_reg<BB, KIND, CT, E, T, R>(v, p0, p1, rcm, tok); break;
         switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
         return;
     }
     const bool one = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
-    diag_thr<KIND, TC, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
+    diag_thr<KIND, CT, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
 // ------------------------------------------------------------- LDS ops --
 // Controlled SWAP or Matchgate (CPU form, operator.rs:993-1004) on the LDS
 // copy; local index e lives at slot swz(e).
-template <typename R, int K, int F>
+template <typename R, int K, int F, int NT>
 __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &op) {
     using T = typename V2<R>::T;
     constexpr int N = 1 << K;
@@ -232,7 +246,7 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
     const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
     const R c0 = R(op.c[0]), c1 = R(op.c[1]);
     const T m23 = mk<T>(R(op.c[2]), R(op.c[3])), m45 = mk<T>(R(op.c[4]), R(op.c[5])), m67 = mk<T>(R(op.c[6]), R(op.c[7]));
-    for (uint32_t q = threadIdx.x; q < N / 4; q += kTT) {
+    for (uint32_t q = threadIdx.x; q < N / 4; q += NT) {
         const uint32_t l = ins0(ins0(q, lo), hi);
         if (op.lkind == LK_SWAP) {
             const uint32_t i = l | b0, j = l | b1;
@@ -253,11 +267,11 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
 }
 
 // ------------------------------------------------------------ the pass --
-template <typename R, int K>
-__global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
-                                                  const RegOp *__restrict__ ops) {
+template <typename R, int K, int TB>
+__global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
+                                                      const RegOp *__restrict__ ops) {
     using T = typename V2<R>::T;
-    constexpr int RB = K - kRegThreadBits, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
+    constexpr int RB = K - TB, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
     __shared__ T lds[1 << K];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
@@ -268,7 +282,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
     {
         uint64_t at = 0;
 #pragma unroll
-        for (int i = 0; i < kRegThreadBits; i++) {
+        for (int i = 0; i < TB; i++) {
             const uint32_t bit = (tid >> i) & 1u;
             tl |= bit << pass.lay_t[i];
             at |= uint64_t(bit) << pass.qt_in[i];
@@ -302,7 +316,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
             if (kind >= RO_TRANSPOSE) break;
             nxt = ops[pass.op0 + o + 1];
             if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-            const bool tc = op.tcm != 0;              // thread-held controls (uniform)
+            const bool tc = (op.tcm | op.rcm) != 0;   // controls inside the tile (uniform)
             const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
 #define QI_OP(KD, FN)                                                                     \
     case KD:                                                                              \
@@ -314,7 +328,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
                 QI_OP(RO_X, pair_op) QI_OP(RO_RR, pair_op) QI_OP(RO_GEN, pair_op)
 #undef ARGS
 #define ARGS tl, base
-                QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op)
+                QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
 #undef ARGS
             default: break;
             }
@@ -338,7 +352,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
         if (op.kind == RO_TRANSPOSE) {
             tl = 0;
 #pragma unroll
-            for (int i = 0; i < kRegThreadBits; i++) tl |= ((tid >> i) & 1u) << op.lay_t[i];
+            for (int i = 0; i < TB; i++) tl |= ((tid >> i) & 1u) << op.lay_t[i];
             uint32_t mr[RB];
 #pragma unroll
             for (int j = 0; j < RB; j++) mr[j] = 1u << op.lay_r[j];
@@ -350,7 +364,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
                 rl[r] = e;
             }
         } else {
-            lds_gate<R, K, F>(lds, op);
+            lds_gate<R, K, F, (1 << TB)>(lds, op);
             __syncthreads();
         }
         {
@@ -373,7 +387,7 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
         // store: thread and register bits hold the qubits of the final layout
         uint64_t at = 0;
 #pragma unroll
-        for (int i = 0; i < kRegThreadBits; i++) at |= uint64_t((tid >> i) & 1u) << pass.qt_out[i];
+        for (int i = 0; i < TB; i++) at |= uint64_t((tid >> i) & 1u) << pass.qt_out[i];
         uint64_t ar[RB];
 #pragma unroll
         for (int j = 0; j < RB; j++) ar[j] = uint64_t(1) << pass.qr_out[j];
@@ -388,31 +402,41 @@ __global__ __launch_bounds__(kTT) void k_reg_pass(typename V2<R>::T *__restrict_
     }
 }
 
-template <typename R, int K> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
+template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     using T = typename V2<R>::T;
     const unsigned tiles = (unsigned)(uint64_t(1) << (s.n - K));
-    hipLaunchKernelGGL((k_reg_pass<R, K>), dim3(tiles), dim3(kTT), 0, st, (T *)s.d, p, ops);
+    hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3(tiles), dim3(1 << TB), 0, st, (T *)s.d, p, ops);
+}
+
+template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
+    switch (p.tb) {
+    case 8: launch_pass<R, K, 8>(s, p, ops, st); return true;
+    case 9: launch_pass<R, K, 9>(s, p, ops, st); return true;
+    default: return false;
+    }
 }
 
 }  // namespace
 
 Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st) {
+    bool ok = false;
     if (s.prec == QI_C128) {
         switch (p.K) {
-        case 10: launch_pass<double, 10>(s, p, dops, st); break;
-        case 11: launch_pass<double, 11>(s, p, dops, st); break;
-        case 12: launch_pass<double, 12>(s, p, dops, st); break;
-        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width");
+        case 10: ok = launch_k<double, 10>(s, p, dops, st); break;
+        case 11: ok = launch_k<double, 11>(s, p, dops, st); break;
+        case 12: ok = launch_k<double, 12>(s, p, dops, st); break;
+        default: break;
         }
     } else {
         switch (p.K) {
-        case 10: launch_pass<float, 10>(s, p, dops, st); break;
-        case 11: launch_pass<float, 11>(s, p, dops, st); break;
-        case 12: launch_pass<float, 12>(s, p, dops, st); break;
-        case 13: launch_pass<float, 13>(s, p, dops, st); break;
-        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width");
+        case 10: ok = launch_k<float, 10>(s, p, dops, st); break;
+        case 11: ok = launch_k<float, 11>(s, p, dops, st); break;
+        case 12: ok = launch_k<float, 12>(s, p, dops, st); break;
+        case 13: ok = launch_k<float, 13>(s, p, dops, st); break;
+        default: break;
         }
     }
+    if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width / threads");
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }

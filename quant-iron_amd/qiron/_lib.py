@@ -69,7 +69,7 @@ class QiPauliTerm(C.Structure):
 
 class QiExecOpts(C.Structure):
     _fields_ = [("mode", C.c_int32), ("tile_qubits", C.c_int32), ("block_qubits", C.c_int32),
-                ("use_graph", C.c_int32)]
+                ("use_graph", C.c_int32), ("tile_threads_log2", C.c_int32)]
 
 
 # numpy mirrors, for building large gate lists without per-gate ctypes objects

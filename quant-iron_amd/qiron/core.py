@@ -367,9 +367,10 @@ class State:
         return self
 
     def apply_gates_inplace(self, gates, n_circuit: Optional[int] = None, mode: int = L.EXEC_AUTO,
-                            tile_qubits: int = 0, block_qubits: int = 0, use_graph: bool = False) -> "State":
+                            tile_qubits: int = 0, block_qubits: int = 0, use_graph: bool = False,
+                            tile_threads_log2: int = 0) -> "State":
         arr = gates if isinstance(gates, np.ndarray) else gate_array(gates)
-        opts = L.QiExecOpts(mode, tile_qubits, block_qubits, 1 if use_graph else 0)
+        opts = L.QiExecOpts(mode, tile_qubits, block_qubits, 1 if use_graph else 0, tile_threads_log2)
         err = L.QiError()
         n_c = self.num_qubits if n_circuit is None else n_circuit
         _check(self.lib.qi_apply_gates(self._h, n_c, arr.ctypes.data_as(C.c_void_p), arr.size, C.byref(opts),

@@ -12,8 +12,8 @@ import pytest
 
 import oracle as O
 import qiron
-from kat_util import (F32_EPS, cvec, gates_array, load_kats, random_gate, random_state, random_terms, term_array,
-                      to_gate)
+from kat_util import (F32_EPS, ONE_Q, cvec, gates_array, load_kats, random_gate, random_state, random_terms,
+                      random_unitary2, term_array, to_gate)
 from qiron import _lib as L
 from qiron import workloads as W
 
@@ -130,6 +130,44 @@ def test_c128_gates_are_bit_exact_with_the_oracle(ctx):
     got = s.state_vector
     # equal up to the sign of zero
     assert np.array_equal(got.real, ref.real) and np.array_equal(got.imag, ref.imag)
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_streaming_gates_at_every_qubit_0_placement(ctx, prec):
+    """The complex64 kernels move two amplitudes per 16-byte unit, so qubit 0
+    (inside the unit) as target, control, swap partner or bystander takes a
+    different code path from the higher qubits; each placement against the
+    oracle, on states of 1 to 5 qubits."""
+    rng = np.random.default_rng(4242)
+    placements = [  # (n, targets, controls)
+        (1, [0], []), (2, [0], []), (2, [1], []), (2, [0], [1]), (2, [1], [0]),
+        (4, [0], [2]), (4, [2], [0]), (4, [3], [0, 1]), (4, [1], [2, 3]), (5, [4], []),
+    ]
+    swaps = [(2, [0, 1], []), (4, [0, 2], []), (4, [3, 0], [1]), (4, [1, 3], [0]), (4, [2, 3], []), (5, [1, 4], [2])]
+    cases = []
+    for kind in ONE_Q:
+        for n, t, c in placements:
+            cases.append(dict(kind=kind, targets=t, controls=c))
+    cases += [dict(kind="SWAP", targets=t, controls=c) for n, t, c in swaps]
+    for n in (1, 2, 4, 5):
+        psi = random_state(n, rng)
+        s = upload(psi, prec)
+        ref = psi
+        for d in cases:
+            if max(d["targets"] + d["controls"]) >= n:
+                continue
+            d = dict(d, params=[float(rng.uniform(0, 2 * np.pi)) for _ in range(3)])
+            if d["kind"] == "U2":
+                u = random_unitary2(rng)
+                d["matrix"] = [[z.real, z.imag] for z in u.reshape(-1)]
+            s.apply_gate_inplace(qiron.Gate(to_gate(d)))
+            ref = O.apply_gate(ref, to_gate(d))
+            got = s.state_vector
+            if prec == "c128":
+                assert np.array_equal(got.real, ref.real) and np.array_equal(got.imag, ref.imag), (n, d)
+            else:
+                check(got, ref, prec, f"n={n} {d['kind']} t={d['targets']} c={d['controls']}")
+                ref = got.astype(np.complex128)  # compare each gate from the engine's own state
 
 
 MODES = [
