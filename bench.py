@@ -277,6 +277,58 @@ def bench_surrogate(W, args, plain_eps):
     return out
 
 
+def profiled_traffic(kernel: str, path: str):
+    """HBM bytes per dispatch of `kernel` from a committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE runs of this same workload, scaled by a plain
+    copy's counters); None when the summary does not cover this kernel."""
+    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    if not os.path.exists(full):
+        return None
+    lines = open(full).read().splitlines()
+    for i, line in enumerate(lines):
+        if line.startswith(kernel) and i + 2 < len(lines) and "traffic_bytes_per_dispatch:" in lines[i + 2]:
+            return int(lines[i + 2].split("traffic_bytes_per_dispatch:")[1].split()[0])
+    return None
+
+
+def single_gates(qiron, L, W, ctx, n, results):
+    """One H gate on the middle qubit of an n-qubit state, streaming kernel:
+    the HBM-roofline case (reads and writes the whole state once)."""
+    for prec, ab in (("c64", 8), ("c128", 16)):
+        S = (1 << n) * ab
+        s = qiron.State.new_plus(n, prec)
+        one = W.GateList(4)
+        one.add("H", [n // 2])
+        arr = one.array()
+        ts = []
+        for rep in range(6):
+            s.apply_gates_inplace(arr, mode=L.EXEC_PER_GATE)
+            if rep:
+                ts.append(ctx.last_exec_ms())
+        del s
+        ms = min(ts)
+        gbps = 2 * S / (ms * 1e-3) / 1e9
+        log(f"{prec} single H gate at {n}q: {ms:.3f} ms, {gbps:.0f} GB/s ({gbps / HBM_PEAK_GBPS:.1%} of 8 TB/s)")
+        results[f"{prec}_single_gate"] = dict(ms=ms, GBps=gbps, frac=gbps / HBM_PEAK_GBPS,
+                                              ms_all=[round(t, 4) for t in ts])
+
+
+def copy_calibration(qiron, ctx, n=29, reps=5):
+    """A plain device copy of known size (the library's state copy kernel,
+    2^29 complex64 amplitudes = 4 GiB read and 4 GiB written per launch): the
+    reference that scales the HBM byte counters of a profiled run."""
+    src = qiron.State.new_plus(n, "c64")
+    dst = qiron.State.new_plus(n, "c64")
+    nbytes = (1 << n) * 8
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_from(src)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    log(f"copy of {nbytes} bytes: {dt * 1e3:.3f} ms per copy (wall), {2 * nbytes / dt / 1e9:.0f} GB/s (read + write)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,7 +336,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--only", default="", help="run only the headline executor (for profiling)")
+    ap.add_argument("--only", default="", choices=["", "tiled", "gate", "copy"],
+                    help="profiling runs: only the headline executor, only the single-gate kernels, or only a plain "
+                         "copy of known size (the reference for HBM byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
@@ -313,11 +367,18 @@ def main():
     G = int(gates.size)
     log(f"circuit: {G} gates on {n} qubits")
 
+    if args.only == "copy":
+        copy_calibration(qiron, ctx)
+        return
+    results = {}
+    if args.only == "gate":
+        single_gates(qiron, L, W, ctx, n, results)
+        return
+
     tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)
     # the library's complex64 defaults: 4096-amplitude tiles, 256 threads (512 from 8192 amplitudes)
     tile_k = args.tile_qubits or 12
     tile_tb = 9 if tile_k >= 13 else 8
-    results = {}
     # ---- headline: tiled executor, complex64
     wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c64", gates, tiled, args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3
@@ -332,25 +393,14 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
                 "kernel": f"k_reg_pass<float, {tile_k}, {tile_tb}>", "algorithmic_bytes_per_launch": 2 * S64}
+    if n == 30 and depth == 40:
+        src = "profiles/k_reg_pass_c64_30q_random_circuit.txt"
+        roofline["traffic"] = profiled_traffic(roofline["kernel"], src)
+        if roofline["traffic"] is not None:
+            roofline["traffic_source"] = src
 
     if not args.only:
-        # ---- single 30-qubit gate application (streaming kernel): the HBM-roofline target
-        for prec, ab in (("c64", 8), ("c128", 16)):
-            S = (1 << n) * ab
-            s = qiron.State.new_plus(n, prec)
-            one = W.GateList(4)
-            one.add("H", [n // 2])
-            arr = one.array()
-            ts = []
-            for rep in range(6):
-                s.apply_gates_inplace(arr, mode=L.EXEC_PER_GATE)
-                if rep:
-                    ts.append(ctx.last_exec_ms())
-            del s
-            ms = min(ts)
-            gbps = 2 * S / (ms * 1e-3) / 1e9
-            log(f"{prec} single H gate at {n}q: {ms:.3f} ms, {gbps:.0f} GB/s ({gbps / HBM_PEAK_GBPS:.1%} of 8 TB/s)")
-            results[f"{prec}_single_gate"] = dict(ms=ms, GBps=gbps, frac=gbps / HBM_PEAK_GBPS)
+        single_gates(qiron, L, W, ctx, n, results)
         # ---- complex128 tiled
         wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c128", gates, tiled, 1, 1)
         results["c128_tiled"] = dict(gates_per_s=G / wall, ms_per_step=wall * 1e3, passes=passes,

@@ -348,6 +348,13 @@ __global__ __launch_bounds__(256) void k_axpy(typename V2<R>::T *y, const typena
         y[i] = cadd(y[i], cmul(x[i], mk<T>(re, im)));
 }
 
+// Whole-state copy in 16-byte units, streamed like the gate kernels.
+__global__ __launch_bounds__(kThreads) void k_copy16(f32x4 *__restrict__ dst, const f32x4 *__restrict__ src,
+                                                     uint64_t units) {
+    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < units) stnt(ldnt(src + i), dst + i);
+}
+
 }  // namespace
 
 int pick_blocks(uint64_t items, int per_block) {
@@ -456,7 +463,10 @@ Status launch_axpy(State &y, double re, double im, const State &x, hipStream_t s
 }
 
 Status launch_copy(State &dst, const State &src, hipStream_t st) {
-    QI_HIP_TRY(hipMemcpyAsync(dst.d, src.d, src.bytes(), hipMemcpyDeviceToDevice, st));
+    const uint64_t units = src.bytes() / 16;  // a state is at least two complex64 amplitudes
+    hipLaunchKernelGGL(k_copy16, dim3(grid_for(units, kThreads)), dim3(kThreads), 0, st,
+                       reinterpret_cast<f32x4 *>(dst.d), reinterpret_cast<const f32x4 *>(src.d), units);
+    QI_HIP_TRY(hipGetLastError());
     return Status{};
 }
 

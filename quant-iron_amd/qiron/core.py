@@ -360,6 +360,13 @@ class State:
         _check(self.lib.qi_state_clone(self._h, C.byref(h), C.byref(err)), err)
         return State(h, self.ctx)
 
+    def copy_from(self, src: "State") -> "State":
+        """Overwrite this state's amplitudes with src's (same qubit count and
+        precision), device to device."""
+        err = L.QiError()
+        _check(self.lib.qi_state_copy(self._h, src._h, C.byref(err)), err)
+        return self
+
     # ---- in-place engine entry points
     def apply_gate_inplace(self, gate: Gate) -> "State":
         err = L.QiError()
