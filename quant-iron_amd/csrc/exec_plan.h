@@ -68,6 +68,7 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     uint8_t lay_t[12];   //               local bit of each thread bit
     uint32_t pad1;
     double c[8];         // coefficients
+    float cf[8];         // the same, rounded to float (read by the complex64 kernels)
 };
 
 struct RegPass {

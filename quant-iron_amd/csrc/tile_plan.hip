@@ -383,6 +383,8 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
         rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
+        for (RegOp &op : P.out)
+            for (int q = 0; q < 8; q++) op.cf[q] = (float)op.c[q];
         plan.reg_ops.insert(plan.reg_ops.end(), P.out.begin(), P.out.end());
         RegOp end{};
         end.kind = RO_NOP;

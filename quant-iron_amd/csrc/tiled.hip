@@ -106,6 +106,72 @@ template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R 
     }
 }
 
+// ------------------------------------------------ in-place float updates --
+// A complex64 tile lives in registers across a loop whose body dispatches on
+// the op kind. Each op writes every amplitude component back into the
+// register it was read from (tied asm operands), so all op paths leave the
+// tile in the same registers and no copies are needed where they merge.
+// Temporaries are ordinary values. Only VALU instructions appear here; the
+// compiler places the waits for loads and LDS results as for its own code.
+// Ops stay straight-line: with controls, new values are computed into
+// temporaries and selected in place under a lane mask (no branches, whose
+// joins would again need copies).
+__device__ __forceinline__ void ip_fma(float &d, float k, float t) {  // d = k * d + t
+    asm("v_fma_f32 %0, %1, %0, %2" : "+v"(d) : "v"(k), "v"(t));
+}
+__device__ __forceinline__ void ip_fma3(float &d, float k, float x, float t) {  // d = k * x + t (old d dead)
+    asm("v_fma_f32 %0, %1, %2, %3" : "+v"(d) : "v"(k), "v"(x), "v"(t));
+}
+__device__ __forceinline__ void ip_fmac(float &d, float k, float x) {  // d += k * x
+    asm("v_fmac_f32 %0, %1, %2" : "+v"(d) : "v"(k), "v"(x));
+}
+__device__ __forceinline__ void ip_set(float &d, float x) { asm("v_mov_b32 %0, %1" : "+v"(d) : "v"(x)); }
+__device__ __forceinline__ void ip_swap(float &x, float &y) { asm("v_swap_b32 %0, %1" : "+v"(x), "+v"(y)); }
+__device__ __forceinline__ void ip_neg(float &d) { asm("v_xor_b32 %0, 0x80000000, %0" : "+v"(d)); }
+// d = lane in mask ? x : d
+__device__ __forceinline__ void ip_sel(float &d, float x, uint64_t mask) {
+    asm("v_cndmask_b32 %0, %0, %1, %2" : "+v"(d) : "v"(x), "s"(mask));
+}
+// lane in mask ? x : y, as a new value
+__device__ __forceinline__ float selm(uint64_t mask, float x, float y) {
+    float r;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(y), "v"(x), "s"(mask));
+    return r;
+}
+__device__ __forceinline__ void ip_xor(float &d, uint32_t m) { asm("v_xor_b32 %0, %1, %0" : "+v"(d) : "v"(m)); }
+
+// (a, b) <- (m0 a + m1 b, m2 a + m3 b), real m, one component
+__device__ __forceinline__ void ip_rr(float &a, float &b, const float *k) {
+    const float t1 = k[2] * a, t2 = k[1] * b;
+    ip_fma(b, k[3], t1);
+    ip_fma(a, k[0], t2);
+}
+// (a, b) <- (c0 a + c1 b, c2 a + c3 b), complex c_i = (k[2i], k[2i+1]); nk = -k
+__device__ __forceinline__ void ip_gen(float &ax, float &ay, float &bx, float &by, const float *k, const float *nk) {
+    const float p = __builtin_fmaf(nk[1], ay, k[0] * ax), q = __builtin_fmaf(k[0], ay, k[1] * ax);
+    const float r = __builtin_fmaf(nk[5], ay, k[4] * ax), s = __builtin_fmaf(k[4], ay, k[5] * ax);
+    ip_fma3(ax, k[2], bx, p);
+    ip_fmac(ax, nk[3], by);
+    ip_fma3(ay, k[2], by, q);
+    ip_fmac(ay, k[3], bx);
+    const float z2 = __builtin_fmaf(nk[7], by, r), z = __builtin_fmaf(k[6], by, s);
+    ip_fma3(by, k[7], bx, z);
+    ip_fma(bx, k[6], z2);
+}
+// a <- (px, py) a
+__device__ __forceinline__ void ip_cmul(float &ax, float &ay, float px, float py, float npy) {
+    const float t1 = npy * ay, t2 = py * ax;
+    ip_fma(ax, px, t1);
+    ip_fma(ay, px, t2);
+}
+
+// Op coefficient q in the kernel's precision (complex64 kernels read the
+// table's float copy: no conversions per op).
+template <typename R> __device__ __forceinline__ R coef(const RegOp &op, int q) {
+    if constexpr (sizeof(R) == 4) return op.cf[q];
+    else return op.c[q];
+}
+
 // ------------------------------------------------------------- 2x2 ops --
 // Pair (a, b) = registers (r, r | 2^B) of one thread. With controls in the
 // tile (CT) every register computes and keeps the old value unless its
@@ -113,6 +179,42 @@ template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R 
 // thread-held controls are all 1; ops without controls compute straight.
 template <int B, int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rcm, bool tok) {
+    if constexpr (sizeof(R) == 4) {
+        float nk[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) nk[q] = -k[q];
+        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            if (r & (1 << B)) continue;
+            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+            float &bx = v.d[2 * (r | (1 << B))], &by = v.d[2 * (r | (1 << B)) + 1];
+            if constexpr (!CT) {
+                if constexpr (KIND == RO_X) { ip_swap(ax, bx); ip_swap(ay, by); }
+                else if constexpr (KIND == RO_RR) { ip_rr(ax, bx, k); ip_rr(ay, by, k); }
+                else ip_gen(ax, ay, bx, by, k, nk);
+            } else {
+                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;  // uniform per register
+                if constexpr (KIND == RO_X) {  // controlled swap: new b first, then a and b in place
+                    const float tx = selm(m, ax, bx), ty = selm(m, ay, by);
+                    ip_sel(ax, bx, m); ip_sel(ay, by, m);
+                    ip_set(bx, tx); ip_set(by, ty);
+                    continue;
+                }
+                float nax, nay, nbx, nby;
+                if constexpr (KIND == RO_RR) {
+                    nax = __builtin_fmaf(k[1], bx, k[0] * ax); nbx = __builtin_fmaf(k[3], bx, k[2] * ax);
+                    nay = __builtin_fmaf(k[1], by, k[0] * ay); nby = __builtin_fmaf(k[3], by, k[2] * ay);
+                } else {
+                    const float2 na = cmul_add<float2, float>(k, mk<float2>(ax, ay), mk<float2>(bx, by));
+                    const float2 nb = cmul_add<float2, float>(k + 4, mk<float2>(ax, ay), mk<float2>(bx, by));
+                    nax = na.x; nay = na.y; nbx = nb.x; nby = nb.y;
+                }
+                ip_sel(ax, nax, m); ip_sel(ay, nay, m); ip_sel(bx, nbx, m); ip_sel(by, nby, m);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < E; r++) {
         if (r & (1 << B)) continue;
@@ -137,6 +239,36 @@ __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rc
 // the matrix (own coefficient first).
 template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, uint32_t rcm, bool tok) {
+    if constexpr (sizeof(R) == 4) {
+        const float nk1 = -k[1], nk3 = -k[3];
+        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            float &sx = v.d[2 * r], &sy = v.d[2 * r + 1];
+            const float px = xlane(addr, sx), py = xlane(addr, sy);
+            if constexpr (!CT) {
+                if constexpr (KIND == RO_X) { ip_set(sx, px); ip_set(sy, py); }
+                else if constexpr (KIND == RO_RR) { ip_fma(sx, k[0], k[1] * px); ip_fma(sy, k[0], k[1] * py); }
+                else {
+                    const float ux = __builtin_fmaf(nk3, py, k[2] * px), uy = __builtin_fmaf(k[3], px, k[2] * py);
+                    const float z = __builtin_fmaf(nk1, sy, ux), w = __builtin_fmaf(k[1], sx, uy);
+                    ip_fma(sy, k[0], w);
+                    ip_fma(sx, k[0], z);
+                }
+            } else {
+                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+                float nx, ny;
+                if constexpr (KIND == RO_X) { nx = px; ny = py; }
+                else if constexpr (KIND == RO_RR) { nx = __builtin_fmaf(k[1], px, k[0] * sx); ny = __builtin_fmaf(k[1], py, k[0] * sy); }
+                else {
+                    const float2 n = cmul_add<float2, float>(k, mk<float2>(sx, sy), mk<float2>(px, py));
+                    nx = n.x; ny = n.y;
+                }
+                ip_sel(sx, nx, m); ip_sel(sy, ny, m);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < E; r++) {
         const T s = v.get(r);
@@ -158,20 +290,20 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
         const bool bit = ((lane >> i) & 1u) != 0;
         R k[4];
         if constexpr (KIND == RO_RR) {  // row 0: (m00, m01); row 1: (m11, m10)
-            k[0] = R(bit ? op.c[3] : op.c[0]);
-            k[1] = R(bit ? op.c[2] : op.c[1]);
+            k[0] = bit ? coef<R>(op, 3) : coef<R>(op, 0);
+            k[1] = bit ? coef<R>(op, 2) : coef<R>(op, 1);
         } else if constexpr (KIND == RO_GEN) {
-            k[0] = R(bit ? op.c[6] : op.c[0]);
-            k[1] = R(bit ? op.c[7] : op.c[1]);
-            k[2] = R(bit ? op.c[4] : op.c[2]);
-            k[3] = R(bit ? op.c[5] : op.c[3]);
+            k[0] = bit ? coef<R>(op, 6) : coef<R>(op, 0);
+            k[1] = bit ? coef<R>(op, 7) : coef<R>(op, 1);
+            k[2] = bit ? coef<R>(op, 4) : coef<R>(op, 2);
+            k[3] = bit ? coef<R>(op, 5) : coef<R>(op, 3);
         }
         pairs_lane<KIND, CT, E, T, R>(v, addr, k, rcm, tok);
         return;
     }
     R k[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) k[q] = R(op.c[q]);
+    for (int q = 0; q < 8; q++) k[q] = coef<R>(op, q);
 #define QI_RB(BB) case BB: if constexpr (BB < RB) pairs_reg<BB, KIND, CT, E, T, R>(v, k, rcm, tok); break;
     switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
@@ -182,6 +314,29 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
 // compile time (RO_Z and RO_P1 leave the target-0 registers alone).
 template <int B, int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, uint32_t rcm, bool tok) {
+    if constexpr (sizeof(R) == 4) {
+        const float n0 = -p0.y, n1 = -p1.y;
+        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            const bool one = ((r >> B) & 1) != 0;
+            if (!one && KIND != RO_DG) continue;
+            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+            const T p = one ? p1 : p0;
+            if constexpr (!CT) {
+                if constexpr (KIND == RO_Z) { ip_neg(ax); ip_neg(ay); }
+                else ip_cmul(ax, ay, p.x, p.y, one ? n1 : n0);
+            } else {
+                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+                if constexpr (KIND == RO_Z) { ip_sel(ax, -ax, m); ip_sel(ay, -ay, m); }
+                else {
+                    const float2 n = cmul(mk<float2>(ax, ay), mk<float2>(p.x, p.y));
+                    ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < E; r++) {
         const bool one = ((r >> B) & 1) != 0;
@@ -197,6 +352,29 @@ __device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, uint32_t rcm
 // Target fixed for the whole thread (thread bit or outside the tile).
 template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t rcm, bool tok) {
+    if constexpr (sizeof(R) == 4) {
+        const uint32_t sgn = one ? 0x80000000u : 0u;  // Z: sign flip on the target-1 lanes
+        const float nfy = -f.y;
+        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+            if constexpr (!CT) {
+                if constexpr (KIND == RO_Z) { ip_xor(ax, sgn); ip_xor(ay, sgn); }
+                else ip_cmul(ax, ay, f.x, f.y, nfy);
+            } else {
+                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+                if constexpr (KIND == RO_Z) {
+                    ip_sel(ax, __uint_as_float(__float_as_uint(ax) ^ sgn), m);
+                    ip_sel(ay, __uint_as_float(__float_as_uint(ay) ^ sgn), m);
+                } else {
+                    const float2 n = cmul(mk<float2>(ax, ay), mk<float2>(f.x, f.y));
+                    ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
+                }
+            }
+        }
+        return;
+    }
     if constexpr (KIND == RO_Z && !CT) {
         // sign flip as an XOR of the sign bits
         const uint32_t m = one ? 0x80000000u : 0u;
@@ -224,7 +402,7 @@ __device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t 
 
 template <int KIND, bool CT, int RB, int E, typename T, typename R>
 __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t tl, uint64_t base, uint32_t rcm, bool tok) {
-    const T p0 = mk<T>(R(op.c[0]), R(op.c[1])), p1 = mk<T>(R(op.c[2]), R(op.c[3]));
+    const T p0 = mk<T>(coef<R>(op, 0), coef<R>(op, 1)), p1 = mk<T>(coef<R>(op, 2), coef<R>(op, 3));
     if (op.dloc == 0) {
 #define QI_RB(BB) case BB: if constexpr (BB < RB) diag_reg<BB, KIND, CT, E, T, R>(v, p0, p1, rcm, tok); break;
         switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
@@ -244,8 +422,9 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
     constexpr int N = 1 << K;
     const uint32_t lcm = op.lcm, b0 = 1u << op.lp0, b1 = 1u << op.lp1;
     const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
-    const R c0 = R(op.c[0]), c1 = R(op.c[1]);
-    const T m23 = mk<T>(R(op.c[2]), R(op.c[3])), m45 = mk<T>(R(op.c[4]), R(op.c[5])), m67 = mk<T>(R(op.c[6]), R(op.c[7]));
+    const R c0 = coef<R>(op, 0), c1 = coef<R>(op, 1);
+    const T m23 = mk<T>(coef<R>(op, 2), coef<R>(op, 3)), m45 = mk<T>(coef<R>(op, 4), coef<R>(op, 5)),
+            m67 = mk<T>(coef<R>(op, 6), coef<R>(op, 7));
     for (uint32_t q = threadIdx.x; q < N / 4; q += NT) {
         const uint32_t l = ins0(ins0(q, lo), hi);
         if (op.lkind == LK_SWAP) {

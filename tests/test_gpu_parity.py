@@ -225,6 +225,32 @@ def test_tiled_handles_controls_diagonals_and_matchgates(ctx):
         assert np.max(np.abs(s.state_vector - ref)) <= 1e-12
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_tiled_walks_many_tiles_per_workgroup(ctx, prec):
+    """At 23 qubits each workgroup of a tile pass walks several tiles (the
+    next tile's loads in flight while the current one is computed), with
+    controlled SWAPs, matchgates and relabeling SWAPs forcing permuted stores;
+    every tile width and thread count against the streaming kernels."""
+    rng = np.random.default_rng(23)
+    n = 23
+    kinds = ["H", "RX", "RY", "U2", "RZ", "P", "Z", "X", "CNOT", "TOFFOLI", "SWAP", "MATCHGATE", "Y"]
+    gates = gates_array([random_gate(n, rng, kinds=kinds, max_controls=2) for _ in range(160)])
+    psi = random_state(n, rng)
+    s = upload(psi, prec)
+    s.apply_gates_inplace(gates, mode=L.EXEC_PER_GATE)
+    ref = s.state_vector
+    kmax = 12 if prec == "c128" else 13
+    for K in range(10, kmax + 1):
+        for tb in (8, 9):
+            s = upload(psi, prec)
+            s.apply_gates_inplace(gates, mode=L.EXEC_TILED, tile_qubits=K, tile_threads_log2=tb)
+            got = s.state_vector
+            if prec == "c128":
+                assert np.max(np.abs(got - ref)) <= 1e-12, (K, tb)
+            else:
+                assert fidelity(got, ref) >= 1 - 1e-9 and np.max(np.abs(got - ref)) <= 1e-5, (K, tb)
+
+
 def test_circuit_with_measurement_and_pauli_gates(ctx):
     """Gate::Measurement / PauliString / PauliTimeEvolution inside Circuit::execute."""
     n = 12
