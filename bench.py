@@ -376,9 +376,9 @@ def main():
         return
 
     tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)
-    # the library's complex64 defaults: 4096-amplitude tiles, 256 threads (512 from 8192 amplitudes)
+    # the library's complex64 defaults: 4096-amplitude tiles on one wave (512 threads for 8192 amplitudes)
     tile_k = args.tile_qubits or 12
-    tile_tb = 9 if tile_k >= 13 else 8
+    tile_tb = 9 if tile_k >= 13 else 6
     # ---- headline: tiled executor, complex64
     wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c64", gates, tiled, args.steps, args.warmup)
     ms_step = wall / args.steps * 1e3

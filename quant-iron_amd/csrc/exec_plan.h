@@ -28,12 +28,13 @@ struct TilePass {
 };
 
 // ---- register-resident tile passes (single-state executor) ----
-// 2^TB threads (TB = 8 or 9) hold a tile of 2^K amplitudes, 2^(K-TB) per
+// 2^TB threads (TB = 6, 8 or 9) hold a tile of 2^K amplitudes, 2^(K-TB) per
 // thread in registers. A layout assigns each of the K local index bits to a
 // thread bit (lane bits 0..5, wave bits 6..TB-1) or a register bit. A 2x2
 // gate acts on a target held in registers (pairs inside a thread) or on lanes
 // (the partner comes from another lane of the wave); wave bits are never
-// targets.
+// targets. With TB = 6 a tile is one wave: no wave bits, no LDS, and layout
+// changes are lane/register exchanges (RO_XLR, RO_XLL).
 constexpr int kMaxThreadBits = 10;
 constexpr int kLaneBits = 6;
 constexpr int kLaneTarget = 8;   // RegOp::tgt >= kLaneTarget: lane bit tgt - kLaneTarget
@@ -45,6 +46,8 @@ enum RegOpKind : int32_t {
     RO_Z,              // -1 where the target bit is 1
     RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (RZ)
     RO_P1,             // p1 = (c2, c3) where the target bit is 1, nothing where it is 0 (S, T, P)
+    RO_XLR,            // exchange the local bits of lane bit tgt and register bit dbit
+    RO_XLL,            // exchange the local bits of lane bits tgt and dbit
     RO_TRANSPOSE,      // new layout through LDS
     RO_LDS,            // gate on the LDS copy (controlled SWAP, Matchgate)
     RO_NOP             // closes each pass's slice (read ahead, never run)

@@ -29,10 +29,13 @@ int tile_low_qubits(int prec) { return prec == QI_C128 ? 3 : 4; }  // 128-byte r
 
 int default_tile_qubits(const State &s) { return s.prec == QI_C128 ? 11 : 12; }
 
-// 512 threads for the 64 KiB tiles (twice the waves per CU at the same LDS
-// use), 256 below.  The 30-qubit random circuit runs fastest with 32 KiB
-// tiles on 256 threads: 4096 complex64 or 2048 complex128 amplitudes.
-int default_tile_threads_log2(int prec, int K) { return K >= 12 + (prec == QI_C128 ? 0 : 1) ? 9 : 8; }
+// complex64 tiles up to 4096 amplitudes run as one wave (no LDS, no
+// barriers, 64 amplitudes per lane); larger and complex128 tiles use 256
+// threads, 512 for the 64 KiB tiles (twice the waves per CU at the same LDS).
+int default_tile_threads_log2(int prec, int K) {
+    if (prec == QI_C64 && K <= 12) return 6;
+    return K >= 12 + (prec == QI_C128 ? 0 : 1) ? 9 : 8;
+}
 
 namespace {
 
@@ -60,14 +63,14 @@ struct Blocked {
 };
 
 // Members of the next pass (list order) and the tile's qubit set.
-uint64_t select_pass(const std::vector<GateSets> &gs, std::vector<char> &done, size_t first, int K, int n, int low,
-                     std::vector<size_t> &members) {
+uint64_t select_pass(const std::vector<GateSets> &gs, const std::vector<char> &streamed, std::vector<char> &done,
+                     size_t first, int K, int n, int low, std::vector<size_t> &members) {
     const uint64_t all = n >= 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
     uint64_t active = (uint64_t(1) << low) - 1;
     Blocked blk;
     for (size_t g = first; g < gs.size(); g++) {
         if (done[g]) continue;
-        if (blk.blocks(gs[g])) blk.add(gs[g]);
+        if (streamed[g] || blk.blocks(gs[g])) blk.add(gs[g]);
         else if (__builtin_popcountll(active | gs[g].need) <= K) {
             active |= gs[g].need;
             members.push_back(g);
@@ -260,9 +263,12 @@ struct Phase {
 Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
     if (K <= 0) K = default_tile_qubits(s);
     if (TB <= 0) TB = default_tile_threads_log2(s.prec, K);
-    if (TB < 8 || TB > 9) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile threads out of range");
+    if (TB != 6 && TB != 8 && TB != 9) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile threads out of range");
     const int kmax = s.prec == QI_C128 ? 12 : 13;
     if (K < 10 || K > kmax) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width out of range");
+    // one-wave tiles keep up to 64 amplitudes per lane in registers: complex64, K <= 12
+    if (TB == 6 && (s.prec != QI_C64 || K > 12))
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "one-wave tiles need complex64 and at most 12 tile qubits");
     if ((int)s.n < K) {  // state smaller than a tile: stream gate by gate
         for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
         return Status{};
@@ -271,10 +277,22 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
     std::vector<GateSets> gs(gates.size());
     for (size_t g = 0; g < gates.size(); g++) gs[g] = gate_sets(gates[g]);
     std::vector<char> done(gates.size(), 0);
+    // One-wave tiles have no LDS: controlled SWAPs and matchgates run as
+    // streaming steps between passes (later gates pass them only if they commute).
+    std::vector<char> streamed(gates.size(), 0);
+    if (TB == 6)
+        for (size_t g = 0; g < gates.size(); g++)
+            streamed[g] = (gates[g].kind == LK_SWAP || gates[g].kind == LK_MATCH) && !is_relabel(gates[g]);
     size_t first = 0;
     while (first < gates.size()) {
+        if (streamed[first]) {
+            plan.steps.push_back(ExecStep::gate(gates[first]));
+            done[first] = 1;
+            while (first < gates.size() && done[first]) first++;
+            continue;
+        }
         std::vector<size_t> members;
-        const uint64_t q = select_pass(gs, done, first, K, (int)s.n, low, members);
+        const uint64_t q = select_pass(gs, streamed, done, first, K, (int)s.n, low, members);
         RegPlanner P;
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
@@ -368,6 +386,18 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
             }
         }
         // ---- store: row qubits back on lanes 0..L-1 (relabels may have moved them)
+        if (TB == kLaneBits) {  // one-wave tiles: lane/register exchanges, lane k in turn
+            for (int k = 0; k < low; k++) {
+                const int lb = P.lbit[k];
+                if (P.ct[k] == lb) continue;
+                RegOp op{};
+                const int i2 = P.lane_index(lb), j = P.reg_index(lb);
+                op.tgt = k;
+                if (i2 >= 0) { op.kind = RO_XLL; op.dbit = i2; std::swap(P.ct[k], P.ct[i2]); }
+                else { op.kind = RO_XLR; op.dbit = j; std::swap(P.ct[k], P.cr[j]); }
+                P.out.push_back(op);
+            }
+        }
         if (!P.io_ok()) {
             int zero[16] = {0}, far[16];
             for (int b = 0; b < 16; b++) far[b] = 1 << 30;
@@ -407,10 +437,11 @@ Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K,
     std::vector<GateSets> gs(gates.size());
     for (size_t g = 0; g < gates.size(); g++) gs[g] = gate_sets(gates[g]);
     std::vector<char> done(gates.size(), 0);
+    const std::vector<char> none(gates.size(), 0);  // every gate can join an LDS pass
     size_t first = 0;
     while (first < gates.size()) {
         std::vector<size_t> members;
-        const uint64_t q = select_pass(gs, done, first, K, (int)s.n, tile_low_qubits(s.prec), members);
+        const uint64_t q = select_pass(gs, none, done, first, K, (int)s.n, tile_low_qubits(s.prec), members);
         TilePass p{};
         p.qmask = q;
         p.K = K;

@@ -1,16 +1,18 @@
 // tiled.hip — register-resident execution of a tile pass (planned in
 // tile_plan.hip).
 //
-// 256 threads hold a tile of 2^K amplitudes, 2^(K-8) per thread in
-// registers. A layout assigns the K local index bits to 8 thread bits (6 lane
-// bits, 2 wave bits) and K-8 register bits. A 2x2 gate acts on a target held
-// in registers (the pair lives in one thread) or on lanes (each lane fetches
-// its partner with ds_bpermute, the LDS crossbar without LDS memory, and
-// computes its half of the pair). Controls and diagonal targets on register
-// bits are compile-time facts of each register, so untouched registers cost
-// nothing; on thread bits they are a per-thread select. When a target sits
-// on a wave bit, the tile is transposed to a new layout through LDS.
-// Controlled SWAPs and Matchgates run on the LDS copy of the tile.
+// 2^TB threads hold a tile of 2^K amplitudes, 2^(K-TB) per thread in
+// registers. A layout assigns the K local index bits to TB thread bits (6
+// lane bits, TB-6 wave bits) and K-TB register bits. A 2x2 gate acts on a
+// target held in registers (the pair lives in one thread) or on lanes (each
+// lane fetches its partner with ds_bpermute, the LDS crossbar without LDS
+// memory, and computes its half of the pair). Controls and diagonal targets
+// on register bits are compile-time facts of each register; on thread bits
+// they are a per-lane mask. With wave bits (TB = 8, 9), a target on a wave bit
+// takes a transpose of the tile through LDS, and controlled SWAPs and
+// Matchgates run on the LDS copy. A one-wave tile (TB = 6: complex64, up to
+// 4096 amplitudes, 64 per lane) has no wave bits: no LDS and no barriers;
+// its only layout changes are lane/register exchanges at the end of a pass.
 //
 // Arithmetic: complex128 keeps each product of the reference's formulas (no
 // contraction); complex64 uses fused multiply-adds.
@@ -413,6 +415,43 @@ __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t
     diag_thr<KIND, CT, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
+// ------------------------------------------------------- exchange ops --
+// One-wave tiles change layout without LDS. Lane bit i <-> register bit B:
+// the amplitude at (lane bit i = a, register bit B = b) moves to
+// (lane bit i = b, register bit B = a); lanes with lane bit 1 send their
+// register-bit-0 value, lanes with lane bit 0 their register-bit-1 value.
+template <int B, int E>
+__device__ __forceinline__ void xlr_reg(Regs<float, E> &v, uint32_t lane, int i) {
+    const int addr = int((lane ^ (1u << i)) << 2);
+    const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0), lo = ~hi;
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if (r & (1 << B)) continue;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float &a = v.d[2 * r + c], &b = v.d[2 * (r | (1 << B)) + c];
+            const float y = xlane(addr, selm(hi, a, b));
+            ip_sel(b, y, lo);
+            ip_sel(a, y, hi);
+        }
+    }
+}
+template <int E>
+__device__ __forceinline__ void xlr_op(Regs<float, E> &v, uint32_t lane, int i, int j) {
+    constexpr int RB = __builtin_ctz(E);
+#define QI_RB(BB) case BB: if constexpr (BB < RB) xlr_reg<BB, E>(v, lane, i); break;
+    switch (j) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+// Lane bits i <-> i2: lanes whose two bits differ trade with lane ^ (2^i | 2^i2).
+template <int E>
+__device__ __forceinline__ void xll_op(Regs<float, E> &v, uint32_t lane, int i, int i2) {
+    const int addr = int((lane ^ ((1u << i) | (1u << i2))) << 2);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != ((lane >> i2) & 1u));
+#pragma unroll
+    for (int k = 0; k < 2 * E; k++) ip_sel(v.d[k], xlane(addr, v.d[k]), m);
+}
+
 // ------------------------------------------------------------- LDS ops --
 // Controlled SWAP or Matchgate (CPU form, operator.rs:993-1004) on the LDS
 // copy; local index e lives at slot swz(e).
@@ -451,7 +490,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
                                                       const RegOp *__restrict__ ops) {
     using T = typename V2<R>::T;
     constexpr int RB = K - TB, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
-    __shared__ T lds[1 << K];
+    __shared__ T lds[TB > kLaneBits ? (1 << K) : 1];  // one-wave tiles use no LDS
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
 
@@ -509,11 +548,22 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
 #define ARGS tl, base
                 QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
 #undef ARGS
+            case RO_XLR:
+                if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
+                break;
+            case RO_XLL:
+                if constexpr (TB == kLaneBits && sizeof(R) == 4) xll_op<E>(v, lane, op.tgt, op.dbit);
+                break;
             default: break;
             }
 #undef QI_OP
         }
         if (o >= pass.nops) break;
+        if constexpr (TB == kLaneBits) {  // no LDS ops in one-wave passes (the planner streams them)
+            nxt = ops[pass.op0 + o + 1];
+            o++;
+            continue;
+        }
         // through LDS: a transpose to a new layout, or a gate on the LDS copy
         const RegOp op = nxt;
         nxt = ops[pass.op0 + o + 1];
@@ -589,6 +639,9 @@ template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p
 
 template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     switch (p.tb) {
+    case 6:
+        if constexpr (sizeof(R) == 4 && K <= 12) { launch_pass<R, K, 6>(s, p, ops, st); return true; }
+        return false;
     case 8: launch_pass<R, K, 8>(s, p, ops, st); return true;
     case 9: launch_pass<R, K, 9>(s, p, ops, st); return true;
     default: return false;

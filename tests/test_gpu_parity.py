@@ -226,11 +226,11 @@ def test_tiled_handles_controls_diagonals_and_matchgates(ctx):
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_tiled_walks_many_tiles_per_workgroup(ctx, prec):
-    """At 23 qubits each workgroup of a tile pass walks several tiles (the
-    next tile's loads in flight while the current one is computed), with
-    controlled SWAPs, matchgates and relabeling SWAPs forcing permuted stores;
-    every tile width and thread count against the streaming kernels."""
+def test_tiled_every_width_and_thread_count(ctx, prec):
+    """A 23-qubit state with controlled SWAPs, matchgates (streamed between
+    one-wave passes, LDS operations otherwise) and relabeling SWAPs that force
+    permuted stores and lane/register exchanges; every tile width and thread
+    count (one-wave tiles included) against the streaming kernels."""
     rng = np.random.default_rng(23)
     n = 23
     kinds = ["H", "RX", "RY", "U2", "RZ", "P", "Z", "X", "CNOT", "TOFFOLI", "SWAP", "MATCHGATE", "Y"]
@@ -241,7 +241,7 @@ def test_tiled_walks_many_tiles_per_workgroup(ctx, prec):
     ref = s.state_vector
     kmax = 12 if prec == "c128" else 13
     for K in range(10, kmax + 1):
-        for tb in (8, 9):
+        for tb in ((6, 8, 9) if prec == "c64" and K <= 12 else (8, 9)):
             s = upload(psi, prec)
             s.apply_gates_inplace(gates, mode=L.EXEC_TILED, tile_qubits=K, tile_threads_log2=tb)
             got = s.state_vector
