@@ -42,7 +42,9 @@ constexpr int kLaneTarget = 8;   // RegOp::tgt >= kLaneTarget: lane bit tgt - kL
 enum RegOpKind : int32_t {
     RO_X = 0,          // swap the pair
     RO_RR,             // real 2x2 [[c0, c1], [c2, c3]] (H, RY)
-    RO_GEN,            // complex 2x2, row-major (c0,c1) (c2,c3) / (c4,c5) (c6,c7) (Y, RX, U2)
+    RO_GEN,            // complex 2x2, row-major (c0,c1) (c2,c3) / (c4,c5) (c6,c7) (U2)
+    RO_RXI,            // [[c0, i c1], [i c2, c3]] with real c (RX)
+    RO_Y,              // [[0, -i], [i, 0]]
     RO_Z,              // -1 where the target bit is 1
     RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (RZ)
     RO_P1,             // p1 = (c2, c3) where the target bit is 1, nothing where it is 0 (S, T, P)

@@ -231,11 +231,17 @@ struct RegPlanner {
                 op.kind = RO_RR; op.c[0] = G.c[0]; op.c[1] = G.c[0]; op.c[2] = G.c[0]; op.c[3] = -G.c[0]; break;
             case LK_RY:  // [[c, -s], [s, c]]
                 op.kind = RO_RR; op.c[0] = G.c[0]; op.c[1] = -G.c[1]; op.c[2] = G.c[1]; op.c[3] = G.c[0]; break;
-            case LK_Y:  // [[0, (c0,c1)], [(c2,c3), 0]]
+            case LK_Y:  // [[0, (c0,c1)], [(c2,c3), 0]]; as lowered: [[0, -i], [i, 0]]
+                if (G.c[0] == 0 && G.c[1] == -1 && G.c[2] == 0 && G.c[3] == 1) { op.kind = RO_Y; break; }
                 op.kind = RO_GEN;
                 op.c[2] = G.c[0]; op.c[3] = G.c[1]; op.c[4] = G.c[2]; op.c[5] = G.c[3];
                 break;
-            case LK_RX:  // [[c0, -(c1,c2)], [(c3,c4), c0]]
+            case LK_RX:  // [[c0, -(c1,c2)], [(c3,c4), c0]]; as lowered [[cos, -i sin], [-i sin, cos]]
+                if (G.c[1] == 0 && G.c[3] == 0) {
+                    op.kind = RO_RXI;
+                    op.c[0] = G.c[0]; op.c[1] = -G.c[2]; op.c[2] = G.c[4]; op.c[3] = G.c[0];
+                    break;
+                }
                 op.kind = RO_GEN;
                 op.c[0] = G.c[0]; op.c[2] = -G.c[1]; op.c[3] = -G.c[2]; op.c[4] = G.c[3]; op.c[5] = G.c[4];
                 op.c[6] = G.c[0];

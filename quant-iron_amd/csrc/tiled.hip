@@ -141,6 +141,9 @@ __device__ __forceinline__ float selm(uint64_t mask, float x, float y) {
     return r;
 }
 __device__ __forceinline__ void ip_xor(float &d, uint32_t m) { asm("v_xor_b32 %0, %1, %0" : "+v"(d) : "v"(m)); }
+__device__ __forceinline__ void ip_xor3(float &d, float x, uint32_t m) {  // d = x ^ m (old d dead)
+    asm("v_xor_b32 %0, %1, %2" : "+v"(d) : "v"(m), "v"(x));
+}
 
 // (a, b) <- (m0 a + m1 b, m2 a + m3 b), real m, one component
 __device__ __forceinline__ void ip_rr(float &a, float &b, const float *k) {
@@ -159,6 +162,14 @@ __device__ __forceinline__ void ip_gen(float &ax, float &ay, float &bx, float &b
     const float z2 = __builtin_fmaf(nk[7], by, r), z = __builtin_fmaf(k[6], by, s);
     ip_fma3(by, k[7], bx, z);
     ip_fma(bx, k[6], z2);
+}
+// (a, b) <- (c0 a + i c1 b, i c2 a + c3 b), real c; nk = -k
+__device__ __forceinline__ void ip_rxi(float &ax, float &ay, float &bx, float &by, const float *k, const float *nk) {
+    const float t1 = nk[1] * by, t2 = k[1] * bx, t3 = nk[2] * ay, t4 = k[2] * ax;
+    ip_fma(ax, k[0], t1);
+    ip_fma(ay, k[0], t2);
+    ip_fma(bx, k[3], t3);
+    ip_fma(by, k[3], t4);
 }
 // a <- (px, py) a
 __device__ __forceinline__ void ip_cmul(float &ax, float &ay, float px, float py, float npy) {
@@ -194,7 +205,11 @@ __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rc
             if constexpr (!CT) {
                 if constexpr (KIND == RO_X) { ip_swap(ax, bx); ip_swap(ay, by); }
                 else if constexpr (KIND == RO_RR) { ip_rr(ax, bx, k); ip_rr(ay, by, k); }
-                else ip_gen(ax, ay, bx, by, k, nk);
+                else if constexpr (KIND == RO_RXI) ip_rxi(ax, ay, bx, by, k, nk);
+                else if constexpr (KIND == RO_Y) {  // (a, b) <- (-i b, i a)
+                    ip_swap(ax, by); ip_swap(ay, bx);
+                    ip_neg(ay); ip_neg(bx);
+                } else ip_gen(ax, ay, bx, by, k, nk);
             } else {
                 const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;  // uniform per register
                 if constexpr (KIND == RO_X) {  // controlled swap: new b first, then a and b in place
@@ -203,10 +218,18 @@ __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rc
                     ip_set(bx, tx); ip_set(by, ty);
                     continue;
                 }
+                if constexpr (KIND == RO_Y) {  // all four new values first, then in place
+                    const float n0 = selm(m, by, ax), n1 = selm(m, -bx, ay), n2 = selm(m, -ay, bx), n3 = selm(m, ax, by);
+                    ip_set(ax, n0); ip_set(ay, n1); ip_set(bx, n2); ip_set(by, n3);
+                    continue;
+                }
                 float nax, nay, nbx, nby;
                 if constexpr (KIND == RO_RR) {
                     nax = __builtin_fmaf(k[1], bx, k[0] * ax); nbx = __builtin_fmaf(k[3], bx, k[2] * ax);
                     nay = __builtin_fmaf(k[1], by, k[0] * ay); nby = __builtin_fmaf(k[3], by, k[2] * ay);
+                } else if constexpr (KIND == RO_RXI) {
+                    nax = __builtin_fmaf(nk[1], by, k[0] * ax); nay = __builtin_fmaf(k[1], bx, k[0] * ay);
+                    nbx = __builtin_fmaf(nk[2], ay, k[3] * bx); nby = __builtin_fmaf(k[2], ax, k[3] * by);
                 } else {
                     const float2 na = cmul_add<float2, float>(k, mk<float2>(ax, ay), mk<float2>(bx, by));
                     const float2 nb = cmul_add<float2, float>(k + 4, mk<float2>(ax, ay), mk<float2>(bx, by));
@@ -224,6 +247,10 @@ __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rc
         T na, nb;
         if constexpr (KIND == RO_X) { na = b; nb = a; }
         else if constexpr (KIND == RO_RR) { na = rmul_add<T, R>(k[0], a, k[1], b); nb = rmul_add<T, R>(k[2], a, k[3], b); }
+        else if constexpr (KIND == RO_RXI) {
+            na = mk<T>(k[0] * a.x - k[1] * b.y, k[0] * a.y + k[1] * b.x);
+            nb = mk<T>(k[3] * b.x - k[2] * a.y, k[3] * b.y + k[2] * a.x);
+        } else if constexpr (KIND == RO_Y) { na = mk<T>(b.y, -b.x); nb = mk<T>(-a.y, a.x); }
         else { na = cmul_add<T, R>(k, a, b); nb = cmul_add<T, R>(k + 4, a, b); }
         if constexpr (CT) {
             const bool on = tok && ((uint32_t(r) & rcm) == rcm);
@@ -244,29 +271,48 @@ __device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, 
     if constexpr (sizeof(R) == 4) {
         const float nk1 = -k[1], nk3 = -k[3];
         const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+        // partners fetched a group of registers ahead: the exchange latency
+        // overlaps the arithmetic of the previous registers
+        constexpr int G = E < 4 ? E : 4;
 #pragma unroll
-        for (int r = 0; r < E; r++) {
-            float &sx = v.d[2 * r], &sy = v.d[2 * r + 1];
-            const float px = xlane(addr, sx), py = xlane(addr, sy);
-            if constexpr (!CT) {
-                if constexpr (KIND == RO_X) { ip_set(sx, px); ip_set(sy, py); }
-                else if constexpr (KIND == RO_RR) { ip_fma(sx, k[0], k[1] * px); ip_fma(sy, k[0], k[1] * py); }
-                else {
-                    const float ux = __builtin_fmaf(nk3, py, k[2] * px), uy = __builtin_fmaf(k[3], px, k[2] * py);
-                    const float z = __builtin_fmaf(nk1, sy, ux), w = __builtin_fmaf(k[1], sx, uy);
-                    ip_fma(sy, k[0], w);
-                    ip_fma(sx, k[0], z);
+        for (int r0 = 0; r0 < E; r0 += G) {
+            float px[G], py[G];
+#pragma unroll
+            for (int g = 0; g < G; g++) { px[g] = xlane(addr, v.d[2 * (r0 + g)]); py[g] = xlane(addr, v.d[2 * (r0 + g) + 1]); }
+#pragma unroll
+            for (int g = 0; g < G; g++) {
+                const int r = r0 + g;
+                float &sx = v.d[2 * r], &sy = v.d[2 * r + 1];
+                if constexpr (!CT) {
+                    if constexpr (KIND == RO_X) { ip_set(sx, px[g]); ip_set(sy, py[g]); }
+                    else if constexpr (KIND == RO_RR) { ip_fma(sx, k[0], k[1] * px[g]); ip_fma(sy, k[0], k[1] * py[g]); }
+                    else if constexpr (KIND == RO_RXI) { ip_fma(sx, k[0], nk1 * py[g]); ip_fma(sy, k[0], k[1] * px[g]); }
+                    else if constexpr (KIND == RO_Y) {  // own = (p.y, -p.x) on target-0 lanes, (-p.y, p.x) on target-1
+                        ip_xor3(sx, py[g], __float_as_uint(k[0]));
+                        ip_xor3(sy, px[g], __float_as_uint(k[1]));
+                    } else {
+                        const float ux = __builtin_fmaf(nk3, py[g], k[2] * px[g]), uy = __builtin_fmaf(k[3], px[g], k[2] * py[g]);
+                        const float z = __builtin_fmaf(nk1, sy, ux), w = __builtin_fmaf(k[1], sx, uy);
+                        ip_fma(sy, k[0], w);
+                        ip_fma(sx, k[0], z);
+                    }
+                } else {
+                    const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+                    float nx, ny;
+                    if constexpr (KIND == RO_X) { nx = px[g]; ny = py[g]; }
+                    else if constexpr (KIND == RO_RR) {
+                        nx = __builtin_fmaf(k[1], px[g], k[0] * sx); ny = __builtin_fmaf(k[1], py[g], k[0] * sy);
+                    } else if constexpr (KIND == RO_RXI) {
+                        nx = __builtin_fmaf(nk1, py[g], k[0] * sx); ny = __builtin_fmaf(k[1], px[g], k[0] * sy);
+                    } else if constexpr (KIND == RO_Y) {
+                        nx = __uint_as_float(__float_as_uint(py[g]) ^ __float_as_uint(k[0]));
+                        ny = __uint_as_float(__float_as_uint(px[g]) ^ __float_as_uint(k[1]));
+                    } else {
+                        const float2 n = cmul_add<float2, float>(k, mk<float2>(sx, sy), mk<float2>(px[g], py[g]));
+                        nx = n.x; ny = n.y;
+                    }
+                    ip_sel(sx, nx, m); ip_sel(sy, ny, m);
                 }
-            } else {
-                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
-                float nx, ny;
-                if constexpr (KIND == RO_X) { nx = px; ny = py; }
-                else if constexpr (KIND == RO_RR) { nx = __builtin_fmaf(k[1], px, k[0] * sx); ny = __builtin_fmaf(k[1], py, k[0] * sy); }
-                else {
-                    const float2 n = cmul_add<float2, float>(k, mk<float2>(sx, sy), mk<float2>(px, py));
-                    nx = n.x; ny = n.y;
-                }
-                ip_sel(sx, nx, m); ip_sel(sy, ny, m);
             }
         }
         return;
@@ -278,6 +324,8 @@ __device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, 
         T n;
         if constexpr (KIND == RO_X) n = p;
         else if constexpr (KIND == RO_RR) n = rmul_add<T, R>(k[0], s, k[1], p);
+        else if constexpr (KIND == RO_RXI) n = mk<T>(k[0] * s.x - k[1] * p.y, k[0] * s.y + k[1] * p.x);
+        else if constexpr (KIND == RO_Y) n = k[0] > 0 ? mk<T>(p.y, -p.x) : mk<T>(-p.y, p.x);
         else n = cmul_add<T, R>(k, s, p);
         if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, s);
         v.set(r, n);
@@ -290,10 +338,17 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
         const int i = op.tgt - kLaneTarget;
         const int addr = int((lane ^ (1u << i)) << 2);
         const bool bit = ((lane >> i) & 1u) != 0;
-        R k[4];
-        if constexpr (KIND == RO_RR) {  // row 0: (m00, m01); row 1: (m11, m10)
+        R k[4] = {R(0), R(0), R(0), R(0)};
+        if constexpr (KIND == RO_RR || KIND == RO_RXI) {  // row 0: (m00, m01); row 1: (m11, m10)
             k[0] = bit ? coef<R>(op, 3) : coef<R>(op, 0);
             k[1] = bit ? coef<R>(op, 2) : coef<R>(op, 1);
+        } else if constexpr (KIND == RO_Y) {
+            if constexpr (sizeof(R) == 4) {  // sign masks for the two components
+                k[0] = __uint_as_float(bit ? 0x80000000u : 0u);
+                k[1] = __uint_as_float(bit ? 0u : 0x80000000u);
+            } else {
+                k[0] = bit ? R(-1) : R(1);
+            }
         } else if constexpr (KIND == RO_GEN) {
             k[0] = bit ? coef<R>(op, 6) : coef<R>(op, 0);
             k[1] = bit ? coef<R>(op, 7) : coef<R>(op, 1);
@@ -424,16 +479,31 @@ template <int B, int E>
 __device__ __forceinline__ void xlr_reg(Regs<float, E> &v, uint32_t lane, int i) {
     const int addr = int((lane ^ (1u << i)) << 2);
     const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0), lo = ~hi;
+    constexpr int G = 4;  // exchanges issued a group ahead of their use
+    int rs[E / 2];
+    {
+        int n = 0;
 #pragma unroll
-    for (int r = 0; r < E; r++) {
-        if (r & (1 << B)) continue;
+        for (int r = 0; r < E; r++) if (!(r & (1 << B))) rs[n++] = r;
+    }
 #pragma unroll
-        for (int c = 0; c < 2; c++) {
-            float &a = v.d[2 * r + c], &b = v.d[2 * (r | (1 << B)) + c];
-            const float y = xlane(addr, selm(hi, a, b));
-            ip_sel(b, y, lo);
-            ip_sel(a, y, hi);
-        }
+    for (int g0 = 0; g0 < E / 2; g0 += G) {
+        float y[G][2];
+#pragma unroll
+        for (int g = 0; g < G && g0 + g < E / 2; g++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int r = rs[g0 + g];
+                y[g][c] = xlane(addr, selm(hi, v.d[2 * r + c], v.d[2 * (r | (1 << B)) + c]));
+            }
+#pragma unroll
+        for (int g = 0; g < G && g0 + g < E / 2; g++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int r = rs[g0 + g];
+                ip_sel(v.d[2 * (r | (1 << B)) + c], y[g][c], lo);
+                ip_sel(v.d[2 * r + c], y[g][c], hi);
+            }
     }
 }
 template <int E>
@@ -448,8 +518,15 @@ template <int E>
 __device__ __forceinline__ void xll_op(Regs<float, E> &v, uint32_t lane, int i, int i2) {
     const int addr = int((lane ^ ((1u << i) | (1u << i2))) << 2);
     const uint64_t m = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != ((lane >> i2) & 1u));
+    constexpr int G = 8;
 #pragma unroll
-    for (int k = 0; k < 2 * E; k++) ip_sel(v.d[k], xlane(addr, v.d[k]), m);
+    for (int k0 = 0; k0 < 2 * E; k0 += G) {
+        float y[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) y[g] = xlane(addr, v.d[k0 + g]);
+#pragma unroll
+        for (int g = 0; g < G; g++) ip_sel(v.d[k0 + g], y[g], m);
+    }
 }
 
 // ------------------------------------------------------------- LDS ops --
@@ -485,12 +562,43 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
 }
 
 // ------------------------------------------------------------ the pass --
+// One in-register op on the thread's share of the tile.
+template <typename R, int K, int TB>
+__device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegOp &op, uint64_t base, uint32_t tl,
+                                         uint32_t lane) {
+    using T = typename V2<R>::T;
+    constexpr int RB = K - TB, E = 1 << RB;
+    const bool tc = (op.tcm | op.rcm) != 0;   // controls inside the tile (uniform)
+    const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
+#define QI_OP(KD, FN)                                                                     \
+    case KD:                                                                              \
+        if (tc) FN<KD, true, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                      \
+        else FN<KD, false, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                        \
+        break;
+#define ARGS lane
+    switch (op.kind) {
+        QI_OP(RO_X, pair_op) QI_OP(RO_RR, pair_op) QI_OP(RO_GEN, pair_op) QI_OP(RO_RXI, pair_op) QI_OP(RO_Y, pair_op)
+#undef ARGS
+#define ARGS tl, base
+        QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
+#undef ARGS
+    case RO_XLR:
+        if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
+        break;
+    case RO_XLL:
+        if constexpr (TB == kLaneBits && sizeof(R) == 4) xll_op<E>(v, lane, op.tgt, op.dbit);
+        break;
+    default: break;
+    }
+#undef QI_OP
+}
+
 template <typename R, int K, int TB>
 __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
                                                       const RegOp *__restrict__ ops) {
     using T = typename V2<R>::T;
     constexpr int RB = K - TB, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
-    __shared__ T lds[TB > kLaneBits ? (1 << K) : 1];  // one-wave tiles use no LDS
+    __shared__ T lds[1 << K];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
 
@@ -534,36 +642,9 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
             if (kind >= RO_TRANSPOSE) break;
             nxt = ops[pass.op0 + o + 1];
             if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-            const bool tc = (op.tcm | op.rcm) != 0;   // controls inside the tile (uniform)
-            const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
-#define QI_OP(KD, FN)                                                                     \
-    case KD:                                                                              \
-        if (tc) FN<KD, true, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                      \
-        else FN<KD, false, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                        \
-        break;
-#define ARGS lane
-            switch (kind) {
-                QI_OP(RO_X, pair_op) QI_OP(RO_RR, pair_op) QI_OP(RO_GEN, pair_op)
-#undef ARGS
-#define ARGS tl, base
-                QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
-#undef ARGS
-            case RO_XLR:
-                if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
-                break;
-            case RO_XLL:
-                if constexpr (TB == kLaneBits && sizeof(R) == 4) xll_op<E>(v, lane, op.tgt, op.dbit);
-                break;
-            default: break;
-            }
-#undef QI_OP
+            apply_op<R, K, TB>(v, op, base, tl, lane);
         }
         if (o >= pass.nops) break;
-        if constexpr (TB == kLaneBits) {  // no LDS ops in one-wave passes (the planner streams them)
-            nxt = ops[pass.op0 + o + 1];
-            o++;
-            continue;
-        }
         // through LDS: a transpose to a new layout, or a gate on the LDS copy
         const RegOp op = nxt;
         nxt = ops[pass.op0 + o + 1];
@@ -631,10 +712,69 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
     }
 }
 
+// One-wave tiles (complex64, TB = 6): a wave holds a whole tile (64
+// amplitudes per lane for 4096-amplitude tiles) and needs no LDS and no
+// barriers. (A second register set to prefetch the next tile costs more in
+// occupancy than it gains: measured slower at 2048 and 1024 amplitudes.)
+template <int K>
+__global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, const RegPass pass,
+                                                  const RegOp *__restrict__ ops) {
+    using R = float;
+    constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
+    uint32_t tl = 0;  // local index bits held by this lane (the layout at load)
+    uint64_t at_in = 0, at_out = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        const uint32_t bit = (lane >> i) & 1u;
+        tl |= bit << pass.lay_t[i];
+        at_in |= uint64_t(bit) << pass.qt_in[i];
+        at_out |= uint64_t(bit) << pass.qt_out[i];
+    }
+    Regs<R, E> v;
+    {
+        const float2 *src = psi + (base | at_in);
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_in[j];
+            v.set(r, src[o]);
+        }
+    }
+    // The op table of a pass ends with a no-op, so op o + 1 can be fetched
+    // while op o runs. One-wave passes hold no LDS ops (the planner streams them).
+    RegOp nxt = ops[pass.op0];
+    for (int o = 0; o < pass.nops; o++) {
+        const RegOp op = nxt;
+        nxt = ops[pass.op0 + o + 1];
+        if (op.kind >= RO_TRANSPOSE) continue;
+        if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
+        apply_op<R, K, TB>(v, op, base, tl, lane);
+    }
+    // In-place permutation: a store may go where this wave loaded another
+    // amplitude, so every load must be complete first.
+    if (pass.sync_store) __builtin_amdgcn_s_waitcnt(0);
+    float2 *dst = psi + (base | at_out);
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_out[j];
+        dst[o] = v.get(r);
+    }
+}
+
 template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     using T = typename V2<R>::T;
-    const unsigned tiles = (unsigned)(uint64_t(1) << (s.n - K));
-    hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3(tiles), dim3(1 << TB), 0, st, (T *)s.d, p, ops);
+    const uint64_t tiles = uint64_t(1) << (s.n - K);
+    if constexpr (TB == kLaneBits) {
+        static_assert(sizeof(R) == 4 && K <= 12, "one-wave tiles: complex64, at most 4096 amplitudes");
+        hipLaunchKernelGGL((k_wave_pass<K>), dim3((unsigned)tiles), dim3(64), 0, st, (float2 *)s.d, p, ops);
+    } else {
+        hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)s.d, p, ops);
+    }
 }
 
 template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
