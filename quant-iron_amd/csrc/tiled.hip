@@ -65,6 +65,20 @@ template <typename R, int E> struct Regs {
     __device__ __forceinline__ void set(int r, T x) { d[2 * r] = x.x; d[2 * r + 1] = x.y; }
 };
 
+// Tile loads and stores stream the state once per pass: non-temporal, so
+// they do not displace cached data (the op table) for data never re-read.
+template <typename T> __device__ __forceinline__ T ld_stream(const T *p) {
+    typedef decltype(T::x) E2 __attribute__((ext_vector_type(2)));
+    const E2 v = __builtin_nontemporal_load(reinterpret_cast<const E2 *>(p));
+    T r; r.x = v.x; r.y = v.y;
+    return r;
+}
+template <typename T> __device__ __forceinline__ void st_stream(T x, T *p) {
+    typedef decltype(T::x) E2 __attribute__((ext_vector_type(2)));
+    E2 v; v.x = x.x; v.y = x.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<E2 *>(p));
+}
+
 // LDS slot of local index e: the bits above the low F fold into the low F
 // (XOR-linear, a bijection) so strided thread bits spread over the banks.
 template <int F> __device__ __forceinline__ uint32_t swz(uint32_t e) {
@@ -626,7 +640,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
             for (int j = 0; j < RB; j++)
                 if ((r >> j) & 1) { o |= ar[j]; e |= mr[j]; }
             rl[r] = e;
-            v.set(r, src[o]);
+            v.set(r, ld_stream(src + o));
         }
     }
     bool dirty = false;  // LDS written since the last barrier
@@ -707,7 +721,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
             uint64_t o = 0;
 #pragma unroll
             for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= ar[j];
-            dst[o] = v.get(r);
+            st_stream(v.get(r), dst + o);
         }
     }
 }
@@ -740,7 +754,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
             uint64_t o = 0;
 #pragma unroll
             for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_in[j];
-            v.set(r, src[o]);
+            v.set(r, ld_stream(src + o));
         }
     }
     // The op table of a pass ends with a no-op, so op o + 1 can be fetched
@@ -762,7 +776,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
         uint64_t o = 0;
 #pragma unroll
         for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_out[j];
-        dst[o] = v.get(r);
+        st_stream(v.get(r), dst + o);
     }
 }
 
