@@ -410,6 +410,29 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
     bool all_ops = true;
     for (size_t k = 0; k < ng; k++) all_ops = all_ops && is_operator(gates[k].kind);
     Ctx &C = *s.ctx;
+    const bool use_graph = o.use_graph && all_ops;
+    // graph key: options, gates, state shape and the op-table buffer the graph reads
+    auto make_key = [&](std::vector<unsigned char> &key) {
+        key.resize(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 16);
+        memcpy(key.data(), &o, sizeof o);
+        if (ng) memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
+        uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
+        uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
+        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
+        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate) + 8, &pp, 8);
+    };
+    std::vector<unsigned char> key;
+    if (use_graph) {
+        // the same gate list on the same state again: replay the cached graph
+        // (its op table is still in the plan buffer), no planning
+        make_key(key);
+        const bool hit = C.graph_exec && C.graph_state == (uint64_t)(uintptr_t)s.d && C.graph_key == key;
+        if (hit) {
+            C.last_plan_steps = C.graph_plan_steps;
+            auto never = []() -> Status { return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, "graph cache"); };
+            return run_maybe_graph(C, s, true, key.data(), key.size(), never);
+        }
+    }
     ExecPlan plan;
     Status e = build_plan(s, gates, ng, mode, o, plan);
     if (!e.ok()) return e;
@@ -417,24 +440,18 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
     if (plan.blob_bytes() > 0) {
         if (!(e = ensure(C.plan, plan.blob_bytes())).ok()) return e;
         QI_HIP_TRY(hipStreamSynchronize(C.stream));  // the previous plan may still be in use
+        // a cached graph reads the op table about to be overwritten
+        if (C.graph_exec) { (void)hipGraphExecDestroy(C.graph_exec); C.graph_exec = nullptr; C.graph_key.clear(); }
         std::vector<unsigned char> blob(plan.blob_bytes(), 0);
         if (!plan.reg_ops.empty()) memcpy(blob.data(), plan.reg_ops.data(), plan.reg_ops.size() * sizeof(RegOp));
         if (!plan.mats.empty()) memcpy(blob.data() + plan.mats_offset(), plan.mats.data(), plan.mats.size());
         QI_HIP_TRY(hipMemcpy(C.plan.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
     }
-    bool use_graph = o.use_graph && all_ops;
-    std::vector<unsigned char> key;
-    if (use_graph) {
-        key.resize(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 16);
-        memcpy(key.data(), &o, sizeof o);
-        memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
-        uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
-        uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
-        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
-        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate) + 8, &pp, 8);
-    }
+    if (use_graph) make_key(key);  // the plan buffer may have moved
     auto launch = [&]() -> Status { return launch_plan(s, plan, gates); };
-    return run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
+    e = run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
+    if (use_graph && e.ok()) C.graph_plan_steps = plan.steps.size();
+    return e;
 }
 
 static Status check_pair(const State &a, const State &b) {

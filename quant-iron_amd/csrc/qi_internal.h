@@ -64,6 +64,7 @@ struct Ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
     uint64_t graph_state = 0;
+    size_t graph_plan_steps = 0;  // plan steps of the cached gate list
 };
 
 struct State {

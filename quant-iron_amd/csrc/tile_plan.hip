@@ -62,26 +62,69 @@ struct Blocked {
     void add(const GateSets &g) { supp |= g.supp; nd |= g.nd; }
 };
 
-// Members of the next pass (list order) and the tile's qubit set.
+// Number of gates (from `first`, list order) a pass on qubit set q can take:
+// a gate joins unless it fails to commute with an earlier skipped gate or
+// acts non-diagonally outside q; skipped gates block later ones.
+size_t pass_members(const std::vector<GateSets> &gs, const std::vector<char> &streamed, const std::vector<char> &done,
+                    size_t first, uint64_t q, uint64_t all, std::vector<size_t> *members) {
+    Blocked blk;
+    size_t count = 0;
+    for (size_t g = first; g < gs.size(); g++) {
+        if (done[g]) continue;
+        if (streamed[g] || blk.blocks(gs[g]) || (gs[g].need & ~q)) blk.add(gs[g]);
+        else {
+            count++;
+            if (members) members->push_back(g);
+        }
+        if ((blk.nd & all) == all) break;  // every later gate is blocked
+    }
+    return count;
+}
+
+// The next pass: its tile's qubit set and members. The set starts greedy (the
+// qubits of the earliest gates that fit, then the lowest unused qubits: longer
+// HBM rows); with `search`, single-qubit exchanges that let the pass take more
+// gates are then applied while they help (a few sweeps), which on random
+// circuits removes a quarter or more of the passes.
 uint64_t select_pass(const std::vector<GateSets> &gs, const std::vector<char> &streamed, std::vector<char> &done,
-                     size_t first, int K, int n, int low, std::vector<size_t> &members) {
+                     size_t first, int K, int n, int low, bool search, std::vector<size_t> &members) {
     const uint64_t all = n >= 64 ? ~uint64_t(0) : ((uint64_t(1) << n) - 1);
-    uint64_t active = (uint64_t(1) << low) - 1;
+    const uint64_t fixed = (uint64_t(1) << low) - 1;
+    uint64_t active = fixed;
     Blocked blk;
     for (size_t g = first; g < gs.size(); g++) {
         if (done[g]) continue;
         if (streamed[g] || blk.blocks(gs[g])) blk.add(gs[g]);
-        else if (__builtin_popcountll(active | gs[g].need) <= K) {
-            active |= gs[g].need;
-            members.push_back(g);
-            done[g] = 1;
-        } else blk.add(gs[g]);
-        if ((blk.nd & all) == all) break;  // every later gate is blocked
+        else if (__builtin_popcountll(active | gs[g].need) <= K) active |= gs[g].need;
+        else blk.add(gs[g]);
+        if ((blk.nd & all) == all) break;
     }
-    uint64_t q = active;  // fill with the lowest unused qubits (longer HBM rows)
+    uint64_t q = active;
     for (int b = 0; __builtin_popcountll(q) < K && b < n; b++) q |= uint64_t(1) << b;
+    if (search) {
+        size_t best = pass_members(gs, streamed, done, first, q, all, nullptr);
+        for (int sweep = 0; sweep < 3; sweep++) {
+            bool improved = false;
+            for (int out = 0; out < n; out++) {
+                if (!((q >> out) & 1) || ((fixed >> out) & 1)) continue;
+                for (int in = 0; in < n; in++) {
+                    if ((q >> in) & 1) continue;
+                    const uint64_t q2 = (q & ~(uint64_t(1) << out)) | (uint64_t(1) << in);
+                    const size_t c = pass_members(gs, streamed, done, first, q2, all, nullptr);
+                    if (c > best) { best = c; q = q2; improved = true; break; }
+                }
+            }
+            if (!improved) break;
+        }
+    }
+    pass_members(gs, streamed, done, first, q, all, &members);
+    for (size_t g : members) done[g] = 1;
     return q;
 }
+
+// States from this size on spend planning time on the tile qubit search (a
+// pass over them costs milliseconds).
+constexpr uint32_t kSearchQubits = 20;
 
 bool is_relabel(const Lowered &L) { return L.kind == LK_SWAP && L.ctrl_mask == 0; }
 bool is_pair(const Lowered &L) { return L.kind <= LK_U2; }
@@ -298,7 +341,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
             continue;
         }
         std::vector<size_t> members;
-        const uint64_t q = select_pass(gs, streamed, done, first, K, (int)s.n, low, members);
+        const uint64_t q = select_pass(gs, streamed, done, first, K, (int)s.n, low, s.n >= kSearchQubits, members);
         RegPlanner P;
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
@@ -447,7 +490,7 @@ Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K,
     size_t first = 0;
     while (first < gates.size()) {
         std::vector<size_t> members;
-        const uint64_t q = select_pass(gs, none, done, first, K, (int)s.n, tile_low_qubits(s.prec), members);
+        const uint64_t q = select_pass(gs, none, done, first, K, (int)s.n, tile_low_qubits(s.prec), false, members);
         TilePass p{};
         p.qmask = q;
         p.K = K;
