@@ -411,47 +411,41 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
     for (size_t k = 0; k < ng; k++) all_ops = all_ops && is_operator(gates[k].kind);
     Ctx &C = *s.ctx;
     const bool use_graph = o.use_graph && all_ops;
-    // graph key: options, gates, state shape and the op-table buffer the graph reads
-    auto make_key = [&](std::vector<unsigned char> &key) {
-        key.resize(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 16);
-        memcpy(key.data(), &o, sizeof o);
-        if (ng) memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
-        uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
-        uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
-        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
-        memcpy(key.data() + sizeof o + ng * sizeof(qi_gate) + 8, &pp, 8);
-    };
-    std::vector<unsigned char> key;
-    if (use_graph) {
-        // the same gate list on the same state again: replay the cached graph
-        // (its op table is still in the plan buffer), no planning
-        make_key(key);
-        const bool hit = C.graph_exec && C.graph_state == (uint64_t)(uintptr_t)s.d && C.graph_key == key;
-        if (hit) {
-            C.last_plan_steps = C.graph_plan_steps;
-            auto never = []() -> Status { return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, "graph cache"); };
-            return run_maybe_graph(C, s, true, key.data(), key.size(), never);
+    // plan key: options, gates and state shape; the same gate list again reuses
+    // the last plan and its op tables in the plan buffer (no planning, no upload)
+    std::vector<unsigned char> key(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 8);
+    memcpy(key.data(), &o, sizeof o);
+    if (ng) memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
+    const uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
+    memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
+    std::shared_ptr<ExecPlan> plan;
+    if (C.cached_plan && C.plan_key == key) {
+        plan = std::static_pointer_cast<ExecPlan>(C.cached_plan);
+    } else {
+        plan = std::make_shared<ExecPlan>();
+        Status e = build_plan(s, gates, ng, mode, o, *plan);
+        if (!e.ok()) return e;
+        C.cached_plan.reset();
+        C.plan_key.clear();
+        if (plan->blob_bytes() > 0) {
+            if (!(e = ensure(C.plan, plan->blob_bytes())).ok()) return e;
+            QI_HIP_TRY(hipStreamSynchronize(C.stream));  // the previous plan may still be in use
+            // a cached graph reads the op table about to be overwritten
+            if (C.graph_exec) { (void)hipGraphExecDestroy(C.graph_exec); C.graph_exec = nullptr; C.graph_key.clear(); }
+            std::vector<unsigned char> blob(plan->blob_bytes(), 0);
+            if (!plan->reg_ops.empty()) memcpy(blob.data(), plan->reg_ops.data(), plan->reg_ops.size() * sizeof(RegOp));
+            if (!plan->mats.empty()) memcpy(blob.data() + plan->mats_offset(), plan->mats.data(), plan->mats.size());
+            QI_HIP_TRY(hipMemcpy(C.plan.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
         }
+        C.cached_plan = plan;
+        C.plan_key = key;
     }
-    ExecPlan plan;
-    Status e = build_plan(s, gates, ng, mode, o, plan);
-    if (!e.ok()) return e;
-    C.last_plan_steps = plan.steps.size();
-    if (plan.blob_bytes() > 0) {
-        if (!(e = ensure(C.plan, plan.blob_bytes())).ok()) return e;
-        QI_HIP_TRY(hipStreamSynchronize(C.stream));  // the previous plan may still be in use
-        // a cached graph reads the op table about to be overwritten
-        if (C.graph_exec) { (void)hipGraphExecDestroy(C.graph_exec); C.graph_exec = nullptr; C.graph_key.clear(); }
-        std::vector<unsigned char> blob(plan.blob_bytes(), 0);
-        if (!plan.reg_ops.empty()) memcpy(blob.data(), plan.reg_ops.data(), plan.reg_ops.size() * sizeof(RegOp));
-        if (!plan.mats.empty()) memcpy(blob.data() + plan.mats_offset(), plan.mats.data(), plan.mats.size());
-        QI_HIP_TRY(hipMemcpy(C.plan.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
-    }
-    if (use_graph) make_key(key);  // the plan buffer may have moved
-    auto launch = [&]() -> Status { return launch_plan(s, plan, gates); };
-    e = run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
-    if (use_graph && e.ok()) C.graph_plan_steps = plan.steps.size();
-    return e;
+    C.last_plan_steps = plan->steps.size();
+    // graph key: the plan key and the op-table buffer the graph reads
+    const uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
+    key.insert(key.end(), (const unsigned char *)&pp, (const unsigned char *)&pp + 8);
+    auto launch = [&]() -> Status { return launch_plan(s, *plan, gates); };
+    return run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
 }
 
 static Status check_pair(const State &a, const State &b) {

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -64,7 +65,9 @@ struct Ctx {
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
     uint64_t graph_state = 0;
-    size_t graph_plan_steps = 0;  // plan steps of the cached gate list
+    // the last gate-list plan (an ExecPlan), whose op tables are in `plan`, and its key
+    std::shared_ptr<void> cached_plan;
+    std::vector<unsigned char> plan_key;
 };
 
 struct State {

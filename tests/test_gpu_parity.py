@@ -251,6 +251,30 @@ def test_tiled_every_width_and_thread_count(ctx, prec):
                 assert fidelity(got, ref) >= 1 - 1e-9 and np.max(np.abs(got - ref)) <= 1e-5, (K, tb)
 
 
+def test_plan_and_graph_caches_follow_the_gate_list(ctx):
+    """Repeated gate lists reuse the cached plan (and graph); a different list
+    in between, with or without a graph, must not leave a stale op table."""
+    rng = np.random.default_rng(31)
+    n = 21
+    kinds = ["H", "RX", "RY", "RZ", "X", "Y", "Z", "P", "CNOT", "SWAP", "TOFFOLI"]
+    a = gates_array([random_gate(n, rng, kinds=kinds, max_controls=2) for _ in range(120)])
+    b = gates_array([random_gate(n, rng, kinds=kinds, max_controls=2) for _ in range(90)])
+    b2 = b.copy()
+    b2["params"][:, 0] += 0.25  # same structure, new angles: a different plan
+    psi = random_state(n, rng)
+    expect = {}
+    for name, g in (("a", a), ("b", b), ("b2", b2)):
+        s = upload(psi, "c128")
+        s.apply_gates_inplace(g, mode=L.EXEC_PER_GATE)
+        expect[name] = s.state_vector
+    for seq in ((("a", False), ("a", False), ("b", True), ("a", True), ("a", True), ("b2", False), ("b", True), ("b", True)),):
+        for name, graph in seq:
+            g = {"a": a, "b": b, "b2": b2}[name]
+            s = upload(psi, "c128")
+            s.apply_gates_inplace(g, mode=L.EXEC_TILED, use_graph=graph)
+            assert np.max(np.abs(s.state_vector - expect[name])) <= 1e-12, (name, graph)
+
+
 def test_circuit_with_measurement_and_pauli_gates(ctx):
     """Gate::Measurement / PauliString / PauliTimeEvolution inside Circuit::execute."""
     n = 12
