@@ -394,10 +394,13 @@ static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates) 
     return Status{};
 }
 
-static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, size_t ng, const qi_exec_opts *opts) {
+// `internal`: a gate list the library built itself (Trotter steps), whose
+// scaled basis changes are deliberately not unitary; it skips validation.
+static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, size_t ng, const qi_exec_opts *opts,
+                          bool internal = false) {
     if (n_circuit != s.n) return make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, s.n);  // circuit.rs:161-163
     if (ng > 0 && !gates) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gates is NULL");
-    for (size_t k = 0; k < ng; k++) {
+    for (size_t k = 0; k < ng && !internal; k++) {
         Status e = validate_gate(s.n, gates[k]);
         if (!e.ok()) return e;
     }
@@ -726,6 +729,52 @@ qi_status qi_sumop_expectation(qi_state_t s, const qi_pauli_term *terms, size_t 
     QI_RET(sumop_expectation(s->s, terms, nterms, re, im, s->s.ctx->stream));
 }
 
+// exp(-i theta P) for a Pauli string P as gates, so the tiled executor fuses
+// a whole Trotter step into a few HBM passes: every factor turned to Z, a CNOT
+// ladder gathering the parity on the last factor's qubit, RZ(2 theta) there,
+// then the ladder and the basis change undone. The basis changes are H and
+// RX(pi/2) scaled by sqrt 2 ([[1, 1], [1, -1]], [[1, -i], [-i, 1]]) and their
+// inverses scaled by 1/2: their products are exact in binary floating point,
+// so no norm drifts from rounded 1/sqrt 2 coefficients over many steps. The
+// same unitary as the reference's cos/sin form (pauli_string.rs:237-262, real
+// coefficient), up to rounding.
+static void pauli_exp_gates(const qi_pauli_term &t, double theta, std::vector<qi_gate> &out) {
+    const uint64_t sup = t.x_mask | t.z_mask;
+    int qs[64], m = 0;
+    for (uint64_t b = sup; b; b &= b - 1) qs[m++] = __builtin_ctzll(b);
+    auto u2 = [&](int q, const double (&mat)[8]) {
+        qi_gate g{};
+        g.kind = QI_GATE_U2; g.ntargets = 1; g.targets[0] = (uint64_t)q;
+        memcpy(g.matrix, mat, sizeof mat);
+        out.push_back(g);
+    };
+    auto cnot = [&](int c, int q) {
+        qi_gate g{};
+        g.kind = QI_GATE_CNOT; g.ntargets = 1; g.targets[0] = (uint64_t)q; g.ncontrols = 1; g.controls[0] = (uint64_t)c;
+        out.push_back(g);
+    };
+    // row-major (re, im) pairs
+    static const double x_in[8] = {1, 0, 1, 0, 1, 0, -1, 0}, x_out[8] = {0.5, 0, 0.5, 0, 0.5, 0, -0.5, 0};
+    static const double y_in[8] = {1, 0, 0, -1, 0, -1, 1, 0}, y_out[8] = {0.5, 0, 0, 0.5, 0, 0.5, 0.5, 0};
+    auto basis = [&](bool undo) {
+        for (int k = 0; k < m; k++) {
+            const bool x = (t.x_mask >> qs[k]) & 1, z = (t.z_mask >> qs[k]) & 1;
+            if (x && !z) u2(qs[k], undo ? x_out : x_in);
+            else if (x && z) u2(qs[k], undo ? y_out : y_in);
+        }
+    };
+    basis(false);
+    for (int k = 0; k + 1 < m; k++) cnot(qs[k], qs[m - 1]);
+    qi_gate rz{};
+    rz.kind = QI_GATE_RZ; rz.ntargets = 1; rz.targets[0] = (uint64_t)qs[m - 1]; rz.params[0] = 2.0 * theta;
+    out.push_back(rz);
+    for (int k = m - 2; k >= 0; k--) cnot(qs[k], qs[m - 1]);
+    basis(true);
+}
+
+// States from this size on run Trotter steps through the tiled executor.
+constexpr uint32_t kFusedTrotterQubits = 16;
+
 qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt, uint64_t steps, int order,
                             qi_error *err) {
     QI_NULLCHK(s);
@@ -737,6 +786,26 @@ qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nte
         if (!(e = check_mask(s->s.n, terms[k].x_mask | terms[k].z_mask)).ok()) QI_RET(e);
     QI_LOCK(s->s.ctx);
     State &S = s->s;
+    // Hermitian terms on a large state: one step as one gate list, planned
+    // once and replayed (cached plan and graph) for every step
+    bool fused = S.n >= kFusedTrotterQubits && steps > 0;
+    for (size_t k = 0; k < nterms; k++)
+        fused = fused && terms[k].c_im == 0.0 && (terms[k].x_mask | terms[k].z_mask) != 0;
+    if (fused) {
+        std::vector<qi_gate> step_gates;
+        if (order == 1) {
+            for (size_t k = 0; k < nterms; k++) pauli_exp_gates(terms[k], dt * terms[k].c_re, step_gates);
+        } else {
+            for (size_t k = 0; k < nterms; k++) pauli_exp_gates(terms[k], 0.5 * dt * terms[k].c_re, step_gates);
+            for (size_t k = nterms; k-- > 0;) pauli_exp_gates(terms[k], 0.5 * dt * terms[k].c_re, step_gates);
+        }
+        qi_exec_opts o{};
+        o.mode = QI_EXEC_TILED;
+        o.use_graph = 1;
+        for (uint64_t step = 0; step < steps; step++)
+            if (!(e = apply_gates(S, S.n, step_gates.data(), step_gates.size(), &o, true)).ok()) QI_RET(e);
+        QI_RET(Status{});
+    }
     for (uint64_t step = 0; step < steps; step++) {
         if (order == 1) {
             for (size_t k = 0; k < nterms; k++)

@@ -400,6 +400,30 @@ def test_trotter_lattice_matches_oracle(ctx, order):
         check(got, ref, prec, f"trotter {prec}")
 
 
+@pytest.mark.parametrize("shape,order", [((4, 4), 1), ((4, 4), 2), ((4, 5), 1)])
+def test_trotter_fused_steps_match_oracle(ctx, shape, order):
+    """From 16 qubits a Trotter step runs as one gate list through the tiled
+    executor (basis change, CNOT ladder, RZ per term): the same unitary as the
+    per-term cos/sin form up to rounding."""
+    terms, xs = W.trotter_2d(*shape, seed=43)
+    n = shape[0] * shape[1]
+    psi = O.init(n, 0)
+    for q in xs:
+        psi = O.apply_gate(psi, to_gate({"kind": "X", "targets": [q]}))
+    ref = O.trotter(psi, terms, 0.02, 8, order)
+    s = upload(psi, "c128")
+    got = qiron.trotter_evolve_state(_sumop_from_arr(terms), s, 0.02, 8, order).state_vector
+    check(got, ref, "c128", f"trotter c128 {shape}")
+    # complex64: each step is ~7 gates per term (about 7000 gate applications in
+    # all), so FP32 rounding accumulates past the per-amplitude bar of short
+    # circuits; the state must still match to a 1e-7 fidelity deficit
+    s = upload(psi, "c64")
+    got = qiron.trotter_evolve_state(_sumop_from_arr(terms), s, 0.02, 8, order).state_vector
+    d = float(np.max(np.abs(got - ref)))
+    f = fidelity(got, ref)
+    assert d <= 1e-4 and f >= 1 - 1e-7, (shape, order, d, 1 - f)
+
+
 def _sumop_from_arr(arr):
     return qiron.SumOp([_ps_from_term(t) for t in arr])
 
