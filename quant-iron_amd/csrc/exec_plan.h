@@ -48,6 +48,8 @@ enum RegOpKind : int32_t {
     RO_Z,              // -1 where the target bit is 1
     RO_DG,             // p0 = (c0, c1) where the target bit is 0, p1 = (c2, c3) where it is 1 (RZ)
     RO_P1,             // p1 = (c2, c3) where the target bit is 1, nothing where it is 0 (S, T, P)
+    RO_XC1,            // RO_X with one register-bit control (lp0) and no lane controls (CNOT, Toffoli)
+    RO_ZC1,            // RO_Z with one register-bit control (lp0) and no lane controls (CZ)
     RO_XLR,            // exchange the local bits of lane bit tgt and register bit dbit
     RO_XLL,            // exchange the local bits of lane bits tgt and dbit
     RO_TRANSPOSE,      // new layout through LDS

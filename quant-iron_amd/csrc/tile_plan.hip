@@ -247,6 +247,10 @@ struct RegPlanner {
             else if (reg_index(lb) >= 0) { op.dloc = 0; op.dbit = reg_index(lb); }
             else { op.dloc = 1; op.dbit = lb; }
             controls(G.ctrl_mask, op);
+            if (op.kind == RO_Z && op.tcm == 0 && __builtin_popcount(op.rcm) == 1) {
+                op.kind = RO_ZC1;
+                op.lp0 = __builtin_ctz(op.rcm);
+            }
         } break;
         case LK_SWAP: case LK_MATCH: {
             if (is_relabel(G)) {  // no data moves: the two local bits trade qubits
@@ -274,7 +278,7 @@ struct RegPlanner {
             else if (i >= 0) op.tgt = kLaneTarget + i;
             else return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile planner: target on a wave bit");
             switch (G.kind) {
-            case LK_X: op.kind = RO_X; break;
+            case LK_X: op.kind = RO_X; break;  // one register control: RO_XC1 below
             case LK_H:  // [[h, h], [h, -h]]
                 op.kind = RO_RR; op.c[0] = G.c[0]; op.c[1] = G.c[0]; op.c[2] = G.c[0]; op.c[3] = -G.c[0]; break;
             case LK_RY:  // [[c, -s], [s, c]]
@@ -306,6 +310,10 @@ struct RegPlanner {
                 break;
             }
             controls(G.ctrl_mask, op);
+            if (op.kind == RO_X && op.tcm == 0 && __builtin_popcount(op.rcm) == 1) {
+                op.kind = RO_XC1;
+                op.lp0 = __builtin_ctz(op.rcm);
+            }
         } break;
         }
         out.push_back(op);

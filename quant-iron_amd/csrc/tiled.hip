@@ -484,6 +484,118 @@ __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t
     diag_thr<KIND, CT, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
+// ------------------------------- single-register-control X and Z (c64) --
+// CNOT, Toffoli and CZ whose one in-tile control sits on register bit C (no
+// lane controls): registers without bit C are skipped at compile time and
+// the rest is updated uncontrolled, instead of computing every register and
+// selecting under a mask.
+template <int B, int C, int E>
+__device__ __forceinline__ void x_reg_c1(Regs<float, E> &v) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if (((r >> B) & 1) || !((r >> C) & 1)) continue;
+        const int s = r | (1 << B);
+        ip_swap(v.d[2 * r], v.d[2 * s]);
+        ip_swap(v.d[2 * r + 1], v.d[2 * s + 1]);
+    }
+}
+template <int C, int E>
+__device__ __forceinline__ void x_lane_c1(Regs<float, E> &v, int addr) {
+    constexpr int H = E / 2, G = H < 4 ? H : 4;
+    int rs[H];
+    {
+        int n = 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) if ((r >> C) & 1) rs[n++] = r;
+    }
+#pragma unroll
+    for (int g0 = 0; g0 < H; g0 += G) {
+        float px[G], py[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) { px[g] = xlane(addr, v.d[2 * rs[g0 + g]]); py[g] = xlane(addr, v.d[2 * rs[g0 + g] + 1]); }
+#pragma unroll
+        for (int g = 0; g < G; g++) { ip_set(v.d[2 * rs[g0 + g]], px[g]); ip_set(v.d[2 * rs[g0 + g] + 1], py[g]); }
+    }
+}
+template <int B, int C, int E>
+__device__ __forceinline__ void z_reg_c1(Regs<float, E> &v) {
+#pragma unroll
+    for (int r = 0; r < E; r++)
+        if (((r >> B) & 1) && ((r >> C) & 1)) { ip_neg(v.d[2 * r]); ip_neg(v.d[2 * r + 1]); }
+}
+template <int C, int E>
+__device__ __forceinline__ void z_thr_c1(Regs<float, E> &v, uint32_t sgn) {
+#pragma unroll
+    for (int r = 0; r < E; r++)
+        if ((r >> C) & 1) { ip_xor(v.d[2 * r], sgn); ip_xor(v.d[2 * r + 1], sgn); }
+}
+template <int B, int E> __device__ __forceinline__ void x_reg_c1_any(Regs<float, E> &v, int c) {
+    constexpr int RB = __builtin_ctz(E);
+    switch (c) {
+    case 0: if constexpr (0 < RB && B != 0) x_reg_c1<B, 0, E>(v); break;
+    case 1: if constexpr (1 < RB && B != 1) x_reg_c1<B, 1, E>(v); break;
+    case 2: if constexpr (2 < RB && B != 2) x_reg_c1<B, 2, E>(v); break;
+    case 3: if constexpr (3 < RB && B != 3) x_reg_c1<B, 3, E>(v); break;
+    case 4: if constexpr (4 < RB && B != 4) x_reg_c1<B, 4, E>(v); break;
+    case 5: if constexpr (5 < RB && B != 5) x_reg_c1<B, 5, E>(v); break;
+    default: break;
+    }
+}
+template <int B, int E> __device__ __forceinline__ void z_reg_c1_any(Regs<float, E> &v, int c) {
+    constexpr int RB = __builtin_ctz(E);
+    switch (c) {
+    case 0: if constexpr (0 < RB && B != 0) z_reg_c1<B, 0, E>(v); break;
+    case 1: if constexpr (1 < RB && B != 1) z_reg_c1<B, 1, E>(v); break;
+    case 2: if constexpr (2 < RB && B != 2) z_reg_c1<B, 2, E>(v); break;
+    case 3: if constexpr (3 < RB && B != 3) z_reg_c1<B, 3, E>(v); break;
+    case 4: if constexpr (4 < RB && B != 4) z_reg_c1<B, 4, E>(v); break;
+    case 5: if constexpr (5 < RB && B != 5) z_reg_c1<B, 5, E>(v); break;
+    default: break;
+    }
+}
+template <int E> __device__ __forceinline__ void xc1_op(Regs<float, E> &v, const RegOp &op, uint32_t lane) {
+    constexpr int RB = __builtin_ctz(E);
+    const int c = op.lp0;
+    if (op.tgt >= kLaneTarget) {
+        const int addr = int((lane ^ (1u << (op.tgt - kLaneTarget))) << 2);
+        switch (c) {
+        case 0: if constexpr (0 < RB) x_lane_c1<0, E>(v, addr); break;
+        case 1: if constexpr (1 < RB) x_lane_c1<1, E>(v, addr); break;
+        case 2: if constexpr (2 < RB) x_lane_c1<2, E>(v, addr); break;
+        case 3: if constexpr (3 < RB) x_lane_c1<3, E>(v, addr); break;
+        case 4: if constexpr (4 < RB) x_lane_c1<4, E>(v, addr); break;
+        case 5: if constexpr (5 < RB) x_lane_c1<5, E>(v, addr); break;
+        default: break;
+        }
+        return;
+    }
+#define QI_RB(BB) case BB: if constexpr (BB < RB) x_reg_c1_any<BB, E>(v, c); break;
+    switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+template <int E>
+__device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint32_t tl, uint64_t base) {
+    constexpr int RB = __builtin_ctz(E);
+    const int c = op.lp0;
+    if (op.dloc == 0) {
+#define QI_RB(BB) case BB: if constexpr (BB < RB) z_reg_c1_any<BB, E>(v, c); break;
+        switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+        return;
+    }
+    const bool one = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+    const uint32_t sgn = one ? 0x80000000u : 0u;
+    switch (c) {
+    case 0: if constexpr (0 < RB) z_thr_c1<0, E>(v, sgn); break;
+    case 1: if constexpr (1 < RB) z_thr_c1<1, E>(v, sgn); break;
+    case 2: if constexpr (2 < RB) z_thr_c1<2, E>(v, sgn); break;
+    case 3: if constexpr (3 < RB) z_thr_c1<3, E>(v, sgn); break;
+    case 4: if constexpr (4 < RB) z_thr_c1<4, E>(v, sgn); break;
+    case 5: if constexpr (5 < RB) z_thr_c1<5, E>(v, sgn); break;
+    default: break;
+    }
+}
+
 // ------------------------------------------------------- exchange ops --
 // One-wave tiles change layout without LDS. Lane bit i <-> register bit B:
 // the amplitude at (lane bit i = a, register bit B = b) moves to
@@ -596,6 +708,14 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
 #define ARGS tl, base
         QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
 #undef ARGS
+    case RO_XC1:
+        if constexpr (sizeof(R) == 4) xc1_op<E>(v, op, lane);
+        else pair_op<RO_X, true, RB, E, T, R>(v, op, lane, op.rcm, tok);
+        break;
+    case RO_ZC1:
+        if constexpr (sizeof(R) == 4) zc1_op<E>(v, op, tl, base);
+        else diag_op<RO_Z, true, RB, E, T, R>(v, op, tl, base, op.rcm, tok);
+        break;
     case RO_XLR:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
         break;
