@@ -136,6 +136,10 @@ bool is_pair(const Lowered &L) { return L.kind <= LK_U2; }
 // and controls of a group go to registers.
 struct RegPlanner {
     int K = 0, RB = 0, L = 0, TB = 8;
+    // Uncontrolled RZ = e^{-i theta/2} diag(1, e^{i theta}): emitted as a
+    // one-sided phase (half the work on a register target) while the global
+    // factor collects here, applied once by the plan's last pass.
+    double *gphase = nullptr;  // (re, im)
     int lbit[64];       // qubit -> local bit (-1: outside the tile)
     int qof[16];        // local bit -> qubit
     uint8_t cr[8], ct[12];  // layout: local bit of each register / thread bit
@@ -241,7 +245,16 @@ struct RegPlanner {
         case LK_Z: case LK_PHASE: case LK_RZ: {
             if (G.kind == LK_Z) op.kind = RO_Z;
             else if (G.kind == LK_PHASE) { op.kind = RO_P1; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; }
-            else { op.kind = RO_DG; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
+            else if (gphase && G.ctrl_mask == 0 && (G.c[0] != 0 || G.c[1] != 0)) {
+                // diag(p0, p1) = p0 diag(1, p1 / p0)
+                const double d = G.c[0] * G.c[0] + G.c[1] * G.c[1];
+                op.kind = RO_P1; op.c[0] = 1;
+                op.c[2] = (G.c[2] * G.c[0] + G.c[3] * G.c[1]) / d;
+                op.c[3] = (G.c[3] * G.c[0] - G.c[2] * G.c[1]) / d;
+                const double re = gphase[0] * G.c[0] - gphase[1] * G.c[1];
+                const double im = gphase[0] * G.c[1] + gphase[1] * G.c[0];
+                gphase[0] = re; gphase[1] = im;
+            } else { op.kind = RO_DG; for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; }
             const int lb = lbit[G.t0];
             if (lb < 0) { op.dloc = 2; op.dbit = (int)G.t0; }
             else if (reg_index(lb) >= 0) { op.dloc = 0; op.dbit = reg_index(lb); }
@@ -351,6 +364,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
     if (TB == 6)
         for (size_t g = 0; g < gates.size(); g++)
             streamed[g] = (gates[g].kind == LK_SWAP || gates[g].kind == LK_MATCH) && !is_relabel(gates[g]);
+    double gphase[2] = {1.0, 0.0};
     size_t first = 0;
     while (first < gates.size()) {
         if (streamed[first]) {
@@ -363,6 +377,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
         const uint64_t q = select_pass(gs, streamed, done, first, K, (int)s.n, low, s.n >= kSearchQubits, members);
         RegPlanner P;
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
+        P.gphase = gphase;
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
         RegPass rp{};
         rp.qmask = q;
@@ -489,6 +504,21 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
         plan.reg_ops.push_back(end);  // the kernel reads one op ahead
         plan.steps.push_back(ExecStep::reg_step(rp));
         while (first < gates.size() && done[first]) first++;
+    }
+    if (gphase[0] != 1.0 || gphase[1] != 0.0) {
+        // the collected global phase, as a last op of the last pass (whose ops
+        // close the op table): it commutes with everything
+        for (size_t k = plan.steps.size(); k-- > 0;) {
+            if (plan.steps[k].type != ExecStep::REG) continue;
+            RegOp op{};
+            op.kind = RO_DG; op.dloc = 0; op.dbit = 0;
+            op.c[0] = op.c[2] = gphase[0];
+            op.c[1] = op.c[3] = gphase[1];
+            for (int q = 0; q < 8; q++) op.cf[q] = (float)op.c[q];
+            plan.reg_ops.insert(plan.reg_ops.end() - 1, op);
+            plan.steps[k].reg.nops++;
+            break;
+        }
     }
     return Status{};
 }
