@@ -140,6 +140,15 @@ struct RegPlanner {
     // one-sided phase (half the work on a register target) while the global
     // factor collects here, applied once by the plan's last pass.
     double *gphase = nullptr;  // (re, im)
+    // The last uncontrolled diagonal op on each diagonal-op position (register
+    // bit, local bit, or qubit outside the tile) that no later op has acted on
+    // non-diagonally: a new uncontrolled diagonal there folds into it
+    // (diagonals commute with everything diagonal on their target).
+    int last_diag[3][64];
+    void forget_diagonals() {
+        for (auto &a : last_diag)
+            for (int &x : a) x = -1;
+    }
     int lbit[64];       // qubit -> local bit (-1: outside the tile)
     int qof[16];        // local bit -> qubit
     uint8_t cr[8], ct[12];  // layout: local bit of each register / thread bit
@@ -225,6 +234,7 @@ struct RegPlanner {
             memcpy(op.lay_r, r, 8);
             memcpy(op.lay_t, t, 12);
             out.push_back(op);
+            forget_diagonals();  // diagonal positions are layout-relative
         }
         memcpy(cr, r, 8);
         memcpy(ct, t, 12);
@@ -329,8 +339,46 @@ struct RegPlanner {
             }
         } break;
         }
+        if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
+            if (!op.rcm && !op.tcm && !op.gcm) {
+                int &last = last_diag[op.dloc][op.dbit];
+                if (last >= 0) {
+                    fold_diagonal(out[(size_t)last], op);
+                    return Status{};
+                }
+                last = (int)out.size();
+            }
+        } else if (op.kind <= RO_Y || op.kind == RO_XC1) {  // non-diagonal on its target
+            if (op.tgt >= kLaneTarget) last_diag[1][ct[op.tgt - kLaneTarget]] = -1;
+            else last_diag[0][op.tgt] = -1;
+        } else if (op.kind != RO_ZC1) {
+            forget_diagonals();
+        }
         out.push_back(op);
         return Status{};
+    }
+    // diag(p0, p1) of an uncontrolled diagonal op
+    static void diag_of(const RegOp &o, double *p) {
+        if (o.kind == RO_Z) { p[0] = 1; p[1] = 0; p[2] = -1; p[3] = 0; }
+        else if (o.kind == RO_P1) { p[0] = 1; p[1] = 0; p[2] = o.c[2]; p[3] = o.c[3]; }
+        else for (int k = 0; k < 4; k++) p[k] = o.c[k];
+    }
+    // prev <- the product of prev and op (both uncontrolled, same position);
+    // an identity product becomes a no-op dropped when the pass is closed
+    static void fold_diagonal(RegOp &prev, const RegOp &op) {
+        double a[4], b[4];
+        diag_of(prev, a);
+        diag_of(op, b);
+        const double p0r = a[0] * b[0] - a[1] * b[1], p0i = a[0] * b[1] + a[1] * b[0];
+        const double p1r = a[2] * b[2] - a[3] * b[3], p1i = a[2] * b[3] + a[3] * b[2];
+        prev.c[0] = p0r; prev.c[1] = p0i; prev.c[2] = p1r; prev.c[3] = p1i;
+        if (p0r == 1 && p0i == 0) {
+            if (p1r == 1 && p1i == 0) prev.kind = RO_NOP;
+            else if (p1r == -1 && p1i == 0) prev.kind = RO_Z;
+            else prev.kind = RO_P1;
+        } else {
+            prev.kind = RO_DG;
+        }
     }
 };
 
@@ -378,6 +426,7 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
         RegPlanner P;
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
         P.gphase = gphase;
+        P.forget_diagonals();
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
         RegPass rp{};
         rp.qmask = q;
@@ -494,6 +543,8 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
         // SWAP relabels or transposes may send an amplitude to an address another
         // thread loads: the kernel then waits for every load before storing
         rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
+        P.out.erase(std::remove_if(P.out.begin(), P.out.end(), [](const RegOp &op) { return op.kind == RO_NOP; }),
+                    P.out.end());
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
         for (RegOp &op : P.out)
