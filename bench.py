@@ -348,7 +348,7 @@ def main():
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--plain-circuits", type=int, default=64, help="circuits timed on the plain FP64 path")
-    ap.add_argument("--train-samples", type=int, default=262144)
+    ap.add_argument("--train-samples", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--candidates", type=int, default=1 << 20)
     ap.add_argument("--no-energies", action="store_true")

@@ -83,7 +83,7 @@ struct RegPass {
     int32_t K, op0, nops;
     int32_t sync_store;  // 1: amplitudes are stored to other addresses than they were loaded from
     int32_t tb;          // thread bits (2^tb threads per tile)
-    int32_t pad;
+    int32_t chunk_log2;  // tile order: 0 = tile b is workgroup b; else see the kernels (set at launch)
     uint8_t lay_r[8];    // layout at load: local bit of each register bit
     uint8_t lay_t[12];   //                 local bit of each thread bit
     uint8_t qr_in[8];    // qubit of each register bit at load

@@ -62,6 +62,8 @@ template <typename R> struct GateK {
     uint64_t pmask;   // pair partner = base ^ pmask
     uint64_t tb2;     // second target bit (swap / matchgate)
     uint64_t cmask;   // control mask tested per element (matchgate only)
+    int32_t chunk;    // workgroup order (xcd_block_order) of the pair and diagonal kernels
+    int32_t pad;
     R c[10];
 };
 
@@ -121,7 +123,7 @@ __device__ __forceinline__ f64x2 unamp(double2 a) { f64x2 r; r.x = a.x; r.y = a.
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, GateK<float> g) {
     if constexpr (MODE == 2) {
-        const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+        const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
         const bool h0 = k0 < g.items, h1 = k1 < g.items;
         uint64_t j0 = 0, j1 = 0;
         f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, 
         if (h0) { pair_update<float, KIND>(g, lo(u0), hi(u0), na, nb); stnt(pack(na, nb), psi + j0); }
         if (h1) { pair_update<float, KIND>(g, lo(u1), hi(u1), na, nb); stnt(pack(na, nb), psi + j1); }
     } else {
-        const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+        const uint64_t k = xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
         if (k >= g.items) return;
         const uint64_t ja = insert_zeros(k, g.fixed) | g.set, jb = ja ^ g.pmask;
         const f32x4 x = ldnt(psi + ja), y = ldnt(psi + jb);
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, 
 // the target), so only the odd halves change.
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, GateK<float> g) {
-    const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;
     uint64_t j0 = 0, j1 = 0;
     f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
@@ -161,7 +163,7 @@ __global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, 
 // complex128: one amplitude is already a 16-byte unit; one pair per thread.
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi, GateK<double> g) {
-    const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    const uint64_t k = xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
     if (k >= g.items) return;
     const uint64_t ia = insert_zeros(k, g.fixed) | g.set, ib = ia ^ g.pmask;
     const double2 a = amp(ldnt(psi + ia)), b = amp(ldnt(psi + ib));
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi,
 
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_diag_c128(f64x2 *__restrict__ psi, GateK<double> g) {
-    const uint64_t k0 = uint64_t(blockIdx.x) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;
     uint64_t j0 = 0, j1 = 0;
     f64x2 u0 = {0, 0}, u1 = {0, 0};
@@ -241,6 +243,10 @@ template <typename R> GateK<R> make_args(const Lowered &L, uint32_t n) {
 
 inline unsigned grid_for(uint64_t items, uint64_t per_block) { return (unsigned)((items + per_block - 1) / per_block); }
 
+// Runs of consecutive workgroups per XCD (log2) for the pair and diagonal
+// kernels: 1-5% faster single gates at 30 qubits than the identity order.
+constexpr int kChunkLog2 = 8;
+
 #define QI_PAIR_KINDS(M) M(LK_H) M(LK_X) M(LK_Y) M(LK_RX) M(LK_RY) M(LK_U2) M(LK_RZ)
 
 Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
@@ -267,18 +273,21 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     u.items = uint64_t(1) << (s.n - 1 - __builtin_popcountll(u.fixed));
     f32x4 *psi = reinterpret_cast<f32x4 *>(s.d);
     const dim3 g1(grid_for(u.items, kThreads)), g2(grid_for(u.items, 2 * kThreads));
+    GateK<float> u2 = u;  // the same gate for the two-units-per-thread grid
+    u.chunk = xcd_chunk_log2(g1.x, kChunkLog2);
+    u2.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
     const int kind = L.kind == LK_SWAP ? LK_X : L.kind;  // SWAP: exchange base|tb with base|tb2
     switch (kind * 4 + mode) {
 #define QI_PAIR(K) \
     case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), g1, block, 0, st, psi, u); break; \
     case K * 4 + 1: hipLaunchKernelGGL((k_pair_c64<K, 1>), g1, block, 0, st, psi, u); break; \
-    case K * 4 + 2: hipLaunchKernelGGL((k_pair_c64<K, 2>), g2, block, 0, st, psi, u); break;
+    case K * 4 + 2: hipLaunchKernelGGL((k_pair_c64<K, 2>), g2, block, 0, st, psi, u2); break;
     QI_PAIR_KINDS(QI_PAIR)
 #undef QI_PAIR
-    case LK_Z * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_Z, 0>), g2, block, 0, st, psi, u); break;
-    case LK_Z * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_Z, 1>), g2, block, 0, st, psi, u); break;
-    case LK_PHASE * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 0>), g2, block, 0, st, psi, u); break;
-    case LK_PHASE * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 1>), g2, block, 0, st, psi, u); break;
+    case LK_Z * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_Z, 0>), g2, block, 0, st, psi, u2); break;
+    case LK_Z * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_Z, 1>), g2, block, 0, st, psi, u2); break;
+    case LK_PHASE * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 0>), g2, block, 0, st, psi, u2); break;
+    case LK_PHASE * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 1>), g2, block, 0, st, psi, u2); break;
     default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
     }
     QI_HIP_TRY(hipGetLastError());
@@ -289,12 +298,15 @@ Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
     GateK<double> g = make_args<double>(L, s.n);
     const dim3 block(kThreads), g1(grid_for(g.items, kThreads)), g2(grid_for(g.items, 2 * kThreads));
     f64x2 *psi = reinterpret_cast<f64x2 *>(s.d);
+    GateK<double> g2k = g;  // the same gate for the two-amplitudes-per-thread grid
+    g2k.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
     switch (L.kind == LK_SWAP ? LK_X : L.kind) {
-#define QI_PAIR(K) case K: hipLaunchKernelGGL((k_pair_c128<K>), g1, block, 0, st, psi, g); break;
+#define QI_PAIR(K) case K: { GateK<double> g1k = g; g1k.chunk = xcd_chunk_log2(g1.x, kChunkLog2); \
+        hipLaunchKernelGGL((k_pair_c128<K>), g1, block, 0, st, psi, g1k); } break;
     QI_PAIR_KINDS(QI_PAIR)
 #undef QI_PAIR
-    case LK_Z: hipLaunchKernelGGL((k_diag_c128<LK_Z>), g2, block, 0, st, psi, g); break;
-    case LK_PHASE: hipLaunchKernelGGL((k_diag_c128<LK_PHASE>), g2, block, 0, st, psi, g); break;
+    case LK_Z: hipLaunchKernelGGL((k_diag_c128<LK_Z>), g2, block, 0, st, psi, g2k); break;
+    case LK_PHASE: hipLaunchKernelGGL((k_diag_c128<LK_PHASE>), g2, block, 0, st, psi, g2k); break;
     case LK_MATCH: hipLaunchKernelGGL((k_match<double>), g1, block, 0, st, (double2 *)s.d, g); break;
     default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
     }

@@ -136,6 +136,24 @@ Status measure_sample_collapse(State &s, State &dst, const uint64_t *qubits, uin
 
 int pick_blocks(uint64_t items, int per_block);
 
+// ------------------------------------------------------- workgroup order --
+// Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8), so with the
+// identity order every XCD walks the whole state with stride 8. With c > 0,
+// XCD x's r-th workgroup takes work item (r >> c) * 8 * 2^c + x * 2^c +
+// (r mod 2^c): each XCD works through runs of 2^c consecutive items. A
+// bijection on [0, grid) when 8 * 2^c divides the grid.
+__device__ __forceinline__ uint64_t xcd_block_order(uint32_t b, int c) {
+    if (c == 0) return b;
+    const uint32_t x = b & 7u, r = b >> 3;
+    return (uint64_t(r >> c) << (c + 3)) | (uint64_t(x) << c) | (r & ((1u << c) - 1u));
+}
+// Largest c <= cmax for which xcd_block_order is a bijection on `grid`.
+inline int xcd_chunk_log2(uint64_t grid, int cmax) {
+    if (grid == 0 || (grid & (grid - 1))) return 0;
+    const int lg = __builtin_ctzll(grid) - 3;
+    return lg < 0 ? 0 : (lg < cmax ? lg : cmax);
+}
+
 }  // namespace qi
 
 struct qi_ctx_s {

@@ -734,7 +734,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
     constexpr int RB = K - TB, E = 1 << RB, F = sizeof(R) == 8 ? 3 : 4;
     __shared__ T lds[1 << K];
     const uint32_t tid = threadIdx.x, lane = tid & 63;
-    const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
+    const uint64_t base = insert_zeros64(xcd_block_order(blockIdx.x, pass.chunk_log2), pass.qmask);
 
     Regs<R, E> v;
     uint32_t tl = 0;  // local index bits held by this thread
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
     using R = float;
     constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
     const uint32_t lane = threadIdx.x;
-    const uint64_t base = insert_zeros64(uint64_t(blockIdx.x), pass.qmask);
+    const uint64_t base = insert_zeros64(xcd_block_order(blockIdx.x, pass.chunk_log2), pass.qmask);
     uint32_t tl = 0;  // local index bits held by this lane (the layout at load)
     uint64_t at_in = 0, at_out = 0;
 #pragma unroll
@@ -900,14 +900,21 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
     }
 }
 
+// Runs of consecutive tiles per XCD (log2; shorter when the grid is small):
+// 2-3% faster passes than the identity order on 30-qubit states.
+constexpr int kTileChunkLog2 = 8;
+
 template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     using T = typename V2<R>::T;
-    const uint64_t tiles = uint64_t(1) << (s.n - K);
+    const int tile_bits = (int)s.n - K;
+    const uint64_t tiles = uint64_t(1) << tile_bits;
+    RegPass q = p;
+    q.chunk_log2 = xcd_chunk_log2(tiles, kTileChunkLog2);
     if constexpr (TB == kLaneBits) {
         static_assert(sizeof(R) == 4 && K <= 12, "one-wave tiles: complex64, at most 4096 amplitudes");
-        hipLaunchKernelGGL((k_wave_pass<K>), dim3((unsigned)tiles), dim3(64), 0, st, (float2 *)s.d, p, ops);
+        hipLaunchKernelGGL((k_wave_pass<K>), dim3((unsigned)tiles), dim3(64), 0, st, (float2 *)s.d, q, ops);
     } else {
-        hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)s.d, p, ops);
+        hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)s.d, q, ops);
     }
 }
 
