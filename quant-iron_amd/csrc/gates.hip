@@ -63,7 +63,7 @@ template <typename R> struct GateK {
     uint64_t tb2;     // second target bit (swap / matchgate)
     uint64_t cmask;   // control mask tested per element (matchgate only)
     int32_t chunk;    // workgroup order (xcd_block_order) of the pair and diagonal kernels
-    int32_t pad;
+    int32_t lane;     // lane-exchange kernels: lane offset of the partner (1, 2 or 4)
     R c[10];
 };
 
@@ -145,6 +145,43 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, 
     }
 }
 
+// Low targets (partner 16-64 bytes away): in the pair layout above each load
+// instruction would touch twice the cache lines it uses. Here a wave reads
+// 128 consecutive units (index space without the control bits), two fully
+// used 1 KiB rows, and takes each partner from the lane that holds it. `g`:
+// fixed = controls only, items = every unit with the controls satisfied.
+template <typename T, typename R, int KIND>
+__device__ __forceinline__ T lane_pick(const GateK<R> &g, T own, T other, bool upper) {
+    T na, nb;
+    pair_update<R, KIND>(g, upper ? other : own, upper ? own : other, na, nb);
+    return upper ? nb : na;
+}
+
+__device__ __forceinline__ f32x4 shfl_xor4(f32x4 v, int m) {
+    f32x4 r;
+    r.x = __shfl_xor(v.x, m); r.y = __shfl_xor(v.y, m); r.z = __shfl_xor(v.z, m); r.w = __shfl_xor(v.w, m);
+    return r;
+}
+
+template <int KIND, int MODE>
+__global__ __launch_bounds__(kThreads) void k_pair_c64_lane(f32x4 *__restrict__ psi, GateK<float> g) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
+                   k1 = k0 + 64;
+    const bool h0 = k0 < g.items, h1 = k1 < g.items;  // a partner is in range iff its lane's unit is
+    uint64_t j0 = 0, j1 = 0;
+    f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
+    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
+    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+    const f32x4 p0 = shfl_xor4(u0, g.lane), p1 = shfl_xor4(u1, g.lane);
+    const bool upper = (lane & uint32_t(g.lane)) != 0;  // this lane holds the target-1 units
+    // MODE 0: both halves of a unit pair up; MODE 1: qubit 0 is a control, only odd halves change
+    const float2 a0 = MODE == 0 ? lane_pick<float2, float, KIND>(g, lo(u0), lo(p0), upper) : lo(u0);
+    const float2 a1 = MODE == 0 ? lane_pick<float2, float, KIND>(g, lo(u1), lo(p1), upper) : lo(u1);
+    if (h0) stnt(pack(a0, lane_pick<float2, float, KIND>(g, hi(u0), hi(p0), upper)), psi + j0);
+    if (h1) stnt(pack(a1, lane_pick<float2, float, KIND>(g, hi(u1), hi(p1), upper)), psi + j1);
+}
+
 // complex64 diagonal gates (Z, phases) on two-amplitude units. MODE 0: bit 0
 // free (both halves change); MODE 1: bit 0 is set by the gate (a control or
 // the target), so only the odd halves change.
@@ -171,6 +208,24 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi,
     pair_update<double, KIND>(g, a, b, na, nb);
     stnt(unamp(na), psi + ia);
     stnt(unamp(nb), psi + ib);
+}
+
+// complex128 low targets (partner 16-64 bytes away), as k_pair_c64_lane.
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_pair_c128_lane(f64x2 *__restrict__ psi, GateK<double> g) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
+                   k1 = k0 + 64;
+    const bool h0 = k0 < g.items, h1 = k1 < g.items;
+    uint64_t j0 = 0, j1 = 0;
+    f64x2 u0 = {0, 0}, u1 = {0, 0};
+    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
+    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+    const double2 p0 = make_double2(__shfl_xor(u0.x, g.lane), __shfl_xor(u0.y, g.lane));
+    const double2 p1 = make_double2(__shfl_xor(u1.x, g.lane), __shfl_xor(u1.y, g.lane));
+    const bool upper = (lane & uint32_t(g.lane)) != 0;
+    if (h0) stnt(unamp(lane_pick<double2, double, KIND>(g, amp(u0), p0, upper)), psi + j0);
+    if (h1) stnt(unamp(lane_pick<double2, double, KIND>(g, amp(u1), p1, upper)), psi + j1);
 }
 
 template <int KIND>
@@ -248,6 +303,23 @@ inline unsigned grid_for(uint64_t items, uint64_t per_block) { return (unsigned)
 constexpr int kChunkLog2 = 8;
 
 #define QI_PAIR_KINDS(M) M(LK_H) M(LK_X) M(LK_Y) M(LK_RX) M(LK_RY) M(LK_U2) M(LK_RZ)
+#define QI_LANE_KINDS(M) M(LK_H) M(LK_X) M(LK_Y) M(LK_RX) M(LK_RY) M(LK_U2)
+
+// Lane-exchange form of a one-target gate whose (unit-space) target bit `tb`
+// lands on bit 0..2 of the index with the controls `ctrl` removed: returns
+// the launch arguments, or false when the pair layout is the better one.
+template <typename R> bool lane_args(const GateK<R> &g, uint64_t tb, uint64_t ctrl, int kind, GateK<R> &out) {
+    if (kind == LK_RZ || (tb & (tb - 1)) || !tb) return false;
+    const int b = __builtin_ctzll(tb);
+    const int kb = b - __builtin_popcountll(ctrl & (tb - 1));
+    if (kb > 2) return false;
+    out = g;
+    out.fixed = ctrl;
+    out.items = g.items * 2;
+    out.lane = 1 << kb;
+    out.chunk = xcd_chunk_log2(grid_for(out.items, 2 * kThreads), kChunkLog2);
+    return true;
+}
 
 Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     GateK<float> g = make_args<float>(L, s.n);
@@ -277,6 +349,20 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     u.chunk = xcd_chunk_log2(g1.x, kChunkLog2);
     u2.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
     const int kind = L.kind == LK_SWAP ? LK_X : L.kind;  // SWAP: exchange base|tb with base|tb2
+    GateK<float> ul;
+    if (L.kind != LK_SWAP && mode != 2 && !diag && lane_args(u, u.pmask, u.fixed & ~u.pmask, kind, ul)) {
+        const dim3 gl(grid_for(ul.items, 2 * kThreads));
+        switch (kind * 2 + mode) {
+#define QI_LANE(K) \
+        case K * 2 + 0: hipLaunchKernelGGL((k_pair_c64_lane<K, 0>), gl, block, 0, st, psi, ul); break; \
+        case K * 2 + 1: hipLaunchKernelGGL((k_pair_c64_lane<K, 1>), gl, block, 0, st, psi, ul); break;
+        QI_LANE_KINDS(QI_LANE)
+#undef QI_LANE
+        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+        }
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
     switch (kind * 4 + mode) {
 #define QI_PAIR(K) \
     case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), g1, block, 0, st, psi, u); break; \
@@ -300,6 +386,19 @@ Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
     f64x2 *psi = reinterpret_cast<f64x2 *>(s.d);
     GateK<double> g2k = g;  // the same gate for the two-amplitudes-per-thread grid
     g2k.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
+    GateK<double> gl;
+    if (L.kind != LK_SWAP && L.kind != LK_MATCH && L.kind != LK_Z && L.kind != LK_PHASE &&
+        lane_args(g, g.pmask, g.fixed & ~g.pmask, L.kind, gl)) {
+        const dim3 gg(grid_for(gl.items, 2 * kThreads));
+        switch (L.kind) {
+#define QI_LANE(K) case K: hipLaunchKernelGGL((k_pair_c128_lane<K>), gg, block, 0, st, psi, gl); break;
+        QI_LANE_KINDS(QI_LANE)
+#undef QI_LANE
+        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+        }
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
     switch (L.kind == LK_SWAP ? LK_X : L.kind) {
 #define QI_PAIR(K) case K: { GateK<double> g1k = g; g1k.chunk = xcd_chunk_log2(g1.x, kChunkLog2); \
         hipLaunchKernelGGL((k_pair_c128<K>), g1, block, 0, st, psi, g1k); } break;

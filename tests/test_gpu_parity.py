@@ -170,6 +170,37 @@ def test_streaming_gates_at_every_qubit_0_placement(ctx, prec):
                 ref = got.astype(np.complex128)  # compare each gate from the engine's own state
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_streaming_low_targets_lane_exchange(ctx, prec):
+    """A one-target gate whose target lands on index bit 0..2 once the
+    controls are removed (partner 16-64 bytes away) takes the lane-exchange
+    kernel: every kind with controls below, above and on qubit 0, against the
+    oracle on 8- and 14-qubit states (the latter spans several workgroups)."""
+    rng = np.random.default_rng(777)
+    placements = [  # (targets, controls)
+        ([0], []), ([1], []), ([2], []), ([3], []), ([4], []), ([0], [1]), ([1], [0]), ([2], [0]), ([3], [0, 1]),
+        ([3], [1]), ([2], [5]), ([1], [7]), ([4], [0, 1]), ([4], [1, 2]), ([3], [0, 7]), ([2], [0, 1, 3]),
+        ([5], [0, 2, 4]), ([6], [1, 2, 3, 4]),
+    ]
+    for n in (8, 14):
+        psi = random_state(n, rng)
+        s = upload(psi, prec)
+        ref = psi
+        for kind in ONE_Q:
+            for t, c in placements:
+                d = dict(kind=kind, targets=t, controls=c, params=[float(rng.uniform(0, 2 * np.pi)) for _ in range(3)])
+                if kind == "U2":
+                    d["matrix"] = [[z.real, z.imag] for z in random_unitary2(rng).reshape(-1)]
+                s.apply_gate_inplace(qiron.Gate(to_gate(d)))
+                ref = O.apply_gate(ref, to_gate(d))
+                got = s.state_vector
+                if prec == "c128":
+                    assert np.array_equal(got.real, ref.real) and np.array_equal(got.imag, ref.imag), (n, d)
+                else:
+                    check(got, ref, prec, f"n={n} {kind} t={t} c={c}")
+                    ref = got.astype(np.complex128)
+
+
 MODES = [
     dict(mode=L.EXEC_PER_GATE),
     dict(mode=L.EXEC_TILED, tile_qubits=10),
