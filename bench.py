@@ -315,6 +315,26 @@ def single_gates(qiron, L, W, ctx, n, results):
         log(f"{prec} single H gate at {n}q: {ms:.3f} ms, {gbps:.0f} GB/s ({gbps / HBM_PEAK_GBPS:.1%} of 8 TB/s)")
         results[f"{prec}_single_gate"] = dict(ms=ms, GBps=gbps, frac=gbps / HBM_PEAK_GBPS,
                                               ms_all=[round(t, 4) for t in ts])
+        # the same gate on every target qubit (the stride between partners changes the rate)
+        s = qiron.State.new_plus(n, prec)
+        fr = []
+        for t in range(n):
+            g = W.GateList(4)
+            g.add("H", [t])
+            a = g.array()
+            tt = []
+            for rep in range(4):
+                s.apply_gates_inplace(a, mode=L.EXEC_PER_GATE)
+                if rep:
+                    tt.append(ctx.last_exec_ms())
+            fr.append(2 * S / (min(tt) * 1e-3) / 1e9 / HBM_PEAK_GBPS)
+        del s
+        srt = sorted(fr)
+        results[f"{prec}_single_gate"]["every_target"] = dict(
+            frac_of_peak=[round(x, 4) for x in fr], median=srt[n // 2], min=srt[0], max=srt[-1],
+            at_least_80pct=sum(x >= 0.8 for x in fr))
+        log(f"{prec} H on each of {n} targets: median {srt[n // 2]:.1%}, range {srt[0]:.1%}-{srt[-1]:.1%}, "
+            f"{sum(x >= 0.8 for x in fr)} targets at >= 80%")
 
 
 def copy_calibration(qiron, ctx, n=29, reps=5):

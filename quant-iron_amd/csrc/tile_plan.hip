@@ -389,7 +389,7 @@ struct Phase {
 
 }  // namespace
 
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
+static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
     if (K <= 0) K = default_tile_qubits(s);
     if (TB <= 0) TB = default_tile_threads_log2(s.prec, K);
     if (TB != 6 && TB != 8 && TB != 9) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile threads out of range");
@@ -571,6 +571,75 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
             break;
         }
     }
+    return Status{};
+}
+
+// Uncontrolled SWAPs as one relabeling of the whole circuit: a SWAP moves no
+// data, it renames the two qubits for every later gate (map: logical qubit ->
+// the physical bit holding it). The net renaming is undone at the end by
+// SWAPs on physical bits, emitted as transpositions in ascending or
+// descending order of the logical qubit put back in place. A pass then never
+// needs both qubits of a SWAP in its tile; the price is the final
+// permutation (a few passes).
+static std::vector<Lowered> relabel_swaps(const std::vector<Lowered> &in, int n, bool descending) {
+    int at[64], holds[64];  // logical -> physical, physical -> logical
+    for (int q = 0; q < 64; q++) at[q] = holds[q] = q;
+    std::vector<Lowered> out;
+    out.reserve(in.size() + n);
+    for (const Lowered &L : in) {
+        if (is_relabel(L)) {
+            std::swap(at[L.t0], at[L.t1]);
+            holds[at[L.t0]] = (int)L.t0;
+            holds[at[L.t1]] = (int)L.t1;
+            continue;
+        }
+        Lowered M = L;
+        M.t0 = (uint32_t)at[L.t0];
+        M.t1 = (uint32_t)at[L.t1];
+        M.ctrl_mask = 0;
+        for (uint64_t c = L.ctrl_mask; c; c &= c - 1) M.ctrl_mask |= uint64_t(1) << at[__builtin_ctzll(c)];
+        out.push_back(M);
+    }
+    for (int i = 0; i < n; i++) {
+        const int l = descending ? n - 1 - i : i;
+        if (at[l] == l) continue;
+        const int p = at[l], m = holds[l];  // logical l sits at p; bit l holds logical m
+        Lowered S;
+        S.kind = LK_SWAP;
+        S.t0 = (uint32_t)l;
+        S.t1 = (uint32_t)p;
+        out.push_back(S);
+        at[l] = l; holds[l] = l;
+        at[m] = p; holds[p] = m;
+    }
+    return out;
+}
+
+static void append_plan(ExecPlan &dst, const ExecPlan &src) {
+    const int32_t base = (int32_t)dst.reg_ops.size();
+    for (ExecStep st : src.steps) {
+        if (st.type == ExecStep::REG) st.reg.op0 += base;
+        dst.steps.push_back(st);
+    }
+    dst.reg_ops.insert(dst.reg_ops.end(), src.reg_ops.begin(), src.reg_ops.end());
+}
+
+// The gate list as given and with SWAPs relabeled (both emission orders of
+// the final permutation): the plan with the fewest launches wins (the given
+// list on ties).
+Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
+    ExecPlan best;
+    Status st = plan_tiled_list(s, gates, K, TB, best);
+    if (!st.ok()) return st;
+    const bool any = std::any_of(gates.begin(), gates.end(), is_relabel);
+    if (any && (int)s.n >= (K <= 0 ? default_tile_qubits(s) : K)) {
+        for (int order = 0; order < 2; order++) {
+            ExecPlan alt;
+            if (!plan_tiled_list(s, relabel_swaps(gates, (int)s.n, order == 1), K, TB, alt).ok()) continue;
+            if (alt.steps.size() < best.steps.size()) best = std::move(alt);
+        }
+    }
+    append_plan(plan, best);
     return Status{};
 }
 
