@@ -50,6 +50,10 @@ enum RegOpKind : int32_t {
     RO_P1,             // p1 = (c2, c3) where the target bit is 1, nothing where it is 0 (S, T, P)
     RO_XC1,            // RO_X with one register-bit control (lp0) and no lane controls (CNOT, Toffoli)
     RO_ZC1,            // RO_Z with one register-bit control (lp0) and no lane controls (CZ)
+    RO_RRS,            // rotation [[cos, -sin], [sin, cos]] on a register bit, uncontrolled in the tile, as three
+                       // shears per component: a += c0 b, b += c1 a, a += c0 b (c0 = -tan(phi/2), c1 = sin phi)
+    RO_RXS,            // RO_RXI rotation [[cos, i sin], [i sin, cos]]: shears (c0, c1) on (a.re, b.im) and
+                       // (-c0, -c1) on (a.im, b.re)
     RO_XLR,            // exchange the local bits of lane bit tgt and register bit dbit
     RO_XLL,            // exchange the local bits of lane bits tgt and dbit
     RO_TRANSPOSE,      // new layout through LDS

@@ -19,6 +19,7 @@
 // engine (plan_lds_passes).
 #include <algorithm>
 #include <array>
+#include <cmath>
 
 #include "qi_internal.h"
 #include "exec_plan.h"
@@ -136,6 +137,7 @@ bool is_pair(const Lowered &L) { return L.kind <= LK_U2; }
 // and controls of a group go to registers.
 struct RegPlanner {
     int K = 0, RB = 0, L = 0, TB = 8;
+    bool f32 = false;  // complex64 kernels: rotations may run as shears
     // Uncontrolled RZ = e^{-i theta/2} diag(1, e^{i theta}): emitted as a
     // one-sided phase (half the work on a register target) while the global
     // factor collects here, applied once by the plan's last pass.
@@ -337,6 +339,7 @@ struct RegPlanner {
                 op.kind = RO_XC1;
                 op.lp0 = __builtin_ctz(op.rcm);
             }
+            if (f32) shear_form(op);
         } break;
         }
         if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
@@ -348,7 +351,7 @@ struct RegPlanner {
                 }
                 last = (int)out.size();
             }
-        } else if (op.kind <= RO_Y || op.kind == RO_XC1) {  // non-diagonal on its target
+        } else if (op.kind <= RO_Y || op.kind == RO_XC1 || op.kind == RO_RRS || op.kind == RO_RXS) {  // non-diagonal
             if (op.tgt >= kLaneTarget) last_diag[1][ct[op.tgt - kLaneTarget]] = -1;
             else last_diag[0][op.tgt] = -1;
         } else if (op.kind != RO_ZC1) {
@@ -356,6 +359,30 @@ struct RegPlanner {
         }
         out.push_back(op);
         return Status{};
+    }
+    // A rotation on a register bit with no control inside the tile: three
+    // in-place shears per component pair (3 multiply-adds instead of 4). The
+    // angle is brought into [-pi/2, pi/2] (tan(phi/2) bounded by 1) by
+    // R(phi) = -R(phi - pi), the sign going to the global phase, which needs
+    // the op uncontrolled; otherwise wide angles keep the plain form.
+    void shear_form(RegOp &op) const {
+        if (op.tgt >= kLaneTarget || op.rcm || op.tcm) return;
+        double c, s;
+        if (op.kind == RO_RR && op.c[0] == op.c[3] && op.c[1] == -op.c[2]) { c = op.c[0]; s = op.c[2]; }
+        else if (op.kind == RO_RXI && op.c[0] == op.c[3] && op.c[1] == op.c[2]) { c = op.c[0]; s = op.c[1]; }
+        else return;
+        if (std::fabs(c * c + s * s - 1.0) > 1e-12) return;
+        double phi = std::atan2(s, c);
+        if (std::fabs(phi) > M_PI / 2) {
+            if (op.gcm) return;
+            phi -= std::copysign(M_PI, phi);
+            gphase[0] = -gphase[0];
+            gphase[1] = -gphase[1];
+        }
+        op.kind = op.kind == RO_RR ? RO_RRS : RO_RXS;
+        op.c[0] = -std::tan(phi / 2);
+        op.c[1] = std::sin(phi);
+        op.c[2] = op.c[3] = 0;
     }
     // diag(p0, p1) of an uncontrolled diagonal op
     static void diag_of(const RegOp &o, double *p) {
@@ -425,6 +452,7 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         const uint64_t q = select_pass(gs, streamed, done, first, K, (int)s.n, low, s.n >= kSearchQubits, members);
         RegPlanner P;
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
+        P.f32 = s.prec == QI_C64;
         P.gphase = gphase;
         P.forget_diagonals();
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;

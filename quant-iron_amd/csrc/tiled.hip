@@ -596,6 +596,37 @@ __device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint3
     }
 }
 
+// ------------------------------------------------ rotations as shears (c64) --
+// R(phi) = [[1, a], [0, 1]] [[1, 0], [b, 1]] [[1, a], [0, 1]] with
+// a = -tan(phi/2), b = sin(phi): three in-place multiply-adds per component
+// pair. RO_RRS rotates (re a, re b) and (im a, im b); RO_RXS rotates
+// (re a, im b) by phi and (im a, re b) by -phi.
+template <int B, bool XS, int E>
+__device__ __forceinline__ void shear_reg(Regs<float, E> &v, float a, float b) {
+    const float na = -a, nb = -b;
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if ((r >> B) & 1) continue;
+        const int s = r | (1 << B);
+        float &p0 = v.d[2 * r], &q0 = v.d[2 * s + (XS ? 1 : 0)];   // first pair, angle phi
+        float &p1 = v.d[2 * r + 1], &q1 = v.d[2 * s + (XS ? 0 : 1)];  // second pair, angle +-phi
+        const float a1 = XS ? na : a, b1 = XS ? nb : b;
+        ip_fmac(p0, a, q0);
+        ip_fmac(p1, a1, q1);
+        ip_fmac(q0, b, p0);
+        ip_fmac(q1, b1, p1);
+        ip_fmac(p0, a, q0);
+        ip_fmac(p1, a1, q1);
+    }
+}
+template <bool XS, int E> __device__ __forceinline__ void shear_op(Regs<float, E> &v, const RegOp &op) {
+    constexpr int RB = __builtin_ctz(E);
+    const float a = op.cf[0], b = op.cf[1];
+#define QI_RB(BB) case BB: if constexpr (BB < RB) shear_reg<BB, XS, E>(v, a, b); break;
+    switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+
 // ------------------------------------------------------- exchange ops --
 // One-wave tiles change layout without LDS. Lane bit i <-> register bit B:
 // the amplitude at (lane bit i = a, register bit B = b) moves to
@@ -715,6 +746,12 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
     case RO_ZC1:
         if constexpr (sizeof(R) == 4) zc1_op<E>(v, op, tl, base);
         else diag_op<RO_Z, true, RB, E, T, R>(v, op, tl, base, op.rcm, tok);
+        break;
+    case RO_RRS:
+        if constexpr (sizeof(R) == 4) shear_op<false, E>(v, op);
+        break;
+    case RO_RXS:
+        if constexpr (sizeof(R) == 4) shear_op<true, E>(v, op);
         break;
     case RO_XLR:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
