@@ -22,6 +22,7 @@ sys.path.insert(0, str(ROOT / "quant-iron_amd"))
 
 METRIC = "gates/s + HBM GB/s @30q; ansatz energies/s FP32+surrogate vs FP64"
 HBM_PEAK_GBPS = 8000.0
+VALU_CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 
@@ -295,6 +296,22 @@ def profiled_traffic(kernel: str, path: str):
     return None
 
 
+def profiled_valu(kernel: str, path: str):
+    """wave64 VALU instructions per dispatch of `kernel` from a committed
+    rocprofv3 --pmc summary (SQ_INSTS_VALU) of this same workload; None when
+    the summary is missing or covers another kernel."""
+    full = os.path.join(os.path.dirname(os.path.abspath(__file__)), path)
+    if not os.path.exists(full):
+        return None
+    text = open(full).read()
+    if kernel not in text.splitlines()[0]:
+        return None
+    for line in text.splitlines():
+        if line.startswith("SQ_INSTS_VALU"):
+            return float(line.split()[1])
+    return None
+
+
 def single_gates(qiron, L, W, ctx, n, results):
     """One H gate on the middle qubit of an n-qubit state, streaming kernel:
     the HBM-roofline case (reads and writes the whole state once)."""
@@ -423,6 +440,13 @@ def main():
         roofline["traffic"] = profiled_traffic(roofline["kernel"], src)
         if roofline["traffic"] is not None:
             roofline["traffic_source"] = src
+        # the second resource of the pass: VALU issue (wave64 instruction = 4 cycles of a 16-lane SIMD)
+        vsrc = "profiles/k_wave_pass_valu_c64_30q_random_circuit.txt"
+        valu = profiled_valu(roofline["kernel"], vsrc)
+        if valu is not None:
+            valu_ms = valu * 4 / (info["cu_count"] * 4) / VALU_CLOCK_HZ * 1e3
+            roofline["valu"] = {"instructions_per_launch": valu, "issue_ms": round(valu_ms, 3),
+                                "issue_frac": round(valu_ms / pass_ms, 4), "source": vsrc}
 
     if not args.only:
         single_gates(qiron, L, W, ctx, n, results)
