@@ -164,8 +164,9 @@ typedef struct qi_exec_opts {
     int32_t tile_qubits; /* tile width k for QI_EXEC_TILED: 10..13 (c64), 10..12 (c128); 0 = 12 (c64) / 11 (c128) */
     int32_t block_qubits;/* dense block width k (<= 6) for QI_EXEC_FUSED_MFMA */
     int32_t use_graph;   /* capture the launch sequence in a hipGraph         */
-    int32_t tile_threads_log2; /* QI_EXEC_TILED threads per tile: 6 (one wave; complex64, tile_qubits <= 12), 8 or 9;
-                                  0 = default (6 for complex64 up to 12 tile qubits, else 8; 9 for 13) */
+    int32_t tile_threads_log2; /* QI_EXEC_TILED threads per tile: 6 (one wave; tile_qubits <= 12 for complex64,
+                                  <= 11 for complex128), 8 or 9; 0 = default (6 where allowed, else 8; 9 for
+                                  complex64 13 / complex128 12 tile qubits) */
 } qi_exec_opts;
 
 /* ---------------- context ---------------- */

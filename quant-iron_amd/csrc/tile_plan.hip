@@ -30,11 +30,12 @@ int tile_low_qubits(int prec) { return prec == QI_C128 ? 3 : 4; }  // 128-byte r
 
 int default_tile_qubits(const State &s) { return s.prec == QI_C128 ? 11 : 12; }
 
-// complex64 tiles up to 4096 amplitudes run as one wave (no LDS, no
-// barriers, 64 amplitudes per lane); larger and complex128 tiles use 256
-// threads, 512 for the 64 KiB tiles (twice the waves per CU at the same LDS).
+// Tiles of up to 4096 complex64 / 2048 complex128 amplitudes run as one wave
+// (no LDS, no barriers, 64 / 32 amplitudes per lane, the per-op scalar work
+// paid once per tile); larger tiles use 256 threads, 512 for the 64 KiB tiles
+// (twice the waves per CU at the same LDS).
 int default_tile_threads_log2(int prec, int K) {
-    if (prec == QI_C64 && K <= 12) return 6;
+    if (K <= (prec == QI_C64 ? 12 : 11)) return 6;
     return K >= 12 + (prec == QI_C128 ? 0 : 1) ? 9 : 8;
 }
 
@@ -422,9 +423,10 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
     if (TB != 6 && TB != 8 && TB != 9) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile threads out of range");
     const int kmax = s.prec == QI_C128 ? 12 : 13;
     if (K < 10 || K > kmax) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width out of range");
-    // one-wave tiles keep up to 64 amplitudes per lane in registers: complex64, K <= 12
-    if (TB == 6 && (s.prec != QI_C64 || K > 12))
-        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "one-wave tiles need complex64 and at most 12 tile qubits");
+    // one-wave tiles keep up to 128 amplitude components per lane in registers
+    if (TB == 6 && K > (s.prec == QI_C64 ? 12 : 11))
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0,
+                        "one-wave tiles hold at most 12 (complex64) or 11 (complex128) tile qubits");
     if ((int)s.n < K) {  // state smaller than a tile: stream gate by gate
         for (const Lowered &L : gates) plan.steps.push_back(ExecStep::gate(L));
         return Status{};

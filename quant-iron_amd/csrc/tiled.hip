@@ -154,6 +154,16 @@ __device__ __forceinline__ float selm(uint64_t mask, float x, float y) {
     asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(r) : "v"(y), "v"(x), "s"(mask));
     return r;
 }
+// The same for complex128 components: two 32-bit selects on value operands
+// (a plain ?: between two tile registers can become a load through a selected
+// address, which keeps the whole tile in scratch memory).
+__device__ __forceinline__ double selm(uint64_t mask, double x, double y) {
+    const uint64_t bx = (uint64_t)__double_as_longlong(x), by = (uint64_t)__double_as_longlong(y);
+    uint32_t lo, hi;
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(lo) : "v"((uint32_t)by), "v"((uint32_t)bx), "s"(mask));
+    asm("v_cndmask_b32 %0, %1, %2, %3" : "=v"(hi) : "v"((uint32_t)(by >> 32)), "v"((uint32_t)(bx >> 32)), "s"(mask));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
 __device__ __forceinline__ void ip_xor(float &d, uint32_t m) { asm("v_xor_b32 %0, %1, %0" : "+v"(d) : "v"(m)); }
 __device__ __forceinline__ void ip_xor3(float &d, float x, uint32_t m) {  // d = x ^ m (old d dead)
     asm("v_xor_b32 %0, %1, %2" : "+v"(d) : "v"(m), "v"(x));
@@ -686,6 +696,39 @@ __device__ __forceinline__ void xll_op(Regs<float, E> &v, uint32_t lane, int i, 
     }
 }
 
+// The same exchanges for complex128 one-wave tiles (plain selects: 32
+// amplitudes per lane).
+template <int B, typename R, int E>
+__device__ __forceinline__ void xlr_reg_g(Regs<R, E> &v, uint32_t lane, int i) {
+    const int addr = int((lane ^ (1u << i)) << 2);
+    const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0);
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if (r & (1 << B)) continue;
+        const int s = r | (1 << B);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const R y = xlane(addr, selm(hi, v.d[2 * r + c], v.d[2 * s + c]));
+            v.d[2 * r + c] = selm(hi, y, v.d[2 * r + c]);
+            v.d[2 * s + c] = selm(hi, v.d[2 * s + c], y);
+        }
+    }
+}
+template <typename R, int E>
+__device__ __forceinline__ void xlr_op_g(Regs<R, E> &v, uint32_t lane, int i, int j) {
+    constexpr int RB = __builtin_ctz(E);
+#define QI_RB(BB) case BB: if constexpr (BB < RB) xlr_reg_g<BB, R, E>(v, lane, i); break;
+    switch (j) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+template <typename R, int E>
+__device__ __forceinline__ void xll_op_g(Regs<R, E> &v, uint32_t lane, int i, int i2) {
+    const int addr = int((lane ^ ((1u << i) | (1u << i2))) << 2);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != ((lane >> i2) & 1u));
+#pragma unroll
+    for (int k = 0; k < 2 * E; k++) v.d[k] = selm(m, xlane(addr, v.d[k]), v.d[k]);
+}
+
 // ------------------------------------------------------------- LDS ops --
 // Controlled SWAP or Matchgate (CPU form, operator.rs:993-1004) on the LDS
 // copy; local index e lives at slot swz(e).
@@ -755,9 +798,11 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         break;
     case RO_XLR:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
+        else if constexpr (TB == kLaneBits) xlr_op_g<R, E>(v, lane, op.tgt, op.dbit);
         break;
     case RO_XLL:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xll_op<E>(v, lane, op.tgt, op.dbit);
+        else if constexpr (TB == kLaneBits) xll_op_g<R, E>(v, lane, op.tgt, op.dbit);
         break;
     default: break;
     }
@@ -883,14 +928,15 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
     }
 }
 
-// One-wave tiles (complex64, TB = 6): a wave holds a whole tile (64
-// amplitudes per lane for 4096-amplitude tiles) and needs no LDS and no
-// barriers. (A second register set to prefetch the next tile costs more in
+// One-wave tiles (TB = 6): a wave holds a whole tile (64 amplitudes per lane
+// for 4096-amplitude complex64 tiles, 32 for 2048-amplitude complex128 ones)
+// and needs no LDS and no barriers, and the per-op scalar work is paid once
+// per tile. (A second register set to prefetch the next tile costs more in
 // occupancy than it gains: measured slower at 2048 and 1024 amplitudes.)
-template <int K>
-__global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, const RegPass pass,
+template <typename R, int K>
+__global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
                                                   const RegOp *__restrict__ ops) {
-    using R = float;
+    using T = typename V2<R>::T;
     constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
     const uint32_t lane = threadIdx.x;
     const uint64_t base = insert_zeros64(xcd_block_order(blockIdx.x, pass.chunk_log2), pass.qmask);
@@ -905,7 +951,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
     }
     Regs<R, E> v;
     {
-        const float2 *src = psi + (base | at_in);
+        const T *src = psi + (base | at_in);
 #pragma unroll
         for (int r = 0; r < E; r++) {
             uint64_t o = 0;
@@ -927,7 +973,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(float2 *__restrict__ psi, cons
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.
     if (pass.sync_store) __builtin_amdgcn_s_waitcnt(0);
-    float2 *dst = psi + (base | at_out);
+    T *dst = psi + (base | at_out);
 #pragma unroll
     for (int r = 0; r < E; r++) {
         uint64_t o = 0;
@@ -948,8 +994,8 @@ template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p
     RegPass q = p;
     q.chunk_log2 = xcd_chunk_log2(tiles, kTileChunkLog2);
     if constexpr (TB == kLaneBits) {
-        static_assert(sizeof(R) == 4 && K <= 12, "one-wave tiles: complex64, at most 4096 amplitudes");
-        hipLaunchKernelGGL((k_wave_pass<K>), dim3((unsigned)tiles), dim3(64), 0, st, (float2 *)s.d, q, ops);
+        static_assert(K <= (sizeof(R) == 4 ? 12 : 11), "one-wave tiles: at most 64 amplitude components per lane");
+        hipLaunchKernelGGL((k_wave_pass<R, K>), dim3((unsigned)tiles), dim3(64), 0, st, (T *)s.d, q, ops);
     } else {
         hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)s.d, q, ops);
     }
@@ -958,7 +1004,7 @@ template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p
 template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
     switch (p.tb) {
     case 6:
-        if constexpr (sizeof(R) == 4 && K <= 12) { launch_pass<R, K, 6>(s, p, ops, st); return true; }
+        if constexpr (K <= (sizeof(R) == 4 ? 12 : 11)) { launch_pass<R, K, 6>(s, p, ops, st); return true; }
         return false;
     case 8: launch_pass<R, K, 8>(s, p, ops, st); return true;
     case 9: launch_pass<R, K, 9>(s, p, ops, st); return true;

@@ -272,7 +272,7 @@ def test_tiled_every_width_and_thread_count(ctx, prec):
     ref = s.state_vector
     kmax = 12 if prec == "c128" else 13
     for K in range(10, kmax + 1):
-        for tb in ((6, 8, 9) if prec == "c64" and K <= 12 else (8, 9)):
+        for tb in ((6, 8, 9) if K <= (12 if prec == "c64" else 11) else (8, 9)):
             s = upload(psi, prec)
             s.apply_gates_inplace(gates, mode=L.EXEC_TILED, tile_qubits=K, tile_threads_log2=tb)
             got = s.state_vector
