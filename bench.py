@@ -433,7 +433,7 @@ def main():
                                 pass_GBps=achieved, norm=norm)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": f"k_wave_pass<{tile_k}>" if tile_tb == 6 else f"k_reg_pass<float, {tile_k}, {tile_tb}>",
+                "kernel": f"k_wave_pass<float, {tile_k}>" if tile_tb == 6 else f"k_reg_pass<float, {tile_k}, {tile_tb}>",
                 "algorithmic_bytes_per_launch": 2 * S64}
     if n == 30 and depth == 40:
         src = "profiles/k_wave_pass_c64_30q_random_circuit.txt"
