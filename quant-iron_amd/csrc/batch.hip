@@ -65,18 +65,31 @@ struct OpC {
     double c[8];
 };
 
-__device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ angles, int slot_base, double *c) {
+// Per-slot trigonometry, once per run (not per op and thread): slot i holds
+// (cos th/2, sin th/2, cos th, sin th) of angle i.
+__global__ __launch_bounds__(256) void k_batch_resolve(const double *__restrict__ angles, uint32_t n,
+                                                       double *__restrict__ coef) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const double th = angles[i];
+    double sh, ch, s1, c1;
+    sincos(th / 2.0, &sh, &ch);
+    sincos(th, &s1, &c1);
+    coef[4 * i] = ch; coef[4 * i + 1] = sh; coef[4 * i + 2] = c1; coef[4 * i + 3] = s1;
+}
+
+__device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ coef, int slot_base, double *c) {
 #pragma unroll
     for (int q = 0; q < 8; q++) c[q] = op.c[q];
     const int slot = (int)op.pad;
     if (slot < 0) return;
-    const double th = angles[slot_base + slot];
-    double sh, ch;
+    const double *k = coef + 4 * (size_t)(slot_base + slot);
+    const double ch = k[0], sh = k[1];
     switch (op.kind) {
-    case LK_RY: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = sh; break;
-    case LK_RZ: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = -sh; c[2] = ch; c[3] = sh; break;
-    case LK_RX: sincos(th / 2.0, &sh, &ch); c[0] = ch; c[1] = 0.0 * sh; c[2] = sh; c[3] = -0.0 * sh; c[4] = -sh; break;
-    case LK_PHASE: sincos(th, &sh, &ch); c[2] = ch; c[3] = sh; break;
+    case LK_RY: c[0] = ch; c[1] = sh; break;
+    case LK_RZ: c[0] = ch; c[1] = -sh; c[2] = ch; c[3] = sh; break;
+    case LK_RX: c[0] = ch; c[1] = 0.0 * sh; c[2] = sh; c[3] = -0.0 * sh; c[4] = -sh; break;
+    case LK_PHASE: c[2] = k[2]; c[3] = k[3]; break;
     default: break;
     }
 }
@@ -197,7 +210,7 @@ struct SmallTask {
 
 template <typename R, int KMAX>
 __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict__ tasks, const TileOp *__restrict__ ops,
-                                                     const ETerm *__restrict__ terms, const double *__restrict__ angles,
+                                                     const ETerm *__restrict__ terms, const double *__restrict__ coef,
                                                      double *__restrict__ energies) {
     using T = typename V2<R>::T;
     __shared__ T tile[1 << KMAX];
@@ -211,7 +224,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
     for (uint32_t o = 0; o < tk.nops; o++) {
         const TileOp op = ops[tk.op0 + o];
         double c[8];
-        resolve(op, angles, tk.slot_base, c);
+        resolve(op, coef, tk.slot_base, c);
         tile_apply<R, (1 << KMAX)>(tile, op, c, 0, (int)tk.n);
         __syncthreads();
     }
@@ -249,7 +262,7 @@ __device__ __forceinline__ int find_task(const BigTask *__restrict__ tasks, int 
 template <typename R, int K>
 __global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restrict__ arena, const BigTask *__restrict__ tasks,
                                                     int ntasks, const TileOp *__restrict__ ops,
-                                                    const double *__restrict__ angles) {
+                                                    const double *__restrict__ coef) {
     using T = typename V2<R>::T;
     constexpr int N = 1 << K, PER = N / kBT;
     __shared__ T tile[N];
@@ -275,7 +288,7 @@ __global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restric
         const TileOp op = ops[tk.op0 + o];
         if ((base & op.gcm) != op.gcm) continue;
         double c[8];
-        resolve(op, angles, tk.slot_base, c);
+        resolve(op, coef, tk.slot_base, c);
         tile_apply<R, N>(tile, op, c, base, K);
         __syncthreads();
     }
@@ -362,7 +375,7 @@ struct Batch {
     std::vector<double> angles;  // host copy of the angle table
     std::vector<ETerm> eterms;   // host copy of the term table
     // device
-    DevBuf d_ops, d_terms, d_small, d_big, d_fold, d_angles, d_part, d_energy, d_arena;
+    DevBuf d_ops, d_terms, d_small, d_big, d_fold, d_angles, d_coef, d_part, d_energy, d_arena;
     std::vector<std::pair<size_t, size_t>> levels;  // [task begin, task end) of each pass level
     std::vector<uint32_t> level_blocks;
     std::vector<std::pair<size_t, size_t>> elevels;
@@ -376,7 +389,7 @@ struct Batch {
     hipGraphExec_t graph = nullptr;
     double last_ms = 0.0;
     ~Batch() {
-        for (DevBuf *b : {&d_ops, &d_terms, &d_small, &d_big, &d_fold, &d_angles, &d_part, &d_energy, &d_arena})
+        for (DevBuf *b : {&d_ops, &d_terms, &d_small, &d_big, &d_fold, &d_angles, &d_coef, &d_part, &d_energy, &d_arena})
             if (b->p) (void)hipFree(b->p);
         if (graph) (void)hipGraphExecDestroy(graph);
         if (side) (void)hipStreamDestroy(side);
@@ -565,6 +578,7 @@ Status batch_plan(Batch &B) {
     if (!(e = upload_vec(B.d_big, big)).ok()) return e;
     if (!(e = upload_vec(B.d_fold, folds)).ok()) return e;
     if (!(e = upload_vec(B.d_angles, B.angles)).ok()) return e;
+    if (!(e = ensure(B.d_coef, std::max<size_t>(B.angles.size(), 1) * 4 * sizeof(double))).ok()) return e;
     if (!(e = ensure(B.d_part, std::max<size_t>(nparts, 1) * 4 * sizeof(double))).ok()) return e;
     if (!(e = ensure(B.d_energy, std::max<size_t>(B.circs.size(), 1) * 2 * sizeof(double))).ok()) return e;
     const size_t esz = c128 ? 16 : 8;
@@ -586,7 +600,7 @@ template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st)
         auto r = B.levels[l];
         hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kBT), 0, st, (T *)B.d_arena.p,
                            big + r.first, (int)(r.second - r.first), (const TileOp *)B.d_ops.p,
-                           (const double *)B.d_angles.p);
+                           (const double *)B.d_coef.p);
     }
     for (size_t l = 0; l < B.elevels.size(); l++) {
         auto r = B.elevels[l];
@@ -599,6 +613,9 @@ template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st)
 }
 
 static Status batch_launch(Batch &B, hipStream_t st) {
+    if (!B.angles.empty())
+        hipLaunchKernelGGL(k_batch_resolve, dim3((unsigned)((B.angles.size() + 255) / 256)), dim3(256), 0, st,
+                           (const double *)B.d_angles.p, (uint32_t)B.angles.size(), (double *)B.d_coef.p);
     // fork: LDS-resident circuits on the side stream, tiled levels on `st`
     QI_HIP_TRY(hipEventRecord(B.fork, st));
     QI_HIP_TRY(hipStreamWaitEvent(B.side, B.fork, 0));
@@ -606,11 +623,11 @@ static Status batch_launch(Batch &B, hipStream_t st) {
         if (B.prec == QI_C128)
             hipLaunchKernelGGL((k_batch_small<double, 12>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
-                               (const double *)B.d_angles.p, (double *)B.d_energy.p);
+                               (const double *)B.d_coef.p, (double *)B.d_energy.p);
         else
             hipLaunchKernelGGL((k_batch_small<float, 13>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
-                               (const double *)B.d_angles.p, (double *)B.d_energy.p);
+                               (const double *)B.d_coef.p, (double *)B.d_energy.p);
     }
     if (B.prec == QI_C128) {
         switch (B.K) {
@@ -770,9 +787,11 @@ qi_status qi_batch_set_angles(qi_batch_t b, const double *angles, size_t nangles
     if (!b || (nangles && !angles)) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
     Batch &B = b->b;
     std::lock_guard<std::mutex> lk(B.ctx->mu);
+    const size_t before = B.angles.size();
     B.angles.assign(angles, angles + nangles);
     if (B.planned) {
-        if (B.d_angles.bytes < nangles * sizeof(double)) { B.planned = false; QB_RET(Status{}); }
+        // the graph's slot kernel covers the planned table size
+        if (nangles != before || B.d_angles.bytes < nangles * sizeof(double)) { B.planned = false; QB_RET(Status{}); }
         hipError_t e = hipMemcpyAsync(B.d_angles.p, B.angles.data(), nangles * sizeof(double), hipMemcpyHostToDevice,
                                       B.ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(B.ctx->stream);
