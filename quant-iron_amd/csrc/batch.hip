@@ -58,6 +58,9 @@ __device__ __forceinline__ uint64_t deposit_pos(uint32_t e, const uint8_t *pos, 
 }
 
 constexpr int kBT = 256;
+// Threads of a tiled pass (a 2^K-amplitude tile in LDS; 256 measured fastest
+// against 64, 128 and 512 on the 4096-circuit ansatz batch).
+constexpr int kPT = 256, kPTLog = __builtin_ctz(kPT);
 
 // Coefficients of one op; parametric ops take their angle from the table
 // (same formulas as lower_gate, evaluated on the device).
@@ -95,7 +98,7 @@ __device__ __forceinline__ void resolve(const TileOp &op, const double *__restri
 }
 
 // Apply one op to a tile held in LDS (shared by the LDS-resident and the tiled kernels).
-template <typename R, int N>
+template <typename R, int N, int NT>
 __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp &op, const double *c, uint64_t base,
                                            int nloc) {
     using T = typename V2<R>::T;
@@ -104,7 +107,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
     switch (op.kind) {
     case LK_Z: case LK_PHASE: case LK_RZ: {
         T p0 = mk<T>(R(c[0]), R(c[1])), p1 = mk<T>(R(c[2]), R(c[3]));
-        for (uint32_t e = tid; e < nel; e += kBT) {
+        for (uint32_t e = tid; e < nel; e += NT) {
             if ((e & lcm) != lcm) continue;
             bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
             T a = tile[e];
@@ -117,7 +120,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
     case LK_SWAP: case LK_MATCH: {
         const uint32_t b0 = 1u << op.lp0, b1 = 1u << op.lp1;
         const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
-        for (uint32_t q = tid; q < (nel >> 2); q += kBT) {
+        for (uint32_t q = tid; q < (nel >> 2); q += NT) {
             uint32_t l = ins0(ins0(q, lo), hi);
             if (op.kind == LK_SWAP) {
                 uint32_t i = l | b0, j = l | b1;
@@ -137,7 +140,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
     } break;
     default: {
         const uint32_t tb = 1u << op.lp0;
-        for (uint32_t q = tid; q < (nel >> 1); q += kBT) {
+        for (uint32_t q = tid; q < (nel >> 1); q += NT) {
             uint32_t i = ins0(q, op.lp0), j = i | tb;
             if ((i & lcm) != lcm) continue;
             T a = tile[i], b = tile[j], na, nb;
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
         const TileOp op = ops[tk.op0 + o];
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, (1 << KMAX)>(tile, op, c, 0, (int)tk.n);
+        tile_apply<R, (1 << KMAX), kBT>(tile, op, c, 0, (int)tk.n);
         __syncthreads();
     }
     DD v[2] = {dd_zero(), dd_zero()};
@@ -260,28 +263,28 @@ __device__ __forceinline__ int find_task(const BigTask *__restrict__ tasks, int 
 }
 
 template <typename R, int K>
-__global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restrict__ arena, const BigTask *__restrict__ tasks,
+__global__ __launch_bounds__(kPT) void k_batch_pass(typename V2<R>::T *__restrict__ arena, const BigTask *__restrict__ tasks,
                                                     int ntasks, const TileOp *__restrict__ ops,
                                                     const double *__restrict__ coef) {
     using T = typename V2<R>::T;
-    constexpr int N = 1 << K, PER = N / kBT;
+    constexpr int N = 1 << K, PER = N / kPT;
     __shared__ T tile[N];
     const int ti = find_task(tasks, ntasks, blockIdx.x);
     const BigTask &tk = tasks[ti];
     const uint64_t base = insert_zeros64(uint64_t(blockIdx.x - tk.block0), tk.qmask);
     T *psi = arena + tk.off;
-    const uint64_t off_t = deposit_pos<8>(threadIdx.x, tk.pos, 8);
+    const uint64_t off_t = deposit_pos<kPTLog>(threadIdx.x, tk.pos, kPTLog);
     if (tk.init) {
 #pragma unroll
         for (int it = 0; it < PER; it++) {
-            const uint32_t e = threadIdx.x + it * kBT;
+            const uint32_t e = threadIdx.x + it * kPT;
             T z; z.x = (base == 0 && e == 0) ? R(1) : R(0); z.y = R(0);
             tile[e] = z;
         }
     } else {
 #pragma unroll
         for (int it = 0; it < PER; it++)
-            tile[threadIdx.x + it * kBT] = psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)];
+            tile[threadIdx.x + it * kPT] = psi[base | off_t | deposit_pos<K>(uint32_t(it) << kPTLog, tk.pos, K)];
     }
     __syncthreads();
     for (uint32_t o = 0; o < tk.nops; o++) {
@@ -289,12 +292,12 @@ __global__ __launch_bounds__(kBT) void k_batch_pass(typename V2<R>::T *__restric
         if ((base & op.gcm) != op.gcm) continue;
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, N>(tile, op, c, base, K);
+        tile_apply<R, N, kPT>(tile, op, c, base, K);
         __syncthreads();
     }
 #pragma unroll
     for (int it = 0; it < PER; it++)
-        psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)] = tile[threadIdx.x + it * kBT];
+        psi[base | off_t | deposit_pos<K>(uint32_t(it) << kPTLog, tk.pos, K)] = tile[threadIdx.x + it * kPT];
 }
 
 template <typename R, int K>
@@ -598,7 +601,7 @@ template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st)
     const BigTask *big = (const BigTask *)B.d_big.p;
     for (size_t l = 0; l < B.levels.size(); l++) {
         auto r = B.levels[l];
-        hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kBT), 0, st, (T *)B.d_arena.p,
+        hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kPT), 0, st, (T *)B.d_arena.p,
                            big + r.first, (int)(r.second - r.first), (const TileOp *)B.d_ops.p,
                            (const double *)B.d_coef.p);
     }
