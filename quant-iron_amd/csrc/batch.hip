@@ -183,15 +183,21 @@ __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const
         const uint64_t gi = base | deposit_pos<KMAX>(e, pos, nloc);
         const double ar = tile[e].x, ai = tile[e].y;
         double hr = 0.0, hi = 0.0;
+        double wr = 0.0, wi = 0.0;
+        uint32_t last = ~0u;
         for (int t = 0; t < nt; t++) {
             const ETerm tm = terms[t];
             // local partner: x lies inside the tile, so flipping its global
-            // bits flips the same local bits (tm.pad holds the local x mask)
-            const uint32_t le = e ^ tm.pad;
-            const double br = tile[le].x, bi = tile[le].y;
+            // bits flips the same local bits (tm.pad holds the local x mask).
+            // w = conj(a) * b, shared by consecutive terms with one x mask
+            if (tm.pad != last) {
+                last = tm.pad;
+                const uint32_t le = e ^ tm.pad;
+                const double br = tile[le].x, bi = tile[le].y;
+                wr = ar * br + ai * bi;
+                wi = ar * bi - ai * br;
+            }
             const unsigned q = (tm.q0 + 2u * (unsigned)__popcll(gi & tm.z)) & 3u;
-            // w = conj(a) * b
-            const double wr = ar * br + ai * bi, wi = ar * bi - ai * br;
             // c * i^q
             const double pr = q == 0 ? tm.c[0] : q == 1 ? -tm.c[1] : q == 2 ? -tm.c[0] : tm.c[1];
             const double pi = q == 0 ? tm.c[1] : q == 1 ? tm.c[0] : q == 2 ? -tm.c[1] : -tm.c[0];
@@ -480,7 +486,12 @@ Status batch_plan(Batch &B) {
             SmallTask st{};
             st.n = P.n; st.op0 = P.op0; st.nops = P.nops; st.out = (uint32_t)ci; st.slot_base = C.slot_base;
             st.t0 = (uint32_t)B.eterms.size(); st.nt = (uint32_t)C.terms.size();
-            for (size_t k = 0; k < C.terms.size(); k++) {
+            std::vector<size_t> order(C.terms.size());
+            for (size_t k = 0; k < order.size(); k++) order[k] = k;
+            // terms sharing an x mask next to each other: they share conj(a) b
+            std::stable_sort(order.begin(), order.end(),
+                             [&](size_t a, size_t b) { return C.terms[a].x_mask < C.terms[b].x_mask; });
+            for (size_t k : order) {
                 ETerm t = make_eterm(C.terms[k]);
                 t.pad = (uint32_t)C.terms[k].x_mask;  // local = global inside a whole-state tile
                 C.eterm_idx[k] = (uint32_t)B.eterms.size();
@@ -535,6 +546,9 @@ Status batch_plan(Batch &B) {
             for (int b = 0; b < (int)P.n; b++)
                 if ((q >> b) & 1) { local[b] = j; bt.pos[j++] = (uint8_t)b; }
             bt.t0 = (uint32_t)B.eterms.size(); bt.nt = (uint32_t)take.size();
+            // terms sharing an x mask next to each other: they share conj(a) b
+            std::stable_sort(take.begin(), take.end(),
+                             [&](size_t a, size_t b) { return C.terms[a].x_mask < C.terms[b].x_mask; });
             for (size_t k : take) {
                 ETerm t = make_eterm(C.terms[k]);
                 uint32_t lx = 0;
