@@ -143,6 +143,14 @@ struct RegPlanner {
     // one-sided phase (half the work on a register target) while the global
     // factor collects here, applied once by the plan's last pass.
     double *gphase = nullptr;  // (re, im)
+    // Uncontrolled H on register bits run unscaled (RO_HS); their factors
+    // (1/sqrt(2) as lowered) multiply up here and go into the global phase.
+    // Every kMaxUnscaled of them a scaling diagonal is emitted, so complex64
+    // amplitudes grow by at most 2 per unscaled op between scalings, 2^60
+    // in all: far below overflow.
+    int *unscaled = nullptr;
+    double *hfactor = nullptr;
+    static constexpr int kMaxUnscaled = 60;
     // The last uncontrolled diagonal op on each diagonal-op position (register
     // bit, local bit, or qubit outside the tile) that no later op has acted on
     // non-diagonally: a new uncontrolled diagonal there folds into it
@@ -341,6 +349,24 @@ struct RegPlanner {
                 op.lp0 = __builtin_ctz(op.rcm);
             }
             if (f32) shear_form(op);
+            if (f32 && G.kind == LK_H && op.kind == RO_RR && op.tgt < kLaneTarget && !op.rcm && !op.tcm && !op.gcm) {
+                op.kind = RO_HS;
+                *hfactor *= G.c[0];
+                for (int k = 0; k < 8; k++) op.c[k] = 0;
+                out.push_back(op);
+                if (++*unscaled == kMaxUnscaled) {  // renormalise: the collected factor on every amplitude
+                    RegOp sc{};
+                    sc.kind = RO_DG; sc.dloc = 0; sc.dbit = 0;
+                    sc.c[0] = sc.c[2] = *hfactor;
+                    *unscaled = 0;
+                    *hfactor = 1.0;
+                    sc.src = op.src;
+                    forget_diagonals();  // not folded: it must act on every amplitude now
+                    out.push_back(sc);
+                }
+                if (op.tgt >= 0) last_diag[0][op.tgt] = -1;
+                return Status{};
+            }
         } break;
         }
         if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
@@ -442,6 +468,8 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         for (size_t g = 0; g < gates.size(); g++)
             streamed[g] = (gates[g].kind == LK_SWAP || gates[g].kind == LK_MATCH) && !is_relabel(gates[g]);
     double gphase[2] = {1.0, 0.0};
+    int unscaled = 0;      // unscaled H ops since the last scaling
+    double hfactor = 1.0;  // their factors, not applied yet
     size_t first = 0;
     while (first < gates.size()) {
         if (streamed[first]) {
@@ -456,6 +484,8 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         P.K = K; P.RB = RB; P.L = low; P.TB = TB;
         P.f32 = s.prec == QI_C64;
         P.gphase = gphase;
+        P.unscaled = &unscaled;
+        P.hfactor = &hfactor;
         P.forget_diagonals();
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
         RegPass rp{};
@@ -586,6 +616,8 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         plan.steps.push_back(ExecStep::reg_step(rp));
         while (first < gates.size() && done[first]) first++;
     }
+    gphase[0] *= hfactor;
+    gphase[1] *= hfactor;
     if (gphase[0] != 1.0 || gphase[1] != 0.0) {
         // the collected global phase, as a last op of the last pass (whose ops
         // close the op table): it commutes with everything

@@ -637,6 +637,28 @@ template <bool XS, int E> __device__ __forceinline__ void shear_op(Regs<float, E
 #undef QI_RB
 }
 
+// Unscaled H on register bit B: (a, b) <- (a + b, a - b) per component as
+// a += b, then b = a - 2b (two in-place instructions instead of four).
+template <int B, int E> __device__ __forceinline__ void hs_reg(Regs<float, E> &v) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if ((r >> B) & 1) continue;
+        const int s = r | (1 << B);
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            float &a = v.d[2 * r + c], &b = v.d[2 * s + c];
+            asm("v_add_f32 %0, %0, %1" : "+v"(a) : "v"(b));
+            asm("v_fma_f32 %0, -2.0, %0, %1" : "+v"(b) : "v"(a));
+        }
+    }
+}
+template <int E> __device__ __forceinline__ void hs_op(Regs<float, E> &v, const RegOp &op) {
+    constexpr int RB = __builtin_ctz(E);
+#define QI_RB(BB) case BB: if constexpr (BB < RB) hs_reg<BB, E>(v); break;
+    switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+
 // ------------------------------------------------------- exchange ops --
 // One-wave tiles change layout without LDS. Lane bit i <-> register bit B:
 // the amplitude at (lane bit i = a, register bit B = b) moves to
@@ -795,6 +817,9 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         break;
     case RO_RXS:
         if constexpr (sizeof(R) == 4) shear_op<true, E>(v, op);
+        break;
+    case RO_HS:
+        if constexpr (sizeof(R) == 4) hs_op<E>(v, op);
         break;
     case RO_XLR:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);

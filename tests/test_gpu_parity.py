@@ -306,6 +306,30 @@ def test_plan_and_graph_caches_follow_the_gate_list(ctx):
             assert np.max(np.abs(s.state_vector - expect[name])) <= 1e-12, (name, graph)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_swaps_relabel_qubits_instead_of_moving_data(ctx, prec):
+    """Uncontrolled SWAPs rename qubits for the rest of the circuit: layers of
+    RY on every qubit with SWAPs across the two halves of a 20-qubit state take
+    the three 10-qubit passes the RY layers need (each tile holds the four
+    lowest qubits), with an even number of SWAP layers (nothing to undo) or an
+    odd one (the final permutation rides along). Kept as data moves, each SWAP
+    layer would need tiles spanning both halves. Both against the oracle."""
+    n = 20
+    rng = np.random.default_rng(99)
+    for layers, max_steps in ((6, 3), (5, 3)):
+        ds = []
+        for _ in range(layers):
+            ds += [{"kind": "RY", "targets": [q], "params": [float(rng.uniform(0, 6.28))]} for q in range(n)]
+            ds += [{"kind": "SWAP", "targets": [i, i + n // 2]} for i in range(n // 2)]
+        gates = gates_array(ds)
+        psi = random_state(n, rng)
+        ref = O.apply_gates(psi, gates)
+        s = upload(psi, prec)
+        s.apply_gates_inplace(gates, mode=L.EXEC_TILED, tile_qubits=10)
+        assert s.ctx.last_exec_steps() <= max_steps, (layers, s.ctx.last_exec_steps())
+        check(s.state_vector, ref, prec, f"{layers} layers")
+
+
 def test_circuit_with_measurement_and_pauli_gates(ctx):
     """Gate::Measurement / PauliString / PauliTimeEvolution inside Circuit::execute."""
     n = 12
