@@ -67,6 +67,23 @@ def test_batch_energies_match_one_at_a_time_oracle(ctx, prec):
     assert np.max(np.abs(e3 - ref3)) <= tol
 
 
+def test_batch_angle_table_of_another_size_replans(ctx):
+    """A table with extra (unused) slots changes the size the per-slot
+    trigonometry kernel covers: the batch is planned again and gives the same
+    energies; a same-size table afterwards replays the graph."""
+    specs = W.batched_ansatz(count=12, seed=5, nmin=3, nmax=15, lmin=1, lmax=3)
+    b = qiron.CircuitBatch("c128")
+    angles, refs = build(b, specs)
+    e = b.run(use_graph=True)
+    b.set_angles(np.concatenate([angles, np.full(7, 1.25)]))
+    e2 = b.run(use_graph=True)
+    assert np.max(np.abs(e2 - e)) <= 1e-13
+    b.set_angles(np.concatenate([angles, np.full(7, -0.5)]))
+    e3 = b.run(use_graph=True)
+    assert np.max(np.abs(e3 - e)) <= 1e-13
+    assert np.max(np.abs(e - oracle_energies(refs))) <= 1e-11
+
+
 def test_batch_rejects_invalid_gates(ctx):
     b = qiron.CircuitBatch("c64")
     g = W.GateList()
