@@ -330,6 +330,32 @@ def test_swaps_relabel_qubits_instead_of_moving_data(ctx, prec):
         check(s.state_vector, ref, prec, f"{layers} layers")
 
 
+def _inverse(gates):
+    """The inverse gate list of a circuit of H, X, Y, Z, RX, RY, RZ, P, CNOT, CZ,
+    SWAP and Toffoli: reversed, rotation and phase angles negated."""
+    inv = gates[::-1].copy()
+    rot = np.isin(inv["kind"], [L.GATE_KINDS[k] for k in ("RX", "RY", "RZ", "P")])
+    inv["params"][rot, 0] = -inv["params"][rot, 0]
+    return inv
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_random_circuit_then_inverse_restores_the_state_at_26_qubits(ctx, prec):
+    """A size-independent check of the tiled executor at a large size: the
+    depth-40 random circuit (relabeled SWAPs, shear rotations, unscaled H,
+    controlled ops on every kind of tile position) followed by its inverse
+    gives back |+...+>; the norm stays 1 throughout."""
+    n = 26
+    gates = W.random_circuit(n, 40, seed=4321)
+    s = qiron.State.new_plus(n, prec)
+    s.apply_gates_inplace(gates, mode=L.EXEC_TILED, use_graph=True)
+    assert abs(s.norm_sqr() - 1.0) <= (1e-12 if prec == "c128" else 1e-4)
+    s.apply_gates_inplace(_inverse(gates), mode=L.EXEC_TILED)
+    got = s.state_vector
+    d = float(np.max(np.abs(got - 2.0 ** (-n / 2))))
+    assert d <= (1e-12 if prec == "c128" else 2e-6), d
+
+
 def test_circuit_with_measurement_and_pauli_gates(ctx):
     """Gate::Measurement / PauliString / PauliTimeEvolution inside Circuit::execute."""
     n = 12
