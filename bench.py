@@ -296,6 +296,30 @@ def profiled_traffic(kernel: str, path: str):
     return None
 
 
+def inverse_gates(L, gates):
+    """The inverse of a circuit of H, X, Y, Z, RX, RY, RZ, P, CNOT, CZ, SWAP and
+    Toffoli: reversed, rotation and phase angles negated."""
+    import numpy as np
+
+    inv = gates[::-1].copy()
+    rot = np.isin(inv["kind"], [L.GATE_KINDS[k] for k in ("RX", "RY", "RZ", "P")])
+    inv["params"][rot, 0] = -inv["params"][rot, 0]
+    return inv
+
+
+def round_trip(qiron, L, n, prec, gates, opts):
+    """1 - |<+...+| U^-1 U |+...+>|^2 at the full size: the circuit, then its
+    inverse, through the same executor (a size-independent check of the path
+    being timed; FP64 reduction on the device)."""
+    s = qiron.State.new_plus(n, prec)
+    s.apply_gates_inplace(gates, **opts)
+    s.apply_gates_inplace(inverse_gates(L, gates), **opts)
+    ref = qiron.State.new_plus(n, prec)
+    f = s.fs_fidelity(ref)
+    del s, ref
+    return 1.0 - f
+
+
 def profiled_valu(kernel: str, path: str):
     """wave64 VALU instructions per dispatch of `kernel` from a committed
     rocprofv3 --pmc summary (SQ_INSTS_VALU) of this same workload; None when
@@ -449,12 +473,17 @@ def main():
                                 "issue_frac": round(valu_ms / pass_ms, 4), "source": vsrc}
 
     if not args.only:
+        rt = round_trip(qiron, L, n, "c64", gates, tiled)
+        results["c64_tiled"]["round_trip_infidelity"] = rt
+        log(f"c64 tiled round trip (circuit, then its inverse) at {n}q: 1 - fidelity {rt:.2e}")
         single_gates(qiron, L, W, ctx, n, results)
         # ---- complex128 tiled
         wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c128", gates, tiled, 1, 1)
         results["c128_tiled"] = dict(gates_per_s=G / wall, ms_per_step=wall * 1e3, passes=passes,
                                      pass_GBps=2 * (1 << n) * 16 * passes / (dev_ms * 1e-3) / 1e9)
-        log(f"c128 tiled: {wall * 1e3:.1f} ms/step {G / wall:.1f} gates/s")
+        results["c128_tiled"]["round_trip_infidelity"] = round_trip(qiron, L, n, "c128", gates, tiled)
+        log(f"c128 tiled: {wall * 1e3:.1f} ms/step {G / wall:.1f} gates/s, round trip 1 - fidelity "
+            f"{results['c128_tiled']['round_trip_infidelity']:.2e}")
         # ---- per-gate streaming and fused MFMA on the first 5 layers
         sub = gates[: (G // depth) * 5]
         for name, opts, prec in (("c64_pergate", dict(mode=L.EXEC_PER_GATE), "c64"),
