@@ -143,11 +143,12 @@ struct RegPlanner {
     // one-sided phase (half the work on a register target) while the global
     // factor collects here, applied once by the plan's last pass.
     double *gphase = nullptr;  // (re, im)
-    // Uncontrolled H on register bits run unscaled (RO_HS); their factors
-    // (1/sqrt(2) as lowered) multiply up here and go into the global phase.
-    // Every kMaxUnscaled of them a scaling diagonal is emitted, so complex64
-    // amplitudes grow by at most 2 per unscaled op between scalings, 2^60
-    // in all: far below overflow.
+    // Uncontrolled multiples of H run unscaled (RO_HS) and multiples of a
+    // rotation as unit shears (RO_RRS, RO_RXS) on register bits; their
+    // factors (1/sqrt(2) for H) multiply up here and go into the global
+    // phase. Every kMaxUnscaled of them a scaling diagonal is emitted, so
+    // complex64 amplitudes grow by at most 2 per such op between scalings,
+    // 2^60 in all: far below overflow.
     int *unscaled = nullptr;
     double *hfactor = nullptr;
     static constexpr int kMaxUnscaled = 60;
@@ -348,25 +349,18 @@ struct RegPlanner {
                 op.kind = RO_XC1;
                 op.lp0 = __builtin_ctz(op.rcm);
             }
-            if (f32) shear_form(op);
-            if (f32 && G.kind == LK_H && op.kind == RO_RR && op.tgt < kLaneTarget && !op.rcm && !op.tcm && !op.gcm) {
+            if (f32 && op.kind == RO_RR && op.tgt < kLaneTarget && !op.rcm && !op.tcm && !op.gcm &&
+                op.c[0] == op.c[1] && op.c[0] == op.c[2] && op.c[0] == -op.c[3] && op.c[0] != 0) {
+                // a multiple of [[1, 1], [1, -1]] (H, and the X basis changes of Trotter steps)
+                const double f = op.c[0];
                 op.kind = RO_HS;
-                *hfactor *= G.c[0];
                 for (int k = 0; k < 8; k++) op.c[k] = 0;
                 out.push_back(op);
-                if (++*unscaled == kMaxUnscaled) {  // renormalise: the collected factor on every amplitude
-                    RegOp sc{};
-                    sc.kind = RO_DG; sc.dloc = 0; sc.dbit = 0;
-                    sc.c[0] = sc.c[2] = *hfactor;
-                    *unscaled = 0;
-                    *hfactor = 1.0;
-                    sc.src = op.src;
-                    forget_diagonals();  // not folded: it must act on every amplitude now
-                    out.push_back(sc);
-                }
-                if (op.tgt >= 0) last_diag[0][op.tgt] = -1;
+                defer_factor(f, op.src);
+                last_diag[0][op.tgt] = -1;
                 return Status{};
             }
+            if (f32) shear_form(op);
         } break;
         }
         if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
@@ -392,13 +386,21 @@ struct RegPlanner {
     // angle is brought into [-pi/2, pi/2] (tan(phi/2) bounded by 1) by
     // R(phi) = -R(phi - pi), the sign going to the global phase, which needs
     // the op uncontrolled; otherwise wide angles keep the plain form.
-    void shear_form(RegOp &op) const {
+    void shear_form(RegOp &op) {
         if (op.tgt >= kLaneTarget || op.rcm || op.tcm) return;
         double c, s;
         if (op.kind == RO_RR && op.c[0] == op.c[3] && op.c[1] == -op.c[2]) { c = op.c[0]; s = op.c[2]; }
         else if (op.kind == RO_RXI && op.c[0] == op.c[3] && op.c[1] == op.c[2]) { c = op.c[0]; s = op.c[1]; }
         else return;
-        if (std::fabs(c * c + s * s - 1.0) > 1e-12) return;
+        // a multiple r of a rotation (uncontrolled only: r joins the deferred factors)
+        const double r2 = c * c + s * s;
+        double r = 1.0;
+        if (std::fabs(r2 - 1.0) > 1e-12) {
+            if (op.gcm || r2 == 0.0) return;
+            r = std::sqrt(r2);
+            c /= r;
+            s /= r;
+        }
         double phi = std::atan2(s, c);
         if (std::fabs(phi) > M_PI / 2) {
             if (op.gcm) return;
@@ -410,6 +412,22 @@ struct RegPlanner {
         op.c[0] = -std::tan(phi / 2);
         op.c[1] = std::sin(phi);
         op.c[2] = op.c[3] = 0;
+        if (r != 1.0) defer_factor(r, op.src);
+    }
+    // A scalar factor of an uncontrolled op, applied later: multiplied into
+    // the pending product; every kMaxUnscaled factors the product is applied
+    // by a diagonal on every amplitude (emitted here).
+    void defer_factor(double f, uint32_t src) {
+        *hfactor *= f;
+        if (++*unscaled < kMaxUnscaled) return;
+        RegOp sc{};
+        sc.kind = RO_DG; sc.dloc = 0; sc.dbit = 0;
+        sc.c[0] = sc.c[2] = *hfactor;
+        sc.src = src;
+        *unscaled = 0;
+        *hfactor = 1.0;
+        forget_diagonals();  // not folded: it must act on every amplitude now
+        out.push_back(sc);
     }
     // diag(p0, p1) of an uncontrolled diagonal op
     static void diag_of(const RegOp &o, double *p) {
