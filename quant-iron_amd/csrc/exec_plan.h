@@ -54,8 +54,11 @@ enum RegOpKind : int32_t {
                        // shears per component: a += c0 b, b += c1 a, a += c0 b (c0 = -tan(phi/2), c1 = sin phi)
     RO_RXS,            // RO_RXI rotation [[cos, i sin], [i sin, cos]]: shears (c0, c1) on (a.re, b.im) and
                        // (-c0, -c1) on (a.im, b.re)
-    RO_HS,             // H without its 1/sqrt(2) on a register bit, uncontrolled: (a, b) <- (a + b, a - b); the
-                       // planner applies the collected factor through an uncontrolled diagonal
+    RO_P1S,            // RO_P1 on a register bit, no in-tile control, |phase| <= pi/2: the target-1 amplitudes'
+                       // (re, im) rotated by three shears (c0 = -tan(phi/2), c1 = sin phi)
+    RO_HS,             // a multiple of [[1, 1], [1, -1]] on a register bit, uncontrolled, unscaled:
+                       // (a, b) <- (a + b, a - b); the planner applies the collected factor through an
+                       // uncontrolled diagonal
     RO_XLR,            // exchange the local bits of lane bit tgt and register bit dbit
     RO_XLL,            // exchange the local bits of lane bits tgt and dbit
     RO_TRANSPOSE,      // new layout through LDS

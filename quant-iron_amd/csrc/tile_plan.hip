@@ -625,8 +625,21 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
                     P.out.end());
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
-        for (RegOp &op : P.out)
+        for (RegOp &op : P.out) {
+            // (c64) a one-sided phase on a register bit as a rotation of (re, im) by shears,
+            // once no later diagonal can fold into it
+            if (P.f32 && op.kind == RO_P1 && op.dloc == 0 && !op.rcm && !op.tcm &&
+                std::fabs(op.c[2] * op.c[2] + op.c[3] * op.c[3] - 1.0) <= 1e-12) {
+                const double phi = std::atan2(op.c[3], op.c[2]);
+                if (std::fabs(phi) <= M_PI / 2) {
+                    op.kind = RO_P1S;
+                    op.c[0] = -std::tan(phi / 2);
+                    op.c[1] = std::sin(phi);
+                    op.c[2] = op.c[3] = 0;
+                }
+            }
             for (int q = 0; q < 8; q++) op.cf[q] = (float)op.c[q];
+        }
         plan.reg_ops.insert(plan.reg_ops.end(), P.out.begin(), P.out.end());
         RegOp end{};
         end.kind = RO_NOP;

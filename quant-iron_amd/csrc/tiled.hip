@@ -637,6 +637,26 @@ template <bool XS, int E> __device__ __forceinline__ void shear_op(Regs<float, E
 #undef QI_RB
 }
 
+// Phase e^{i phi} on the amplitudes whose register bit B is 1, as a rotation
+// of (re, im) by three in-place shears (|phi| <= pi/2).
+template <int B, int E> __device__ __forceinline__ void p1s_reg(Regs<float, E> &v, float a, float b) {
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        if (!((r >> B) & 1)) continue;
+        float &x = v.d[2 * r], &y = v.d[2 * r + 1];
+        ip_fmac(x, a, y);
+        ip_fmac(y, b, x);
+        ip_fmac(x, a, y);
+    }
+}
+template <int E> __device__ __forceinline__ void p1s_op(Regs<float, E> &v, const RegOp &op) {
+    constexpr int RB = __builtin_ctz(E);
+    const float a = op.cf[0], b = op.cf[1];
+#define QI_RB(BB) case BB: if constexpr (BB < RB) p1s_reg<BB, E>(v, a, b); break;
+    switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
+#undef QI_RB
+}
+
 // Unscaled H on register bit B: (a, b) <- (a + b, a - b) per component as
 // a += b, then b = a - 2b (two in-place instructions instead of four).
 template <int B, int E> __device__ __forceinline__ void hs_reg(Regs<float, E> &v) {
@@ -820,6 +840,9 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         break;
     case RO_HS:
         if constexpr (sizeof(R) == 4) hs_op<E>(v, op);
+        break;
+    case RO_P1S:
+        if constexpr (sizeof(R) == 4) p1s_op<E>(v, op);
         break;
     case RO_XLR:
         if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
