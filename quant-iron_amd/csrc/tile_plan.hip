@@ -148,10 +148,10 @@ struct RegPlanner {
     // factors (1/sqrt(2) for H) multiply up here and go into the global
     // phase. Every kMaxUnscaled of them a scaling diagonal is emitted, so
     // complex64 amplitudes grow by at most 2 per such op between scalings,
-    // 2^60 in all: far below overflow.
+    // 2^32 in all: far below overflow also for unnormalised states.
     int *unscaled = nullptr;
     double *hfactor = nullptr;
-    static constexpr int kMaxUnscaled = 60;
+    static constexpr int kMaxUnscaled = 32;
     // The last uncontrolled diagonal op on each diagonal-op position (register
     // bit, local bit, or qubit outside the tile) that no later op has acted on
     // non-diagonally: a new uncontrolled diagonal there folds into it

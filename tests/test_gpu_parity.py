@@ -330,6 +330,27 @@ def test_swaps_relabel_qubits_instead_of_moving_data(ctx, prec):
         check(s.state_vector, ref, prec, f"{layers} layers")
 
 
+def test_unscaled_h_keeps_large_unnormalised_states_finite(ctx):
+    """Multiples of H run unscaled with the factors applied later; an
+    unnormalised complex64 state of magnitude 1e18 (State * Complex allows it)
+    through 400 H gates and rotations stays finite and matches the oracle to
+    single-precision relative accuracy."""
+    n = 14
+    rng = np.random.default_rng(8)
+    ds = []
+    for _ in range(40):
+        ds += [{"kind": "H", "targets": [q]} for q in range(4, 14)]
+        ds.append({"kind": "RY", "targets": [int(rng.integers(4, 14))], "params": [float(rng.uniform(0, 6.28))]})
+    gates = gates_array(ds)
+    psi = random_state(n, rng) * 1e18
+    ref = O.apply_gates(psi, gates)
+    s = upload(psi, "c64")
+    s.apply_gates_inplace(gates, mode=L.EXEC_TILED, tile_qubits=12)
+    got = s.state_vector
+    assert np.all(np.isfinite(got))
+    assert np.max(np.abs(got - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
 def _inverse(gates):
     """The inverse gate list of a circuit of H, X, Y, Z, RX, RY, RZ, P, CNOT, CZ,
     SWAP and Toffoli: reversed, rotation and phase angles negated."""
