@@ -107,14 +107,38 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
     switch (op.kind) {
     case LK_Z: case LK_PHASE: case LK_RZ: {
         T p0 = mk<T>(R(c[0]), R(c[1])), p1 = mk<T>(R(c[2]), R(c[3]));
+        bool half = op.kind != LK_RZ;
+        if (op.kind == LK_RZ && lcm == 0 && op.gcm == 0) {
+            // this engine returns energies only, which no global phase changes:
+            // an uncontrolled diag(p0, p1) acts as diag(1, p1 conj(p0))
+            p1 = mk<T>(R(c[2] * c[0] + c[3] * c[1]), R(c[3] * c[0] - c[2] * c[1]));
+            half = true;
+        }
+        if (half) {  // only the target-1 amplitudes change
+            if (op.outside) {
+                if (!((base >> op.gt) & 1)) break;
+                for (uint32_t e = tid; e < nel; e += NT) {
+                    if ((e & lcm) != lcm) continue;
+                    T a = tile[e];
+                    if (op.kind == LK_Z) { a.x = -a.x; a.y = -a.y; } else a = cmul(a, p1);
+                    tile[e] = a;
+                }
+            } else {
+                const uint32_t tb = 1u << op.lp0;
+                for (uint32_t q = tid; q < (nel >> 1); q += NT) {
+                    const uint32_t e = ins0(q, op.lp0) | tb;
+                    if ((e & lcm) != lcm) continue;
+                    T a = tile[e];
+                    if (op.kind == LK_Z) { a.x = -a.x; a.y = -a.y; } else a = cmul(a, p1);
+                    tile[e] = a;
+                }
+            }
+            break;
+        }
         for (uint32_t e = tid; e < nel; e += NT) {
             if ((e & lcm) != lcm) continue;
-            bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
-            T a = tile[e];
-            if (op.kind == LK_Z) { if (one) { a.x = -a.x; a.y = -a.y; } }
-            else if (op.kind == LK_PHASE) { if (one) a = cmul(a, p1); }
-            else a = cmul(a, one ? p1 : p0);
-            tile[e] = a;
+            const bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
+            tile[e] = cmul(tile[e], one ? p1 : p0);
         }
     } break;
     case LK_SWAP: case LK_MATCH: {
