@@ -195,7 +195,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
 struct ETerm {
     uint64_t x, z;
     uint32_t q0, pad;
-    double c[2];
+    double c[2];  // the coefficient times i^q0
 };
 
 // sum over tile elements e of conj(psi_e) * c * i^{q(e)} * psi_{e ^ x}
@@ -207,27 +207,31 @@ __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const
         const uint64_t gi = base | deposit_pos<KMAX>(e, pos, nloc);
         const double ar = tile[e].x, ai = tile[e].y;
         double hr = 0.0, hi = 0.0;
-        double wr = 0.0, wi = 0.0;
+        double wr = 0.0, wi = 0.0, gr = 0.0, gi2 = 0.0;
         uint32_t last = ~0u;
         for (int t = 0; t < nt; t++) {
             const ETerm tm = terms[t];
             // local partner: x lies inside the tile, so flipping its global
             // bits flips the same local bits (tm.pad holds the local x mask).
-            // w = conj(a) * b, shared by consecutive terms with one x mask
+            // Terms with one x mask are consecutive: they share w = conj(a) b,
+            // their signed coefficients sum first (g) and meet w once.
             if (tm.pad != last) {
+                hr += wr * gr - wi * gi2;
+                hi += wr * gi2 + wi * gr;
+                gr = gi2 = 0.0;
                 last = tm.pad;
                 const uint32_t le = e ^ tm.pad;
                 const double br = tile[le].x, bi = tile[le].y;
                 wr = ar * br + ai * bi;
                 wi = ar * bi - ai * br;
             }
-            const unsigned q = (tm.q0 + 2u * (unsigned)__popcll(gi & tm.z)) & 3u;
-            // c * i^q
-            const double pr = q == 0 ? tm.c[0] : q == 1 ? -tm.c[1] : q == 2 ? -tm.c[0] : tm.c[1];
-            const double pi = q == 0 ? tm.c[1] : q == 1 ? tm.c[0] : q == 2 ? -tm.c[1] : -tm.c[0];
-            hr += wr * pr - wi * pi;
-            hi += wr * pi + wi * pr;
+            // c i^q with q = q0 + 2 popcount(index & z): the stored c i^q0 times +-1
+            const double sg = (__popcll(gi & tm.z) & 1) ? -1.0 : 1.0;
+            gr = fma(sg, tm.c[0], gr);
+            gi2 = fma(sg, tm.c[1], gi2);
         }
+        hr += wr * gr - wi * gi2;
+        hi += wr * gi2 + wi * gr;
         dd_acc(sr, hr);
         dd_acc(si, hi);
     }
@@ -431,11 +435,21 @@ struct Batch {
     }
 };
 
+// c * i^q0 (the device adds it with the sign (-1)^popcount(index & z))
+static void set_eterm_coeff(ETerm &t, double re, double im) {
+    switch (t.q0) {
+    case 0: t.c[0] = re; t.c[1] = im; break;
+    case 1: t.c[0] = -im; t.c[1] = re; break;
+    case 2: t.c[0] = -re; t.c[1] = -im; break;
+    default: t.c[0] = im; t.c[1] = -re; break;
+    }
+}
+
 static ETerm make_eterm(const qi_pauli_term &p) {
     ETerm t{};
     t.x = p.x_mask; t.z = p.z_mask;
     t.q0 = (3u * (unsigned)__builtin_popcountll(p.x_mask & p.z_mask)) & 3u;
-    t.c[0] = p.c_re; t.c[1] = p.c_im;
+    set_eterm_coeff(t, p.c_re, p.c_im);
     return t;
 }
 
@@ -857,8 +871,7 @@ qi_status qi_batch_set_term_coeffs(qi_batch_t b, const double *coeffs, size_t nt
             C.terms[t].c_im = coeffs[2 * k + 1];
             if (B.planned) {
                 ETerm &e = B.eterms[C.eterm_idx[t]];
-                e.c[0] = coeffs[2 * k];
-                e.c[1] = coeffs[2 * k + 1];
+                set_eterm_coeff(e, coeffs[2 * k], coeffs[2 * k + 1]);
             }
         }
     if (B.planned) {
