@@ -97,10 +97,17 @@ __device__ __forceinline__ void resolve(const TileOp &op, const double *__restri
     }
 }
 
+// (cos theta, sin theta) of a folded RZ(theta), or nullptr.
+__device__ __forceinline__ const double *phase_of(const TileOp &op, const double *__restrict__ coef, int slot_base) {
+    return op.slot2 >= 0 ? coef + 4 * (size_t)(slot_base + op.slot2) + 2 : nullptr;
+}
+
 // Apply one op to a tile held in LDS (shared by the LDS-resident and the tiled kernels).
+// ph: e^{i theta} applied to the target-1 output of a one-qubit pair op
+// (a folded RZ, see fold_rz), or nullptr.
 template <typename R, int N, int NT>
 __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp &op, const double *c, uint64_t base,
-                                           int nloc) {
+                                           int nloc, const double *ph) {
     using T = typename V2<R>::T;
     const uint32_t tid = threadIdx.x, lcm = op.lcm;
     const uint32_t nel = 1u << nloc;
@@ -185,6 +192,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
                 nb = cadd(cmul(mk<T>(R(c[4]), R(c[5])), a), cmul(mk<T>(R(c[6]), R(c[7])), b));
                 break;
             }
+            if (ph) nb = cmul(nb, mk<T>(R(ph[0]), R(ph[1])));
             tile[i] = na; tile[j] = nb;
         }
     } break;
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
         const TileOp op = ops[tk.op0 + o];
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, (1 << KMAX), kBT>(tile, op, c, 0, (int)tk.n);
+        tile_apply<R, (1 << KMAX), kBT>(tile, op, c, 0, (int)tk.n, phase_of(op, coef, tk.slot_base));
         __syncthreads();
     }
     DD v[2] = {dd_zero(), dd_zero()};
@@ -326,7 +334,7 @@ __global__ __launch_bounds__(kPT) void k_batch_pass(typename V2<R>::T *__restric
         if ((base & op.gcm) != op.gcm) continue;
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, N, kPT>(tile, op, c, base, K);
+        tile_apply<R, N, kPT>(tile, op, c, base, K, phase_of(op, coef, tk.slot_base));
         __syncthreads();
     }
 #pragma unroll
@@ -453,6 +461,37 @@ static ETerm make_eterm(const qi_pauli_term &p) {
     return t;
 }
 
+// This engine returns energies only, which no global phase changes, so an
+// uncontrolled RZ(theta) = e^{-i theta/2} diag(1, e^{i theta}). A parametric
+// one on position p folds into the last op before it on p when that is an
+// uncontrolled one-qubit pair gate: its target-1 output takes e^{i theta}
+// (ops on other positions in between commute with both). In place on a slice
+// whose `pad` holds angle slots; returns the new length.
+static size_t fold_rz(TileOp *ops, size_t n) {
+    int last[64];
+    for (int &x : last) x = -1;
+    size_t w = 0;
+    for (size_t r = 0; r < n; r++) {
+        const TileOp op = ops[r];
+        if (op.kind == LK_RZ && !op.outside && op.lcm == 0 && op.gcm == 0 && (int32_t)op.pad >= 0 && last[op.lp0] >= 0) {
+            TileOp &pv = ops[last[op.lp0]];
+            const bool one_q = pv.kind == LK_H || pv.kind == LK_X || pv.kind == LK_Y || pv.kind == LK_RX ||
+                               pv.kind == LK_RY || pv.kind == LK_U2;
+            if (one_q && pv.lp0 == op.lp0 && pv.lcm == 0 && pv.gcm == 0 && pv.slot2 < 0) {
+                pv.slot2 = (int32_t)op.pad;
+                continue;
+            }
+        }
+        ops[w] = op;
+        const bool diag_outside = op.outside && (op.kind == LK_Z || op.kind == LK_PHASE || op.kind == LK_RZ);
+        if (!diag_outside) last[op.lp0] = (int)w;
+        if (op.kind == LK_SWAP || op.kind == LK_MATCH) last[op.lp1] = (int)w;
+        for (uint32_t m = op.lcm; m; m &= m - 1) last[__builtin_ctz(m)] = (int)w;
+        w++;
+    }
+    return w;
+}
+
 template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &v) {
     Status e = ensure(b, std::max<size_t>(v.size() * sizeof(T), 16));
     if (!e.ok()) return e;
@@ -495,7 +534,8 @@ Status batch_plan(Batch &B) {
                 op.pad = (uint32_t)lslot[g];
                 ops.push_back(op);
             }
-            P.nops = (uint32_t)lw.size();
+            P.nops = (uint32_t)fold_rz(ops.data() + P.op0, lw.size());
+            ops.resize(P.op0 + P.nops);
             continue;
         }
         State fake;
@@ -504,11 +544,18 @@ Status batch_plan(Batch &B) {
         Status e = plan_lds_passes(fake, lw, B.K, plan);
         if (!e.ok()) return e;
         P.ops_base = (uint32_t)ops.size();
-        for (TileOp op : plan.tile_ops) { op.pad = (uint32_t)lslot[op.pad]; ops.push_back(op); }
+        for (TileOp &op : plan.tile_ops) op.pad = (uint32_t)lslot[op.pad];
+        std::vector<TileOp> packed;
         for (const ExecStep &s : plan.steps) {
             if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
-            P.passes.push_back(s.tile);
+            TilePass tp = s.tile;
+            const size_t n = fold_rz(plan.tile_ops.data() + tp.op0, (size_t)tp.nops);
+            tp.op0 = (int32_t)packed.size();
+            tp.nops = (int32_t)n;
+            packed.insert(packed.end(), plan.tile_ops.begin() + s.tile.op0, plan.tile_ops.begin() + s.tile.op0 + n);
+            P.passes.push_back(tp);
         }
+        ops.insert(ops.end(), packed.begin(), packed.end());
     }
     std::vector<SmallTask> small;
     std::vector<std::vector<BigTask>> per_level, per_elevel;

@@ -18,6 +18,8 @@ struct TileOp {
     uint32_t pad;        // planner: source gate index; batched engine: angle slot (-1: none)
     uint64_t gcm;        // controls outside the tile (global bit mask)
     double c[8];         // coefficients as in Lowered (phases in c[2], c[3])
+    int32_t slot2 = -1;  // batched engine: angle slot of a one-sided phase applied after this op (-1: none)
+    int32_t pad2 = 0;
 };
 
 struct TilePass {
