@@ -58,6 +58,10 @@ __device__ __forceinline__ uint64_t deposit_pos(uint32_t e, const uint8_t *pos, 
 }
 
 constexpr int kBT = 256;
+// Op kind of two parametric, uncontrolled RY on different qubits (lp0, lp1)
+// in one sweep of the tile, each optionally followed by its folded phase
+// (slot2, slot4); the first RY's angle in pad, the second's in slot3.
+constexpr int kTwoRY = 100;
 // Threads of a tiled pass (a 2^K-amplitude tile in LDS; 256 measured fastest
 // against 64, 128 and 512 on the 4096-circuit ansatz batch).
 constexpr int kPT = 256, kPTLog = __builtin_ctz(kPT);
@@ -89,7 +93,7 @@ __device__ __forceinline__ void resolve(const TileOp &op, const double *__restri
     const double *k = coef + 4 * (size_t)(slot_base + slot);
     const double ch = k[0], sh = k[1];
     switch (op.kind) {
-    case LK_RY: c[0] = ch; c[1] = sh; break;
+    case LK_RY: case kTwoRY: c[0] = ch; c[1] = sh; break;
     case LK_RZ: c[0] = ch; c[1] = -sh; c[2] = ch; c[3] = sh; break;
     case LK_RX: c[0] = ch; c[1] = 0.0 * sh; c[2] = sh; c[3] = -0.0 * sh; c[4] = -sh; break;
     case LK_PHASE: c[2] = k[2]; c[3] = k[3]; break;
@@ -107,7 +111,7 @@ __device__ __forceinline__ const double *phase_of(const TileOp &op, const double
 // (a folded RZ, see fold_rz), or nullptr.
 template <typename R, int N, int NT>
 __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp &op, const double *c, uint64_t base,
-                                           int nloc, const double *ph) {
+                                           int nloc, const double *ph, const double *__restrict__ coef, int slot_base) {
     using T = typename V2<R>::T;
     const uint32_t tid = threadIdx.x, lcm = op.lcm;
     const uint32_t nel = 1u << nloc;
@@ -146,6 +150,28 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
             if ((e & lcm) != lcm) continue;
             const bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
             tile[e] = cmul(tile[e], one ? p1 : p0);
+        }
+    } break;
+    case kTwoRY: {
+        // (cos, sin) of each half angle from the slot table; c[] holds the first RY's
+        const double c1 = c[0], s1 = c[1];
+        const double *k2 = coef + 4 * (size_t)(slot_base + op.slot3);
+        const double c2 = k2[0], s2 = k2[1];
+        const double *q2 = op.slot4 >= 0 ? coef + 4 * (size_t)(slot_base + op.slot4) + 2 : nullptr;
+        const uint32_t b1 = 1u << op.lp0, b2 = 1u << op.lp1;
+        const unsigned lo = op.lp0 < op.lp1 ? op.lp0 : op.lp1, hi = op.lp0 < op.lp1 ? op.lp1 : op.lp0;
+        for (uint32_t q = tid; q < (nel >> 2); q += NT) {
+            const uint32_t l = ins0(ins0(q, lo), hi);
+            T a00 = tile[l], a01 = tile[l | b1], a10 = tile[l | b2], a11 = tile[l | b1 | b2];
+            // first RY on (x0, x1) pairs along b1, then its phase on the b1-set outputs
+            T n00 = csub(cscale<T>(R(c1), a00), cscale<T>(R(s1), a01)), n01 = cadd(cscale<T>(R(s1), a00), cscale<T>(R(c1), a01));
+            T n10 = csub(cscale<T>(R(c1), a10), cscale<T>(R(s1), a11)), n11 = cadd(cscale<T>(R(s1), a10), cscale<T>(R(c1), a11));
+            if (ph) { n01 = cmul(n01, mk<T>(R(ph[0]), R(ph[1]))); n11 = cmul(n11, mk<T>(R(ph[0]), R(ph[1]))); }
+            // second RY along b2, then its phase on the b2-set outputs
+            T m00 = csub(cscale<T>(R(c2), n00), cscale<T>(R(s2), n10)), m10 = cadd(cscale<T>(R(s2), n00), cscale<T>(R(c2), n10));
+            T m01 = csub(cscale<T>(R(c2), n01), cscale<T>(R(s2), n11)), m11 = cadd(cscale<T>(R(s2), n01), cscale<T>(R(c2), n11));
+            if (q2) { m10 = cmul(m10, mk<T>(R(q2[0]), R(q2[1]))); m11 = cmul(m11, mk<T>(R(q2[0]), R(q2[1]))); }
+            tile[l] = m00; tile[l | b1] = m01; tile[l | b2] = m10; tile[l | b1 | b2] = m11;
         }
     } break;
     case LK_SWAP: case LK_MATCH: {
@@ -270,7 +296,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
         const TileOp op = ops[tk.op0 + o];
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, (1 << KMAX), kBT>(tile, op, c, 0, (int)tk.n, phase_of(op, coef, tk.slot_base));
+        tile_apply<R, (1 << KMAX), kBT>(tile, op, c, 0, (int)tk.n, phase_of(op, coef, tk.slot_base), coef, tk.slot_base);
         __syncthreads();
     }
     DD v[2] = {dd_zero(), dd_zero()};
@@ -334,7 +360,7 @@ __global__ __launch_bounds__(kPT) void k_batch_pass(typename V2<R>::T *__restric
         if ((base & op.gcm) != op.gcm) continue;
         double c[8];
         resolve(op, coef, tk.slot_base, c);
-        tile_apply<R, N, kPT>(tile, op, c, base, K, phase_of(op, coef, tk.slot_base));
+        tile_apply<R, N, kPT>(tile, op, c, base, K, phase_of(op, coef, tk.slot_base), coef, tk.slot_base);
         __syncthreads();
     }
 #pragma unroll
@@ -461,6 +487,28 @@ static ETerm make_eterm(const qi_pauli_term &p) {
     return t;
 }
 
+// Adjacent uncontrolled parametric RY ops on two positions become one kTwoRY
+// op (both act on every amplitude; they commute). In place; returns the new
+// length.
+static size_t pair_ry(TileOp *ops, size_t n) {
+    size_t w = 0;
+    for (size_t r = 0; r < n; r++) {
+        TileOp op = ops[r];
+        const auto single = [](const TileOp &o) {
+            return o.kind == LK_RY && o.lcm == 0 && o.gcm == 0 && (int32_t)o.pad >= 0;
+        };
+        if (r + 1 < n && single(op) && single(ops[r + 1]) && ops[r + 1].lp0 != op.lp0) {
+            op.kind = kTwoRY;
+            op.lp1 = ops[r + 1].lp0;
+            op.slot3 = (int32_t)ops[r + 1].pad;
+            op.slot4 = ops[r + 1].slot2;
+            r++;
+        }
+        ops[w++] = op;
+    }
+    return w;
+}
+
 // This engine returns energies only, which no global phase changes, so an
 // uncontrolled RZ(theta) = e^{-i theta/2} diag(1, e^{i theta}). A parametric
 // one on position p folds into the last op before it on p when that is an
@@ -534,7 +582,7 @@ Status batch_plan(Batch &B) {
                 op.pad = (uint32_t)lslot[g];
                 ops.push_back(op);
             }
-            P.nops = (uint32_t)fold_rz(ops.data() + P.op0, lw.size());
+            P.nops = (uint32_t)pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size()));
             ops.resize(P.op0 + P.nops);
             continue;
         }
@@ -549,7 +597,7 @@ Status batch_plan(Batch &B) {
         for (const ExecStep &s : plan.steps) {
             if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
             TilePass tp = s.tile;
-            const size_t n = fold_rz(plan.tile_ops.data() + tp.op0, (size_t)tp.nops);
+            const size_t n = pair_ry(plan.tile_ops.data() + tp.op0, fold_rz(plan.tile_ops.data() + tp.op0, (size_t)tp.nops));
             tp.op0 = (int32_t)packed.size();
             tp.nops = (int32_t)n;
             packed.insert(packed.end(), plan.tile_ops.begin() + s.tile.op0, plan.tile_ops.begin() + s.tile.op0 + n);

@@ -19,6 +19,8 @@ struct TileOp {
     uint64_t gcm;        // controls outside the tile (global bit mask)
     double c[8];         // coefficients as in Lowered (phases in c[2], c[3])
     int32_t slot2 = -1;  // batched engine: angle slot of a one-sided phase applied after this op (-1: none)
+    int32_t slot3 = -1;  // batched engine, two RY in one sweep: the second RY's angle slot (its qubit in lp1)
+    int32_t slot4 = -1;  //   and the slot of its trailing phase (-1: none)
     int32_t pad2 = 0;
 };
 
