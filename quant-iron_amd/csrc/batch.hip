@@ -62,6 +62,10 @@ constexpr int kBT = 256;
 // in one sweep of the tile, each optionally followed by its folded phase
 // (slot2, slot4); the first RY's angle in pad, the second's in slot3.
 constexpr int kTwoRY = 100;
+// Op kind of a run of in-tile CNOTs as one permutation of the local index:
+// the new amplitude at e is the old one at G(e) = XOR of column j for each set
+// bit j of e; the columns are uint32 masks packed into c[] (at most 16).
+constexpr int kPerm = 101;
 // Threads of a tiled pass (a 2^K-amplitude tile in LDS; 256 measured fastest
 // against 64, 128 and 512 on the 4096-circuit ansatz batch).
 constexpr int kPT = 256, kPTLog = __builtin_ctz(kPT);
@@ -150,6 +154,27 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
             if ((e & lcm) != lcm) continue;
             const bool one = op.outside ? ((base >> op.gt) & 1) : ((e >> op.lp0) & 1);
             tile[e] = cmul(tile[e], one ? p1 : p0);
+        }
+    } break;
+    case kPerm: {
+        uint32_t cols[16];
+        memcpy(cols, op.c, sizeof cols);
+        T v[N / NT];
+#pragma unroll
+        for (int k = 0; k < N / NT; k++) {
+            const uint32_t e = tid + (uint32_t)k * NT;
+            if (e < nel) {
+                uint32_t src = 0;
+                for (int j = 0; j < nloc; j++)
+                    if ((e >> j) & 1u) src ^= cols[j];
+                v[k] = tile[src];
+            }
+        }
+        __syncthreads();  // every gather done before any element is overwritten
+#pragma unroll
+        for (int k = 0; k < N / NT; k++) {
+            const uint32_t e = tid + (uint32_t)k * NT;
+            if (e < nel) tile[e] = v[k];
         }
     } break;
     case kTwoRY: {
@@ -487,6 +512,42 @@ static ETerm make_eterm(const qi_pauli_term &p) {
     return t;
 }
 
+// Runs of consecutive CNOTs inside the tile (one in-tile control, no other
+// control) become one kPerm op. A CNOT c -> t maps index e to
+// F(e) = e ^ (bit c of e) << t, and the state to new[F(e)] = old[e]; for the
+// run F_k ... F_1 the gather map is G = F_1 o ... o F_k (each F its own
+// inverse), built column by column. In place; returns the new length.
+static size_t perm_cnots(TileOp *ops, size_t n, int nloc) {
+    const auto cnot = [](const TileOp &o) {
+        return o.kind == LK_X && !o.outside && o.gcm == 0 && __builtin_popcount(o.lcm) == 1 && (int32_t)o.pad < 0 &&
+               o.slot2 < 0;
+    };
+    size_t w = 0;
+    for (size_t r = 0; r < n;) {
+        size_t e = r;
+        while (e < n && cnot(ops[e])) e++;
+        if (e - r < 2 || nloc > 16) {
+            if (e == r) e = r + 1;
+            for (size_t k = r; k < e; k++) ops[w++] = ops[k];
+            r = e;
+            continue;
+        }
+        uint32_t cols[16];
+        for (int j = 0; j < 16; j++) cols[j] = j < nloc ? 1u << j : 0u;
+        for (size_t k = e; k-- > r;) {  // G = F_r o ... o F_(e-1): apply the later maps first
+            const int c = __builtin_ctz(ops[k].lcm), t = ops[k].lp0;
+            for (int j = 0; j < nloc; j++) cols[j] ^= ((cols[j] >> c) & 1u) << t;
+        }
+        TileOp p{};
+        p.kind = kPerm;
+        p.pad = (uint32_t)-1;
+        memcpy(p.c, cols, sizeof cols);
+        ops[w++] = p;
+        r = e;
+    }
+    return w;
+}
+
 // Adjacent uncontrolled parametric RY ops on two positions become one kTwoRY
 // op (both act on every amplitude; they commute). In place; returns the new
 // length.
@@ -582,7 +643,8 @@ Status batch_plan(Batch &B) {
                 op.pad = (uint32_t)lslot[g];
                 ops.push_back(op);
             }
-            P.nops = (uint32_t)pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size()));
+            P.nops = (uint32_t)perm_cnots(ops.data() + P.op0, pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size())),
+                                          (int)P.n);
             ops.resize(P.op0 + P.nops);
             continue;
         }
@@ -597,7 +659,8 @@ Status batch_plan(Batch &B) {
         for (const ExecStep &s : plan.steps) {
             if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
             TilePass tp = s.tile;
-            const size_t n = pair_ry(plan.tile_ops.data() + tp.op0, fold_rz(plan.tile_ops.data() + tp.op0, (size_t)tp.nops));
+            TileOp *slice = plan.tile_ops.data() + tp.op0;
+            const size_t n = perm_cnots(slice, pair_ry(slice, fold_rz(slice, (size_t)tp.nops)), B.K);
             tp.op0 = (int32_t)packed.size();
             tp.nops = (int32_t)n;
             packed.insert(packed.end(), plan.tile_ops.begin() + s.tile.op0, plan.tile_ops.begin() + s.tile.op0 + n);
