@@ -156,7 +156,7 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
             tile[e] = cmul(tile[e], one ? p1 : p0);
         }
     } break;
-    case kPerm: {
+    case kPerm: if constexpr (sizeof(R) == 4) {  // complex64 only (complex128 keeps one sweep per CNOT)
         uint32_t cols[16];
         memcpy(cols, op.c, sizeof cols);
         T v[N / NT];
@@ -165,8 +165,9 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
             const uint32_t e = tid + (uint32_t)k * NT;
             if (e < nel) {
                 uint32_t src = 0;
-                for (int j = 0; j < nloc; j++)
-                    if ((e >> j) & 1u) src ^= cols[j];
+#pragma unroll
+                for (int j = 0; j < 16; j++)  // fixed trip count: cols stays in registers
+                    if (j < nloc && ((e >> j) & 1u)) src ^= cols[j];
                 v[k] = tile[src];
             }
         }
@@ -643,8 +644,8 @@ Status batch_plan(Batch &B) {
                 op.pad = (uint32_t)lslot[g];
                 ops.push_back(op);
             }
-            P.nops = (uint32_t)perm_cnots(ops.data() + P.op0, pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size())),
-                                          (int)P.n);
+            const size_t nf = pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size()));
+            P.nops = (uint32_t)(c128 ? nf : perm_cnots(ops.data() + P.op0, nf, (int)P.n));
             ops.resize(P.op0 + P.nops);
             continue;
         }
@@ -660,7 +661,9 @@ Status batch_plan(Batch &B) {
             if (s.type != ExecStep::TILE) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "batched circuit not tileable");
             TilePass tp = s.tile;
             TileOp *slice = plan.tile_ops.data() + tp.op0;
-            const size_t n = perm_cnots(slice, pair_ry(slice, fold_rz(slice, (size_t)tp.nops)), B.K);
+            const size_t nf = pair_ry(slice, fold_rz(slice, (size_t)tp.nops));
+            // complex64 only: the complex128 gather buffer would not stay in registers
+            const size_t n = c128 ? nf : perm_cnots(slice, nf, B.K);
             tp.op0 = (int32_t)packed.size();
             tp.nops = (int32_t)n;
             packed.insert(packed.end(), plan.tile_ops.begin() + s.tile.op0, plan.tile_ops.begin() + s.tile.op0 + n);
