@@ -545,6 +545,29 @@ def test_config1_full_size_matches_oracle(ctx):
         assert abs(e - e_ref) <= 1e-12, (opts, e, e_ref)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_qft_matches_the_fft_closed_form_at_20_qubits(ctx, prec):
+    """Subroutine::qft (subroutine.rs:90-112) on the engine, checked with no
+    oracle in the loop: the gate list computes R F R, with F the unitary DFT
+    (omega = e^{+2 pi i/N}, numpy's ifft times sqrt(N)) and R the bit
+    reversal of the index, on a random 20-qubit state, in every executor."""
+    n = 20
+    N = 1 << n
+    gl = W.GateList()
+    W.qft_gates(gl, list(range(n)))
+    gates = gl.array()
+    idx = np.arange(N)
+    rev = np.zeros(N, dtype=np.int64)
+    for b in range(n):
+        rev |= ((idx >> b) & 1) << (n - 1 - b)
+    psi = random_state(n, np.random.default_rng(2024))
+    exp = (np.fft.ifft(psi[rev]) * math.sqrt(N))[rev]
+    for mode in (L.EXEC_PER_GATE, L.EXEC_TILED, L.EXEC_FUSED_MFMA):
+        s = upload(psi, prec)
+        s.apply_gates_inplace(gates, mode=mode)
+        check(s.state_vector, exp, prec, f"qft mode {mode}")
+
+
 def test_large_state_properties(ctx):
     """26 qubits (complex64): norm preserved and the three executors agree."""
     n = 26
