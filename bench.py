@@ -296,24 +296,13 @@ def profiled_traffic(kernel: str, path: str):
     return None
 
 
-def inverse_gates(L, gates):
-    """The inverse of a circuit of H, X, Y, Z, RX, RY, RZ, P, CNOT, CZ, SWAP and
-    Toffoli: reversed, rotation and phase angles negated."""
-    import numpy as np
-
-    inv = gates[::-1].copy()
-    rot = np.isin(inv["kind"], [L.GATE_KINDS[k] for k in ("RX", "RY", "RZ", "P")])
-    inv["params"][rot, 0] = -inv["params"][rot, 0]
-    return inv
-
-
-def round_trip(qiron, L, n, prec, gates, opts):
+def round_trip(qiron, W, n, prec, gates, opts):
     """1 - |<+...+| U^-1 U |+...+>|^2 at the full size: the circuit, then its
     inverse, through the same executor (a size-independent check of the path
     being timed; FP64 reduction on the device)."""
     s = qiron.State.new_plus(n, prec)
     s.apply_gates_inplace(gates, **opts)
-    s.apply_gates_inplace(inverse_gates(L, gates), **opts)
+    s.apply_gates_inplace(W.inverse_gates(gates), **opts)
     ref = qiron.State.new_plus(n, prec)
     f = s.fs_fidelity(ref)
     del s, ref
@@ -473,7 +462,7 @@ def main():
                                 "issue_frac": round(valu_ms / pass_ms, 4), "source": vsrc}
 
     if not args.only:
-        rt = round_trip(qiron, L, n, "c64", gates, tiled)
+        rt = round_trip(qiron, W, n, "c64", gates, tiled)
         results["c64_tiled"]["round_trip_infidelity"] = rt
         log(f"c64 tiled round trip (circuit, then its inverse) at {n}q: 1 - fidelity {rt:.2e}")
         single_gates(qiron, L, W, ctx, n, results)
@@ -481,7 +470,7 @@ def main():
         wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c128", gates, tiled, 1, 1)
         results["c128_tiled"] = dict(gates_per_s=G / wall, ms_per_step=wall * 1e3, passes=passes,
                                      pass_GBps=2 * (1 << n) * 16 * passes / (dev_ms * 1e-3) / 1e9)
-        results["c128_tiled"]["round_trip_infidelity"] = round_trip(qiron, L, n, "c128", gates, tiled)
+        results["c128_tiled"]["round_trip_infidelity"] = round_trip(qiron, W, n, "c128", gates, tiled)
         log(f"c128 tiled: {wall * 1e3:.1f} ms/step {G / wall:.1f} gates/s, round trip 1 - fidelity "
             f"{results['c128_tiled']['round_trip_infidelity']:.2e}")
         # ---- per-gate streaming and fused MFMA on the first 5 layers

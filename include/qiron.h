@@ -126,7 +126,8 @@ typedef enum qi_basis {
  * entries are read.
  *   QI_GATE_MEASURE:  controls[0..ncontrols) are the measured qubits (empty =
  *                     all), `basis`, custom basis in `matrix`, and the injected
- *                     uniform draw in params[0] (replaces rand::rng(), state.rs:601).
+ *                     uniform draw in params[0] (replaces rand::rng(), state.rs:601);
+ *                     NaN draws from the context's stream each time the gate runs.
  *   QI_GATE_PAULI_STRING / QI_GATE_PAULI_EVOLUTION: x_mask/z_mask (Y = X&Z),
  *                     coefficient (params[0], params[1]); evolution time params[2].
  *   QI_GATE_U2:       matrix = [[m00, m01], [m10, m11]] row-major, each (re, im). */
@@ -174,6 +175,11 @@ typedef struct qi_exec_opts {
 qi_status qi_ctx_create(int device, qi_ctx_t *out, qi_error *err);
 qi_status qi_ctx_destroy(qi_ctx_t ctx);
 qi_status qi_ctx_synchronize(qi_ctx_t ctx, qi_error *err);
+/* Seeds the context's SplitMix64 stream of uniform draws, used by
+ * measurements whose draw is not injected (a NaN uniform, NULL uniforms).
+ * Replaces the OS-seeded rand::rng() of State::measure (state.rs:601-602);
+ * an unseeded context starts from an OS-seeded state. */
+qi_status qi_ctx_seed(qi_ctx_t ctx, uint64_t seed, qi_error *err);
 /* Device name and HBM size, for reports. */
 qi_status qi_ctx_info(qi_ctx_t ctx, char *name, size_t name_len, uint64_t *hbm_bytes,
                       int *cu_count, qi_error *err);
@@ -242,13 +248,25 @@ qi_status qi_inner_product(qi_state_t a, qi_state_t b, double *re, double *im, q
 qi_status qi_norm_sqr(qi_state_t s, double *out, qi_error *err);
 /* State::normalise (state.rs:924-945), in place. */
 qi_status qi_normalise(qi_state_t s, qi_error *err);
-/* State::fs_fidelity (state.rs:492-498). */
+/* State::fs_fidelity (state.rs:492-498): |<a/|a| | b/|b|>|^2. Errors in the
+ * reference's order: ZeroNorm for a, then for b (normalise, state.rs:924-935),
+ * then InvalidNumberOfQubits(n_a) when the qubit counts differ
+ * (inner_product, state.rs:890-897). */
 qi_status qi_fidelity(qi_state_t a, qi_state_t b, double *out, qi_error *err);
+/* State::fs_dist (state.rs:470-478): acos(|<a/|a| | b/|b|>|), not clamped
+ * (as in the reference); errors as qi_fidelity. */
+qi_status qi_fs_dist(qi_state_t a, qi_state_t b, double *out, qi_error *err);
 /* State * Complex (state.rs:2687-2705), in place. */
 qi_status qi_state_scale(qi_state_t s, double re, double im, qi_error *err);
-/* y <- y + alpha * x (State Add, state.rs:2773-2800, with a scalar). */
+/* y <- y + alpha * x (a scaled State Add, state.rs:2773-2800). */
 qi_status qi_state_axpy(qi_state_t y, double alpha_re, double alpha_im, qi_state_t x,
                         qi_error *err);
+/* out <- a + b (impl Add for State, state.rs:2773-2800) and out <- a - b
+ * (impl Sub, state.rs:2836-2862), elementwise with no norm check; out may be
+ * a or b. A qubit-count mismatch (a panic in the reference) returns
+ * InvalidNumberOfQubits(n_a). */
+qi_status qi_state_add(qi_state_t a, qi_state_t b, qi_state_t out, qi_error *err);
+qi_status qi_state_sub(qi_state_t a, qi_state_t b, qi_state_t out, qi_error *err);
 
 /* ---------------- measurement ---------------- */
 /* Outcome probabilities over `qubits` (bit b of the outcome = qubit qubits[b]),
@@ -258,19 +276,69 @@ qi_status qi_measure_probs(qi_state_t s, const uint64_t *qubits, uint32_t m, dou
 /* State::measure (state.rs:525-730) with the uniform draw injected:
  * outcome = first i with u < C_i (sequential cumsum of normalised
  * probabilities), else the last bin. The collapsed state is written to
- * `out_state` (may equal s); `*outcome` holds the outcome bits. */
+ * `out_state` (may equal s); `*outcome` holds the outcome bits. A NaN
+ * `uniform` is drawn from the context's stream (qi_ctx_seed). */
 qi_status qi_measure(qi_state_t s, qi_basis basis, const double *custom_u, const uint64_t *qubits,
                      uint32_t m, double uniform, qi_state_t out_state, uint64_t *outcome,
                      qi_error *err);
 /* State::measure_n (state.rs:750-784): `shots` independent measurements of the
- * same state with injected uniforms; outcomes only (the collapsed state of a
- * given outcome is qi_measure_collapse). */
+ * same state with injected uniforms (NULL: `shots` draws of the context's
+ * stream); outcomes only. Shot k's MeasurementResult::new_state is
+ * qi_measure_result_state of outcomes[k]. */
 qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u,
                        const uint64_t *qubits, uint32_t m, const double *uniforms, size_t shots,
                        uint64_t *outcomes, qi_error *err);
 /* Collapse onto a given computational outcome and renormalise (state.rs:622-654). */
 qi_status qi_measure_collapse(qi_state_t src, qi_state_t dst, const uint64_t *qubits, uint32_t m,
                               uint64_t outcome, qi_error *err);
+/* MeasurementResult::new_state (measurement.rs:15-72) of a measurement of
+ * `src` in `basis` that gave `outcome`: the basis change, the collapse onto
+ * the outcome with renormalisation and State::new's check, and the basis
+ * change back (state.rs:622-728). dst may be src. */
+qi_status qi_measure_result_state(qi_state_t src, qi_basis basis, const double *custom_u,
+                                  const uint64_t *qubits, uint32_t m, uint64_t outcome,
+                                  qi_state_t dst, qi_error *err);
+
+/* ---------------- parametric circuits ---------------- */
+/* A circuit whose RX / RY / RZ / P gates may read their angle from an angle
+ * table: Gate::Parametric (gate.rs:107-114) over shared Parameter<N> values
+ * (parameter.rs:12-68), for loops (VQE) that re-run one circuit with new
+ * parameters. The circuit is validated and planned once (tiled or per gate,
+ * as qi_apply_gates; opts may be NULL); angle_slots[g] >= 0 makes gate g read
+ * table entry angle_slots[g] (NULL: no parametric gates). The table starts
+ * from the gates' own angles. qi_param_circuit_set_angles replaces the whole
+ * table (MismatchedNumberOfParameters{expected, given} on a length
+ * mismatch); qi_param_circuit_run applies the circuit to a state of the same
+ * qubit count and precision in place (Circuit::execute, circuit.rs:160-172),
+ * replaying one captured launch sequence per state. Operator gates only. */
+typedef struct qi_param_circuit_s *qi_param_circuit_t;
+qi_status qi_param_circuit_create(qi_ctx_t ctx, uint32_t n, qi_prec prec, const qi_gate *gates, size_t ngates,
+                                  const int32_t *angle_slots, const qi_exec_opts *opts, qi_param_circuit_t *out,
+                                  qi_error *err);
+qi_status qi_param_circuit_destroy(qi_param_circuit_t pc);
+qi_status qi_param_circuit_num_slots(qi_param_circuit_t pc, uint64_t *nslots, qi_error *err);
+qi_status qi_param_circuit_set_angles(qi_param_circuit_t pc, const double *angles, size_t nangles, qi_error *err);
+qi_status qi_param_circuit_run(qi_param_circuit_t pc, qi_state_t s, qi_error *err);
+
+/* ---------------- OpenQASM 3.0 input ---------------- */
+/* Reads an OpenQASM 3.0 program into gate records: the programs written by
+ * Circuit::to_qasm (src/circuit.rs:244-278, src/compiler/qasm.rs,
+ * src/compiler/ir.rs) and the same gate set in standard syntax (stdgates.inc
+ * gates, ctrl / negctrl / inv modifiers, register broadcasting, measure in the
+ * computational, X (xmeasure) and Y (ymeasure) bases). The reference exports
+ * QASM but has no reader; this is its input direction, for Circuit::execute
+ * (qi_apply_gates) and qi_batch_add. Qubits are numbered across the qubit
+ * registers in declaration order. Measurements become QI_GATE_MEASURE records
+ * whose uniform is drawn when they run (NaN). `input float`/`input angle`
+ * declarations are circuit parameters, numbered in declaration order: a gate
+ * whose whole angle is parameter k gets angle_slots[g] = k and params[0] =
+ * param_values[k] (0 when param_values is NULL or shorter); every other gate
+ * gets -1. With gates == NULL only *ngates, *nqubits and *nparams are
+ * returned (the size query). Errors: QI_ERR_INVALID_INPUT_VALUE with the line
+ * number in err->a and the reason in err->msg. */
+qi_status qi_qasm_parse(const char *src, size_t len, const double *param_values, size_t nvalues,
+                        qi_gate *gates, int32_t *angle_slots, size_t capacity, size_t *ngates,
+                        uint32_t *nqubits, uint32_t *nparams, qi_error *err);
 
 /* ---------------- batched circuits ---------------- */
 /* A batch of parametrised circuits of uneven qubit counts, each run from
@@ -343,6 +411,14 @@ qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_e
  * parameters of the candidates it keeps from their indices. */
 qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang,
                                 float *y, qi_error *err);
+
+/* Predictions for `count` candidates of a local optimisation round: angles
+ * theta_j = theta0[j] + sigma z_j, z_j = sqrt(-2 ln(1 - u1)) cos(2 pi u2) with
+ * u1, u2 the SplitMix64 draws draw0 + 2 (c nang + j) + 1 and + 2 of the stream
+ * `seed`, couplings J[0..2] fixed for the round. */
+qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang,
+                                      const double *theta0, double sigma, const double *J, float *y,
+                                      qi_error *err);
 
 /* ---------------- timing hooks (for benchmarks) ---------------- */
 /* Device time of the last qi_apply_gates call's launch sequence, from HIP

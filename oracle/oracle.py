@@ -76,6 +76,10 @@ def lib():
             "or_measure_collapse": [DP, DP, C.c_uint32, U64P, C.c_uint32, C.c_uint64, ERR],
             "or_sample_outcome": [DP, C.c_uint64, C.c_double],
             "rf_apply_gates": [C.POINTER(DP), C.c_uint32, P, C.c_size_t, C.c_int, ERR],
+            "or_fs_metric": [DP, C.c_uint32, DP, C.c_uint32, C.c_int, DP, ERR],
+            "or_state_add": [DP, DP, DP, C.c_uint32, C.c_int],
+            "or_state_scale": [DP, C.c_uint32, C.c_double, C.c_double],
+            "or_sample_cdf": [DP, C.c_uint64, DP],
         }
         for name, args in sig.items():
             f = getattr(_lib, name)
@@ -84,6 +88,8 @@ def lib():
         _lib.or_norm_sqr.restype = C.c_double
         _lib.or_inner_product.restype = None
         _lib.or_sample_outcome.restype = C.c_uint64
+        for name in ("or_state_add", "or_state_scale", "or_sample_cdf"):
+            getattr(_lib, name).restype = None
     return _lib
 
 
@@ -240,6 +246,51 @@ def fidelity(a, b) -> float:
     e = _err()
     _check(lib().or_fidelity(_dp(a), _dp(b), n_of(a), C.byref(v), C.byref(e)), e)
     return v.value
+
+
+def fs_fidelity(a, b) -> float:
+    """fs_fidelity of states of any sizes, with the reference's error order."""
+    return _fs_metric(a, b, 0)
+
+
+def fs_dist(a, b) -> float:
+    return _fs_metric(a, b, 1)
+
+
+def _fs_metric(a, b, dist):
+    a, b = _vec(a), _vec(b)
+    v = C.c_double()
+    e = _err()
+    _check(lib().or_fs_metric(_dp(a), n_of(a), _dp(b), n_of(b), dist, C.byref(v), C.byref(e)), e)
+    return v.value
+
+
+def state_add(a, b) -> np.ndarray:
+    a, b = _vec(a), _vec(b)
+    out = np.empty_like(a)
+    lib().or_state_add(_dp(a), _dp(b), _dp(out), n_of(a), 0)
+    return out
+
+
+def state_sub(a, b) -> np.ndarray:
+    a, b = _vec(a), _vec(b)
+    out = np.empty_like(a)
+    lib().or_state_add(_dp(a), _dp(b), _dp(out), n_of(a), 1)
+    return out
+
+
+def state_scale(a, c) -> np.ndarray:
+    out = _vec(a)
+    c = complex(c)
+    lib().or_state_scale(_dp(out), n_of(out), c.real, c.imag)
+    return out
+
+
+def sample_cdf(probs) -> np.ndarray:
+    p = np.ascontiguousarray(np.asarray(probs, dtype=np.float64))
+    out = np.empty_like(p)
+    lib().or_sample_cdf(p.ctypes.data_as(C.POINTER(C.c_double)), p.size, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out
 
 
 def check_norm(a) -> None:

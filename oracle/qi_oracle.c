@@ -494,6 +494,39 @@ int or_fidelity(const double *a, const double *b, uint32_t n, double *out, qi_er
     return st;
 }
 
+/* fs_fidelity / fs_dist (state.rs:470-498) on states of na and nb qubits, in
+ * the reference's order: normalise self, normalise other (ZeroNorm each,
+ * state.rs:924-935), then inner_product's checks (state.rs:890-897:
+ * InvalidNumberOfQubits(0) for an empty state, InvalidNumberOfQubits(na) when
+ * the lengths differ). `dist` != 0 returns acos(|<a|b>|) (Complex::norm is
+ * hypot), else |<a|b>|^2 (Complex::norm_sqr). */
+int or_fs_metric(const double *a, uint32_t na, const double *b, uint32_t nb, int dist, double *out, qi_error *err) {
+    uint64_t da = 1ULL << na, db = 1ULL << nb;
+    cx *ca = (cx *)malloc(da * sizeof(cx)), *cb = (cx *)malloc(db * sizeof(cx));
+    if (!ca || !cb) { free(ca); free(cb); return set_err(err, QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0); }
+    memcpy(ca, a, da * sizeof(cx)); memcpy(cb, b, db * sizeof(cx));
+    int st = or_normalise((double *)ca, na, err);
+    if (!st) st = or_normalise((double *)cb, nb, err);
+    if (!st && (na == 0 || nb == 0)) st = set_err(err, QI_ERR_INVALID_NUMBER_OF_QUBITS, 0, 0, 0, 0);
+    if (!st && da != db) st = set_err(err, QI_ERR_INVALID_NUMBER_OF_QUBITS, na, 0, 0, 0);
+    if (!st) {
+        double r, i;
+        or_inner_product((double *)ca, (double *)cb, na, &r, &i);
+        *out = dist ? acos(hypot(r, i)) : r * r + i * i;
+    }
+    free(ca); free(cb);
+    return st;
+}
+
+/* impl Add / Sub for State (state.rs:2773-2800, 2836-2862): elementwise a + b
+ * or a - b; impl Mul<Complex<f64>> (state.rs:2687-2705): amplitude * rhs. */
+void or_state_add(const double *a, const double *b, double *out, uint32_t n, int sub) {
+    uint64_t dim = 1ULL << n;
+    for (uint64_t i = 0; i < dim; i++) AMP(out, i) = sub ? cx_sub(AMP(a, i), AMP(b, i)) : cx_add(AMP(a, i), AMP(b, i));
+}
+
+void or_state_scale(double *psi, uint32_t n, double re, double im) { scale_state(psi, n, cx_make(re, im)); }
+
 /* ----------------------------------------------------------- measurement -- */
 static uint64_t outcome_of(uint64_t idx, const uint64_t *q, uint32_t m) {
     uint64_t o = 0;
@@ -525,6 +558,15 @@ uint64_t or_sample_outcome(const double *probs, uint64_t nb, double u) {
     }
     if (!found && u >= cum && nb > 0) pick = nb - 1;
     return pick;
+}
+
+/* The cumulative probabilities C_i of that rule (sequential sums of
+ * p_i / total), the bin boundaries the uniform draw is compared with. */
+void or_sample_cdf(const double *probs, uint64_t nb, double *cdf) {
+    double total = 0.0;
+    for (uint64_t i = 0; i < nb; i++) total += probs[i];
+    double cum = 0.0;
+    for (uint64_t i = 0; i < nb; i++) { cum += probs[i] / total; cdf[i] = cum; }
 }
 
 /* Collapse and renormalise, state.rs:622-667 (including State::new's check). */

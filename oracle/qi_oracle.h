@@ -42,6 +42,13 @@ void or_inner_product(const double *a, const double *b, uint32_t n, double *re, 
 double or_norm_sqr(const double *psi, uint32_t n);
 int or_normalise(double *psi, uint32_t n, qi_error *err);
 int or_fidelity(const double *a, const double *b, uint32_t n, double *out, qi_error *err);
+/* fs_fidelity (dist = 0) / fs_dist (dist = 1) on states of na and nb qubits,
+ * with the reference's error order (state.rs:470-498, 890-945). */
+int or_fs_metric(const double *a, uint32_t na, const double *b, uint32_t nb, int dist, double *out,
+                 qi_error *err);
+/* State Add / Sub / Mul<Complex> (state.rs:2687-2862). */
+void or_state_add(const double *a, const double *b, double *out, uint32_t n, int sub);
+void or_state_scale(double *psi, uint32_t n, double re, double im);
 
 /* PauliString / SumOp (pauli_string.rs:139-287, 453-507). */
 int or_pauli_apply(const double *in, double *out, uint32_t n, const qi_pauli_term *t,
@@ -75,6 +82,8 @@ int or_measure_collapse(const double *psi, double *out, uint32_t n, const uint64
 /* Sequential cumulative-probability rule of State::measure (state.rs:590-619):
  * index of the first bin with u < C_i, else the last. */
 uint64_t or_sample_outcome(const double *probs, uint64_t nbins, double u);
+/* Its cumulative probabilities C_i, the boundaries u is compared with. */
+void or_sample_cdf(const double *probs, uint64_t nbins, double *cdf);
 
 /* Reference-structured CPU path used as the timing baseline: clone per gate,
  * update list + serial scatter for pair gates, in-place pass on a clone for

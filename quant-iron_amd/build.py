@@ -43,11 +43,24 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+
+
+def _compile_host(src: Path) -> Path:
+    """Host-only sources (no device code): plain g++."""
+    obj = BUILD / (src.stem + ".o")
+    r = subprocess.run(["g++", *HOST_FLAGS, "-c", str(src), "-o", str(obj)], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on {src.name}:\n{r.stderr[-6000:]}")
+    return obj
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip"))
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    host_srcs = sorted(CSRC.glob("*.cpp"))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs) + len(host_srcs))) as ex:
+        objs = list(ex.map(_compile, srcs)) + list(ex.map(_compile_host, host_srcs))
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)

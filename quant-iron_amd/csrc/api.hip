@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <random>
 
 #include "qi_internal.h"
 #include "exec_plan.h"
@@ -57,7 +58,7 @@ Status ensure(DevBuf &b, size_t bytes) {
     }
     size_t want = bytes < 4096 ? 4096 : bytes;
     hipError_t e = hipMalloc(&b.p, want);
-    if (e != hipSuccess) { b.p = nullptr; return make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)); }
+    if (e != hipSuccess) { b.p = nullptr; return oom_err(e); }
     b.bytes = want;
     return Status{};
 }
@@ -137,7 +138,9 @@ Status validate_gate(uint32_t n, const qi_gate &g) {
     case QI_GATE_MEASURE:
         if (g.basis > QI_BASIS_CUSTOM) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "measurement basis");
         if (g.basis == QI_BASIS_CUSTOM && !(s = check_unitary(g.matrix)).ok()) return s;
-        if (!(g.params[0] >= 0.0 && g.params[0] < 1.0)) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)");
+        // a NaN uniform is drawn from the context's stream when the gate runs
+        if (!std::isnan(g.params[0]) && !(g.params[0] >= 0.0 && g.params[0] < 1.0))
+            return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)");
         return check_measure(n, g.controls, nc);
     case QI_GATE_PAULI_STRING:
         return check_mask(n, g.x_mask | g.z_mask);
@@ -179,14 +182,17 @@ static Status normalise(State &s) {
 }
 
 // State::new's check (state.rs:117-122) on a device state: |sum |a|^2 - 1|
-// <= eps * len, with eps the machine epsilon of the stored amplitudes
-// (f64::EPSILON for complex128 as in the reference, f32::EPSILON for
-// complex64, whose renormalised amplitudes carry FP32 rounding).
+// <= tol. complex128 uses the reference's tolerance, f64::EPSILON * len.
+// complex64 amplitudes that were just renormalised carry one FP32 rounding
+// of the quotient and one of the divisor, so sum |a|^2 (summed in FP64) is
+// within about 2 f32::EPSILON of 1 at any size; the bound 16 f32::EPSILON
+// does not grow with the state, so a collapse onto an outcome of zero
+// probability (an all-zero state) is rejected at every size.
 static Status check_normalised(State &s) {
     double ns;
     Status e = norm_sqr(s, &ns);
     if (!e.ok()) return e;
-    double tol = (s.prec == QI_C128 ? DBL_EPSILON : (double)FLT_EPSILON) * double(uint64_t(1) << s.n);
+    double tol = s.prec == QI_C128 ? DBL_EPSILON * double(uint64_t(1) << s.n) : 16.0 * (double)FLT_EPSILON;
     if (std::fabs(ns - 1.0) > tol) return make_err(QI_ERR_STATE_NOT_NORMALISED);
     return Status{};
 }
@@ -194,7 +200,7 @@ static Status check_normalised(State &s) {
 static Status alloc_state(Ctx &C, uint32_t n, int prec, State &out) {
     out.ctx = &C; out.n = n; out.prec = prec; out.d = nullptr;
     hipError_t e = hipMalloc(&out.d, out.bytes());
-    if (e != hipSuccess) { out.d = nullptr; return make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)); }
+    if (e != hipSuccess) { out.d = nullptr; return oom_err(e); }
     return Status{};
 }
 
@@ -252,11 +258,26 @@ static Status measure_one(State &s, int basis, const double *u, const uint64_t *
     Status e = check_measure(s.n, qubits, m);
     if (!e.ok()) return e;
     if (basis == QI_BASIS_CUSTOM && !(e = check_unitary(u)).ok()) return e;
+    if (std::isnan(uniform)) uniform = splitmix_uniform(s.ctx->rng);  // rand::rng(), state.rs:601-602
     if (!(uniform >= 0.0 && uniform < 1.0)) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)");
     hipStream_t st = s.ctx->stream;
     if (dst.d != s.d && !(e = launch_copy(dst, s, st)).ok()) return e;
     if (!(e = basis_in(dst, basis, u, qubits, m)).ok()) return e;
     if (!(e = measure_sample_collapse(dst, dst, qubits, m, &uniform, 1, outcome, true, st)).ok()) return e;
+    if (!(e = check_normalised(dst)).ok()) return e;
+    return basis_out(dst, basis, u, qubits, m);
+}
+
+// MeasurementResult::new_state of a given outcome (state.rs:622-728): the
+// basis change, the collapse onto `outcome` with renormalisation and
+// State::new's check, and the basis change back. `dst` may be `s`.
+static Status measured_state(State &s, int basis, const double *u, const uint64_t *qubits, uint32_t m,
+                             uint64_t outcome, State &dst) {
+    hipStream_t st = s.ctx->stream;
+    Status e;
+    if (dst.d != s.d && !(e = launch_copy(dst, s, st)).ok()) return e;
+    if (!(e = basis_in(dst, basis, u, qubits, m)).ok()) return e;
+    if (!(e = collapse_outcome(dst, dst, qubits, m, outcome, st)).ok()) return e;
     if (!(e = check_normalised(dst)).ok()) return e;
     return basis_out(dst, basis, u, qubits, m);
 }
@@ -353,7 +374,9 @@ static Status run_maybe_graph(Ctx &C, State &s, bool use_graph, const void *key,
 // Plans the launch sequence of a gate list: runs of operator gates go through
 // the chosen executor, other Gate variants (measurement, Pauli strings) become
 // host-orchestrated steps between them.
-static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, const qi_exec_opts &o, ExecPlan &plan) {
+// `slots` (may be NULL): the angle slot of each gate, -1 for none (parametric circuits).
+static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, const qi_exec_opts &o, ExecPlan &plan,
+                         const int32_t *slots = nullptr) {
     size_t k = 0;
     while (k < ng) {
         if (!is_operator(gates[k].kind)) { plan.steps.push_back(ExecStep::host(k)); k++; continue; }
@@ -363,6 +386,7 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
             Lowered L;
             Status e = lower_gate(s.n, gates[j], L);
             if (!e.ok()) return e;
+            if (slots) L.slot = slots[j];
             if (L.kind != LK_NOP) seg.push_back(L);
             j++;
         }
@@ -376,11 +400,12 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
     return Status{};
 }
 
-static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates) {
+// `blob`: the device copy of the plan's op tables (reg_ops, then matrices).
+static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates, const void *blob) {
     Ctx &C = *s.ctx;
     hipStream_t st = C.stream;
-    const RegOp *dops = (const RegOp *)C.plan.p;
-    const unsigned char *dmats = (const unsigned char *)C.plan.p + plan.mats_offset();
+    const RegOp *dops = (const RegOp *)blob;
+    const unsigned char *dmats = (const unsigned char *)blob + plan.mats_offset();
     for (const ExecStep &step : plan.steps) {
         Status e;
         switch (step.type) {
@@ -447,7 +472,7 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
     // graph key: the plan key and the op-table buffer the graph reads
     const uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
     key.insert(key.end(), (const unsigned char *)&pp, (const unsigned char *)&pp + 8);
-    auto launch = [&]() -> Status { return launch_plan(s, *plan, gates); };
+    auto launch = [&]() -> Status { return launch_plan(s, *plan, gates, C.plan.p); };
     return run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
 }
 
@@ -469,12 +494,59 @@ using namespace qi;
         return (qi_status)s_.code; \
     } while (0)
 
+#define QI_HIP_TRY_C(expr)                                                                  \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e_))); \
+    } while (0)
+
 #define QI_NULLCHK(p)                                                                             \
     do {                                                                                          \
         if (!(p)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, #p " is NULL"));          \
     } while (0)
 
 #define QI_LOCK(ctxp) std::lock_guard<std::mutex> lk_((ctxp)->mu)
+
+// A circuit with parametric gates (Gate::Parametric, gate.rs:107-114, whose
+// angles come from shared Parameter<N> values, parameter.rs:12-68), planned
+// once: new angles rewrite the coefficients of the parametric ops in the op
+// tables (one upload), and each run replays the captured launch sequence.
+struct qi_param_circuit_s {
+    Ctx *ctx = nullptr;
+    uint32_t n = 0;
+    int prec = QI_C64;
+    std::vector<qi_gate> gates;
+    std::vector<int32_t> slots;
+    std::vector<double> angles;  // the angle table: entry k is slot k's angle
+    ExecPlan plan;
+    DevBuf blob;                 // device copy of plan.reg_ops
+    bool dirty = true;           // plan.reg_ops changed since the last upload
+    bool graph_ok = true;        // false when parametric gates run as streaming steps (angles in launch arguments)
+    hipGraphExec_t graph = nullptr;
+    void *graph_state = nullptr;
+};
+
+namespace qi {
+
+// New coefficients of every parametric op of the plan for the angle table.
+static void patch_param_plan(qi_param_circuit_s &pc) {
+    for (RegOp &op : pc.plan.reg_ops)
+        if (op.param > 0) param_op_coeffs(op, pc.angles[(size_t)op.param - 1]);
+    static const int gate_of[LK_NOP + 1] = {-1, -1, -1, QI_GATE_RX, QI_GATE_RY, -1, -1, QI_GATE_P, QI_GATE_RZ, -1, -1, -1};
+    for (ExecStep &st : pc.plan.steps) {
+        if (st.type != ExecStep::GATE || st.L.slot < 0) continue;
+        qi_gate g{};
+        g.kind = gate_of[st.L.kind];
+        g.ntargets = 1;
+        g.params[0] = pc.angles[(size_t)st.L.slot];
+        Lowered M;
+        (void)lower_gate(pc.n, g, M);
+        memcpy(st.L.c, M.c, sizeof M.c);
+    }
+    pc.dirty = true;
+}
+
+}  // namespace qi
 
 extern "C" {
 
@@ -497,6 +569,8 @@ qi_status qi_ctx_create(int device, qi_ctx_t *out, qi_error *err) {
     C.host_pinned_words = 64;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) C.cu_count = prop.multiProcessorCount;
+    // an unseeded context draws from an OS-seeded stream, as rand::rng() does
+    C.rng = (uint64_t(std::random_device{}()) << 32) ^ uint64_t(std::random_device{}());
     *out = c.release();
     QI_RET(Status{});
 }
@@ -507,7 +581,7 @@ qi_status qi_ctx_destroy(qi_ctx_t ctx) {
     (void)hipSetDevice(C.device);
     (void)hipStreamSynchronize(C.stream);
     if (C.graph_exec) (void)hipGraphExecDestroy(C.graph_exec);
-    for (DevBuf *b : {&C.red, &C.red_out, &C.ops, &C.probs, &C.aux, &C.plan})
+    for (DevBuf *b : {&C.red, &C.red_out, &C.ops, &C.probs, &C.aux, &C.plan, &C.trot})
         if (b->p) (void)hipFree(b->p);
     if (C.host_pinned) (void)hipHostFree(C.host_pinned);
     if (C.ev0) (void)hipEventDestroy(C.ev0);
@@ -515,6 +589,13 @@ qi_status qi_ctx_destroy(qi_ctx_t ctx) {
     if (C.stream) (void)hipStreamDestroy(C.stream);
     delete ctx;
     return QI_OK;
+}
+
+qi_status qi_ctx_seed(qi_ctx_t ctx, uint64_t seed, qi_error *err) {
+    QI_NULLCHK(ctx);
+    QI_LOCK(&ctx->c);
+    ctx->c.rng = seed;
+    QI_RET(Status{});
 }
 
 qi_status qi_ctx_synchronize(qi_ctx_t ctx, qi_error *err) {
@@ -775,6 +856,114 @@ static void pauli_exp_gates(const qi_pauli_term &t, double theta, std::vector<qi
 // States from this size on run Trotter steps through the tiled executor.
 constexpr uint32_t kFusedTrotterQubits = 16;
 
+// Two Pauli strings commute iff they anticommute on an even number of qubits.
+static bool pauli_commute(uint64_t x1, uint64_t z1, uint64_t x2, uint64_t z2) {
+    return (__builtin_popcountll((x1 & z2) ^ (z1 & x2)) & 1) == 0;
+}
+
+// One Trotter step on the streaming kernels as a sequence of pieces: a
+// non-diagonal term exp(a P), or a run of diagonal terms applied in one pass.
+// A diagonal term joins the last diagonal run when it commutes with every
+// term after that run (moving it back past them changes nothing), and
+// consecutive exponentials of the same string add their factors; both are
+// exact rewrites of the reference's term-by-term product (time_evolution.rs:
+// 45-115), so results differ only by rounding.
+struct TrotterPiece {
+    bool diag = false;
+    std::vector<std::pair<qi_pauli_term, cxd>> terms;  // (term, factor applied to its coefficient)
+    DiagPass pass{};
+    size_t table_off = 0;  // first table entry of this piece in the step's table buffer
+};
+
+static bool diag_fits(const qi_pauli_term &t) { return t.x_mask == 0 && __builtin_popcountll(t.z_mask) <= kDiagTabBits; }
+
+static std::vector<TrotterPiece> trotter_pieces(const qi_pauli_term *terms, size_t nt, double dt, int order) {
+    std::vector<std::pair<qi_pauli_term, cxd>> seq;
+    if (order == 1) {
+        for (size_t k = 0; k < nt; k++) seq.push_back({terms[k], cxd{0.0, -dt}});
+    } else {
+        for (size_t k = 0; k < nt; k++) seq.push_back({terms[k], cxd{0.0, -dt / 2.0}});
+        for (size_t k = nt; k-- > 0;) seq.push_back({terms[k], cxd{0.0, -dt / 2.0}});
+    }
+    std::vector<TrotterPiece> out;
+    for (const auto &tf : seq) {
+        const qi_pauli_term &t = tf.first;
+        if (diag_fits(t)) {
+            // walk back over non-diagonal pieces the term commutes with
+            size_t j = out.size();
+            while (j > 0 && !out[j - 1].diag &&
+                   pauli_commute(0, t.z_mask, out[j - 1].terms[0].first.x_mask, out[j - 1].terms[0].first.z_mask))
+                j--;
+            if (j > 0 && out[j - 1].diag) { out[j - 1].terms.push_back(tf); continue; }
+        } else if (!out.empty() && !out.back().diag && out.back().terms[0].first.x_mask == t.x_mask &&
+                   out.back().terms[0].first.z_mask == t.z_mask && out.back().terms[0].first.c_re == t.c_re &&
+                   out.back().terms[0].first.c_im == t.c_im) {
+            // exp(a P) exp(b P) = exp((a + b) P)
+            out.back().terms[0].second.re += tf.second.re;
+            out.back().terms[0].second.im += tf.second.im;
+            continue;
+        }
+        TrotterPiece p;
+        p.diag = diag_fits(t);
+        p.terms.push_back(tf);
+        out.push_back(p);
+    }
+    return out;
+}
+
+// Splits a diagonal run into passes of at most kDiagTabMax tables over at most
+// kDiagTabBits qubits each, and fills the tables: entry u of a table is the
+// product over its terms of cosh(a) + s sinh(a), s = (-1)^{popcount(u's bits & z)}.
+static void diag_tables(const std::vector<std::pair<qi_pauli_term, cxd>> &run, std::vector<TrotterPiece> &passes,
+                        std::vector<double> &tab) {
+    struct Table { uint64_t mask; std::vector<size_t> members; };
+    std::vector<Table> cur;
+    auto flush = [&]() {
+        if (cur.empty()) return;
+        TrotterPiece p;
+        p.diag = true;
+        p.table_off = tab.size() / 2;
+        int entries = 0;
+        for (size_t b = 0; b < cur.size(); b++) {
+            DiagTab &d = p.pass.tab[b];
+            d.bits = 0;
+            d.off = entries;
+            for (uint64_t m = cur[b].mask; m; m &= m - 1) d.pos[d.bits++] = (uint8_t)__builtin_ctzll(m);
+            for (uint32_t u = 0; u < (1u << d.bits); u++) {
+                uint64_t pattern = 0;
+                for (int k = 0; k < d.bits; k++) if ((u >> k) & 1) pattern |= uint64_t(1) << d.pos[k];
+                cxd prod{1.0, 0.0};
+                for (size_t i : cur[b].members) {
+                    const qi_pauli_term &t = run[i].first;
+                    const cxd a = cmul(cxd{t.c_re, t.c_im}, run[i].second);
+                    const double s = (__builtin_popcountll(pattern & t.z_mask) & 1) ? -1.0 : 1.0;
+                    const cxd ch = ccosh_(a), sh = csinh_(a);
+                    prod = cmul(prod, cxd{ch.re + s * sh.re, ch.im + s * sh.im});
+                }
+                tab.push_back(prod.re);
+                tab.push_back(prod.im);
+            }
+            entries += 1 << d.bits;
+        }
+        p.pass.ntab = (int)cur.size();
+        p.pass.entries = entries;
+        passes.push_back(p);
+        cur.clear();
+    };
+    for (size_t i = 0; i < run.size(); i++) {
+        const uint64_t z = run[i].first.z_mask;
+        size_t b = 0;
+        while (b < cur.size() && __builtin_popcountll(cur[b].mask | z) > kDiagTabBits) b++;
+        if (b == cur.size()) {
+            if (cur.size() == (size_t)kDiagTabMax) { flush(); b = 0; }
+            cur.push_back(Table{0, {}});
+        }
+        cur[b].mask |= z;
+        cur[b].members.push_back(i);
+    }
+    flush();
+}
+
 qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt, uint64_t steps, int order,
                             qi_error *err) {
     QI_NULLCHK(s);
@@ -804,20 +993,29 @@ qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nte
         o.use_graph = 1;
         for (uint64_t step = 0; step < steps; step++)
             if (!(e = apply_gates(S, S.n, step_gates.data(), step_gates.size(), &o, true)).ok()) QI_RET(e);
-        QI_RET(Status{});
+        QI_RET(Status{});  // last_plan_steps: the tile passes of one step
     }
-    for (uint64_t step = 0; step < steps; step++) {
-        if (order == 1) {
-            for (size_t k = 0; k < nterms; k++)
-                if (!(e = pauli_exp(S, terms[k], cxd{0.0, -dt})).ok()) QI_RET(e);
-        } else {
-            cxd half{0.0, -dt / 2.0};
-            for (size_t k = 0; k < nterms; k++)
-                if (!(e = pauli_exp(S, terms[k], half)).ok()) QI_RET(e);
-            for (size_t k = nterms; k-- > 0;)
-                if (!(e = pauli_exp(S, terms[k], half)).ok()) QI_RET(e);
+    // streaming kernels: one pass per non-diagonal term, one per diagonal run
+    std::vector<TrotterPiece> plan;
+    std::vector<double> tab;
+    for (TrotterPiece &p : trotter_pieces(terms, nterms, dt, order)) {
+        if (p.diag) diag_tables(p.terms, plan, tab);
+        else plan.push_back(p);
+    }
+    Ctx &C = *S.ctx;
+    if (!tab.empty()) {
+        if (!(e = ensure(C.trot, tab.size() * sizeof(double))).ok()) QI_RET(e);
+        hipError_t he = hipMemcpyAsync(C.trot.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, C.stream);
+        if (he != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(he)));
+    }
+    C.last_plan_steps = plan.size();
+    const double2 *dtab = (const double2 *)C.trot.p;
+    for (uint64_t step = 0; step < steps; step++)
+        for (const TrotterPiece &p : plan) {
+            if (p.diag) e = launch_diag_tables(S, p.pass, dtab + p.table_off, C.stream);
+            else e = pauli_exp(S, p.terms[0].first, p.terms[0].second);
+            if (!e.ok()) QI_RET(e);
         }
-    }
     QI_RET(device_check(S.ctx->stream));
 }
 
@@ -846,20 +1044,47 @@ qi_status qi_normalise(qi_state_t s, qi_error *err) {
     QI_RET(device_check(s->s.ctx->stream));
 }
 
+// The normalised inner product <a/|a| | b/|b|> of fs_fidelity / fs_dist
+// (state.rs:470-498), with the reference's error order: each state's
+// normalise (ZeroNorm) before the inner product's qubit-count check
+// (state.rs:890-897).
+static Status normalised_inner(State &a, State &b, double *ir, double *ii) {
+    double na, nb, r[2];
+    Status e;
+    if (!(e = norm_sqr(a, &na)).ok()) return e;
+    if (std::sqrt(na) == 0.0) return make_err(QI_ERR_ZERO_NORM);
+    if (!(e = norm_sqr(b, &nb)).ok()) return e;
+    if (std::sqrt(nb) == 0.0) return make_err(QI_ERR_ZERO_NORM);
+    if (!(e = check_pair(a, b)).ok()) return e;
+    if (!(e = reduce_state(a, &b, RK_INNER, r, a.ctx->stream)).ok()) return e;
+    const double scale = std::sqrt(na) * std::sqrt(nb);
+    *ir = r[0] / scale;
+    *ii = r[1] / scale;
+    return Status{};
+}
+
 qi_status qi_fidelity(qi_state_t a, qi_state_t b, double *out, qi_error *err) {
     // fs_fidelity, state.rs:492-498: |<a/|a| | b/|b|>|^2
     QI_NULLCHK(a); QI_NULLCHK(b); QI_NULLCHK(out);
-    Status e = check_pair(a->s, b->s);
-    if (!e.ok()) QI_RET(e);
+    if (a->s.ctx != b->s.ctx) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "states on different contexts"));
     QI_LOCK(a->s.ctx);
-    double na, nb, r[2];
-    if (!(e = norm_sqr(a->s, &na)).ok()) QI_RET(e);
-    if (!(e = norm_sqr(b->s, &nb)).ok()) QI_RET(e);
-    if (std::sqrt(na) == 0.0 || std::sqrt(nb) == 0.0) QI_RET(make_err(QI_ERR_ZERO_NORM));
-    if (!(e = reduce_state(a->s, &b->s, RK_INNER, r, a->s.ctx->stream)).ok()) QI_RET(e);
-    double scale = std::sqrt(na) * std::sqrt(nb);
-    double ir = r[0] / scale, ii = r[1] / scale;
+    double ir, ii;
+    Status e = normalised_inner(a->s, b->s, &ir, &ii);
+    if (!e.ok()) QI_RET(e);
     *out = ir * ir + ii * ii;
+    QI_RET(Status{});
+}
+
+qi_status qi_fs_dist(qi_state_t a, qi_state_t b, double *out, qi_error *err) {
+    // fs_dist, state.rs:470-478: acos(|<a/|a| | b/|b|>|), unclamped as in the
+    // reference (a modulus rounded above 1 gives NaN there too)
+    QI_NULLCHK(a); QI_NULLCHK(b); QI_NULLCHK(out);
+    if (a->s.ctx != b->s.ctx) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "states on different contexts"));
+    QI_LOCK(a->s.ctx);
+    double ir, ii;
+    Status e = normalised_inner(a->s, b->s, &ir, &ii);
+    if (!e.ok()) QI_RET(e);
+    *out = std::acos(std::hypot(ir, ii));  // Complex::norm is hypot
     QI_RET(Status{});
 }
 
@@ -879,6 +1104,19 @@ qi_status qi_state_axpy(qi_state_t y, double re, double im, qi_state_t x, qi_err
     if (!(e = launch_axpy(y->s, re, im, x->s, y->s.ctx->stream)).ok()) QI_RET(e);
     QI_RET(device_check(y->s.ctx->stream));
 }
+
+static qi_status addsub(qi_state_t a, qi_state_t b, qi_state_t out, bool sub, qi_error *err) {
+    QI_NULLCHK(a); QI_NULLCHK(b); QI_NULLCHK(out);
+    Status e = check_pair(a->s, b->s);  // the reference panics on a qubit-count mismatch
+    if (!e.ok()) QI_RET(e);
+    if (!(e = check_pair(a->s, out->s)).ok()) QI_RET(e);
+    QI_LOCK(a->s.ctx);
+    if (!(e = launch_addsub(out->s, a->s, b->s, sub, a->s.ctx->stream)).ok()) QI_RET(e);
+    QI_RET(device_check(a->s.ctx->stream));
+}
+
+qi_status qi_state_add(qi_state_t a, qi_state_t b, qi_state_t out, qi_error *err) { return addsub(a, b, out, false, err); }
+qi_status qi_state_sub(qi_state_t a, qi_state_t b, qi_state_t out, qi_error *err) { return addsub(a, b, out, true, err); }
 
 qi_status qi_measure_probs(qi_state_t s, const uint64_t *qubits, uint32_t m, double *probs, qi_error *err) {
     QI_NULLCHK(s); QI_NULLCHK(probs);
@@ -913,7 +1151,7 @@ qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u, con
                        const double *uniforms, size_t shots, uint64_t *outcomes, qi_error *err) {
     QI_NULLCHK(s);
     if (shots == 0) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_MEASUREMENTS, 0));  // state.rs:756-758
-    QI_NULLCHK(uniforms); QI_NULLCHK(outcomes);
+    QI_NULLCHK(outcomes);
     if (m && !qubits) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "qubits is NULL"));
     if (basis < QI_BASIS_COMPUTATIONAL || basis > QI_BASIS_CUSTOM) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "basis"));
     if (basis == QI_BASIS_CUSTOM && !custom_u) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "custom basis is NULL"));
@@ -923,9 +1161,15 @@ qi_status qi_measure_n(qi_state_t s, qi_basis basis, const double *custom_u, con
     Status e = check_measure(S.n, qubits, m);
     if (!e.ok()) QI_RET(e);
     if (basis == QI_BASIS_CUSTOM && !(e = check_unitary(custom_u)).ok()) QI_RET(e);
-    for (size_t k = 0; k < shots; k++)
+    for (size_t k = 0; uniforms && k < shots; k++)
         if (!(uniforms[k] >= 0.0 && uniforms[k] < 1.0)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "uniform draw outside [0,1)"));
     QI_LOCK(S.ctx);
+    std::vector<double> drawn;
+    if (!uniforms) {  // one draw of the context's stream per shot (rand::rng(), state.rs:601-602)
+        drawn.resize(shots);
+        for (double &u : drawn) u = splitmix_uniform(S.ctx->rng);
+        uniforms = drawn.data();
+    }
     // every shot measures the same state: one histogram and scan, many searches
     TmpState w;
     State *src = &S;
@@ -956,6 +1200,153 @@ qi_status qi_measure_collapse(qi_state_t src, qi_state_t dst, const uint64_t *qu
     QI_LOCK(S.ctx);
     if (!(e = collapse_outcome(S, dst->s, qubits, m, outcome, S.ctx->stream)).ok()) QI_RET(e);
     QI_RET(check_normalised(dst->s));
+}
+
+qi_status qi_measure_result_state(qi_state_t src, qi_basis basis, const double *custom_u, const uint64_t *qubits,
+                                  uint32_t m, uint64_t outcome, qi_state_t dst, qi_error *err) {
+    QI_NULLCHK(src); QI_NULLCHK(dst);
+    if (m && !qubits) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "qubits is NULL"));
+    if (basis < QI_BASIS_COMPUTATIONAL || basis > QI_BASIS_CUSTOM) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "basis"));
+    if (basis == QI_BASIS_CUSTOM && !custom_u) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "custom basis is NULL"));
+    Status e = check_pair(src->s, dst->s);
+    if (!e.ok()) QI_RET(e);
+    State &S = src->s;
+    std::vector<uint64_t> all;
+    if (m == 0) { for (uint32_t i = 0; i < S.n; i++) all.push_back(i); qubits = all.data(); m = S.n; }
+    if (!(e = check_measure(S.n, qubits, m)).ok()) QI_RET(e);
+    if (basis == QI_BASIS_CUSTOM && !(e = check_unitary(custom_u)).ok()) QI_RET(e);
+    if (m < 64 && outcome >= (uint64_t(1) << m)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "outcome out of range"));
+    QI_LOCK(S.ctx);
+    if (!(e = measured_state(S, basis, custom_u, qubits, m, outcome, dst->s)).ok()) QI_RET(e);
+    QI_RET(device_check(S.ctx->stream));
+}
+
+qi_status qi_param_circuit_create(qi_ctx_t ctx, uint32_t n, qi_prec prec, const qi_gate *gates, size_t ngates,
+                                  const int32_t *angle_slots, const qi_exec_opts *opts, qi_param_circuit_t *out,
+                                  qi_error *err) {
+    QI_NULLCHK(ctx); QI_NULLCHK(out);
+    if (ngates && !gates) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gates is NULL"));
+    if (n == 0) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, 0));
+    if (n > QI_MAX_QUBITS) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "more qubits than the engine supports"));
+    if (prec != QI_C64 && prec != QI_C128) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "precision"));
+    std::unique_ptr<qi_param_circuit_s> pc(new qi_param_circuit_s());
+    pc->ctx = &ctx->c; pc->n = n; pc->prec = prec;
+    pc->gates.assign(gates, gates + ngates);
+    pc->slots.assign(ngates, -1);
+    Status e;
+    for (size_t g = 0; g < ngates; g++) {
+        if (!is_operator(gates[g].kind)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, g, 0, 0, 0, "parametric circuits hold operator gates only"));
+        if (!(e = validate_gate(n, gates[g])).ok()) QI_RET(e);
+        const int32_t sl = angle_slots ? angle_slots[g] : -1;
+        if (sl < 0) continue;
+        const int k = gates[g].kind;
+        if (k != QI_GATE_RX && k != QI_GATE_RY && k != QI_GATE_RZ && k != QI_GATE_P)
+            QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, g, 0, 0, 0, "only RX, RY, RZ and P gates take an angle slot"));
+        if (sl >= (1 << 24)) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, g, 0, 0, 0, "angle slot out of range"));
+        pc->slots[g] = sl;
+        if ((size_t)sl >= pc->angles.size()) pc->angles.resize((size_t)sl + 1, 0.0);
+    }
+    // the angle table starts from the gates' own angles (first gate of each slot)
+    std::vector<char> seen(pc->angles.size(), 0);
+    for (size_t g = 0; g < ngates; g++)
+        if (pc->slots[g] >= 0 && !seen[(size_t)pc->slots[g]]) {
+            seen[(size_t)pc->slots[g]] = 1;
+            pc->angles[(size_t)pc->slots[g]] = gates[g].params[0];
+        }
+    qi_exec_opts o{};
+    o.mode = QI_EXEC_AUTO;
+    if (opts) o = *opts;
+    int mode = o.mode;
+    if (mode == QI_EXEC_AUTO) mode = n >= 12 ? QI_EXEC_TILED : QI_EXEC_PER_GATE;
+    if (mode != QI_EXEC_TILED && mode != QI_EXEC_PER_GATE)
+        QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "parametric circuits run tiled or per gate"));
+    State shape;
+    shape.ctx = &ctx->c; shape.n = n; shape.prec = prec;
+    if (!(e = build_plan(shape, pc->gates.data(), ngates, mode, o, pc->plan, pc->slots.data())).ok()) QI_RET(e);
+    if (!pc->plan.mats.empty()) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "dense blocks are not parametric"));
+    for (const ExecStep &st : pc->plan.steps) pc->graph_ok = pc->graph_ok && !(st.type == ExecStep::GATE && st.L.slot >= 0);
+    patch_param_plan(*pc);
+    {
+        QI_LOCK(&ctx->c);
+        if (!pc->plan.reg_ops.empty() && !(e = ensure(pc->blob, pc->plan.reg_ops.size() * sizeof(RegOp))).ok()) QI_RET(e);
+    }
+    *out = pc.release();
+    QI_RET(Status{});
+}
+
+qi_status qi_param_circuit_destroy(qi_param_circuit_t pc) {
+    if (!pc) return QI_OK;
+    {
+        QI_LOCK(pc->ctx);
+        (void)hipStreamSynchronize(pc->ctx->stream);
+        if (pc->graph) (void)hipGraphExecDestroy(pc->graph);
+        if (pc->blob.p) (void)hipFree(pc->blob.p);
+    }
+    delete pc;
+    return QI_OK;
+}
+
+qi_status qi_param_circuit_num_slots(qi_param_circuit_t pc, uint64_t *nslots, qi_error *err) {
+    QI_NULLCHK(pc); QI_NULLCHK(nslots);
+    *nslots = pc->angles.size();
+    QI_RET(Status{});
+}
+
+qi_status qi_param_circuit_set_angles(qi_param_circuit_t pc, const double *angles, size_t nangles, qi_error *err) {
+    QI_NULLCHK(pc);
+    if (nangles != pc->angles.size())
+        QI_RET(make_err(QI_ERR_MISMATCHED_PARAMETERS, pc->angles.size(), nangles));  // MismatchedNumberOfParameters{expected, actual}
+    if (nangles) QI_NULLCHK(angles);
+    QI_LOCK(pc->ctx);
+    std::copy(angles, angles + nangles, pc->angles.begin());
+    patch_param_plan(*pc);
+    QI_RET(Status{});
+}
+
+qi_status qi_param_circuit_run(qi_param_circuit_t pc, qi_state_t s, qi_error *err) {
+    QI_NULLCHK(pc); QI_NULLCHK(s);
+    State &S = s->s;
+    if (S.n != pc->n) QI_RET(make_err(QI_ERR_INVALID_NUMBER_OF_QUBITS, S.n));  // circuit.rs:161-163
+    if (S.prec != pc->prec) QI_RET(make_err(QI_ERR_PRECISION_MISMATCH));
+    if (S.ctx != pc->ctx) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "state on another context"));
+    QI_LOCK(pc->ctx);
+    Ctx &C = *pc->ctx;
+    hipStream_t st = C.stream;
+    Status e;
+    if (pc->dirty && !pc->plan.reg_ops.empty()) {
+        // the previous run may still read the op table: the stream orders the copy after it
+        hipError_t he = hipMemcpyAsync(pc->blob.p, pc->plan.reg_ops.data(), pc->plan.reg_ops.size() * sizeof(RegOp),
+                                       hipMemcpyHostToDevice, st);
+        if (he == hipSuccess) he = hipStreamSynchronize(st);  // the host table may change before the next run
+        if (he != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(he)));
+    }
+    pc->dirty = false;
+    QI_HIP_TRY_C(hipEventRecord(C.ev0, st));
+    if (!pc->graph_ok) {
+        if (!(e = launch_plan(S, pc->plan, pc->gates.data(), pc->blob.p)).ok()) QI_RET(e);
+    } else {
+        if (!pc->graph || pc->graph_state != S.d) {
+            if (pc->graph) { (void)hipGraphExecDestroy(pc->graph); pc->graph = nullptr; }
+            hipGraph_t graph = nullptr;
+            QI_HIP_TRY_C(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            e = launch_plan(S, pc->plan, pc->gates.data(), pc->blob.p);
+            hipError_t ce = hipStreamEndCapture(st, &graph);
+            if (!e.ok()) { if (graph) (void)hipGraphDestroy(graph); QI_RET(e); }
+            if (ce != hipSuccess) QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ce)));
+            hipError_t ie = hipGraphInstantiate(&pc->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) { pc->graph = nullptr; QI_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(ie))); }
+            pc->graph_state = S.d;
+        }
+        QI_HIP_TRY_C(hipGraphLaunch(pc->graph, st));
+    }
+    QI_HIP_TRY_C(hipEventRecord(C.ev1, st));
+    if (!(e = device_check(st)).ok()) QI_RET(e);
+    float ms = 0.f;
+    QI_HIP_TRY_C(hipEventElapsedTime(&ms, C.ev0, C.ev1));
+    C.last_exec_ms = ms;
+    C.last_plan_steps = pc->plan.steps.size();
+    QI_RET(Status{});
 }
 
 qi_status qi_last_exec_ms(qi_ctx_t ctx, double *ms, qi_error *err) {

@@ -77,7 +77,7 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = local bit),
                          //               2 outside the tile (dbit = qubit)
     int32_t dbit;
-    int32_t pad0;
+    int32_t param;       // angle slot + 1 of a parametric op (RO_RR: RY, RO_RXI: RX, RO_DG: RZ, RO_P1: P); 0: none
     int32_t lp0, lp1;    // RO_LDS: local bits of the targets
     uint32_t rcm;        // controls on register bits (register-bit mask)
     uint32_t tcm;        // controls on thread bits (local-bit mask)
@@ -85,7 +85,7 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     uint32_t src;        // source gate index
     uint64_t gcm;        // controls outside the tile (qubit mask)
     uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
-    uint8_t lay_t[12];   //               local bit of each thread bit
+    uint8_t lay_t[12];   //               local bit of each thread bit (also after RO_XLR / RO_XLL)
     uint32_t pad1;
     double c[8];         // coefficients
     float cf[8];         // the same, rounded to float (read by the complex64 kernels)
@@ -139,6 +139,10 @@ struct ExecPlan {
     size_t mats_offset() const { return (reg_ops.size() * sizeof(RegOp) + 255) & ~size_t(255); }
     size_t blob_bytes() const { return mats_offset() + mats.size(); }
 };
+
+// Coefficients of a parametric op (RegOp::param > 0) for the angle theta, in
+// the op's form: the same values plan_tiled writes for that gate at that angle.
+void param_op_coeffs(RegOp &op, double theta);
 
 int default_tile_qubits(const State &s);
 int default_tile_threads_log2(int prec, int K);

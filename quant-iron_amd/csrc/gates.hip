@@ -62,6 +62,7 @@ template <typename R> struct GateK {
     uint64_t pmask;   // pair partner = base ^ pmask
     uint64_t tb2;     // second target bit (swap / matchgate)
     uint64_t cmask;   // control mask tested per element (matchgate only)
+    uint64_t base;    // first item of this launch (a gate over more items than one grid holds is split)
     int32_t chunk;    // workgroup order (xcd_block_order) of the pair and diagonal kernels
     int32_t lane;     // lane-exchange kernels: lane offset of the partner (1, 2 or 4)
     R c[10];
@@ -123,7 +124,7 @@ __device__ __forceinline__ f64x2 unamp(double2 a) { f64x2 r; r.x = a.x; r.y = a.
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, GateK<float> g) {
     if constexpr (MODE == 2) {
-        const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+        const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
         const bool h0 = k0 < g.items, h1 = k1 < g.items;
         uint64_t j0 = 0, j1 = 0;
         f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, 
         if (h0) { pair_update<float, KIND>(g, lo(u0), hi(u0), na, nb); stnt(pack(na, nb), psi + j0); }
         if (h1) { pair_update<float, KIND>(g, lo(u1), hi(u1), na, nb); stnt(pack(na, nb), psi + j1); }
     } else {
-        const uint64_t k = xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+        const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
         if (k >= g.items) return;
         const uint64_t ja = insert_zeros(k, g.fixed) | g.set, jb = ja ^ g.pmask;
         const f32x4 x = ldnt(psi + ja), y = ldnt(psi + jb);
@@ -166,7 +167,7 @@ __device__ __forceinline__ f32x4 shfl_xor4(f32x4 v, int m) {
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pair_c64_lane(f32x4 *__restrict__ psi, GateK<float> g) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
+    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
                    k1 = k0 + 64;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;  // a partner is in range iff its lane's unit is
     uint64_t j0 = 0, j1 = 0;
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64_lane(f32x4 *__restrict__ 
 // the target), so only the odd halves change.
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, GateK<float> g) {
-    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;
     uint64_t j0 = 0, j1 = 0;
     f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, 
 // complex128: one amplitude is already a 16-byte unit; one pair per thread.
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi, GateK<double> g) {
-    const uint64_t k = xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+    const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
     if (k >= g.items) return;
     const uint64_t ia = insert_zeros(k, g.fixed) | g.set, ib = ia ^ g.pmask;
     const double2 a = amp(ldnt(psi + ia)), b = amp(ldnt(psi + ib));
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi,
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_pair_c128_lane(f64x2 *__restrict__ psi, GateK<double> g) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
+    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
                    k1 = k0 + 64;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;
     uint64_t j0 = 0, j1 = 0;
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128_lane(f64x2 *__restrict__
 
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_diag_c128(f64x2 *__restrict__ psi, GateK<double> g) {
-    const uint64_t k0 = xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
+    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
     const bool h0 = k0 < g.items, h1 = k1 < g.items;
     uint64_t j0 = 0, j1 = 0;
     f64x2 u0 = {0, 0}, u1 = {0, 0};
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(kThreads) void k_diag_c128(f64x2 *__restrict__ psi,
 // amplitudes sit in different halves of their units, so this one walks
 // single amplitudes (8-byte accesses, several pairs per thread in flight).
 __global__ __launch_bounds__(kThreads) void k_swap_bit0(float2 *__restrict__ psi, GateK<float> g) {
-    const uint64_t first = (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
+    const uint64_t first = g.base + (uint64_t(blockIdx.x) * kItems) * kThreads + threadIdx.x;
     uint64_t i0[kItems];
     float2 a[kItems], b[kItems];
 #pragma unroll
@@ -266,7 +267,7 @@ __global__ __launch_bounds__(kThreads) void k_swap_bit0(float2 *__restrict__ psi
 template <typename R>
 __global__ __launch_bounds__(kThreads) void k_match(typename V2<R>::T *__restrict__ psi, GateK<R> g) {
     using T = typename V2<R>::T;
-    const uint64_t k = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    const uint64_t k = g.base + uint64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (k >= g.items) return;
     uint64_t l = insert_zeros(k, g.fixed);
     uint64_t i01 = l | g.tb, i10 = l | g.tb2, i11 = l | g.tb | g.tb2;
@@ -296,7 +297,12 @@ template <typename R> GateK<R> make_args(const Lowered &L, uint32_t n) {
     return g;
 }
 
-inline unsigned grid_for(uint64_t items, uint64_t per_block) { return (unsigned)((items + per_block - 1) / per_block); }
+inline uint64_t grid_for(uint64_t items, uint64_t per_block) { return (items + per_block - 1) / per_block; }
+
+// A grid holds at most 2^32 - 1 work-items; a gate on a larger state (2^32
+// unit pairs at 34 complex64 qubits) runs as several launches of at most 2^22
+// workgroups, each with `base` advanced past the items of the earlier ones.
+constexpr uint64_t kMaxGridBlocks = uint64_t(1) << 22;
 
 // Runs of consecutive workgroups per XCD (log2) for the pair and diagonal
 // kernels: 1-5% faster single gates at 30 qubits than the identity order.
@@ -321,17 +327,34 @@ template <typename R> bool lane_args(const GateK<R> &g, uint64_t tb, uint64_t ct
     return true;
 }
 
+// Launches `launch(grid, args)` over `blocks` workgroups of `per_block` items:
+// one launch, or pieces of kMaxGridBlocks workgroups with args.base advanced
+// (and the XCD order of each piece's own grid).
+template <typename R, typename F> void split_launch(uint64_t blocks, uint64_t per_block, const GateK<R> &g, F &&launch) {
+    for (uint64_t b0 = 0; b0 < blocks; b0 += kMaxGridBlocks) {
+        const uint64_t nb = std::min(kMaxGridBlocks, blocks - b0);
+        GateK<R> h = g;
+        h.base = b0 * per_block;
+        h.chunk = xcd_chunk_log2(nb, kChunkLog2);
+        launch(dim3((unsigned)nb), h);
+    }
+}
+
 Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     GateK<float> g = make_args<float>(L, s.n);
     const dim3 block(kThreads);
     if (L.kind == LK_MATCH) {
-        hipLaunchKernelGGL((k_match<float>), dim3(grid_for(g.items, kThreads)), block, 0, st, (float2 *)s.d, g);
+        split_launch(grid_for(g.items, kThreads), kThreads, g, [&](dim3 gr, const GateK<float> &h) {
+            hipLaunchKernelGGL((k_match<float>), gr, block, 0, st, (float2 *)s.d, h);
+        });
         QI_HIP_TRY(hipGetLastError());
         return Status{};
     }
     if (L.kind == LK_SWAP && (L.t0 == 0 || L.t1 == 0)) {
-        hipLaunchKernelGGL(k_swap_bit0, dim3(grid_for(g.items, uint64_t(kThreads) * kItems)), block, 0, st,
-                           (float2 *)s.d, g);
+        const uint64_t per = uint64_t(kThreads) * kItems;
+        split_launch(grid_for(g.items, per), per, g, [&](dim3 gr, const GateK<float> &h) {
+            hipLaunchKernelGGL(k_swap_bit0, gr, block, 0, st, (float2 *)s.d, h);
+        });
         QI_HIP_TRY(hipGetLastError());
         return Status{};
     }
@@ -344,71 +367,82 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     u.pmask = g.pmask >> 1;
     u.items = uint64_t(1) << (s.n - 1 - __builtin_popcountll(u.fixed));
     f32x4 *psi = reinterpret_cast<f32x4 *>(s.d);
-    const dim3 g1(grid_for(u.items, kThreads)), g2(grid_for(u.items, 2 * kThreads));
-    GateK<float> u2 = u;  // the same gate for the two-units-per-thread grid
-    u.chunk = xcd_chunk_log2(g1.x, kChunkLog2);
-    u2.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
+    const uint64_t b1 = grid_for(u.items, kThreads), b2 = grid_for(u.items, 2 * kThreads);
     const int kind = L.kind == LK_SWAP ? LK_X : L.kind;  // SWAP: exchange base|tb with base|tb2
     GateK<float> ul;
     if (L.kind != LK_SWAP && mode != 2 && !diag && lane_args(u, u.pmask, u.fixed & ~u.pmask, kind, ul)) {
-        const dim3 gl(grid_for(ul.items, 2 * kThreads));
-        switch (kind * 2 + mode) {
+        bool ok = true;
+        split_launch(grid_for(ul.items, 2 * kThreads), 2 * kThreads, ul, [&](dim3 gl, const GateK<float> &h) {
+            switch (kind * 2 + mode) {
 #define QI_LANE(K) \
-        case K * 2 + 0: hipLaunchKernelGGL((k_pair_c64_lane<K, 0>), gl, block, 0, st, psi, ul); break; \
-        case K * 2 + 1: hipLaunchKernelGGL((k_pair_c64_lane<K, 1>), gl, block, 0, st, psi, ul); break;
-        QI_LANE_KINDS(QI_LANE)
+            case K * 2 + 0: hipLaunchKernelGGL((k_pair_c64_lane<K, 0>), gl, block, 0, st, psi, h); break; \
+            case K * 2 + 1: hipLaunchKernelGGL((k_pair_c64_lane<K, 1>), gl, block, 0, st, psi, h); break;
+            QI_LANE_KINDS(QI_LANE)
 #undef QI_LANE
-        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
-        }
+            default: ok = false;
+            }
+        });
+        if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
         QI_HIP_TRY(hipGetLastError());
         return Status{};
     }
-    switch (kind * 4 + mode) {
+    bool ok = true;
+    const bool two = mode == 2 || diag;  // two units per thread
+    split_launch(two ? b2 : b1, two ? 2 * kThreads : kThreads, u, [&](dim3 gr, const GateK<float> &h) {
+        switch (kind * 4 + mode) {
 #define QI_PAIR(K) \
-    case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), g1, block, 0, st, psi, u); break; \
-    case K * 4 + 1: hipLaunchKernelGGL((k_pair_c64<K, 1>), g1, block, 0, st, psi, u); break; \
-    case K * 4 + 2: hipLaunchKernelGGL((k_pair_c64<K, 2>), g2, block, 0, st, psi, u2); break;
-    QI_PAIR_KINDS(QI_PAIR)
+        case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), gr, block, 0, st, psi, h); break; \
+        case K * 4 + 1: hipLaunchKernelGGL((k_pair_c64<K, 1>), gr, block, 0, st, psi, h); break; \
+        case K * 4 + 2: hipLaunchKernelGGL((k_pair_c64<K, 2>), gr, block, 0, st, psi, h); break;
+        QI_PAIR_KINDS(QI_PAIR)
 #undef QI_PAIR
-    case LK_Z * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_Z, 0>), g2, block, 0, st, psi, u2); break;
-    case LK_Z * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_Z, 1>), g2, block, 0, st, psi, u2); break;
-    case LK_PHASE * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 0>), g2, block, 0, st, psi, u2); break;
-    case LK_PHASE * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 1>), g2, block, 0, st, psi, u2); break;
-    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
-    }
+        case LK_Z * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_Z, 0>), gr, block, 0, st, psi, h); break;
+        case LK_Z * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_Z, 1>), gr, block, 0, st, psi, h); break;
+        case LK_PHASE * 4 + 0: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 0>), gr, block, 0, st, psi, h); break;
+        case LK_PHASE * 4 + 1: hipLaunchKernelGGL((k_diag_c64<LK_PHASE, 1>), gr, block, 0, st, psi, h); break;
+        default: ok = false;
+        }
+    });
+    if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }
 
 Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
     GateK<double> g = make_args<double>(L, s.n);
-    const dim3 block(kThreads), g1(grid_for(g.items, kThreads)), g2(grid_for(g.items, 2 * kThreads));
+    const dim3 block(kThreads);
+    const uint64_t b1 = grid_for(g.items, kThreads), b2 = grid_for(g.items, 2 * kThreads);
     f64x2 *psi = reinterpret_cast<f64x2 *>(s.d);
-    GateK<double> g2k = g;  // the same gate for the two-amplitudes-per-thread grid
-    g2k.chunk = xcd_chunk_log2(g2.x, kChunkLog2);
     GateK<double> gl;
+    bool ok = true;
     if (L.kind != LK_SWAP && L.kind != LK_MATCH && L.kind != LK_Z && L.kind != LK_PHASE &&
         lane_args(g, g.pmask, g.fixed & ~g.pmask, L.kind, gl)) {
-        const dim3 gg(grid_for(gl.items, 2 * kThreads));
-        switch (L.kind) {
-#define QI_LANE(K) case K: hipLaunchKernelGGL((k_pair_c128_lane<K>), gg, block, 0, st, psi, gl); break;
-        QI_LANE_KINDS(QI_LANE)
+        split_launch(grid_for(gl.items, 2 * kThreads), 2 * kThreads, gl, [&](dim3 gg, const GateK<double> &h) {
+            switch (L.kind) {
+#define QI_LANE(K) case K: hipLaunchKernelGGL((k_pair_c128_lane<K>), gg, block, 0, st, psi, h); break;
+            QI_LANE_KINDS(QI_LANE)
 #undef QI_LANE
-        default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
-        }
+            default: ok = false;
+            }
+        });
+        if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
         QI_HIP_TRY(hipGetLastError());
         return Status{};
     }
-    switch (L.kind == LK_SWAP ? LK_X : L.kind) {
-#define QI_PAIR(K) case K: { GateK<double> g1k = g; g1k.chunk = xcd_chunk_log2(g1.x, kChunkLog2); \
-        hipLaunchKernelGGL((k_pair_c128<K>), g1, block, 0, st, psi, g1k); } break;
-    QI_PAIR_KINDS(QI_PAIR)
+    const int kind = L.kind == LK_SWAP ? LK_X : L.kind;
+    const bool two = kind == LK_Z || kind == LK_PHASE;  // two amplitudes per thread
+    split_launch(two ? b2 : b1, two ? 2 * kThreads : kThreads, g, [&](dim3 gr, const GateK<double> &h) {
+        switch (kind) {
+#define QI_PAIR(K) case K: hipLaunchKernelGGL((k_pair_c128<K>), gr, block, 0, st, psi, h); break;
+        QI_PAIR_KINDS(QI_PAIR)
 #undef QI_PAIR
-    case LK_Z: hipLaunchKernelGGL((k_diag_c128<LK_Z>), g2, block, 0, st, psi, g2k); break;
-    case LK_PHASE: hipLaunchKernelGGL((k_diag_c128<LK_PHASE>), g2, block, 0, st, psi, g2k); break;
-    case LK_MATCH: hipLaunchKernelGGL((k_match<double>), g1, block, 0, st, (double2 *)s.d, g); break;
-    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
-    }
+        case LK_Z: hipLaunchKernelGGL((k_diag_c128<LK_Z>), gr, block, 0, st, psi, h); break;
+        case LK_PHASE: hipLaunchKernelGGL((k_diag_c128<LK_PHASE>), gr, block, 0, st, psi, h); break;
+        case LK_MATCH: hipLaunchKernelGGL((k_match<double>), gr, block, 0, st, (double2 *)s.d, h); break;
+        default: ok = false;
+        }
+    });
+    if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }
@@ -459,10 +493,20 @@ __global__ __launch_bounds__(256) void k_axpy(typename V2<R>::T *y, const typena
         y[i] = cadd(y[i], cmul(x[i], mk<T>(re, im)));
 }
 
+// State Add / Sub (state.rs:2773-2800, 2836-2862): out = a + b or a - b per
+// component, one rounding each, so the result is the reference's bit for bit.
+template <typename R, bool SUB>
+__global__ __launch_bounds__(256) void k_addsub(typename V2<R>::T *out, const typename V2<R>::T *a,
+                                                const typename V2<R>::T *b, uint64_t dim) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        out[i] = SUB ? csub(a[i], b[i]) : cadd(a[i], b[i]);
+}
+
 // Whole-state copy in 16-byte units, streamed like the gate kernels.
 __global__ __launch_bounds__(kThreads) void k_copy16(f32x4 *__restrict__ dst, const f32x4 *__restrict__ src,
-                                                     uint64_t units) {
-    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+                                                     uint64_t units, uint64_t base) {
+    const uint64_t i = base + uint64_t(blockIdx.x) * kThreads + threadIdx.x;
     if (i < units) stnt(ldnt(src + i), dst + i);
 }
 
@@ -573,10 +617,26 @@ Status launch_axpy(State &y, double re, double im, const State &x, hipStream_t s
     return Status{};
 }
 
+Status launch_addsub(State &out, const State &a, const State &b, bool sub, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << out.n;
+    int blocks = pick_blocks(dim, 256);
+    if (out.prec == QI_C128) {
+        auto k = sub ? k_addsub<double, true> : k_addsub<double, false>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, (double2 *)out.d, (const double2 *)a.d, (const double2 *)b.d, dim);
+    } else {
+        auto k = sub ? k_addsub<float, true> : k_addsub<float, false>;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, (float2 *)out.d, (const float2 *)a.d, (const float2 *)b.d, dim);
+    }
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
 Status launch_copy(State &dst, const State &src, hipStream_t st) {
     const uint64_t units = src.bytes() / 16;  // a state is at least two complex64 amplitudes
-    hipLaunchKernelGGL(k_copy16, dim3(grid_for(units, kThreads)), dim3(kThreads), 0, st,
-                       reinterpret_cast<f32x4 *>(dst.d), reinterpret_cast<const f32x4 *>(src.d), units);
+    const uint64_t blocks = grid_for(units, kThreads);
+    for (uint64_t b0 = 0; b0 < blocks; b0 += kMaxGridBlocks)
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)std::min(kMaxGridBlocks, blocks - b0)), dim3(kThreads), 0, st,
+                           reinterpret_cast<f32x4 *>(dst.d), reinterpret_cast<const f32x4 *>(src.d), units, b0 * kThreads);
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }

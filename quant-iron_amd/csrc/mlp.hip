@@ -1,14 +1,16 @@
 // mlp.hip — the energy surrogate: an MLP (n_in -> H -> H -> H -> 1, GELU)
 // trained and evaluated with hand-written kernels. Weights are BF16 for the
-// matrix cores (v_mfma_f32_16x16x16bf16_1k, FP32 accumulation) with FP32
+// matrix cores (gfx950 v_mfma_f32_16x16x32_bf16, FP32 accumulation) with FP32
 // master copies for Adam; activations are stored BF16 (GEMM inputs) and FP32
 // (pre-activations, needed by the GELU derivative).
 //
-// GEMM tile: 64 x 64 per workgroup, four waves of 32 x 32, K in steps of 16
-// staged through LDS with K contiguous per row, so each lane's 4 consecutive
-// K values are one 8-byte LDS read in the operand layout of the instruction
-// (A: row = lane % 16, k = 4 (lane / 16) + v; B: col = lane % 16, same k;
-// D: row = 4 (lane / 16) + r, col = lane % 16).
+// GEMM: a 128 x 128 output tile per workgroup, four waves of 64 x 64 (4 x 4
+// MFMA tiles of 16 x 16 each), K in stages of 64 held in LDS with K
+// contiguous per row (padded rows), double-buffered: the next stage's global
+// loads are in registers while the current stage feeds the matrix cores. The
+// instruction's operands: A row = lane % 16, k = 8 (lane / 16) + v (8 BF16 per
+// lane, one 16-byte LDS read); B col = lane % 16, same k; D row = 4 (lane / 16)
+// + r, col = lane % 16.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -22,7 +24,7 @@ namespace qi {
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef short shortx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef uint16_t bf16;
 
 __device__ __host__ __forceinline__ bf16 f2bf(float x) {
@@ -45,7 +47,58 @@ __device__ __forceinline__ float gelu_grad(float z) {
 
 enum Epi { EPI_GELU = 0, EPI_STORE = 1, EPI_GELU_BWD = 2 };
 
-constexpr int TM = 64, TN = 64, TK = 16, LDK = TK + 4;  // LDS row stride (bf16), padded
+constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;  // LDS row: 72 BF16 = 144 bytes (9 x 16)
+constexpr int kChunks = TM * TK / 8 / 256;                // 16-byte chunks per thread per operand stage
+
+// One operand tile (128 rows of the output dimension x 64 of K) as 16-byte
+// chunks in registers. KC (K contiguous in memory: X row-major M x K, or W^T):
+// chunk c = row c / 8, k = 8 (c % 8). Otherwise (K strided: the operand is
+// stored K x rows): chunk c = k c / 16, rows 8 (c % 16) .. + 7, transposed
+// into LDS on store. Out-of-range rows and k read as zero (K % 8 == 0).
+template <bool KC>
+__device__ __forceinline__ void load_tile(const bf16 *__restrict__ P, int ld, int rows, int K, int r0, int k0, uint4 (&reg)[kChunks]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kChunks; i++) {
+        const int c = tid + 256 * i;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (KC) {
+            const int r = r0 + c / 8, k = k0 + 8 * (c % 8);
+            if (r < rows && k < K) v = *reinterpret_cast<const uint4 *>(P + size_t(r) * ld + k);
+        } else {
+            const int k = k0 + c / 16, r = r0 + 8 * (c % 16);
+            if (k < K) {
+                if (r + 7 < rows) {
+                    v = *reinterpret_cast<const uint4 *>(P + size_t(k) * ld + r);
+                } else {
+                    bf16 t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    for (int q = 0; q < 8; q++)
+                        if (r + q < rows) t[q] = P[size_t(k) * ld + r + q];
+                    memcpy(&v, t, 16);
+                }
+            }
+        }
+        reg[i] = v;
+    }
+}
+
+template <bool KC>
+__device__ __forceinline__ void store_tile(bf16 (*S)[LDK], const uint4 (&reg)[kChunks]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < kChunks; i++) {
+        const int c = tid + 256 * i;
+        if (KC) {
+            *reinterpret_cast<uint4 *>(&S[c / 8][8 * (c % 8)]) = reg[i];
+        } else {
+            bf16 t[8];
+            memcpy(t, &reg[i], 16);
+            const int k = c / 16, r = 8 * (c % 16);
+#pragma unroll
+            for (int q = 0; q < 8; q++) S[r + q][k] = t[q];
+        }
+    }
+}
 
 // C[M x N] = op(A)[M x K] op(B)[K x N]; op(A) = A (row-major M x K, lda) or
 // A^T (A row-major K x M); op(B) = B (row-major K x N) or B^T (B row-major N x K).
@@ -53,83 +106,57 @@ template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B, int ldb,
                                               int M, int N, int K, float *__restrict__ Cf, bf16 *__restrict__ Cb,
                                               int ldc, const float *__restrict__ bias, const float *__restrict__ Zin) {
-    __shared__ bf16 As[TM][LDK];
-    __shared__ bf16 Bs[TN][LDK];
+    __shared__ __attribute__((aligned(16))) bf16 As[2][TM][LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][TN][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int wr = wave >> 1, wc = wave & 1;
-    floatx4 acc[2][2];
+    floatx4 acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0, 0, 0, 0};
-    for (int k0 = 0; k0 < K; k0 += TK) {
-        // ---- stage op(A) tile (64 x 16) as As[m][k]
-        if (!TA) {
-            const int m = tid >> 2, kq = (tid & 3) * 4;
-            const int gm = m0 + m;
-            uint64_t v = 0;
-            if (gm < M) v = *reinterpret_cast<const uint64_t *>(A + size_t(gm) * lda + k0 + kq);
-            *reinterpret_cast<uint64_t *>(&As[m][kq]) = v;
-        } else {
-            const int k = tid >> 4, mq = (tid & 15) * 4;
-            const int gm = m0 + mq;
-            bf16 v[4] = {0, 0, 0, 0};
-            if (gm + 3 < M) {
-                uint64_t w = *reinterpret_cast<const uint64_t *>(A + size_t(k0 + k) * lda + gm);
-                memcpy(v, &w, 8);
-            } else {
-                for (int q = 0; q < 4; q++)
-                    if (gm + q < M) v[q] = A[size_t(k0 + k) * lda + gm + q];
-            }
-            for (int q = 0; q < 4; q++) As[mq + q][k] = v[q];
-        }
-        // ---- stage op(B) tile (16 x 64) as Bs[n][k]
-        if (TB) {
-            const int n = tid >> 2, kq = (tid & 3) * 4;
-            const int gn = n0 + n;
-            uint64_t v = 0;
-            if (gn < N) v = *reinterpret_cast<const uint64_t *>(B + size_t(gn) * ldb + k0 + kq);
-            *reinterpret_cast<uint64_t *>(&Bs[n][kq]) = v;
-        } else {
-            const int k = tid >> 4, nq = (tid & 15) * 4;
-            const int gn = n0 + nq;
-            bf16 v[4] = {0, 0, 0, 0};
-            if (gn + 3 < N) {
-                uint64_t w = *reinterpret_cast<const uint64_t *>(B + size_t(k0 + k) * ldb + gn);
-                memcpy(v, &w, 8);
-            } else {
-                for (int q = 0; q < 4; q++)
-                    if (gn + q < N) v[q] = B[size_t(k0 + k) * ldb + gn + q];
-            }
-            for (int q = 0; q < 4; q++) Bs[nq + q][k] = v[q];
-        }
-        __syncthreads();
-        shortx4 a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            uint64_t w = *reinterpret_cast<const uint64_t *>(&As[wr * 32 + i * 16 + (lane & 15)][4 * (lane >> 4)]);
-            memcpy(&a[i], &w, 8);
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0, 0, 0, 0};
+    uint4 ra[kChunks], rb[kChunks];
+    load_tile<!TA>(A, lda, M, K, m0, 0, ra);
+    load_tile<TB>(B, ldb, N, K, n0, 0, rb);
+    store_tile<!TA>(As[0], ra);
+    store_tile<TB>(Bs[0], rb);
+    __syncthreads();
+    const int nst = (K + TK - 1) / TK;
+    for (int s = 0; s < nst; s++) {
+        const int cur = s & 1;
+        if (s + 1 < nst) {  // next stage in flight while this one computes
+            load_tile<!TA>(A, lda, M, K, m0, (s + 1) * TK, ra);
+            load_tile<TB>(B, ldb, N, K, n0, (s + 1) * TK, rb);
         }
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            uint64_t w = *reinterpret_cast<const uint64_t *>(&Bs[wc * 32 + j * 16 + (lane & 15)][4 * (lane >> 4)]);
-            memcpy(&b[j], &w, 8);
+        for (int kk = 0; kk < TK; kk += 32) {
+            bf16x8 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][wr * 64 + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wc * 64 + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (s + 1 < nst) {
+            store_tile<!TA>(As[cur ^ 1], ra);
+            store_tile<TB>(Bs[cur ^ 1], rb);
+        }
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++)
+        for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wr * 32 + i * 16 + 4 * (lane >> 4) + r;
-                const int col = n0 + wc * 32 + j * 16 + (lane & 15);
+                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
+                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
                 if (row >= M || col >= N) continue;
                 const size_t o = size_t(row) * ldc + col;
                 float v = acc[i][j][r];
@@ -239,6 +266,18 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, bf
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) y[i] = f2bf(x[i]);
 }
 
+// y = BF16(x^T) for a rows x cols FP32 matrix (y is cols x rows): the weights
+// in the K-contiguous layout the forward GEMMs stream.
+__global__ __launch_bounds__(256) void k_transpose_bf16(const float *__restrict__ x, int rows, int cols, bf16 *__restrict__ y) {
+    __shared__ float t[32][33];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8)
+        if (r0 + k < rows && c0 + tx < cols) t[k][tx] = x[size_t(r0 + k) * cols + c0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8)
+        if (c0 + k < cols && r0 + tx < rows) y[size_t(c0 + k) * rows + r0 + tx] = f2bf(t[tx][k]);
+}
+
 // gather rows idx[0..rows) of a BF16 matrix (cols wide) into a contiguous block
 __global__ __launch_bounds__(256) void k_gather_rows(const bf16 *__restrict__ X, const uint32_t *__restrict__ idx, int rows,
                                                      int cols, bf16 *__restrict__ out, const float *__restrict__ t,
@@ -298,6 +337,33 @@ __global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t
     }
 }
 
+// Candidates of an optimisation round around a centre theta0 (a local
+// proposal): theta_j = theta0_j + sigma z_j, z_j a standard normal by the
+// Box-Muller transform of draws draw0 + 2 (c nang + j) + 1 and + 2 of the
+// stream `seed` (z = sqrt(-2 ln(1 - u1)) cos(2 pi u2)); the couplings are the
+// fixed J of the round. Same row layout as k_ansatz_features.
+__global__ __launch_bounds__(256) void k_ansatz_features_local(uint64_t seed, uint64_t draw0, uint64_t c0, int rows, int nang,
+                                                               int k_in, const double *__restrict__ theta0, double sigma,
+                                                               double j0, double j1, double j2, bf16 *__restrict__ X) {
+    const size_t total = size_t(rows) * k_in;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
+        const uint64_t r = i / k_in, col = i % k_in, c = c0 + r;
+        float f = 0.f;
+        if (col < uint64_t(2 * nang)) {
+            const uint64_t j = col < uint64_t(nang) ? col : col - nang;
+            const uint64_t d = draw0 + 2 * (c * nang + j);
+            const double u1 = sm64_uniform(seed, d + 1), u2 = sm64_uniform(seed, d + 2);
+            const double z = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+            const double th = theta0[j] + sigma * z;
+            f = (float)(col < uint64_t(nang) ? cos(th) : sin(th));
+        } else if (col < uint64_t(2 * nang + 3)) {
+            const uint64_t q = col - 2 * nang;
+            f = (float)((q == 0 ? j0 : q == 1 ? j1 : j2) - 1.0);
+        }
+        X[i] = f2bf(f);
+    }
+}
+
 // ------------------------------------------------------------------ host --
 struct Mlp {
     Ctx *ctx = nullptr;
@@ -306,6 +372,7 @@ struct Mlp {
     size_t off[8] = {0}, len[8] = {0}, nparam = 0;
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;  // FP32 masters, Adam moments, grads
     bf16 *wb = nullptr;                                            // BF16 copies (W1, W2, W3)
+    bf16 *wt = nullptr;                                            // BF16 W1^T, W2^T, W3^T (same offsets)
     int step = 0;
     // activations for a batch of up to cap rows
     size_t cap = 0;
@@ -315,14 +382,16 @@ struct Mlp {
     double *lacc = nullptr;                                        // epoch loss accumulator
     float *yall = nullptr;                                         // predictions of a candidate screen
     size_t ycap = 0;
+    double *theta0 = nullptr;                                      // centre of a local candidate round
+    size_t th0cap = 0;
     // resident training set
     bf16 *dataX = nullptr;
     float *dataT = nullptr;
     uint32_t *perm = nullptr;
     size_t ndata = 0;
     ~Mlp() {
-        void *ps[] = {w, m, v, g, wb, X, H[0], H[1], H[2], dZb[0], dZb[1], dZb[2], Z[0], Z[1], Z[2], dZf[0], dZf[1], dZf[2],
-                      y, t, dy, part, lossp, dataX, dataT, perm, lacc, yall};
+        void *ps[] = {w, m, v, g, wb, wt, X, H[0], H[1], H[2], dZb[0], dZb[1], dZb[2], Z[0], Z[1], Z[2], dZf[0], dZf[1], dZf[2],
+                      y, t, dy, part, lossp, dataX, dataT, perm, lacc, yall, theta0};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -368,13 +437,25 @@ static void colsum(Mlp &N, hipStream_t st, const float *X, int rows, int cols, f
     hipLaunchKernelGGL(k_colsum_fold, dim3((cols + 255) / 256), dim3(256), 0, st, (const float *)N.part, chunks, cols, out);
 }
 
+// The BF16 copies of the weights (W and W^T) from the FP32 masters.
+static void refresh_bf16(Mlp &N, hipStream_t st) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, st, (const float *)N.w, N.wb, N.nparam);
+    const int rows[3] = {N.k_in, N.hid, N.hid};
+    for (int l = 0; l < 3; l++) {
+        const size_t o = N.off[2 * l];
+        hipLaunchKernelGGL(k_transpose_bf16, dim3((N.hid + 31) / 32, (rows[l] + 31) / 32), dim3(256), 0, st,
+                           (const float *)(N.w + o), rows[l], N.hid, N.wt + o);
+    }
+}
+
 // Forward over rows already staged in N.X; keeps Z (pre-activations) when training.
+// C = X W with B^T = W^T stored K-contiguous (the fast operand layout).
 static void forward(Mlp &N, hipStream_t st, int rows, bool keepZ) {
-    const bf16 *W1 = N.wb + N.off[0], *W2 = N.wb + N.off[2], *W3 = N.wb + N.off[4];
+    const bf16 *W1t = N.wt + N.off[0], *W2t = N.wt + N.off[2], *W3t = N.wt + N.off[4];
     const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
-    gemm<false, false, EPI_GELU>(st, N.X, N.k_in, W1, N.hid, rows, N.hid, N.k_in, keepZ ? N.Z[0] : nullptr, N.H[0], N.hid, b1);
-    gemm<false, false, EPI_GELU>(st, N.H[0], N.hid, W2, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[1] : nullptr, N.H[1], N.hid, b2);
-    gemm<false, false, EPI_GELU>(st, N.H[1], N.hid, W3, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[2] : nullptr, N.H[2], N.hid, b3);
+    gemm<false, true, EPI_GELU>(st, N.X, N.k_in, W1t, N.k_in, rows, N.hid, N.k_in, keepZ ? N.Z[0] : nullptr, N.H[0], N.hid, b1);
+    gemm<false, true, EPI_GELU>(st, N.H[0], N.hid, W2t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[1] : nullptr, N.H[1], N.hid, b2);
+    gemm<false, true, EPI_GELU>(st, N.H[1], N.hid, W3t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[2] : nullptr, N.H[2], N.hid, b3);
 }
 
 // Gradients of the mean squared error over `rows` staged rows (targets in N.t) into N.g.
@@ -465,12 +546,12 @@ qi_status qi_mlp_create(qi_ctx_t ctx, uint32_t n_in, uint32_t hidden, uint64_t s
     hipStream_t st = N.ctx->stream;
     hipError_t e;
     if ((e = hipMalloc(&N.w, N.nparam * 4)) || (e = hipMalloc(&N.m, N.nparam * 4)) || (e = hipMalloc(&N.v, N.nparam * 4)) ||
-        (e = hipMalloc(&N.g, N.nparam * 4)) || (e = hipMalloc(&N.wb, N.nparam * 2)))
-        QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+        (e = hipMalloc(&N.g, N.nparam * 4)) || (e = hipMalloc(&N.wb, N.nparam * 2)) || (e = hipMalloc(&N.wt, N.nparam * 2)))
+        QM_RET(oom_err(e));
     if ((e = hipMemcpy(N.w, w.data(), N.nparam * 4, hipMemcpyHostToDevice)) || (e = hipMemset(N.m, 0, N.nparam * 4)) ||
         (e = hipMemset(N.v, 0, N.nparam * 4)) || (e = hipMemset(N.g, 0, N.nparam * 4)))
         QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
-    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, st, (const float *)N.w, N.wb, N.nparam);
+    refresh_bf16(N, st);
     if ((e = hipStreamSynchronize(st))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
     *out = h.release();
     QM_RET(Status{});
@@ -508,7 +589,7 @@ qi_status qi_mlp_set_params(qi_mlp_t m, const float *in, qi_error *err) {
     std::lock_guard<std::mutex> lk(N.ctx->mu);
     hipError_t e = hipMemcpy(N.w, in, N.nparam * 4, hipMemcpyHostToDevice);
     if (e) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
-    hipLaunchKernelGGL(k_to_bf16, dim3(1024), dim3(256), 0, N.ctx->stream, (const float *)N.w, N.wb, N.nparam);
+    refresh_bf16(N, N.ctx->stream);
     if ((e = hipStreamSynchronize(N.ctx->stream))) QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
     QM_RET(Status{});
 }
@@ -542,7 +623,7 @@ qi_status qi_mlp_set_data(qi_mlp_t m, const float *x, const float *t, size_t row
     hipError_t e;
     if ((e = hipMalloc(&N.dataX, rows * N.k_in * sizeof(bf16))) || (e = hipMalloc(&N.dataT, rows * 4)) ||
         (e = hipMalloc(&N.perm, rows * 4)))
-        QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+        QM_RET(oom_err(e));
     const size_t chunk = 65536;
     std::vector<bf16> hx(chunk * N.k_in);
     for (size_t r0 = 0; r0 < rows; r0 += chunk) {
@@ -588,6 +669,10 @@ qi_status qi_mlp_train_epoch(qi_mlp_t m, uint32_t batch, float lr, uint64_t shuf
         const float c1 = 1.f - std::pow(b1, (float)N.step), c2 = 1.f - std::pow(b2, (float)N.step);
         hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, st, N.w, N.m, N.v, (const float *)N.g, N.wb, N.nparam, lr, b1, b2,
                            1e-8f, c1, c2);
+        const int rows[3] = {N.k_in, N.hid, N.hid};
+        for (int l = 0; l < 3; l++)
+            hipLaunchKernelGGL(k_transpose_bf16, dim3((N.hid + 31) / 32, (rows[l] + 31) / 32), dim3(256), 0, st,
+                               (const float *)(N.w + N.off[2 * l]), rows[l], N.hid, N.wt + N.off[2 * l]);
     }
     (void)hipEventRecord(N.ctx->ev1, st);
     double lsum = 0.0;
@@ -648,7 +733,7 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
         if (N.yall) (void)hipFree(N.yall);
         N.yall = nullptr;
         N.ycap = 0;
-        if ((e = hipMalloc(&N.yall, count * sizeof(float)))) QM_RET(make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e)));
+        if ((e = hipMalloc(&N.yall, count * sizeof(float)))) QM_RET(oom_err(e));
         N.ycap = count;
     }
     (void)hipEventRecord(N.ctx->ev0, st);
@@ -657,6 +742,55 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
         const size_t cells = size_t(nr) * N.k_in;
         hipLaunchKernelGGL(k_ansatz_features, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 8192)), dim3(256), 0, st, seed,
                            draw0, count, c0, nr, (int)nang, N.k_in, N.X);
+        forward(N, st, nr, false);
+        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
+                           (const float *)(N.w + N.off[7]), nr, N.hid, N.yall + c0, (const float *)nullptr, 0.f,
+                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+    }
+    (void)hipEventRecord(N.ctx->ev1, st);
+    if ((e = hipMemcpyAsync(y, N.yall, count * sizeof(float), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
+        QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, N.ctx->ev0, N.ctx->ev1);
+    N.ctx->last_exec_ms = ms;
+    QM_RET(Status{});
+}
+
+// Predictions for `count` candidates of a local optimisation round around
+// theta0 (k_ansatz_features_local), couplings J[3] fixed; the host regenerates
+// the kept candidates' angles from their indices with the same formula.
+qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang,
+                                      const double *theta0, double sigma, const double *J, float *y, qi_error *err) {
+    if (!m || !y || !theta0 || !J || count == 0) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    Mlp &N = m->m;
+    if ((uint64_t)N.n_in != 2ull * nang + 3) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "network inputs != 2 nang + 3"));
+    std::lock_guard<std::mutex> lk(N.ctx->mu);
+    hipStream_t st = N.ctx->stream;
+    const size_t chunk = 65536;
+    QM_TRY(mlp_alloc(N, std::min<size_t>(chunk, count)));
+    hipError_t e;
+    if (count > N.ycap) {
+        if (N.yall) (void)hipFree(N.yall);
+        N.yall = nullptr;
+        N.ycap = 0;
+        if ((e = hipMalloc(&N.yall, count * sizeof(float)))) QM_RET(oom_err(e));
+        N.ycap = count;
+    }
+    if (nang > N.th0cap) {
+        if (N.theta0) (void)hipFree(N.theta0);
+        N.theta0 = nullptr;
+        N.th0cap = 0;
+        if ((e = hipMalloc(&N.theta0, nang * sizeof(double)))) QM_RET(oom_err(e));
+        N.th0cap = nang;
+    }
+    if ((e = hipMemcpyAsync(N.theta0, theta0, nang * sizeof(double), hipMemcpyHostToDevice, st)))
+        QM_RET(make_err(QI_ERR_DEVICE, 0, 0, 0, 0, hipGetErrorString(e)));
+    (void)hipEventRecord(N.ctx->ev0, st);
+    for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
+        const int nr = (int)std::min<uint64_t>(chunk, count - c0);
+        const size_t cells = size_t(nr) * N.k_in;
+        hipLaunchKernelGGL(k_ansatz_features_local, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 8192)), dim3(256), 0, st,
+                           seed, draw0, c0, nr, (int)nang, N.k_in, (const double *)N.theta0, sigma, J[0], J[1], J[2], N.X);
         forward(N, st, nr, false);
         hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
                            (const float *)(N.w + N.off[7]), nr, N.hid, N.yall + c0, (const float *)nullptr, 0.f,

@@ -155,6 +155,33 @@ __global__ __launch_bounds__(256) void k_expect_group(const T *__restrict__ psi,
     }
 }
 
+// A run of diagonal Pauli terms (x = 0) applied in one pass: amplitude i is
+// multiplied by T_b[bits of i on the qubits of table b] for each table b, a
+// product over the run's terms of cosh(a_t) + (-1)^{popcount(i & z_t)} sinh(a_t)
+// computed on the host in FP64 (apply_exp_factor of a diagonal string,
+// pauli_string.rs:237-262). Tables live in LDS for the whole launch.
+template <typename R>
+__global__ __launch_bounds__(256) void k_diag_tables(typename V2<R>::T *psi, uint64_t dim, DiagPass p,
+                                                     const double2 *__restrict__ tables) {
+    using T = typename V2<R>::T;
+    extern __shared__ double2 lut[];  // p.entries of them
+    for (int e = threadIdx.x; e < p.entries; e += blockDim.x) lut[e] = tables[e];
+    __syncthreads();
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += uint64_t(gridDim.x) * blockDim.x) {
+        const T a = psi[i];
+        double re = a.x, im = a.y;
+        for (int b = 0; b < p.ntab; b++) {
+            uint32_t idx = 0;
+            for (int k = 0; k < p.tab[b].bits; k++) idx |= uint32_t((i >> p.tab[b].pos[k]) & 1u) << k;
+            const double2 f = lut[p.tab[b].off + idx];
+            const double nr = re * f.x - im * f.y, ni = re * f.y + im * f.x;
+            re = nr; im = ni;
+        }
+        T r; r.x = R(re); r.y = R(im);
+        psi[i] = r;
+    }
+}
+
 TermK make_term(const qi_pauli_term &p) {
     TermK t{};
     t.x = p.x_mask; t.z = p.z_mask;
@@ -191,6 +218,20 @@ Status launch_pauli_exp(State &s, const qi_pauli_term &p, double ch_re, double c
         hipLaunchKernelGGL(k_pauli_exp<double>, dim3(blocks), dim3(256), 0, st, (double2 *)s.d, items, t);
     else
         hipLaunchKernelGGL(k_pauli_exp<float>, dim3(blocks), dim3(256), 0, st, (float2 *)s.d, items, t);
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_diag_tables(State &s, const DiagPass &p, const double2 *dtables, hipStream_t st) {
+    uint64_t dim = uint64_t(1) << s.n;
+    int blocks = pick_blocks(dim, 256 * 4);
+    const size_t lds = size_t(p.entries) * sizeof(double2);
+    if (p.entries < 1 || p.entries > (kDiagTabMax << kDiagTabBits) || p.ntab < 1 || p.ntab > kDiagTabMax)
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "diagonal table pass");
+    if (s.prec == QI_C128)
+        hipLaunchKernelGGL(k_diag_tables<double>, dim3(blocks), dim3(256), lds, st, (double2 *)s.d, dim, p, dtables);
+    else
+        hipLaunchKernelGGL(k_diag_tables<float>, dim3(blocks), dim3(256), lds, st, (float2 *)s.d, dim, p, dtables);
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }

@@ -27,6 +27,12 @@ struct Status {
 Status make_err(int code, uint64_t a = 0, uint64_t b = 0, double re = 0, double im = 0,
                 const char *detail = nullptr);
 void fill_err(qi_error *err, const Status &st);
+// A failed device allocation: HIP keeps the error as its last error, which the
+// next launch check would report, so it is read (cleared) here.
+inline Status oom_err(hipError_t e) {
+    (void)hipGetLastError();
+    return make_err(QI_ERR_OUT_OF_MEMORY, 0, 0, 0, 0, hipGetErrorString(e));
+}
 
 #define QI_HIP_TRY(expr)                                                          \
     do {                                                                          \
@@ -55,12 +61,16 @@ struct Ctx {
     DevBuf probs;    // measurement histograms / CDF
     DevBuf aux;      // misc (sample outcomes, uniforms)
     DevBuf plan;     // op tables and fused matrices of the last gate-list plan
+    DevBuf trot;     // phase tables of the diagonal runs of a Trotter step
     size_t last_plan_steps = 0;
     double *host_pinned = nullptr;  // pinned host words for results
     size_t host_pinned_words = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_exec_ms = 0.0;
     int cu_count = 0;
+    // SplitMix64 stream for measurements whose uniform draw is not injected
+    // (the reference draws from rand::rng(), state.rs:601-602)
+    uint64_t rng = 0;
     // cached graph of the last captured gate list
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
@@ -96,6 +106,8 @@ struct Lowered {
     uint32_t t0 = 0, t1 = 0;        // targets
     uint64_t ctrl_mask = 0;         // controls (must be 1)
     double c[8] = {0};              // coefficients (kind-specific, see gates.hip)
+    int32_t slot = -1;              // parametric gate (RX, RY, RZ, P): its angle slot; the planner keeps
+                                    // such gates in forms whose coefficients can be rewritten later
 };
 
 // Argument checks of Operator::apply / Gate::apply (validate_qubits and the
@@ -108,6 +120,8 @@ Status launch_gate(State &s, const Lowered &L, hipStream_t st);
 Status launch_init(State &s, int kind, uint64_t basis, hipStream_t st);
 Status launch_scale(State &s, double re, double im, hipStream_t st);
 Status launch_axpy(State &y, double re, double im, const State &x, hipStream_t st);
+// out <- a + b (sub = false) or a - b (sub = true), elementwise; out may alias a or b.
+Status launch_addsub(State &out, const State &a, const State &b, bool sub, hipStream_t st);
 Status launch_copy(State &dst, const State &src, hipStream_t st);
 Status launch_div_real(State &s, double f, hipStream_t st);
 
@@ -122,6 +136,20 @@ Status launch_pauli_exp(State &s, const qi_pauli_term &t, double ch_re, double c
                         double sh_im, hipStream_t st);
 Status launch_sumop_apply(const State &in, State &out, const qi_pauli_term *terms, size_t nt,
                           hipStream_t st);
+// A run of diagonal Pauli-string exponentials in one pass: per-amplitude
+// factors looked up in tables over at most kDiagTabBits qubits each.
+constexpr int kDiagTabBits = 10;
+constexpr int kDiagTabMax = 4;
+struct DiagTab {
+    int32_t bits, off;            // qubits in the table, first entry
+    uint8_t pos[kDiagTabBits];    // qubit of table-index bit k
+    uint8_t pad[6];
+};
+struct DiagPass {
+    int32_t ntab, entries;
+    DiagTab tab[kDiagTabMax];
+};
+Status launch_diag_tables(State &s, const DiagPass &p, const double2 *dtables, hipStream_t st);
 // Expectation of a list of terms; result (re, im) summed in FP64 compensated.
 Status sumop_expectation(State &s, const qi_pauli_term *terms, size_t nt, double *re, double *im,
                          hipStream_t st);
@@ -135,6 +163,16 @@ Status measure_sample_collapse(State &s, State &dst, const uint64_t *qubits, uin
                                bool collapse, hipStream_t st);
 
 int pick_blocks(uint64_t items, int per_block);
+
+// Next uniform in [0, 1) of a SplitMix64 stream (Steele, Lea & Flood 2014):
+// the top 53 bits of the output word times 2^-53.
+inline double splitmix_uniform(uint64_t &state) {
+    uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return double(z >> 11) * (1.0 / 9007199254740992.0);
+}
 
 // ------------------------------------------------------- workgroup order --
 // Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8), so with the

@@ -251,6 +251,45 @@ struct RegPlanner {
         memcpy(cr, r, 8);
         memcpy(ct, t, 12);
     }
+    // One-wave tiles: before a 2x2 op on a lane-held local bit, exchange that
+    // lane bit with a register bit (RO_XLR) when the pass's remaining ops on
+    // it make that cheaper. Per-op costs in units of a lane-bit 2x2 op (the
+    // partner of every amplitude through ds_bpermute, 128 per op): a
+    // register-bit 2x2 op kRegCost, an exchange kXlrCost (half the amplitudes
+    // cross lanes). The register given up holds the local bit with the fewest
+    // remaining 2x2 ops, which then run on the lane; a row local bit (loaded
+    // and stored on lanes 0..L-1) pays a second exchange to go back at the end.
+    static constexpr double kRegCost = 0.18, kXlrCost = 0.5;
+    void lane_to_register(const std::vector<int> &tgt_lb, size_t at) {
+        const int b = tgt_lb[at];
+        const int i = lane_index(b);
+        if (i < 0) return;  // already on a register bit
+        auto uses = [&](int lb) {
+            int k = 0;
+            for (size_t s = at; s < tgt_lb.size(); s++) k += tgt_lb[s] == lb;
+            return k;
+        };
+        const int k = uses(b);
+        bool row = false;
+        for (int r = 0; r < L; r++) row = row || lbit[r] == b;
+        int best = -1, best_m = 1 << 30;
+        for (int j = 0; j < RB; j++) {
+            const int m = uses(cr[j]);
+            if (m < best_m) { best_m = m; best = j; }
+        }
+        if (best < 0) return;
+        const double keep = k * 1.0;
+        const double move = kXlrCost * (row ? 2 : 1) + k * kRegCost + best_m * (1.0 - kRegCost);
+        if (move >= keep) return;
+        RegOp op{};
+        op.kind = RO_XLR;
+        op.tgt = i;
+        op.dbit = best;
+        std::swap(ct[i], cr[best]);
+        memcpy(op.lay_t, ct, sizeof op.lay_t);  // the kernel's lane-held local bits from here on
+        forget_diagonals();  // diagonal positions are layout-relative
+        out.push_back(op);
+    }
     void controls(uint64_t cm, RegOp &op) const {
         for (; cm; cm &= cm - 1) {
             const int q = __builtin_ctzll(cm);
@@ -263,11 +302,13 @@ struct RegPlanner {
     Status emit(const Lowered &G, uint32_t src) {
         RegOp op{};
         op.src = src;
+        const bool param = G.slot >= 0;  // no rewrite that depends on the angle's value
+        op.param = param ? G.slot + 1 : 0;
         switch (G.kind) {
         case LK_Z: case LK_PHASE: case LK_RZ: {
             if (G.kind == LK_Z) op.kind = RO_Z;
             else if (G.kind == LK_PHASE) { op.kind = RO_P1; op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; }
-            else if (gphase && G.ctrl_mask == 0 && (G.c[0] != 0 || G.c[1] != 0)) {
+            else if (gphase && !param && G.ctrl_mask == 0 && (G.c[0] != 0 || G.c[1] != 0)) {
                 // diag(p0, p1) = p0 diag(1, p1 / p0)
                 const double d = G.c[0] * G.c[0] + G.c[1] * G.c[1];
                 op.kind = RO_P1; op.c[0] = 1;
@@ -349,7 +390,7 @@ struct RegPlanner {
                 op.kind = RO_XC1;
                 op.lp0 = __builtin_ctz(op.rcm);
             }
-            if (f32 && op.kind == RO_RR && op.tgt < kLaneTarget && !op.rcm && !op.tcm && !op.gcm &&
+            if (f32 && !param && op.kind == RO_RR && op.tgt < kLaneTarget && !op.rcm && !op.tcm && !op.gcm &&
                 op.c[0] == op.c[1] && op.c[0] == op.c[2] && op.c[0] == -op.c[3] && op.c[0] != 0) {
                 // a multiple of [[1, 1], [1, -1]] (H, and the X basis changes of Trotter steps)
                 const double f = op.c[0];
@@ -360,10 +401,13 @@ struct RegPlanner {
                 last_diag[0][op.tgt] = -1;
                 return Status{};
             }
-            if (f32) shear_form(op);
+            if (f32 && !param) shear_form(op);
         } break;
         }
-        if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
+        if (param && (op.kind == RO_P1 || op.kind == RO_DG)) {
+            // neither folds into an earlier diagonal nor takes later ones
+            if (!op.rcm && !op.tcm && !op.gcm) last_diag[op.dloc][op.dbit] = -1;
+        } else if (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) {
             if (!op.rcm && !op.tcm && !op.gcm) {
                 int &last = last_diag[op.dloc][op.dbit];
                 if (last >= 0) {
@@ -583,6 +627,21 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
             for (int j = 0; j < RB; j++) rp.qr_in[j] = (uint8_t)P.qof[P.cr[j]];
             for (int i = 0; i < TB; i++) rp.qt_in[i] = (uint8_t)P.qof[P.ct[i]];
         }
+        // The local bit each 2x2 op targets, in emission order (relabels move
+        // qubits between local bits, not data): the lookahead of the register
+        // allocation below.
+        std::vector<int> tgt_lb;
+        {
+            int lb[64];
+            memcpy(lb, P.lbit, sizeof lb);
+            for (size_t p = 0; p < phases.size(); p++)
+                for (size_t i : phases[p].ops) {
+                    const Lowered &G = gates[members[i]];
+                    tgt_lb.push_back(is_pair(G) && !is_relabel(G) ? lb[G.t0] : -1);
+                    if (is_relabel(G)) std::swap(lb[G.t0], lb[G.t1]);
+                }
+        }
+        size_t seq = 0;
         for (size_t p = 0; p < phases.size(); p++) {
             if (phases[p].need & P.wave_set()) {
                 next_use(p, nu);
@@ -591,8 +650,10 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
                 P.set_layout(r, t, true);
             }
             for (size_t i : phases[p].ops) {
+                if (TB == kLaneBits && tgt_lb[seq] >= 0) P.lane_to_register(tgt_lb, seq);
                 Status e = P.emit(gates[members[i]], (uint32_t)members[i]);
                 if (!e.ok()) return e;
+                seq++;
             }
         }
         // ---- store: row qubits back on lanes 0..L-1 (relabels may have moved them)
@@ -605,6 +666,7 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
                 op.tgt = k;
                 if (i2 >= 0) { op.kind = RO_XLL; op.dbit = i2; std::swap(P.ct[k], P.ct[i2]); }
                 else { op.kind = RO_XLR; op.dbit = j; std::swap(P.ct[k], P.cr[j]); }
+                memcpy(op.lay_t, P.ct, sizeof op.lay_t);
                 P.out.push_back(op);
             }
         }
@@ -628,7 +690,7 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         for (RegOp &op : P.out) {
             // (c64) a one-sided phase on a register bit as a rotation of (re, im) by shears,
             // once no later diagonal can fold into it
-            if (P.f32 && op.kind == RO_P1 && op.dloc == 0 && !op.rcm && !op.tcm &&
+            if (P.f32 && op.kind == RO_P1 && !op.param && op.dloc == 0 && !op.rcm && !op.tcm &&
                 std::fabs(op.c[2] * op.c[2] + op.c[3] * op.c[3] - 1.0) <= 1e-12) {
                 const double phi = std::atan2(op.c[3], op.c[2]);
                 if (std::fabs(phi) <= M_PI / 2) {
@@ -734,6 +796,25 @@ Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int 
     }
     append_plan(plan, best);
     return Status{};
+}
+
+void param_op_coeffs(RegOp &op, double theta) {
+    static const int kind_of[] = {-1, QI_GATE_RY, -1, QI_GATE_RX, -1, -1, QI_GATE_RZ, QI_GATE_P};
+    qi_gate g{};
+    g.kind = op.kind < 8 ? kind_of[op.kind] : -1;
+    g.ntargets = 1;
+    g.params[0] = theta;
+    Lowered G;
+    (void)lower_gate(64, g, G);
+    for (int k = 0; k < 8; k++) op.c[k] = 0;
+    switch (op.kind) {  // the forms RegPlanner::emit gives these gates
+    case RO_RR: op.c[0] = G.c[0]; op.c[1] = -G.c[1]; op.c[2] = G.c[1]; op.c[3] = G.c[0]; break;   // RY
+    case RO_RXI: op.c[0] = G.c[0]; op.c[1] = -G.c[2]; op.c[2] = G.c[4]; op.c[3] = G.c[0]; break;  // RX
+    case RO_DG: for (int k = 0; k < 4; k++) op.c[k] = G.c[k]; break;                              // RZ
+    case RO_P1: op.c[0] = 1; op.c[2] = G.c[0]; op.c[3] = G.c[1]; break;                           // P
+    default: break;
+    }
+    for (int k = 0; k < 8; k++) op.cf[k] = (float)op.c[k];
 }
 
 // ------------------------------------------------------------- LDS passes --

@@ -1017,6 +1017,11 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
         if (op.kind >= RO_TRANSPOSE) continue;
         if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
         apply_op<R, K, TB>(v, op, base, tl, lane);
+        if (op.kind == RO_XLR || op.kind == RO_XLL) {  // the lane bits now hold other local bits
+            tl = 0;
+#pragma unroll
+            for (int i = 0; i < TB; i++) tl |= ((lane >> i) & 1u) << op.lay_t[i];
+        }
     }
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.
@@ -1063,6 +1068,9 @@ template <typename R, int K> bool launch_k(State &s, const RegPass &p, const Reg
 }  // namespace
 
 Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st) {
+    // one workgroup per tile: the grid's work-item count must stay below 2^32
+    if (p.K < 10 || (int)s.n - p.K + p.tb >= 32)
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width too small for this state (grid over 2^32 threads)");
     bool ok = false;
     if (s.prec == QI_C128) {
         switch (p.K) {

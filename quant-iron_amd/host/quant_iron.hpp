@@ -246,16 +246,29 @@ class State {
         check(qi_fidelity(h_, o.h_, &f, &e), e);
         return f;
     }
+    double fs_dist(const State &o) const {
+        double d;
+        qi_error e{};
+        check(qi_fs_dist(h_, o.h_, &d, &e), e);
+        return d;
+    }
     State operator*(C64 c) const {
         State s(*this);
         qi_error e{};
         check(qi_state_scale(s.h_, c.real(), c.imag(), &e), e);
         return s;
     }
+    // impl Add / Sub for State (state.rs:2773-2800, 2836-2862)
     State operator+(const State &o) const {
-        State s(*this);
+        State s(num_qubits(), precision());
         qi_error e{};
-        check(qi_state_axpy(s.h_, 1.0, 0.0, o.h_, &e), e);
+        check(qi_state_add(h_, o.h_, s.h_, &e), e);
+        return s;
+    }
+    State operator-(const State &o) const {
+        State s(num_qubits(), precision());
+        qi_error e{};
+        check(qi_state_sub(h_, o.h_, s.h_, &e), e);
         return s;
     }
 
@@ -276,8 +289,9 @@ class State {
         r.new_state = out;
         return r;
     }
-    std::vector<uint64_t> measure_n(const MeasurementBasis &b, const std::vector<size_t> &qubits,
-                                    const std::vector<double> &uniforms) const {
+    // measure_n outcomes (bit b = qubits[b]) for injected uniforms
+    std::vector<uint64_t> measure_n_outcomes(const MeasurementBasis &b, const std::vector<size_t> &qubits,
+                                             const std::vector<double> &uniforms) const {
         std::vector<uint64_t> q(qubits.begin(), qubits.end()), out(uniforms.size());
         double m8[8];
         b.flat(m8);
@@ -285,6 +299,34 @@ class State {
         check(qi_measure_n(h_, b.kind, m8, q.empty() ? nullptr : q.data(), (uint32_t)q.size(), uniforms.data(),
                            uniforms.size(), out.data(), &e), e);
         return out;
+    }
+    // State::measure_n (state.rs:750-784): one MeasurementResult per shot, the
+    // collapsed state computed once per distinct outcome
+    std::vector<MeasurementResult> measure_n(const MeasurementBasis &b, const std::vector<size_t> &qubits,
+                                             const std::vector<double> &uniforms) const {
+        auto outs = measure_n_outcomes(b, qubits, uniforms);
+        std::vector<uint64_t> q(qubits.begin(), qubits.end());
+        double m8[8];
+        b.flat(m8);
+        std::vector<MeasurementResult> res;
+        std::vector<std::pair<uint64_t, std::shared_ptr<State>>> seen;
+        for (uint64_t o : outs) {
+            MeasurementResult r;
+            r.basis = b;
+            if (qubits.empty()) for (size_t k = 0; k < num_qubits(); k++) r.indices.push_back(k);
+            else r.indices = qubits;
+            for (size_t k = 0; k < r.indices.size(); k++) r.outcomes.push_back((uint8_t)((o >> k) & 1));
+            for (auto &p : seen) if (p.first == o) r.new_state = p.second;
+            if (!r.new_state) {
+                r.new_state = std::make_shared<State>(num_qubits(), precision());
+                qi_error e{};
+                check(qi_measure_result_state(h_, b.kind, m8, q.empty() ? nullptr : q.data(), (uint32_t)q.size(), o,
+                                              r.new_state->h_, &e), e);
+                seen.emplace_back(o, r.new_state);
+            }
+            res.push_back(std::move(r));
+        }
+        return res;
     }
 
     Precision precision() const { return static_cast<Precision>(qi_state_precision(h_)); }

@@ -16,6 +16,7 @@ from .core import (  # noqa: F401
     Gate,
     MeasurementBasis,
     MeasurementResult,
+    ParametricCircuit,
     PauliString,
     QuantIronError,
     State,

@@ -95,6 +95,7 @@ SIGNATURES = {
     "qi_ctx_create": [C.c_int, C.POINTER(P), ERR],
     "qi_ctx_destroy": [P],
     "qi_ctx_synchronize": [P, ERR],
+    "qi_ctx_seed": [P, U64, ERR],
     "qi_ctx_info": [P, C.c_char_p, C.c_size_t, U64P, C.POINTER(C.c_int), ERR],
     "qi_state_create": [P, U32, C.c_int, C.POINTER(P), ERR],
     "qi_state_destroy": [P],
@@ -119,12 +120,24 @@ SIGNATURES = {
     "qi_norm_sqr": [P, DP, ERR],
     "qi_normalise": [P, ERR],
     "qi_fidelity": [P, P, DP, ERR],
+    "qi_fs_dist": [P, P, DP, ERR],
     "qi_state_scale": [P, D, D, ERR],
     "qi_state_axpy": [P, D, D, P, ERR],
+    "qi_state_add": [P, P, P, ERR],
+    "qi_state_sub": [P, P, P, ERR],
     "qi_measure_probs": [P, U64P, U32, DP, ERR],
     "qi_measure": [P, C.c_int, DP, U64P, U32, D, P, U64P, ERR],
     "qi_measure_n": [P, C.c_int, DP, U64P, U32, DP, C.c_size_t, U64P, ERR],
     "qi_measure_collapse": [P, P, U64P, U32, U64, ERR],
+    "qi_measure_result_state": [P, C.c_int, DP, U64P, U32, U64, P, ERR],
+    "qi_qasm_parse": [C.c_char_p, C.c_size_t, DP, C.c_size_t, C.c_void_p, C.POINTER(C.c_int32), C.c_size_t,
+                      C.POINTER(C.c_size_t), C.POINTER(U32), C.POINTER(U32), ERR],
+    "qi_param_circuit_create": [P, U32, C.c_int, C.c_void_p, C.c_size_t, C.POINTER(C.c_int32),
+                                C.POINTER(QiExecOpts), C.POINTER(P), ERR],
+    "qi_param_circuit_destroy": [P],
+    "qi_param_circuit_num_slots": [P, U64P, ERR],
+    "qi_param_circuit_set_angles": [P, DP, C.c_size_t, ERR],
+    "qi_param_circuit_run": [P, P, ERR],
     "qi_batch_create": [P, C.c_int, C.POINTER(P), ERR],
     "qi_batch_destroy": [P],
     "qi_batch_add": [P, U32, C.c_void_p, C.c_size_t, C.POINTER(C.c_int32), C.c_void_p, C.c_size_t, ERR],

@@ -69,6 +69,12 @@ class Context:
         err = L.QiError()
         _check(self.lib.qi_ctx_synchronize(self._h, C.byref(err)), err)
 
+    def seed(self, seed: int):
+        """Seeds the stream that measurements without injected uniforms draw
+        from (the reference's rand::rng(), state.rs:601-602)."""
+        err = L.QiError()
+        _check(self.lib.qi_ctx_seed(self._h, int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(err)), err)
+
     def info(self):
         name = C.create_string_buffer(256)
         hbm = C.c_uint64()
@@ -224,9 +230,11 @@ class Gate:
     def p_controlled_gates(targets, controls, angle):
         return Gate.controlled("P", targets, controls, [angle])
     @staticmethod
-    def new_measurement(qubits, basis=MeasurementBasis.Computational, uniform: float = 0.5):
-        """Gate::Measurement with the draw of rand::rng() injected as `uniform`."""
+    def new_measurement(qubits, basis=MeasurementBasis.Computational, uniform: Optional[float] = None):
+        """Gate::Measurement. `uniform` injects the draw of rand::rng(); None draws
+        from the context's stream each time the gate runs (Context.seed)."""
         code, mat = _basis_code(basis)
+        uniform = math.nan if uniform is None else uniform
         return Gate(_gate("MEASURE", [], list(qubits), [uniform], mat, basis=code), "Measure")
     @staticmethod
     def pauli_string_gate(ps: "PauliString"):
@@ -243,18 +251,27 @@ class Gate:
 # ------------------------------------------------------------- measurement --
 @dataclass
 class MeasurementResult:
-    """MeasurementResult (measurement.rs:15-72)."""
+    """MeasurementResult (measurement.rs:15-72). For measure_n results the
+    collapsed state is computed by the engine on first access and shared by
+    the shots with the same outcome (the reference builds one per shot)."""
 
     basis: tuple
     indices: List[int]
     outcomes: List[int]
     _state: Optional["State"] = None
     _parent: Optional["State"] = field(default=None, repr=False)
+    _cache: Optional[dict] = field(default=None, repr=False)
 
     @property
     def new_state(self) -> "State":
         if self._state is None:
-            self._state = self._parent._collapse_result(self.basis, self.indices, self.outcomes)
+            o = sum(int(b) << k for k, b in enumerate(self.outcomes))
+            if self._cache is not None and o in self._cache:
+                self._state = self._cache[o]
+            else:
+                self._state = self._parent._measured_state(self.basis, self.indices, o)
+                if self._cache is not None:
+                    self._cache[o] = self._state
         return self._state
 
     def get_new_state(self) -> "State":
@@ -484,9 +501,14 @@ class State:
         return v.value
 
     def fs_dist(self, other: "State") -> float:
-        return math.acos(min(1.0, math.sqrt(self.fs_fidelity(other))))
+        """fs_dist (state.rs:470-478): acos(|<a|b>|) of the normalised states."""
+        v = C.c_double()
+        err = L.QiError()
+        _check(self.lib.qi_fs_dist(self._h, other._h, C.byref(v), C.byref(err)), err)
+        return v.value
 
     def __mul__(self, c) -> "State":
+        """impl Mul<Complex<f64>> / Mul<f64> for State (state.rs:2687-2771)."""
         out = self.clone()
         c = complex(c)
         err = L.QiError()
@@ -495,19 +517,37 @@ class State:
 
     __rmul__ = __mul__
 
-    def __add__(self, other: "State") -> "State":
-        if self.num_qubits != other.num_qubits:
-            raise ValueError(f"Cannot add states with different numbers of qubits: {self.num_qubits} != {other.num_qubits}")
-        out = self.clone()
+    def _addsub(self, other: "State", fn, verb: str) -> "State":
+        if self.num_qubits != other.num_qubits:  # the reference panics with this message
+            raise ValueError(f"Cannot {verb} states with different numbers of qubits: "
+                             f"{self.num_qubits} != {other.num_qubits}")
+        out = State._alloc(self.num_qubits, self.precision, self.ctx)
         err = L.QiError()
-        _check(self.lib.qi_state_axpy(out._h, 1.0, 0.0, other._h, C.byref(err)), err)
+        _check(fn(self._h, other._h, out._h, C.byref(err)), err)
         return out
 
+    def __add__(self, other: "State") -> "State":
+        """impl Add for State (state.rs:2773-2800)."""
+        return self._addsub(other, self.lib.qi_state_add, "add")
+
     def __sub__(self, other: "State") -> "State":
-        out = self.clone()
-        err = L.QiError()
-        _check(self.lib.qi_state_axpy(out._h, -1.0, 0.0, other._h, C.byref(err)), err)
-        return out
+        """impl Sub for State (state.rs:2836-2862)."""
+        return self._addsub(other, self.lib.qi_state_sub, "subtract")
+
+    @staticmethod
+    def sum(states: Iterable["State"]) -> "State":
+        """impl Sum for State (state.rs:2802-2834): a left fold of Add."""
+        it = iter(states)
+        try:
+            acc = next(it)
+        except StopIteration:
+            raise ValueError("Cannot sum an empty iterator of State objects: num_qubits for the sum is undefined.")
+        for st in it:
+            if acc.num_qubits != st.num_qubits:
+                raise ValueError(f"Cannot sum states with different numbers of qubits: "
+                                 f"{acc.num_qubits} != {st.num_qubits}")
+            acc = acc + st
+        return acc
 
     def __eq__(self, other) -> bool:
         """PartialEq (state.rs:2348-2372): |dre|, |dim| <= f32::EPSILON per amplitude."""
@@ -519,10 +559,12 @@ class State:
 
     __hash__ = None
 
-    # ---- measurement (state.rs:525-784), uniforms injected
+    # ---- measurement (state.rs:525-784): uniforms injected, or drawn by the
+    # engine from the context's stream (Context.seed) when not given
     def measure(self, basis=MeasurementBasis.Computational, measured_qubits: Sequence[int] = (),
-                uniform: float = 0.5) -> MeasurementResult:
+                uniform: Optional[float] = None) -> MeasurementResult:
         code, mat = _basis_code(basis)
+        uniform = math.nan if uniform is None else uniform
         qs = np.asarray(list(measured_qubits), dtype=np.uint64)
         out = State._alloc(self.num_qubits, self.precision, self.ctx)
         o = C.c_uint64()
@@ -537,17 +579,25 @@ class State:
 
     def measure_n(self, basis=MeasurementBasis.Computational, measured_qubits: Sequence[int] = (), n: int = 1,
                   uniforms: Optional[Sequence[float]] = None) -> List[MeasurementResult]:
+        """State::measure_n (state.rs:750-784): n shots, each with its outcome
+        bits and collapsed state. `uniforms` injects the n draws; None draws
+        them from the context's stream."""
         code, mat = _basis_code(basis)
         qs = np.asarray(list(measured_qubits), dtype=np.uint64)
-        u = np.asarray(uniforms if uniforms is not None else np.full(max(n, 0), 0.5), dtype=np.float64)
-        outs = np.zeros(max(u.size, 1), dtype=np.uint64)
+        u = None
+        if uniforms is not None:
+            u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64))
+            if u.size < n:
+                raise QuantIronError(17, msg=f"{u.size} uniforms for {n} shots")
+        outs = np.zeros(max(int(n), 1), dtype=np.uint64)
         m8 = _mat8(mat)
         err = L.QiError()
         _check(self.lib.qi_measure_n(self._h, code, L.dptr(m8) if m8 is not None else None,
-                                     L.u64ptr(qs) if qs.size else None, qs.size, L.dptr(u), int(n), L.u64ptr(outs),
-                                     C.byref(err)), err)
+                                     L.u64ptr(qs) if qs.size else None, qs.size,
+                                     L.dptr(u) if u is not None else None, int(n), L.u64ptr(outs), C.byref(err)), err)
         idx = list(measured_qubits) if len(measured_qubits) else list(range(self.num_qubits))
-        return [MeasurementResult(basis, idx, [(int(o) >> b) & 1 for b in range(len(idx))], None, self)
+        src, cache = self.clone(), {}  # the shots' states derive from this state as it is now
+        return [MeasurementResult(basis, idx, [(int(o) >> b) & 1 for b in range(len(idx))], None, src, cache)
                 for o in outs[:n]]
 
     def measure_outcomes(self, measured_qubits: Sequence[int], uniforms, basis=MeasurementBasis.Computational) -> np.ndarray:
@@ -572,26 +622,16 @@ class State:
                                          C.byref(err)), err)
         return out
 
-    def _collapse_result(self, basis, idx, bits) -> "State":
+    def _measured_state(self, basis, idx, outcome: int) -> "State":
+        """MeasurementResult::new_state of `outcome` (qi_measure_result_state)."""
         code, mat = _basis_code(basis)
-        w = self.clone()
-        if code == L.BASIS_X:
-            w = w.h_multi(idx)
-        elif code == L.BASIS_Y:
-            w = w.s_dag_multi(idx).h_multi(idx)
-        elif code == L.BASIS_CUSTOM:
-            w = w.unitary_multi(idx, mat)
-        qs = np.asarray(idx, dtype=np.uint64)
-        o = sum(int(b) << k for k, b in enumerate(bits))
+        qs = np.asarray(list(idx), dtype=np.uint64)
+        m8 = _mat8(mat)
         out = State._alloc(self.num_qubits, self.precision, self.ctx)
         err = L.QiError()
-        _check(self.lib.qi_measure_collapse(w._h, out._h, L.u64ptr(qs), qs.size, o, C.byref(err)), err)
-        if code == L.BASIS_X:
-            out = out.h_multi(idx)
-        elif code == L.BASIS_Y:
-            out = out.h_multi(idx).s_multi(idx)
-        elif code == L.BASIS_CUSTOM:
-            out = out.unitary_multi(idx, np.conj(mat).T)
+        _check(self.lib.qi_measure_result_state(self._h, code, L.dptr(m8) if m8 is not None else None,
+                                                L.u64ptr(qs) if qs.size else None, qs.size, int(outcome), out._h,
+                                                C.byref(err)), err)
         return out
 
 
@@ -819,6 +859,16 @@ class Circuit:
             self._arr = gate_array(self.gates)
         return self._arr
 
+    @classmethod
+    def from_qasm(cls, text: str, params=None) -> "Circuit":
+        """A circuit from OpenQASM 3.0 text (the exporter's output of
+        Circuit::to_qasm, circuit.rs:244-278, or standard syntax); `params`
+        binds the program's `input` parameters in declaration order."""
+        n, arr, _slots, _np = parse_qasm(text, params)
+        c = cls(n, [Gate(L.QiGate.from_buffer_copy(arr[i].tobytes())) for i in range(arr.size)])
+        c._arr = arr
+        return c
+
     def execute(self, state: State, **opts) -> State:
         """Circuit::execute: a new state; all gates in one engine call."""
         out = state.clone()
@@ -838,6 +888,24 @@ class Circuit:
             cur.apply_gate_inplace(g)
             states.append(cur.clone())
         return states
+
+
+def parse_qasm(text: str, params=None):
+    """OpenQASM 3.0 text -> (qubit count, qi_gate array, angle slots, parameter
+    count) through the engine's reader (qi_qasm_parse). `params` gives the
+    values of the program's `input` parameters in declaration order."""
+    lib = L.load()
+    raw = text.encode()
+    vals = None if params is None else np.ascontiguousarray(np.asarray(params, dtype=np.float64))
+    ng, nq, npar = C.c_size_t(), C.c_uint32(), C.c_uint32()
+    err = L.QiError()
+    args = (raw, len(raw), L.dptr(vals) if vals is not None else None, 0 if vals is None else vals.size)
+    _check(lib.qi_qasm_parse(*args, None, None, 0, C.byref(ng), C.byref(nq), C.byref(npar), C.byref(err)), err)
+    arr = np.zeros(ng.value, dtype=L.GATE_DTYPE)
+    slots = np.full(ng.value, -1, dtype=np.int32)
+    _check(lib.qi_qasm_parse(*args, arr.ctypes.data_as(C.c_void_p), slots.ctypes.data_as(C.POINTER(C.c_int32)),
+                             arr.size, C.byref(ng), C.byref(nq), C.byref(npar), C.byref(err)), err)
+    return nq.value, arr, slots, npar.value
 
 
 class CircuitBuilder:
@@ -864,7 +932,7 @@ class CircuitBuilder:
     def cnot_gate(self, target, control): return self.add_gate(Gate.cnot_gate(target, control))
     def swap_gate(self, q1, q2): return self.add_gate(Gate.swap_gate(q1, q2))
     def toffoli_gate(self, c1, c2, target): return self.add_gate(Gate.toffoli_gate(target, [c1, c2]))
-    def measure_gate(self, basis, qubits, uniform=0.5): return self.add_gate(Gate.new_measurement(qubits, basis, uniform))
+    def measure_gate(self, basis, qubits, uniform=None): return self.add_gate(Gate.new_measurement(qubits, basis, uniform))
     def pauli_string_gate(self, ps): return self.add_gate(Gate.pauli_string_gate(ps))
     def pauli_time_evolution_gate(self, ps, t): return self.add_gate(Gate.pauli_time_evolution_gate(ps, t))
     def build(self) -> Circuit: return Circuit(self.num_qubits, self.gates)
@@ -907,6 +975,64 @@ class Subroutine:
         return b.build()
 
 
+# ------------------------------------------------------ parametric circuits --
+class ParametricCircuit:
+    """A circuit whose RX/RY/RZ/P gates may read their angle from a table
+    (Gate::Parametric over Parameter<N> values, gate.rs:107-114,
+    parameter.rs:12-68), planned once and re-run with new angles: set_angles
+    rewrites the parametric ops' coefficients, run replays the captured
+    launch sequence on a state (qi_param_circuit_* in include/qiron.h)."""
+
+    def __init__(self, num_qubits: int, gates, angle_slots=None, precision: str = "c64", ctx=None,
+                 mode: int = L.EXEC_AUTO, tile_qubits: int = 0):
+        self.ctx = ctx or default_context()
+        self.lib = self.ctx.lib
+        self.num_qubits = num_qubits
+        self.precision = precision
+        arr = gates if isinstance(gates, np.ndarray) else gate_array(gates)
+        slots = None
+        if angle_slots is not None:
+            slots = np.ascontiguousarray(np.asarray(angle_slots, dtype=np.int32))
+        opts = L.QiExecOpts(mode, tile_qubits, 0, 1, 0)
+        self._h = C.c_void_p()
+        err = L.QiError()
+        _check(self.lib.qi_param_circuit_create(self.ctx.handle, int(num_qubits), PRECISIONS[precision],
+                                                arr.ctypes.data_as(C.c_void_p), arr.size,
+                                                slots.ctypes.data_as(C.POINTER(C.c_int32)) if slots is not None else None,
+                                                C.byref(opts), C.byref(self._h), C.byref(err)), err)
+        n = C.c_uint64()
+        _check(self.lib.qi_param_circuit_num_slots(self._h, C.byref(n), C.byref(err)), err)
+        self.num_slots = n.value
+
+    @classmethod
+    def from_qasm(cls, text: str, precision: str = "c64", ctx=None, **kw) -> "ParametricCircuit":
+        """The program's `input` parameters become the angle slots, in declaration order."""
+        n, arr, slots, _np = parse_qasm(text)
+        return cls(n, arr, slots, precision, ctx, **kw)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.lib.qi_param_circuit_destroy(self._h)
+        except Exception:
+            pass
+
+    def set_angles(self, angles) -> None:
+        a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64))
+        err = L.QiError()
+        _check(self.lib.qi_param_circuit_set_angles(self._h, L.dptr(a), a.size, C.byref(err)), err)
+
+    def run(self, state: State) -> State:
+        """Applies the circuit to `state` in place."""
+        err = L.QiError()
+        _check(self.lib.qi_param_circuit_run(self._h, state._h, C.byref(err)), err)
+        return state
+
+    def execute(self, state: State) -> State:
+        """Circuit::execute: a new state."""
+        return self.run(state.clone())
+
+
 # ------------------------------------------------------------------- batches --
 class CircuitBatch:
     """Many parametrised circuits, each with its own SumOp, evaluated to
@@ -939,6 +1065,13 @@ class CircuitBatch:
                                      t.ctypes.data_as(C.c_void_p), t.size, C.byref(err)), err)
         self.count += 1
         return self.count - 1
+
+    def add_qasm(self, text: str, terms: np.ndarray, slot_offset: int = 0) -> int:
+        """Adds a circuit given as OpenQASM 3.0 text; its k-th `input`
+        parameter reads entry slot_offset + k of the batch's angle table."""
+        n, arr, slots, npar = parse_qasm(text)
+        slots = np.where(slots >= 0, slots + int(slot_offset), -1).astype(np.int32)
+        return self.add(n, arr, terms, slots if npar else None)
 
     def set_angles(self, angles) -> None:
         a = np.ascontiguousarray(np.asarray(angles, dtype=np.float64))

@@ -34,6 +34,8 @@ L.SIGNATURES.update({
     "qi_mlp_train_epoch": [L.P, C.c_uint32, C.c_float, C.c_uint64, C.POINTER(C.c_float), L.ERR],
     "qi_mlp_predict": [L.P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float), L.ERR],
     "qi_mlp_predict_ansatz": [L.P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.POINTER(C.c_float), L.ERR],
+    "qi_mlp_predict_ansatz_local": [L.P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, L.DP, C.c_double, L.DP,
+                                    C.POINTER(C.c_float), L.ERR],
 })
 
 
@@ -49,7 +51,7 @@ class MLP:
         self.lib = L.load()
         for name in ("qi_mlp_create", "qi_mlp_destroy", "qi_mlp_num_params", "qi_mlp_get_params", "qi_mlp_set_params",
                      "qi_mlp_gradients", "qi_mlp_set_data", "qi_mlp_train_epoch", "qi_mlp_predict",
-                     "qi_mlp_predict_ansatz"):
+                     "qi_mlp_predict_ansatz", "qi_mlp_predict_ansatz_local"):
             f = getattr(self.lib, name)
             f.argtypes = L.SIGNATURES[name]
             f.restype = C.c_int

@@ -190,6 +190,19 @@ def random_circuit(n: int = 30, depth: int = 40, seed: int = SEED_BASE + 2, toff
     return gl.array()
 
 
+def inverse_gates(gates: np.ndarray) -> np.ndarray:
+    """The inverse of a circuit of H, X, Y, Z, CNOT, CZ, SWAP, Toffoli (each its
+    own inverse) and RX, RY, RZ, P (inverted by negating the angle): the list
+    reversed with rotation and phase angles negated."""
+    bad = np.isin(gates["kind"], [L.GATE_KINDS[k] for k in ("S", "T", "SDAG", "TDAG", "U2", "MATCHGATE")])
+    if bad.any():
+        raise ValueError("inverse_gates covers self-inverse gates and RX/RY/RZ/P only")
+    inv = gates[::-1].copy()
+    rot = np.isin(inv["kind"], [L.GATE_KINDS[k] for k in ("RX", "RY", "RZ", "P")])
+    inv["params"][rot, 0] = -inv["params"][rot, 0]
+    return inv
+
+
 def minimal_touch_bytes(gates: np.ndarray, n: int, amp_bytes: int) -> float:
     """Bytes a gate must move at the least: dense or permutation 1-qubit gates and
     RZ read and write the state (2S); Z/S/T/P touch the target-1 half (S);

@@ -99,8 +99,22 @@ int main() {
         auto r1 = State::new_plus(1).measure(MeasurementBasis::computational(), {0}, 0.75);
         CHECK(r0.outcomes[0] == 0 && *r0.new_state == State::new_zero(1));
         CHECK(r1.outcomes[0] == 1 && *r1.new_state == State::new_basis_n(1, 1));
-        auto outs = State::new_plus(2).measure_n(MeasurementBasis::computational(), {}, {0.1, 0.3, 0.6, 0.9});
+        auto outs = State::new_plus(2).measure_n_outcomes(MeasurementBasis::computational(), {}, {0.1, 0.3, 0.6, 0.9});
         CHECK(outs == (std::vector<uint64_t>{0, 1, 2, 3}));
+        // per-shot results: shot k collapses |++> onto basis state k
+        auto shots = State::new_plus(2).measure_n(MeasurementBasis::computational(), {}, {0.1, 0.3, 0.6, 0.9});
+        for (uint64_t k = 0; k < 4; k++)
+            CHECK(shots[k].outcomes[0] == (k & 1) && shots[k].outcomes[1] == (k >> 1) &&
+                  *shots[k].new_state == State::new_basis_n(2, k));
+    }
+    // Fubini-Study metrics (state_tests.rs:225-295) and State Add / Sub
+    {
+        CHECK(std::abs(State::new_plus(1).fs_fidelity(State::new_basis_n(1, 1)) - 0.5) < 1e-15);
+        CHECK(std::abs(State::new_plus(2).fs_dist(State::new_plus(2))) < 1e-15);
+        auto p = State::new_plus(1), m = State::new_minus(1);
+        // |+> + |-> = sqrt(2)|0>, |+> - |-> = sqrt(2)|1>
+        CHECK((p + m) == State::new_zero(1) * std::sqrt(2.0));
+        CHECK((p - m) == State::new_basis_n(1, 1) * std::sqrt(2.0));
     }
     // error variants (operator_tests.rs:2513-2703)
     expect_error([] { State::new_zero(2).h(2); }, QI_ERR_INVALID_QUBIT_INDEX, 2, 2);

@@ -208,6 +208,41 @@ CASES.append({"name": "new_ghz_3", "cite": "state.rs:277-300", "n": 3, "op": "in
 CASES.append({"name": "new_plus_3", "cite": "state.rs:225-237", "n": 3, "op": "init", "kind": 2,
               "expected": [[z.real, z.imag] for z in plus(3)]})
 
+# ---- normalise and the Fubini-Study metrics (state_tests.rs:149-166, 225-295).
+# The reference builds some of these states with a num_qubits field that does
+# not match their length (e.g. two amplitudes labelled 2 qubits); the length is
+# what the arithmetic uses, so here a state of 2^k amplitudes is k qubits.
+def vec(v):
+    return [[complex(z).real, complex(z).imag] for z in v]
+
+
+CASES.append({"name": "normalise_3_4", "cite": "state_tests.rs:149-156", "n": 1, "op": "normalise",
+              "init": vec([3.0, 4.0]), "expected": vec([0.6, 0.8]), "bit_exact": True})
+METRICS = [
+    {"name": "fs_dist_zero", "cite": "state_tests.rs:225-229", "metric": "fs_dist",
+     "a": vec(plus(2)), "b": vec(plus(2)), "expected": 0.0},
+    {"name": "fs_dist_pi_over_4", "cite": "state_tests.rs:231-244", "metric": "fs_dist",
+     "a": vec([1.0, 1j]), "b": vec([0.0, 1.0]), "expected": PI / 4.0},
+    {"name": "fs_fidelity_zero", "cite": "state_tests.rs:271-278", "metric": "fs_fidelity",
+     "a": vec([1.0, 0.0]), "b": vec([0.0, 1.0]), "expected": 0.0},
+    {"name": "fs_fidelity_one", "cite": "state_tests.rs:280-283", "metric": "fs_fidelity",
+     "a": vec(plus(2)), "b": vec(plus(2)), "expected": 1.0},
+    {"name": "fs_fidelity_half", "cite": "state_tests.rs:285-294", "metric": "fs_fidelity",
+     "a": vec(plus(1)), "b": vec(basis(1, 1)), "expected": 0.5},
+]
+# errors: (variant code, payload a); the empty state of state_tests.rs:248-252
+# (no amplitudes) cannot be built in the engine (State::new rejects it too)
+METRIC_ERRORS = [
+    {"name": "fs_dist_qubit_count", "cite": "state_tests.rs:254-259", "metric": "fs_dist",
+     "a": vec(plus(2)), "b": vec(plus(1)), "code": 3, "a_payload": 2},
+    {"name": "fs_dist_zero_norm", "cite": "state_tests.rs:261-267", "metric": "fs_dist",
+     "a": vec([0.0, 0.0]), "b": vec(basis(2, 1)), "code": 14, "a_payload": 0},
+    {"name": "fs_fidelity_zero_norm_other", "cite": "state.rs:492-498", "metric": "fs_fidelity",
+     "a": vec(basis(2, 1)), "b": vec([0.0, 0.0, 0.0, 0.0]), "code": 14, "a_payload": 0},
+    {"name": "normalise_zero_norm", "cite": "state_tests.rs:159-166", "metric": "normalise",
+     "a": vec([0.0, 0.0]), "b": None, "code": 14, "a_payload": 0},
+]
+
 # ---- error cases: (gate, n, expected variant, payload) from operator_tests.rs:2545-2703 etc.
 ERRORS = [
     {"name": "h_bad_target", "cite": "operator_tests.rs:2513-2545", "n": 2, "gate": g("H", [2]),
@@ -245,5 +280,7 @@ ERRORS = [
 ]
 
 if __name__ == "__main__":
-    OUT.write_text(json.dumps({"cases": CASES, "errors": ERRORS}, indent=1))
-    print(f"wrote {len(CASES)} cases and {len(ERRORS)} error cases to {OUT}")
+    OUT.write_text(json.dumps({"cases": CASES, "errors": ERRORS, "metrics": METRICS,
+                               "metric_errors": METRIC_ERRORS}, indent=1))
+    print(f"wrote {len(CASES)} cases, {len(ERRORS)} error cases, {len(METRICS)} metric cases and "
+          f"{len(METRIC_ERRORS)} metric error cases to {OUT}")

@@ -255,3 +255,89 @@ def test_reference_structured_baseline_equals_oracle():
     a = O.apply_gates(psi, gates)
     b = O.reference_structured_apply(psi, gates, threads=3)
     np.testing.assert_allclose(a, b, atol=1e-13)
+
+
+def _dense_basis_measure(psi, n, qubits, U, u):
+    """Measurement in the basis of U on `qubits` from dense algebra: w = (x)U psi,
+    the outcome by the sequential-CDF rule on the marginal of w, the projector,
+    renormalisation and (x)U^dagger back (state.rs:706-728)."""
+    V = np.eye(1, dtype=complex)
+    for q in reversed(range(n)):  # qubit q is bit q: the highest qubit is the leftmost factor
+        V = np.kron(V, U if q in qubits else I2)
+    w = V @ psi
+    outcome_of = np.array([sum(((i >> q) & 1) << b for b, q in enumerate(qubits)) for i in range(1 << n)])
+    probs = np.bincount(outcome_of, weights=np.abs(w) ** 2, minlength=1 << len(qubits))
+    cum = 0.0
+    o = len(probs) - 1
+    for i, p in enumerate(probs):  # first bin with u < C_i, else the last (state.rs:603-619)
+        cum += p / probs.sum()
+        if u < cum:
+            o = i
+            break
+    col = np.where(outcome_of == o, w, 0)
+    col = col / np.linalg.norm(col)
+    return o, V.conj().T @ col
+
+
+def test_custom_basis_measurement_matches_dense_algebra():
+    """MeasurementBasis::Custom(U) (state.rs:706-728) with a general complex
+    unitary on two measured qubits of four: outcome and collapsed state against
+    the dense product, projector and adjoint."""
+    from kat_util import random_unitary2
+
+    rng = np.random.default_rng(77)
+    n = 4
+    for trial in range(6):
+        psi = random_state(n, rng)
+        U = random_unitary2(rng)
+        qs = [3, 1] if trial % 2 else [0, 2]
+        for u in (0.02, 0.31, 0.64, 0.97):
+            o, post = O.measure(psi, 3, qs, u, U)
+            o_d, post_d = _dense_basis_measure(psi, n, qs, U, u)
+            assert o == o_d, (trial, u)
+            np.testing.assert_allclose(post, post_d, atol=1e-13)
+    # a basis that is not unitary is rejected (Unitary2::new, operator.rs:2092-2118)
+    with pytest.raises(O.OracleError) as ei:
+        O.measure(random_state(2, rng), 3, [0], 0.5, np.array([[1, 1], [0, 1]]))
+    assert ei.value.code == 6
+
+
+def test_fs_metrics_against_numpy():
+    """fs_fidelity = |<a|b>|^2 / (|a|^2 |b|^2) and fs_dist = acos of its root,
+    computed here with numpy's vdot on unnormalised random vectors."""
+    rng = np.random.default_rng(5)
+    for n in (1, 3, 7):
+        a = random_state(n, rng) * 1.7
+        b = random_state(n, rng) * 0.4
+        f = abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+        assert abs(O.fs_fidelity(a, b) - f) < 1e-14
+        assert abs(O.fs_dist(a, b) - math.acos(math.sqrt(f))) < 1e-12
+
+
+def test_state_arithmetic_against_numpy():
+    """State Add / Sub / Mul (state.rs:2687-2862): one IEEE rounding per
+    component, the same as numpy's complex add, subtract and multiply."""
+    rng = np.random.default_rng(6)
+    a, b = random_state(6, rng), random_state(6, rng)
+    np.testing.assert_array_equal(O.state_add(a, b), a + b)
+    np.testing.assert_array_equal(O.state_sub(a, b), a - b)
+    c = complex(0.3, -1.25)
+    # (ar*cr - ai*ci, ar*ci + ai*cr), the num-complex product
+    exp = (a.real * c.real - a.imag * c.imag) + 1j * (a.real * c.imag + a.imag * c.real)
+    np.testing.assert_array_equal(O.state_scale(a, c), exp)
+
+
+def test_sample_cdf_is_the_sequential_cumulative_sum():
+    rng = np.random.default_rng(8)
+    p = rng.random(1000)
+    cdf = O.sample_cdf(p)
+    total = 0.0
+    for x in p:
+        total += x
+    cum, exp = 0.0, []
+    for x in p:
+        cum += x / total
+        exp.append(cum)
+    np.testing.assert_array_equal(cdf, np.array(exp))
+    for u in rng.random(100):
+        assert O.sample_outcome(p, u) == int(np.argmax(u < cdf)) if u < cdf[-1] else len(p) - 1

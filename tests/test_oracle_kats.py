@@ -26,6 +26,8 @@ def run_oracle(case):
         return np.array([O.expectation(psi, term_array(case["terms"]))])
     if op == "trotter":
         return O.trotter(psi, term_array(case["terms"]), case["dt"], case["steps"], case["order"])
+    if op == "normalise":
+        return O.normalise(psi)
     raise ValueError(op)
 
 
@@ -36,6 +38,8 @@ def test_oracle_matches_reference_kat(case):
     # the reference's PartialEq tolerance, and the FP64 closed form to 1e-12
     assert np.all(np.abs(got.real - exp.real) <= F32_EPS) and np.all(np.abs(got.imag - exp.imag) <= F32_EPS)
     assert np.max(np.abs(got - exp)) <= 1e-12, case["cite"]
+    if case.get("bit_exact"):
+        np.testing.assert_array_equal(got, exp)
 
 
 ERRS = KATS["errors"]
@@ -48,3 +52,31 @@ def test_oracle_error_variants(case):
         O.apply_gate(psi, to_gate(case["gate"]))
     e = ei.value
     assert (e.code, e.a, e.b) == (case["code"], case["a"], case["b"]), case["cite"]
+
+
+METRICS = KATS["metrics"]
+
+
+def oracle_metric(name, a, b):
+    return O.fs_dist(a, b) if name == "fs_dist" else O.fs_fidelity(a, b)
+
+
+@pytest.mark.parametrize("case", METRICS, ids=[c["name"] for c in METRICS])
+def test_oracle_fs_metrics(case):
+    """fs_fidelity / fs_dist known answers (state_tests.rs:225-295); the
+    reference asserts them within f64::EPSILON."""
+    got = oracle_metric(case["metric"], cvec(case["a"]), cvec(case["b"]))
+    assert abs(got - case["expected"]) < np.finfo(np.float64).eps, case["cite"]
+
+
+MERRS = KATS["metric_errors"]
+
+
+@pytest.mark.parametrize("case", MERRS, ids=[c["name"] for c in MERRS])
+def test_oracle_fs_metric_errors(case):
+    with pytest.raises(O.OracleError) as ei:
+        if case["metric"] == "normalise":
+            O.normalise(cvec(case["a"]))
+        else:
+            oracle_metric(case["metric"], cvec(case["a"]), cvec(case["b"]))
+    assert (ei.value.code, ei.value.a) == (case["code"], case["a_payload"]), case["cite"]
