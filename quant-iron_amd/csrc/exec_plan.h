@@ -74,13 +74,13 @@ struct RegOp {            // every field dword-sized: the kernel reads ops with 
     int32_t kind;        // RegOpKind
     int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
     int32_t tgt;         // pair ops: register bit, or kLaneTarget + lane bit
-    int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = local bit),
+    int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = thread-index bit),
                          //               2 outside the tile (dbit = qubit)
     int32_t dbit;
     int32_t param;       // angle slot + 1 of a parametric op (RO_RR: RY, RO_RXI: RX, RO_DG: RZ, RO_P1: P); 0: none
     int32_t lp0, lp1;    // RO_LDS: local bits of the targets
     uint32_t rcm;        // controls on register bits (register-bit mask)
-    uint32_t tcm;        // controls on thread bits (local-bit mask)
+    uint32_t tcm;        // controls on thread bits (mask over thread-index bits)
     uint32_t lcm;        // RO_LDS: controls inside the tile (local-bit mask)
     uint32_t src;        // source gate index
     uint64_t gcm;        // controls outside the tile (qubit mask)

@@ -186,6 +186,11 @@ struct RegPlanner {
             if (ct[i] == lb) return i;
         return -1;
     }
+    int thread_index(int lb) const {  // thread-index bit holding local bit lb
+        for (int i = 0; i < TB; i++)
+            if (ct[i] == lb) return i;
+        return -1;
+    }
     bool io_ok() const {
         for (int k = 0; k < L; k++)
             if (ct[k] != lbit[k]) return false;
@@ -296,7 +301,7 @@ struct RegPlanner {
             const int lb = lbit[q];
             if (lb < 0) op.gcm |= uint64_t(1) << q;
             else if (reg_index(lb) >= 0) op.rcm |= 1u << reg_index(lb);
-            else op.tcm |= 1u << lb;
+            else op.tcm |= 1u << thread_index(lb);
         }
     }
     Status emit(const Lowered &G, uint32_t src) {
@@ -321,7 +326,7 @@ struct RegPlanner {
             const int lb = lbit[G.t0];
             if (lb < 0) { op.dloc = 2; op.dbit = (int)G.t0; }
             else if (reg_index(lb) >= 0) { op.dloc = 0; op.dbit = reg_index(lb); }
-            else { op.dloc = 1; op.dbit = lb; }
+            else { op.dloc = 1; op.dbit = thread_index(lb); }
             controls(G.ctrl_mask, op);
             if (op.kind == RO_Z && op.tcm == 0 && __builtin_popcount(op.rcm) == 1) {
                 op.kind = RO_ZC1;
@@ -417,7 +422,7 @@ struct RegPlanner {
                 last = (int)out.size();
             }
         } else if (op.kind <= RO_Y || op.kind == RO_XC1 || op.kind == RO_RRS || op.kind == RO_RXS) {  // non-diagonal
-            if (op.tgt >= kLaneTarget) last_diag[1][ct[op.tgt - kLaneTarget]] = -1;
+            if (op.tgt >= kLaneTarget) last_diag[1][op.tgt - kLaneTarget] = -1;  // thread-index key
             else last_diag[0][op.tgt] = -1;
         } else if (op.kind != RO_ZC1) {
             forget_diagonals();

@@ -482,7 +482,7 @@ __device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t 
 }
 
 template <int KIND, bool CT, int RB, int E, typename T, typename R>
-__device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t tl, uint64_t base, uint32_t rcm, bool tok) {
+__device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t tid, uint64_t base, uint32_t rcm, bool tok) {
     const T p0 = mk<T>(coef<R>(op, 0), coef<R>(op, 1)), p1 = mk<T>(coef<R>(op, 2), coef<R>(op, 3));
     if (op.dloc == 0) {
 #define QI_RB(BB) case BB: if constexpr (BB < RB) diag_reg<BB, KIND, CT, E, T, R>(v, p0, p1, rcm, tok); break;
@@ -490,7 +490,7 @@ __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t
 #undef QI_RB
         return;
     }
-    const bool one = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+    const bool one = op.dloc == 1 ? ((tid >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
     diag_thr<KIND, CT, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
@@ -584,7 +584,7 @@ template <int E> __device__ __forceinline__ void xc1_op(Regs<float, E> &v, const
 #undef QI_RB
 }
 template <int E>
-__device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint32_t tl, uint64_t base) {
+__device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint32_t tid, uint64_t base) {
     constexpr int RB = __builtin_ctz(E);
     const int c = op.lp0;
     if (op.dloc == 0) {
@@ -593,7 +593,7 @@ __device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint3
 #undef QI_RB
         return;
     }
-    const bool one = op.dloc == 1 ? ((tl >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+    const bool one = op.dloc == 1 ? ((tid >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
     const uint32_t sgn = one ? 0x80000000u : 0u;
     switch (c) {
     case 0: if constexpr (0 < RB) z_thr_c1<0, E>(v, sgn); break;
@@ -806,12 +806,12 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
 // ------------------------------------------------------------ the pass --
 // One in-register op on the thread's share of the tile.
 template <typename R, int K, int TB>
-__device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegOp &op, uint64_t base, uint32_t tl,
+__device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegOp &op, uint64_t base, uint32_t tid,
                                          uint32_t lane) {
     using T = typename V2<R>::T;
     constexpr int RB = K - TB, E = 1 << RB;
     const bool tc = (op.tcm | op.rcm) != 0;   // controls inside the tile (uniform)
-    const bool tok = (tl & op.tcm) == op.tcm;  // this thread's thread-held controls
+    const bool tok = (tid & op.tcm) == op.tcm;  // this thread's thread-held controls
 #define QI_OP(KD, FN)                                                                     \
     case KD:                                                                              \
         if (tc) FN<KD, true, RB, E, T, R>(v, op, ARGS, op.rcm, tok);                      \
@@ -821,7 +821,7 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
     switch (op.kind) {
         QI_OP(RO_X, pair_op) QI_OP(RO_RR, pair_op) QI_OP(RO_GEN, pair_op) QI_OP(RO_RXI, pair_op) QI_OP(RO_Y, pair_op)
 #undef ARGS
-#define ARGS tl, base
+#define ARGS tid, base
         QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
 #undef ARGS
     case RO_XC1:
@@ -829,8 +829,8 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         else pair_op<RO_X, true, RB, E, T, R>(v, op, lane, op.rcm, tok);
         break;
     case RO_ZC1:
-        if constexpr (sizeof(R) == 4) zc1_op<E>(v, op, tl, base);
-        else diag_op<RO_Z, true, RB, E, T, R>(v, op, tl, base, op.rcm, tok);
+        if constexpr (sizeof(R) == 4) zc1_op<E>(v, op, tid, base);
+        else diag_op<RO_Z, true, RB, E, T, R>(v, op, tid, base, op.rcm, tok);
         break;
     case RO_RRS:
         if constexpr (sizeof(R) == 4) shear_op<false, E>(v, op);
@@ -906,7 +906,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
             if (kind >= RO_TRANSPOSE) break;
             nxt = ops[pass.op0 + o + 1];
             if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-            apply_op<R, K, TB>(v, op, base, tl, lane);
+            apply_op<R, K, TB>(v, op, base, tid, lane);
         }
         if (o >= pass.nops) break;
         // through LDS: a transpose to a new layout, or a gate on the LDS copy
@@ -988,12 +988,10 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
     constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
     const uint32_t lane = threadIdx.x;
     const uint64_t base = insert_zeros64(xcd_block_order(blockIdx.x, pass.chunk_log2), pass.qmask);
-    uint32_t tl = 0;  // local index bits held by this lane (the layout at load)
     uint64_t at_in = 0, at_out = 0;
 #pragma unroll
     for (int i = 0; i < TB; i++) {
         const uint32_t bit = (lane >> i) & 1u;
-        tl |= bit << pass.lay_t[i];
         at_in |= uint64_t(bit) << pass.qt_in[i];
         at_out |= uint64_t(bit) << pass.qt_out[i];
     }
@@ -1016,12 +1014,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
         nxt = ops[pass.op0 + o + 1];
         if (op.kind >= RO_TRANSPOSE) continue;
         if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-        apply_op<R, K, TB>(v, op, base, tl, lane);
-        if (op.kind == RO_XLR || op.kind == RO_XLL) {  // the lane bits now hold other local bits
-            tl = 0;
-#pragma unroll
-            for (int i = 0; i < TB; i++) tl |= ((lane >> i) & 1u) << op.lay_t[i];
-        }
+        apply_op<R, K, TB>(v, op, base, lane, lane);
     }
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.
