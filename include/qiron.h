@@ -237,6 +237,18 @@ qi_status qi_sumop_apply(qi_state_t in, qi_state_t out, const qi_pauli_term *ter
 /* SumOp::expectation_value (pauli_string.rs:485-507): FP64 compensated sums. */
 qi_status qi_sumop_expectation(qi_state_t s, const qi_pauli_term *terms, size_t nterms,
                                double *re, double *im, qi_error *err);
+/* qi_sumop_expectation with no allocation of any kind: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it). The term
+ * table, block partials and group results live in a fixed static device
+ * array (one per device, 76 KiB, held until the process ends) instead
+ * of the context's growable buffers. At most 512 terms (more: InvalidArgument).
+ * Same kernels, same summation order and the same result bits as
+ * qi_sumop_expectation. Calls on one context are serialised by its lock;
+ * calls that overlap on one device from different contexts race on the
+ * array. */
+qi_status qi_sumop_expectation_noalloc(qi_state_t s, const qi_pauli_term *terms, size_t nterms,
+                                       double *re, double *im, qi_error *err);
 /* trotter_evolve_state (time_evolution.rs:140-166); order 1 = First, 2 = Second. */
 qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt,
                             uint64_t steps, int order, qi_error *err);

@@ -569,6 +569,11 @@ qi_status qi_ctx_create(int device, qi_ctx_t *out, qi_error *err) {
     C.host_pinned_words = 64;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) C.cu_count = prop.multiProcessorCount;
+    // the no-allocation scratch array's address on this device (the device is current)
+    {
+        Status ns = noalloc_scratch(device, &C.noalloc);
+        if (!ns.ok()) QI_RET(ns);
+    }
     // an unseeded context draws from an OS-seeded stream, as rand::rng() does
     C.rng = (uint64_t(std::random_device{}()) << 32) ^ uint64_t(std::random_device{}());
     *out = c.release();
@@ -962,6 +967,17 @@ static void diag_tables(const std::vector<std::pair<qi_pauli_term, cxd>> &run, s
         cur[b].members.push_back(i);
     }
     flush();
+}
+
+qi_status qi_sumop_expectation_noalloc(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double *re, double *im,
+                                       qi_error *err) {
+    QI_NULLCHK(s); QI_NULLCHK(re); QI_NULLCHK(im);
+    if (nterms && !terms) QI_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "terms is NULL"));
+    Status e;
+    for (size_t k = 0; k < nterms; k++)
+        if (!(e = check_mask(s->s.n, terms[k].x_mask | terms[k].z_mask)).ok()) QI_RET(e);
+    QI_LOCK(s->s.ctx);
+    QI_RET(sumop_expectation_noalloc(s->s, terms, nterms, re, im, s->s.ctx->stream));
 }
 
 qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nterms, double dt, uint64_t steps, int order,

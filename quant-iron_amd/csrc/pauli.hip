@@ -182,6 +182,16 @@ __global__ __launch_bounds__(256) void k_diag_tables(typename V2<R>::T *psi, uin
     }
 }
 
+// Sequential sum of the group results in group order (the order the host
+// sums them in sumop_expectation), so both entry points return the same bits.
+__global__ void k_fold_groups(const double *__restrict__ g, int ng, double *__restrict__ out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < ng; k++) { sr += g[2 * k]; si += g[2 * k + 1]; }
+    out[0] = sr;
+    out[1] = si;
+}
+
 TermK make_term(const qi_pauli_term &p) {
     TermK t{};
     t.x = p.x_mask; t.z = p.z_mask;
@@ -308,6 +318,95 @@ Status sumop_expectation(State &s, const qi_pauli_term *terms, size_t nt, double
     double sr = 0.0, si = 0.0;
     for (size_t g = 0; g < ng; g++) { sr += C.host_pinned[2 * g]; si += C.host_pinned[2 * g + 1]; }
     *re = sr; *im = si;
+    return Status{};
+}
+
+// ---------------------------------------------------- no-allocation variant --
+// qi_sumop_expectation_noalloc keeps its term table, block partials and group
+// results in this fixed device array instead of the context's growable
+// buffers: no device allocation on any call. Each device that the library
+// runs on holds its own copy until the process ends. Every location a call
+// reads is written earlier in the same call, so nothing needs clearing; calls
+// that overlap on one device (different contexts, or threads without the
+// context's lock) race on it.
+namespace {
+constexpr int kNoAllocBlocks = 1024;
+struct NoAllocScratch {
+    TermK terms[kNoAllocMaxTerms];
+    double part[kNoAllocBlocks * 4];
+    double group[kNoAllocMaxTerms * 2];
+    double total[2];
+};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[sizeof(NoAllocScratch)];
+
+Status noalloc_scratch(int device, void **out) {
+    static std::mutex mu;
+    static void *ptrs[64] = {nullptr};
+    if (device < 0 || device >= 64) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "device index");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptrs[device]) {
+        // the host-side name of `scratch` is not a device address: take the
+        // address of this device's copy once, with the device current
+        void *p = nullptr;
+        QI_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ptrs[device] = p;
+    }
+    *out = ptrs[device];
+    return Status{};
+}
+
+Status sumop_expectation_noalloc(State &s, const qi_pauli_term *terms, size_t nt, double *re, double *im, hipStream_t st) {
+    void *base = s.ctx->noalloc;
+    if (!base) return make_err(QI_ERR_DEVICE, 0, 0, 0, 0, "no scratch address for this device");
+    if (nt > (size_t)kNoAllocMaxTerms)
+        return make_err(QI_ERR_INVALID_ARGUMENT, nt, kNoAllocMaxTerms, 0, 0, "more terms than the no-allocation maximum (512)");
+    *re = 0.0; *im = 0.0;
+    if (nt == 0) return Status{};
+    NoAllocScratch *sc = reinterpret_cast<NoAllocScratch *>(base);
+    // group by x mask in first-appearance order, as sumop_expectation does (host stack only)
+    uint64_t xs[kNoAllocMaxTerms];
+    uint16_t gid[kNoAllocMaxTerms];
+    uint32_t cnt[kNoAllocMaxTerms + 1];
+    TermK flat[kNoAllocMaxTerms];
+    size_t ng = 0;
+    for (size_t k = 0; k < nt; k++) {
+        size_t g = 0;
+        while (g < ng && xs[g] != terms[k].x_mask) g++;
+        if (g == ng) { xs[ng] = terms[k].x_mask; cnt[ng] = 0; ng++; }
+        gid[k] = (uint16_t)g;
+        cnt[g]++;
+    }
+    uint32_t off[kNoAllocMaxTerms + 1];
+    off[0] = 0;
+    for (size_t g = 0; g < ng; g++) off[g + 1] = off[g] + cnt[g];
+    uint32_t fill[kNoAllocMaxTerms];
+    for (size_t g = 0; g < ng; g++) fill[g] = off[g];
+    for (size_t k = 0; k < nt; k++) flat[fill[gid[k]]++] = make_term(terms[k]);
+    QI_HIP_TRY(hipMemcpyAsync(sc->terms, flat, nt * sizeof(TermK), hipMemcpyHostToDevice, st));
+    const uint64_t dim = uint64_t(1) << s.n;
+    const int blocks = (int)std::min<uint64_t>(kNoAllocBlocks, (dim + 2047) / 2048);
+    for (size_t g = 0; g < ng; g++) {
+        const uint64_t x = xs[g];
+        const uint64_t items = x ? dim / 2 : dim;
+        const unsigned pivot = x ? (unsigned)__builtin_ctzll(x) : 0;
+        if (s.prec == QI_C128)
+            hipLaunchKernelGGL(k_expect_group<double2>, dim3(blocks), dim3(256), 0, st, (const double2 *)s.d, items, x, pivot,
+                               (const TermK *)(sc->terms + off[g]), (int)(off[g + 1] - off[g]), sc->part);
+        else
+            hipLaunchKernelGGL(k_expect_group<float2>, dim3(blocks), dim3(256), 0, st, (const float2 *)s.d, items, x, pivot,
+                               (const TermK *)(sc->terms + off[g]), (int)(off[g + 1] - off[g]), sc->part);
+        // the partials are reused by the next group: this stream orders it after the fold
+        hipLaunchKernelGGL(k_reduce_partials<2>, dim3(1), dim3(256), 0, st, (const double *)sc->part, blocks, sc->group + 2 * g);
+    }
+    hipLaunchKernelGGL(k_fold_groups, dim3(1), dim3(64), 0, st, (const double *)sc->group, (int)ng, sc->total);
+    QI_HIP_TRY(hipGetLastError());
+    double res[2];
+    QI_HIP_TRY(hipMemcpyAsync(res, sc->total, sizeof res, hipMemcpyDeviceToHost, st));
+    QI_HIP_TRY(hipStreamSynchronize(st));  // also retires the copy from `flat` on this stack
+    *re = res[0];
+    *im = res[1];
     return Status{};
 }
 

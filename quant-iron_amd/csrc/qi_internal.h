@@ -71,6 +71,7 @@ struct Ctx {
     // SplitMix64 stream for measurements whose uniform draw is not injected
     // (the reference draws from rand::rng(), state.rs:601-602)
     uint64_t rng = 0;
+    void *noalloc = nullptr;  // this device's copy of the no-allocation scratch array (pauli.hip)
     // cached graph of the last captured gate list
     hipGraphExec_t graph_exec = nullptr;
     std::vector<unsigned char> graph_key;
@@ -150,6 +151,12 @@ struct DiagPass {
     DiagTab tab[kDiagTabMax];
 };
 Status launch_diag_tables(State &s, const DiagPass &p, const double2 *dtables, hipStream_t st);
+// The no-allocation expectation: at most kNoAllocMaxTerms terms, all scratch in
+// a static device array whose per-device address noalloc_scratch returns.
+constexpr int kNoAllocMaxTerms = 512;
+Status noalloc_scratch(int device, void **out);
+Status sumop_expectation_noalloc(State &s, const qi_pauli_term *terms, size_t nt, double *re, double *im,
+                                 hipStream_t st);
 // Expectation of a list of terms; result (re, im) summed in FP64 compensated.
 Status sumop_expectation(State &s, const qi_pauli_term *terms, size_t nt, double *re, double *im,
                          hipStream_t st);

@@ -115,6 +115,7 @@ SIGNATURES = {
     "qi_pauli_exp_neg_i_dt": [P, C.POINTER(QiPauliTerm), D, ERR],
     "qi_sumop_apply": [P, P, C.c_void_p, C.c_size_t, ERR],
     "qi_sumop_expectation": [P, C.c_void_p, C.c_size_t, DP, DP, ERR],
+    "qi_sumop_expectation_noalloc": [P, C.c_void_p, C.c_size_t, DP, DP, ERR],
     "qi_trotter_evolve": [P, C.c_void_p, C.c_size_t, D, U64, C.c_int, ERR],
     "qi_inner_product": [P, P, DP, DP, ERR],
     "qi_norm_sqr": [P, DP, ERR],
