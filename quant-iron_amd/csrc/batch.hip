@@ -518,7 +518,11 @@ static ETerm make_eterm(const qi_pauli_term &p) {
 // F(e) = e ^ (bit c of e) << t, and the state to new[F(e)] = old[e]; for the
 // run F_k ... F_1 the gather map is G = F_1 o ... o F_k (each F its own
 // inverse), built column by column. In place; returns the new length.
-static size_t perm_cnots(TileOp *ops, size_t n, int nloc) {
+// complex128 ops are returned unchanged: the kPerm case is compiled for
+// complex64 kernels only (the complex128 gather buffer would not stay in
+// registers), so no kPerm op may reach a complex128 kernel.
+static size_t perm_cnots(TileOp *ops, size_t n, int nloc, bool c128) {
+    if (c128) return n;
     const auto cnot = [](const TileOp &o) {
         return o.kind == LK_X && !o.outside && o.gcm == 0 && __builtin_popcount(o.lcm) == 1 && (int32_t)o.pad < 0 &&
                o.slot2 < 0;
@@ -645,7 +649,7 @@ Status batch_plan(Batch &B) {
                 ops.push_back(op);
             }
             const size_t nf = pair_ry(ops.data() + P.op0, fold_rz(ops.data() + P.op0, lw.size()));
-            P.nops = (uint32_t)(c128 ? nf : perm_cnots(ops.data() + P.op0, nf, (int)P.n));
+            P.nops = (uint32_t)perm_cnots(ops.data() + P.op0, nf, (int)P.n, c128);
             ops.resize(P.op0 + P.nops);
             continue;
         }
@@ -662,8 +666,7 @@ Status batch_plan(Batch &B) {
             TilePass tp = s.tile;
             TileOp *slice = plan.tile_ops.data() + tp.op0;
             const size_t nf = pair_ry(slice, fold_rz(slice, (size_t)tp.nops));
-            // complex64 only: the complex128 gather buffer would not stay in registers
-            const size_t n = c128 ? nf : perm_cnots(slice, nf, B.K);
+            const size_t n = perm_cnots(slice, nf, B.K, c128);
             tp.op0 = (int32_t)packed.size();
             tp.nops = (int32_t)n;
             packed.insert(packed.end(), plan.tile_ops.begin() + s.tile.op0, plan.tile_ops.begin() + s.tile.op0 + n);

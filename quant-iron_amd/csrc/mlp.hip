@@ -10,7 +10,7 @@
 // loads are in registers while the current stage feeds the matrix cores. The
 // instruction's operands: A row = lane % 16, k = 8 (lane / 16) + v (8 BF16 per
 // lane, one 16-byte LDS read); B col = lane % 16, same k; D row = 4 (lane / 16)
-// + r, col = lane % 16.
+// + r, col = lane % 16 (the GEMM feeds B first, so D is the tile's transpose).
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -40,15 +40,42 @@ __device__ __host__ __forceinline__ float bf2f(bf16 h) {
     return x;
 }
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf(x) by Abramowitz & Stegun 7.1.26 (absolute error below 1.5e-7, under
+// the BF16 rounding of the activations): 1 - t (a1 + t (a2 + ...)) e^(-x^2),
+// t = 1 / (1 + p |x|), odd in x. Returns erf(x) and e^(-x^2) (the GELU
+// derivative reuses the exponential).
+__device__ __forceinline__ float erf_as(float x, float &ex2) {
+    const float ax = fabsf(x);
+    const float t = __frcp_rn(1.0f + 0.3275911f * ax);
+    float p = 1.061405429f;
+    p = fmaf(p, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    ex2 = __expf(-ax * ax);
+    const float e = 1.0f - p * t * ex2;
+    return copysignf(e, x);
+}
+// GELU(z) = 0.5 z (1 + erf(z / sqrt 2)); gelu'(z) = 0.5 (1 + erf(z / sqrt 2)) + z phi(z).
+__device__ __forceinline__ float gelu(float z) {
+    float ex2;
+    return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f, ex2));
+}
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    float ex2;  // e^(-z^2 / 2)
+    const float e = erf_as(z * 0.70710678118654752f, ex2);
+    return 0.5f * (1.0f + e) + z * 0.39894228040143268f * ex2;
 }
 
-enum Epi { EPI_GELU = 0, EPI_STORE = 1, EPI_GELU_BWD = 2 };
+// EPI_GELU_HEAD (prediction only): the last hidden layer's GELU output is not
+// stored; its dot product with the head weights (passed as Zin) over this
+// wave's 64 columns goes to slot 2 bx + wc of the row's partials (Cf, 2 *
+// gridDim.x per row), summed with the head bias by k_head_fold.
+enum Epi { EPI_GELU = 0, EPI_STORE = 1, EPI_GELU_BWD = 2, EPI_GELU_HEAD = 3 };
 
 constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;  // LDS row: 72 BF16 = 144 bytes (9 x 16)
-constexpr int kChunks = TM * TK / 8 / 256;                // 16-byte chunks per thread per operand stage
+constexpr int kChunks = TM * TK / 8 / 256;
+constexpr int kMaxSplits = 16;  // K slices of a weight-gradient GEMM                // 16-byte chunks per thread per operand stage
 
 // One operand tile (128 rows of the output dimension x 64 of K) as 16-byte
 // chunks in registers. KC (K contiguous in memory: X row-major M x K, or W^T):
@@ -105,30 +132,45 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[LDK], const uint4 (&reg)[kC
 template <bool TA, bool TB, int EPI>
 __global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B, int ldb,
                                               int M, int N, int K, float *__restrict__ Cf, bf16 *__restrict__ Cb,
-                                              int ldc, const float *__restrict__ bias, const float *__restrict__ Zin) {
+                                              int ldc, const float *__restrict__ bias, const float *__restrict__ Zin,
+                                              int kper) {
     __shared__ __attribute__((aligned(16))) bf16 As[2][TM][LDK];
     __shared__ __attribute__((aligned(16))) bf16 Bs[2][TN][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    // Workgroup w runs on XCD w % 8. The N tiles of one M tile read the same
+    // A rows: they go to one XCD (consecutive r below), so A crosses HBM once
+    // and the other N tiles find it in that XCD's L2.
+    int bx = blockIdx.x, by = blockIdx.y;
+    if ((gridDim.y & 7) == 0) {
+        const uint32_t w = blockIdx.y * gridDim.x + blockIdx.x, r = w >> 3;
+        bx = int(r % gridDim.x);
+        by = int((r / gridDim.x) * 8 + (w & 7));
+    }
+    const int m0 = by * TM, n0 = bx * TN;
+    // split K: slice z takes K range [kb, ke) and writes its partial product
+    // to slice z of the output (EPI_STORE into an FP32 workspace)
+    const int kb = blockIdx.z * kper, ke = min(K, kb + kper);
+    if (gridDim.z > 1) Cf += size_t(blockIdx.z) * M * ldc;
     const int wr = wave >> 1, wc = wave & 1;
     floatx4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0, 0, 0, 0};
-    uint4 ra[kChunks], rb[kChunks];
-    load_tile<!TA>(A, lda, M, K, m0, 0, ra);
-    load_tile<TB>(B, ldb, N, K, n0, 0, rb);
-    store_tile<!TA>(As[0], ra);
-    store_tile<TB>(Bs[0], rb);
+    // Stage s's global loads are issued two stages ahead (register sets 0 and
+    // 1 alternate, the loop runs two stages per trip so the set is static),
+    // then stored to LDS buffer s % 2 one stage ahead of its use.
+    // The stage count is rounded up to even: stages past K load zeros.
+    uint4 ra0[kChunks], rb0[kChunks], ra1[kChunks], rb1[kChunks];
+    const int nst = ((ke - kb + 2 * TK - 1) / (2 * TK)) * 2;
+    load_tile<!TA>(A, lda, M, ke, m0, kb, ra0);
+    load_tile<TB>(B, ldb, N, ke, n0, kb, rb0);
+    load_tile<!TA>(A, lda, M, ke, m0, kb + TK, ra1);
+    load_tile<TB>(B, ldb, N, ke, n0, kb + TK, rb1);
+    store_tile<!TA>(As[0], ra0);
+    store_tile<TB>(Bs[0], rb0);
     __syncthreads();
-    const int nst = (K + TK - 1) / TK;
-    for (int s = 0; s < nst; s++) {
-        const int cur = s & 1;
-        if (s + 1 < nst) {  // next stage in flight while this one computes
-            load_tile<!TA>(A, lda, M, K, m0, (s + 1) * TK, ra);
-            load_tile<TB>(B, ldb, N, K, n0, (s + 1) * TK, rb);
-        }
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 32) {
             bf16x8 a[4], b[4];
@@ -141,38 +183,103 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int ld
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
-        if (s + 1 < nst) {
-            store_tile<!TA>(As[cur ^ 1], ra);
-            store_tile<TB>(Bs[cur ^ 1], rb);
+    };
+    for (int s = 0; s < nst; s += 2) {
+        // even stage s: LDS buffer 0; set 0 (stage s, stored) takes stage s + 2
+        if (s + 2 < nst) {
+            load_tile<!TA>(A, lda, M, ke, m0, kb + (s + 2) * TK, ra0);
+            load_tile<TB>(B, ldb, N, ke, n0, kb + (s + 2) * TK, rb0);
+        }
+        compute(0);
+        store_tile<!TA>(As[1], ra1);
+        store_tile<TB>(Bs[1], rb1);
+        __syncthreads();
+        // odd stage s + 1: LDS buffer 1; set 1 takes stage s + 3
+        if (s + 3 < nst) {
+            load_tile<!TA>(A, lda, M, ke, m0, kb + (s + 3) * TK, ra1);
+            load_tile<TB>(B, ldb, N, ke, n0, kb + (s + 3) * TK, rb1);
+        }
+        compute(1);
+        if (s + 2 < nst) {
+            store_tile<!TA>(As[0], ra0);
+            store_tile<TB>(Bs[0], rb0);
         }
         __syncthreads();
+    }
+    // acc[i][j] holds C^T of its 16 x 16 tile (B fed as the first operand):
+    // this lane has row 16 i + lane % 16 and columns 16 j + 4 (lane / 16) + r,
+    // four consecutive columns stored as one 16-byte FP32 / 8-byte BF16 access
+    const bool vec = (ldc & 3) == 0;
+    if constexpr (EPI == EPI_GELU_HEAD) {
+        const int slots = 2 * gridDim.x;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = m0 + wr * 64 + i * 16 + (lane & 15);
+            float hs = 0.f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int col = n0 + wc * 64 + j * 16 + 4 * (lane >> 4);
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (col + r < N) hs += gelu(acc[i][j][r] + bias[col + r]) * Zin[col + r];
+            }
+            hs += __shfl_xor(hs, 16);
+            hs += __shfl_xor(hs, 32);
+            if (lane < 16 && row < M) Cf[size_t(row) * slots + 2 * bx + wc] = hs;
+        }
+        return;
     }
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++)
+        for (int j = 0; j < 4; j++) {
+            const int row = m0 + wr * 64 + i * 16 + (lane & 15);
+            const int col = n0 + wc * 64 + j * 16 + 4 * (lane >> 4);
+            if (row >= M) continue;
+            const size_t o = size_t(row) * ldc + col;
+            float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+            if (vec && col + 3 < N) {
+                if (EPI == EPI_GELU) {
+                    const float4 bb = *reinterpret_cast<const float4 *>(bias + col);
+                    v[0] += bb.x; v[1] += bb.y; v[2] += bb.z; v[3] += bb.w;
+                    if (Cf) *reinterpret_cast<float4 *>(Cf + o) = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) v[r] = gelu(v[r]);
+                } else if (EPI == EPI_GELU_BWD) {
+                    const float4 z = *reinterpret_cast<const float4 *>(Zin + o);
+                    v[0] *= gelu_grad(z.x); v[1] *= gelu_grad(z.y); v[2] *= gelu_grad(z.z); v[3] *= gelu_grad(z.w);
+                    *reinterpret_cast<float4 *>(Cf + o) = make_float4(v[0], v[1], v[2], v[3]);
+                } else if (Cf) {
+                    *reinterpret_cast<float4 *>(Cf + o) = make_float4(v[0], v[1], v[2], v[3]);
+                }
+                if (EPI != EPI_STORE || Cb) {
+                    uint2 pk;
+                    pk.x = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+                    pk.y = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+                    *reinterpret_cast<uint2 *>(Cb + o) = pk;
+                }
+                continue;
+            }
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = m0 + wr * 64 + i * 16 + 4 * (lane >> 4) + r;
-                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
-                if (row >= M || col >= N) continue;
-                const size_t o = size_t(row) * ldc + col;
-                float v = acc[i][j][r];
+                if (col + r >= N) continue;
+                float x = v[r];
                 if (EPI == EPI_GELU) {
-                    v += bias[col];
-                    if (Cf) Cf[o] = v;
-                    Cb[o] = f2bf(gelu(v));
+                    x += bias[col + r];
+                    if (Cf) Cf[o + r] = x;
+                    Cb[o + r] = f2bf(gelu(x));
                 } else if (EPI == EPI_GELU_BWD) {
-                    v *= gelu_grad(Zin[o]);
-                    Cf[o] = v;
-                    Cb[o] = f2bf(v);
+                    x *= gelu_grad(Zin[o + r]);
+                    Cf[o + r] = x;
+                    Cb[o + r] = f2bf(x);
                 } else {
-                    if (Cf) Cf[o] = v;
-                    if (Cb) Cb[o] = f2bf(v);
+                    if (Cf) Cf[o + r] = x;
+                    if (Cb) Cb[o + r] = f2bf(x);
                 }
             }
+        }
 }
 
 // y[r] = H[r,:] . w + b ; with targets: dy = 2 (y - t) / B, loss partials, and
@@ -230,20 +337,30 @@ __global__ __launch_bounds__(256) void k_colsum_fold(const float *__restrict__ p
     out[c] = s;
 }
 
-// dw[k] = sum_r H[r,k] dy[r]  (head weights), db = sum_r dy[r]
-__global__ __launch_bounds__(256) void k_head_grad(const bf16 *__restrict__ H, const float *__restrict__ dy, int rows, int hid,
-                                                   float *__restrict__ dw, float *__restrict__ db) {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k < hid) {
-        float s = 0.f;
-        for (int r = 0; r < rows; r++) s += bf2f(H[size_t(r) * hid + k]) * dy[r];
-        dw[k] = s;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        float s = 0.f;
-        for (int r = 0; r < rows; r++) s += dy[r];
-        *db = s;
-    }
+// y[r] = b + the row's head partials in slot order (EPI_GELU_HEAD).
+__global__ __launch_bounds__(256) void k_head_fold(const float *__restrict__ part, int slots, int rows,
+                                                   const float *__restrict__ bp, float *__restrict__ y) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= rows) return;
+    float s = 0.f;
+    for (int k = 0; k < slots; k++) s += part[size_t(r) * slots + k];
+    y[r] = s + *bp;
+}
+
+// Partial sums of dw[k] = sum_r H[r,k] dy[r] (head weights) over chunks of 256
+// rows, as k_colsum_part (folded by k_colsum_fold).
+__global__ __launch_bounds__(256) void k_head_grad_part(const bf16 *__restrict__ H, const float *__restrict__ dy, int rows,
+                                                        int hid, float *__restrict__ part) {
+    __shared__ float sh[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * 256;
+    float s = 0.f;
+    if (c < hid)
+        for (int r = r0 + g; r < r0 + 256 && r < rows; r += 4) s += bf2f(H[size_t(r) * hid + c]) * dy[r];
+    sh[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0 && c < hid)
+        part[size_t(blockIdx.y) * hid + c] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
 
 // Adam on FP32 masters; refreshes the BF16 copy (nullable) of each parameter.
@@ -276,6 +393,26 @@ __global__ __launch_bounds__(256) void k_transpose_bf16(const float *__restrict_
     __syncthreads();
     for (int k = ty; k < 32; k += 8)
         if (c0 + k < cols && r0 + tx < rows) y[size_t(c0 + k) * rows + r0 + tx] = f2bf(t[tx][k]);
+}
+
+// y = x^T for a BF16 matrix x (rows x cols), through a 32 x 32 LDS tile.
+__global__ __launch_bounds__(256) void k_transpose_b16(const bf16 *__restrict__ x, int rows, int cols, bf16 *__restrict__ y) {
+    __shared__ bf16 t[32][34];
+    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32, tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8)
+        if (r0 + k < rows && c0 + tx < cols) t[k][tx] = x[size_t(r0 + k) * cols + c0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8)
+        if (c0 + k < cols && r0 + tx < rows) y[size_t(c0 + k) * rows + r0 + tx] = t[tx][k];
+}
+
+// out[i] = sum over z of ws[z count + i], z in order (split-K partials).
+__global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ ws, int splits, size_t count, float *__restrict__ out) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < count; i += size_t(gridDim.x) * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < splits; z++) s += ws[size_t(z) * count + i];
+        out[i] = s;
+    }
 }
 
 // gather rows idx[0..rows) of a BF16 matrix (cols wide) into a contiguous block
@@ -318,22 +455,28 @@ __device__ __forceinline__ double sm64_uniform(uint64_t seed, uint64_t k) {
 // Network inputs of ansatz candidates c0 .. c0 + rows drawn from a SplitMix64
 // stream: candidate c's angles are draws draw0 + c nang + j + 1 (theta = 2 pi u),
 // its couplings draws draw0 + count nang + 3 c + i + 1 (J = 0.5 + u). Row
-// layout: cos theta_j, sin theta_j, J_i - 1, zero padding to k_in.
+// layout: cos theta_j, sin theta_j, J_i - 1, zero padding to k_in. One item
+// per (row, angle), the last item of a row writes its couplings and padding.
+// The inputs are rounded to BF16, so the trigonometry runs in FP32 (the host
+// regenerates the kept candidates' angles in FP64 for the exact evaluation).
 __global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t draw0, uint64_t count, uint64_t c0, int rows,
                                                          int nang, int k_in, bf16 *__restrict__ X) {
-    const size_t total = size_t(rows) * k_in;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
-        const uint64_t r = i / k_in, col = i % k_in, c = c0 + r;
-        float f = 0.f;
-        if (col < uint64_t(2 * nang)) {
-            const uint64_t j = col < uint64_t(nang) ? col : col - nang;
-            const double th = 2.0 * 3.14159265358979323846 * sm64_uniform(seed, draw0 + c * nang + j + 1);
-            f = (float)(col < uint64_t(nang) ? cos(th) : sin(th));
-        } else if (col < uint64_t(2 * nang + 3)) {
-            const uint64_t q = col - 2 * nang;
-            f = (float)(0.5 + sm64_uniform(seed, draw0 + count * nang + 3 * c + q + 1) - 1.0);
+    const uint32_t per = uint32_t(nang) + 1, items = uint32_t(rows) * per;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
+        const uint32_t r = i / per, j = i - r * per;
+        const uint64_t c = c0 + r;
+        bf16 *row = X + size_t(r) * k_in;
+        if (j < uint32_t(nang)) {
+            const float th = (float)(2.0 * 3.14159265358979323846 * sm64_uniform(seed, draw0 + c * nang + j + 1));
+            float sn, cs;
+            sincosf(th, &sn, &cs);
+            row[j] = f2bf(cs);
+            row[nang + j] = f2bf(sn);
+        } else {
+            for (int q = 0; q < 3; q++)
+                row[2 * nang + q] = f2bf((float)(0.5 + sm64_uniform(seed, draw0 + count * nang + 3 * c + q + 1) - 1.0));
+            for (int k = 2 * nang + 3; k < k_in; k++) row[k] = f2bf(0.f);
         }
-        X[i] = f2bf(f);
     }
 }
 
@@ -341,28 +484,35 @@ __global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t
 // proposal): theta_j = theta0_j + sigma z_j, z_j a standard normal by the
 // Box-Muller transform of draws draw0 + 2 (c nang + j) + 1 and + 2 of the
 // stream `seed` (z = sqrt(-2 ln(1 - u1)) cos(2 pi u2)); the couplings are the
-// fixed J of the round. Same row layout as k_ansatz_features.
+// fixed J of the round. Same row layout and items as k_ansatz_features.
 __global__ __launch_bounds__(256) void k_ansatz_features_local(uint64_t seed, uint64_t draw0, uint64_t c0, int rows, int nang,
                                                                int k_in, const double *__restrict__ theta0, double sigma,
                                                                double j0, double j1, double j2, bf16 *__restrict__ X) {
-    const size_t total = size_t(rows) * k_in;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += size_t(gridDim.x) * blockDim.x) {
-        const uint64_t r = i / k_in, col = i % k_in, c = c0 + r;
-        float f = 0.f;
-        if (col < uint64_t(2 * nang)) {
-            const uint64_t j = col < uint64_t(nang) ? col : col - nang;
+    const uint32_t per = uint32_t(nang) + 1, items = uint32_t(rows) * per;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
+        const uint32_t r = i / per, j = i - r * per;
+        const uint64_t c = c0 + r;
+        bf16 *row = X + size_t(r) * k_in;
+        if (j < uint32_t(nang)) {
             const uint64_t d = draw0 + 2 * (c * nang + j);
             const double u1 = sm64_uniform(seed, d + 1), u2 = sm64_uniform(seed, d + 2);
-            const double z = sqrt(-2.0 * log(1.0 - u1)) * cos(2.0 * 3.14159265358979323846 * u2);
-            const double th = theta0[j] + sigma * z;
-            f = (float)(col < uint64_t(nang) ? cos(th) : sin(th));
-        } else if (col < uint64_t(2 * nang + 3)) {
-            const uint64_t q = col - 2 * nang;
-            f = (float)((q == 0 ? j0 : q == 1 ? j1 : j2) - 1.0);
+            const float z = sqrtf(-2.f * logf((float)(1.0 - u1))) * cosf(6.2831853071795865f * (float)u2);
+            float sn, cs;
+            sincosf((float)(theta0[j] + sigma * (double)z), &sn, &cs);
+            row[j] = f2bf(cs);
+            row[nang + j] = f2bf(sn);
+        } else {
+            row[2 * nang] = f2bf((float)(j0 - 1.0));
+            row[2 * nang + 1] = f2bf((float)(j1 - 1.0));
+            row[2 * nang + 2] = f2bf((float)(j2 - 1.0));
+            for (int k = 2 * nang + 3; k < k_in; k++) row[k] = f2bf(0.f);
         }
-        X[i] = f2bf(f);
     }
 }
+
+// Rows per launch group of the candidate screens (activations of this many
+// rows stay allocated: about 1.1 GB at 512 hidden units).
+constexpr size_t kScreenChunk = 262144;
 
 // ------------------------------------------------------------------ host --
 struct Mlp {
@@ -379,6 +529,8 @@ struct Mlp {
     bf16 *X = nullptr, *H[3] = {nullptr, nullptr, nullptr}, *dZb[3] = {nullptr, nullptr, nullptr};
     float *Z[3] = {nullptr, nullptr, nullptr}, *dZf[3] = {nullptr, nullptr, nullptr};
     float *y = nullptr, *t = nullptr, *dy = nullptr, *part = nullptr, *lossp = nullptr;
+    bf16 *T1 = nullptr, *T2 = nullptr;                             // transposed GEMM operands (weight gradients)
+    float *ws = nullptr;                                           // split-K partials
     double *lacc = nullptr;                                        // epoch loss accumulator
     float *yall = nullptr;                                         // predictions of a candidate screen
     size_t ycap = 0;
@@ -391,7 +543,7 @@ struct Mlp {
     size_t ndata = 0;
     ~Mlp() {
         void *ps[] = {w, m, v, g, wb, wt, X, H[0], H[1], H[2], dZb[0], dZb[1], dZb[2], Z[0], Z[1], Z[2], dZf[0], dZf[1], dZf[2],
-                      y, t, dy, part, lossp, dataX, dataT, perm, lacc, yall, theta0};
+                      y, t, dy, part, lossp, T1, T2, ws, dataX, dataT, perm, lacc, yall, theta0};
         for (void *p : ps)
             if (p) (void)hipFree(p);
     }
@@ -402,7 +554,7 @@ static Status mlp_alloc(Mlp &N, size_t rows) {
     void **ps[] = {(void **)&N.X, (void **)&N.H[0], (void **)&N.H[1], (void **)&N.H[2], (void **)&N.dZb[0], (void **)&N.dZb[1],
                    (void **)&N.dZb[2], (void **)&N.Z[0], (void **)&N.Z[1], (void **)&N.Z[2], (void **)&N.dZf[0],
                    (void **)&N.dZf[1], (void **)&N.dZf[2], (void **)&N.y, (void **)&N.t, (void **)&N.dy, (void **)&N.part,
-                   (void **)&N.lossp};
+                   (void **)&N.lossp, (void **)&N.T1, (void **)&N.T2, (void **)&N.ws};
     QI_HIP_TRY(hipDeviceSynchronize());
     for (void **p : ps)
         if (*p) { (void)hipFree(*p); *p = nullptr; }
@@ -419,6 +571,10 @@ static Status mlp_alloc(Mlp &N, size_t rows) {
     QI_HIP_TRY(hipMalloc(&N.dy, rows * sizeof(float)));
     QI_HIP_TRY(hipMalloc(&N.part, ((rows + 255) / 256) * std::max<size_t>(H, N.k_in) * sizeof(float)));
     QI_HIP_TRY(hipMalloc(&N.lossp, ((rows + 3) / 4) * sizeof(float)));
+    const size_t wide = std::max<size_t>(H, N.k_in);
+    QI_HIP_TRY(hipMalloc(&N.T1, rows * wide * sizeof(bf16)));
+    QI_HIP_TRY(hipMalloc(&N.T2, rows * H * sizeof(bf16)));
+    QI_HIP_TRY(hipMalloc(&N.ws, size_t(kMaxSplits) * wide * H * sizeof(float)));
     if (!N.lacc) QI_HIP_TRY(hipMalloc(&N.lacc, sizeof(double)));
     N.cap = rows;
     return Status{};
@@ -428,7 +584,25 @@ template <bool TA, bool TB, int EPI>
 static void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int Nn, int K, float *Cf, bf16 *Cb,
                  int ldc, const float *bias = nullptr, const float *Zin = nullptr) {
     dim3 grid((Nn + TN - 1) / TN, (M + TM - 1) / TM);
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, A, lda, B, ldb, M, Nn, K, Cf, Cb, ldc, bias, Zin);
+    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, A, lda, B, ldb, M, Nn, K, Cf, Cb, ldc, bias, Zin, K);
+}
+
+// C = A^T B for A (K x M) and B (K x N) row-major with a long K (weight
+// gradients: K = rows of the batch): both operands transposed first (BF16
+// copies in N.T1, N.T2), then K split over `splits` slices whose FP32
+// partials (N.ws) are summed in slice order into C (FP32, ldc = N).
+static void gemm_at_b(Mlp &N, hipStream_t st, const bf16 *A, int M, const bf16 *B, int Nn, int K, float *C) {
+    hipLaunchKernelGGL(k_transpose_b16, dim3((M + 31) / 32, (K + 31) / 32), dim3(256), 0, st, A, K, M, N.T1);
+    hipLaunchKernelGGL(k_transpose_b16, dim3((Nn + 31) / 32, (K + 31) / 32), dim3(256), 0, st, B, K, Nn, N.T2);
+    const int tiles = ((M + TM - 1) / TM) * ((Nn + TN - 1) / TN);
+    int splits = std::max(1, std::min({kMaxSplits, 256 / std::max(tiles, 1), K / (2 * TK)}));
+    const int kper = ((K + splits - 1) / splits + 2 * TK - 1) / (2 * TK) * (2 * TK);
+    splits = (K + kper - 1) / kper;
+    dim3 grid((Nn + TN - 1) / TN, (M + TM - 1) / TM, splits);
+    hipLaunchKernelGGL((k_gemm<false, true, EPI_STORE>), grid, dim3(256), 0, st, (const bf16 *)N.T1, K, (const bf16 *)N.T2, K, M,
+                       Nn, K, splits > 1 ? N.ws : C, (bf16 *)nullptr, Nn, (const float *)nullptr, (const float *)nullptr, kper);
+    if (splits > 1)
+        hipLaunchKernelGGL(k_sum_splits, dim3(1024), dim3(256), 0, st, (const float *)N.ws, splits, size_t(M) * Nn, C);
 }
 
 static void colsum(Mlp &N, hipStream_t st, const float *X, int rows, int cols, float *out) {
@@ -450,6 +624,20 @@ static void refresh_bf16(Mlp &N, hipStream_t st) {
 
 // Forward over rows already staged in N.X; keeps Z (pre-activations) when training.
 // C = X W with B^T = W^T stored K-contiguous (the fast operand layout).
+// Predictions y for `rows` staged rows: layers 1 and 2, then layer 3 with the
+// head in its epilogue (the last hidden activations are never stored).
+static void predict_rows(Mlp &N, hipStream_t st, int rows, float *y) {
+    const bf16 *W1t = N.wt + N.off[0], *W2t = N.wt + N.off[2], *W3t = N.wt + N.off[4];
+    const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
+    gemm<false, true, EPI_GELU>(st, N.X, N.k_in, W1t, N.k_in, rows, N.hid, N.k_in, nullptr, N.H[0], N.hid, b1);
+    gemm<false, true, EPI_GELU>(st, N.H[0], N.hid, W2t, N.hid, rows, N.hid, N.hid, nullptr, N.H[1], N.hid, b2);
+    const int slots = 2 * ((N.hid + TN - 1) / TN);
+    gemm<false, true, EPI_GELU_HEAD>(st, N.H[1], N.hid, W3t, N.hid, rows, N.hid, N.hid, N.Z[2], nullptr, slots, b3,
+                                     N.w + N.off[6]);
+    hipLaunchKernelGGL(k_head_fold, dim3((rows + 255) / 256), dim3(256), 0, st, (const float *)N.Z[2], slots, rows,
+                       (const float *)(N.w + N.off[7]), y);
+}
+
 static void forward(Mlp &N, hipStream_t st, int rows, bool keepZ) {
     const bf16 *W1t = N.wt + N.off[0], *W2t = N.wt + N.off[2], *W3t = N.wt + N.off[4];
     const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
@@ -466,19 +654,25 @@ static void backward(Mlp &N, hipStream_t st, int rows) {
     hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, (const bf16 *)N.H[2], w4, (const float *)(N.w + N.off[7]), rows, H,
                        N.y, (const float *)N.t,
                        1.0f / rows, (const float *)N.Z[2], N.dZf[2], N.dZb[2], N.dy, N.lossp);
-    hipLaunchKernelGGL(k_head_grad, dim3((H + 255) / 256), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)N.dy, rows,
-                       H, N.g + N.off[6], N.g + N.off[7]);
+    {  // head: dw4 = H3^T dy, db4 = sum dy
+        const int chunks = (rows + 255) / 256;
+        hipLaunchKernelGGL(k_head_grad_part, dim3((H + 63) / 64, chunks), dim3(256), 0, st, (const bf16 *)N.H[2],
+                           (const float *)N.dy, rows, H, N.part);
+        hipLaunchKernelGGL(k_colsum_fold, dim3((H + 255) / 256), dim3(256), 0, st, (const float *)N.part, chunks, H,
+                           N.g + N.off[6]);
+        colsum(N, st, N.dy, rows, 1, N.g + N.off[7]);
+    }
     // layer 3: dW3 = H2^T dZ3, db3 = colsum dZ3, dZ2 = (dZ3 W3^T) * gelu'(Z2)
     const bf16 *W2 = N.wb + N.off[2], *W3 = N.wb + N.off[4];
-    gemm<true, false, EPI_STORE>(st, N.H[1], H, N.dZb[2], H, H, H, rows, N.g + N.off[4], nullptr, H);
+    gemm_at_b(N, st, N.H[1], H, N.dZb[2], H, rows, N.g + N.off[4]);
     colsum(N, st, N.dZf[2], rows, H, N.g + N.off[5]);
     gemm<false, true, EPI_GELU_BWD>(st, N.dZb[2], H, W3, H, rows, H, H, N.dZf[1], N.dZb[1], H, nullptr, N.Z[1]);
     // layer 2
-    gemm<true, false, EPI_STORE>(st, N.H[0], H, N.dZb[1], H, H, H, rows, N.g + N.off[2], nullptr, H);
+    gemm_at_b(N, st, N.H[0], H, N.dZb[1], H, rows, N.g + N.off[2]);
     colsum(N, st, N.dZf[1], rows, H, N.g + N.off[3]);
     gemm<false, true, EPI_GELU_BWD>(st, N.dZb[1], H, W2, H, rows, H, H, N.dZf[0], N.dZb[0], H, nullptr, N.Z[0]);
     // layer 1: dW1 = X^T dZ1, db1
-    gemm<true, false, EPI_STORE>(st, N.X, N.k_in, N.dZb[0], H, N.k_in, H, rows, N.g + N.off[0], nullptr, H);
+    gemm_at_b(N, st, N.X, N.k_in, N.dZb[0], H, rows, N.g + N.off[0]);
     colsum(N, st, N.dZf[0], rows, H, N.g + N.off[1]);
 }
 
@@ -700,10 +894,7 @@ qi_status qi_mlp_predict(qi_mlp_t m, const float *x, size_t rows, float *y, qi_e
         QM_TRY(stage_rows(N, st, x + r0 * N.n_in, nullptr, nr));
         hipEvent_t a = N.ctx->ev0, b = N.ctx->ev1;
         (void)hipEventRecord(a, st);
-        forward(N, st, (int)nr, false);
-        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
-                           (const float *)(N.w + N.off[7]), (int)nr, N.hid, N.y, (const float *)nullptr, 0.f,
-                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+        predict_rows(N, st, (int)nr, N.y);
         (void)hipEventRecord(b, st);
         if ((e = hipMemcpyAsync(y + r0, N.y, nr * 4, hipMemcpyDeviceToHost, st))) break;
         if ((e = hipStreamSynchronize(st))) break;
@@ -726,7 +917,7 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
     if ((uint64_t)N.n_in != 2ull * nang + 3) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "network inputs != 2 nang + 3"));
     std::lock_guard<std::mutex> lk(N.ctx->mu);
     hipStream_t st = N.ctx->stream;
-    const size_t chunk = 65536;
+    const size_t chunk = std::min<size_t>(kScreenChunk, (size_t(1) << 31) / (size_t(nang) + 1));  // 32-bit item indices
     QM_TRY(mlp_alloc(N, std::min<size_t>(chunk, count)));
     hipError_t e;
     if (count > N.ycap) {
@@ -739,13 +930,10 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
     (void)hipEventRecord(N.ctx->ev0, st);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
         const int nr = (int)std::min<uint64_t>(chunk, count - c0);
-        const size_t cells = size_t(nr) * N.k_in;
-        hipLaunchKernelGGL(k_ansatz_features, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 8192)), dim3(256), 0, st, seed,
+        const size_t items = size_t(nr) * (nang + 1);
+        hipLaunchKernelGGL(k_ansatz_features, dim3((unsigned)std::min<size_t>((items + 255) / 256, 16384)), dim3(256), 0, st, seed,
                            draw0, count, c0, nr, (int)nang, N.k_in, N.X);
-        forward(N, st, nr, false);
-        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
-                           (const float *)(N.w + N.off[7]), nr, N.hid, N.yall + c0, (const float *)nullptr, 0.f,
-                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+        predict_rows(N, st, nr, N.yall + c0);
     }
     (void)hipEventRecord(N.ctx->ev1, st);
     if ((e = hipMemcpyAsync(y, N.yall, count * sizeof(float), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))
@@ -766,7 +954,7 @@ qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0,
     if ((uint64_t)N.n_in != 2ull * nang + 3) QM_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "network inputs != 2 nang + 3"));
     std::lock_guard<std::mutex> lk(N.ctx->mu);
     hipStream_t st = N.ctx->stream;
-    const size_t chunk = 65536;
+    const size_t chunk = std::min<size_t>(kScreenChunk, (size_t(1) << 31) / (size_t(nang) + 1));  // 32-bit item indices
     QM_TRY(mlp_alloc(N, std::min<size_t>(chunk, count)));
     hipError_t e;
     if (count > N.ycap) {
@@ -788,13 +976,10 @@ qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0,
     (void)hipEventRecord(N.ctx->ev0, st);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
         const int nr = (int)std::min<uint64_t>(chunk, count - c0);
-        const size_t cells = size_t(nr) * N.k_in;
-        hipLaunchKernelGGL(k_ansatz_features_local, dim3((unsigned)std::min<size_t>((cells + 255) / 256, 8192)), dim3(256), 0, st,
+        const size_t items = size_t(nr) * (nang + 1);
+        hipLaunchKernelGGL(k_ansatz_features_local, dim3((unsigned)std::min<size_t>((items + 255) / 256, 16384)), dim3(256), 0, st,
                            seed, draw0, c0, nr, (int)nang, N.k_in, (const double *)N.theta0, sigma, J[0], J[1], J[2], N.X);
-        forward(N, st, nr, false);
-        hipLaunchKernelGGL(k_head, dim3((nr + 3) / 4), dim3(256), 0, st, (const bf16 *)N.H[2], (const float *)(N.w + N.off[6]),
-                           (const float *)(N.w + N.off[7]), nr, N.hid, N.yall + c0, (const float *)nullptr, 0.f,
-                           (const float *)nullptr, (float *)nullptr, (bf16 *)nullptr, (float *)nullptr, (float *)nullptr);
+        predict_rows(N, st, nr, N.yall + c0);
     }
     (void)hipEventRecord(N.ctx->ev1, st);
     if ((e = hipMemcpyAsync(y, N.yall, count * sizeof(float), hipMemcpyDeviceToHost, st)) || (e = hipStreamSynchronize(st)))

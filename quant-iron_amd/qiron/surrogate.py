@@ -118,6 +118,20 @@ class MLP:
         _check(self.lib.qi_mlp_predict_ansatz(self._h, seed, draw0, count, nang, _fp(y), C.byref(err)), err)
         return y
 
+    def predict_local(self, seed: int, draw0: int, count: int, theta0: np.ndarray, sigma: float,
+                      J: np.ndarray) -> np.ndarray:
+        """Predictions for `count` candidates theta0 + sigma * z drawn on the
+        device (z standard normal from the SplitMix64 stream (seed, draw0),
+        see local_params), couplings J fixed."""
+        th0 = np.ascontiguousarray(theta0, dtype=np.float64)
+        jj = np.ascontiguousarray(J, dtype=np.float64)
+        y = np.empty(count, dtype=np.float32)
+        err = L.QiError()
+        _check(self.lib.qi_mlp_predict_ansatz_local(
+            self._h, seed, draw0, count, th0.size, th0.ctypes.data_as(L.DP), float(sigma), jj.ctypes.data_as(L.DP),
+            _fp(y), C.byref(err)), err)
+        return y
+
     def last_ms(self) -> float:
         return self.ctx.last_exec_ms()
 
@@ -195,6 +209,19 @@ def params_at(seed: int, draw0: int, count: int, nang: int, idx: np.ndarray):
     return th.reshape(len(idx), nang), J.reshape(len(idx), 3)
 
 
+def local_params(seed: int, draw0: int, nang: int, theta0: np.ndarray, sigma: float, idx: np.ndarray) -> np.ndarray:
+    """The angles qi_mlp_predict_ansatz_local draws for candidates `idx`:
+    theta0[j] + sigma * z, z = sqrt(-2 ln(1 - u1)) cos(2 pi u2) with u1, u2
+    the draws draw0 + 2 (c nang + j) + 1 and + 2 of the stream `seed`."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    j = np.arange(nang, dtype=np.uint64)
+    d = np.uint64(draw0) + np.uint64(2) * (idx[:, None] * np.uint64(nang) + j)
+    u1 = W.splitmix_at(seed, d + np.uint64(1))
+    u2 = W.splitmix_at(seed, d + np.uint64(2))
+    z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
+    return np.asarray(theta0, dtype=np.float64)[None, :] + sigma * z
+
+
 def ansatz_batch(specs, precision: str = "c64", ctx: Optional[Context] = None) -> CircuitBatch:
     """One CircuitBatch over (n, L, thetas, J) specs: every (n, L) shape is one
     shared program whose rotations take their angles from the batch's table."""
@@ -228,6 +255,10 @@ class ScreenResult:
     mean_exact_random: float
     surrogate_mae_kept: float
     max_abs_err_kept: float
+    sd_random: float = float("nan")
+    heldout_r2: float = float("nan")  # on the random subset (not trained on)
+    best_theta: Optional[np.ndarray] = None
+    best_J: Optional[np.ndarray] = None
 
 
 def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: int, keep_frac: float, t_mean: float,
@@ -245,14 +276,17 @@ def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: i
     th, J = params_at(seed, draw0, count, nang, top)
     exact = sim.energies(th, J)
     t2 = time.perf_counter()
-    mean_rnd = float("nan")
+    mean_rnd = sd_rnd = r2 = float("nan")
     if random_baseline:
         rnd = rng_np.choice(count, size=k, replace=False)
         thr, Jr = params_at(seed, draw0, count, nang, rnd)
-        mean_rnd = float(sim.energies(thr, Jr).mean())
+        er = sim.energies(thr, Jr)
+        mean_rnd, sd_rnd = float(er.mean()), float(er.std())
+        r2 = float(1.0 - np.mean((pred[rnd] - er) ** 2) / np.var(er))
     err = np.abs(pred[top] - exact)
+    b = int(np.argmin(exact))
     return ScreenResult(count, k, t1 - t0, t2 - t1, float(exact.min()), float(exact.mean()), mean_rnd,
-                        float(err.mean()), float(err.max()))
+                        float(err.mean()), float(err.max()), sd_rnd, r2, th[b].copy(), J[b].copy())
 
 
 def screen(mlp: MLP, sim: AnsatzEnergies, thetas: np.ndarray, J: np.ndarray, keep_frac: float, t_mean: float,
@@ -272,3 +306,90 @@ def screen(mlp: MLP, sim: AnsatzEnergies, thetas: np.ndarray, J: np.ndarray, kee
     err = np.abs(pred[top] - exact)
     return ScreenResult(thetas.shape[0], k, t1 - t0, t2 - t1, float(exact.min()), float(exact.mean()),
                         float(exact_rnd.mean()), float(err.mean()), float(err.max()))
+
+
+# ------------------------------------------------------- optimisation rounds --
+@dataclass
+class RoundResult:
+    round: int
+    centre_energy: float       # exact energy at the round's centre
+    train_samples: int
+    heldout_r2: float          # on the random control set (unbiased, never trained on)
+    mean_exact_kept: float     # top keep_frac by the surrogate, evaluated exactly
+    mean_exact_random: float   # the same number of candidates chosen at random
+    sd_random: float
+    gain_sd: float             # (random mean - kept mean) / sd of the random set
+    best_exact: float          # best exact energy among the kept candidates
+    surrogate_mae_kept: float
+    surrogate_max_err_kept: float
+    baseline_best: float       # the chain without the network: best energy so far
+    seconds: dict
+    centre_after: float = float("nan")  # the network chain's best energy after the round
+
+
+def optimise_local(mlp: MLP, sim: AnsatzEnergies, J: np.ndarray, theta: np.ndarray, rounds: int, sigma: float,
+                   n_local: int, candidates: int, keep_frac: float, epochs: int, lr: float, seed: int,
+                   batch: int = 512) -> list:
+    """Surrogate-screened local search. Each round: exact energies of n_local
+    points theta + sigma z around the centre (the training data, from the
+    batched simulator), a few epochs of training from the current weights,
+    the network ranks `candidates` local candidates drawn on the device, the
+    top keep_frac are evaluated exactly. A random keep_frac of the same
+    candidates is evaluated too: held-out R^2 and the selection gain are
+    measured on it. The best of the round's exact energies (training points,
+    kept and random candidates) becomes the next centre if it improves on it.
+    A second chain without the network spends the same number of exact
+    evaluations per round (n_local + 2 kept) on random local candidates and
+    moves to its best; its best energy is reported beside the network's."""
+    nang = sim.nang
+    J = np.asarray(J, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    k = max(1, int(round(keep_frac * candidates)))
+    centre = np.asarray(theta, dtype=np.float64).copy()
+    e_centre = float(sim.energies(centre[None, :], J[None, :])[0])
+    base_centre, base_best = centre.copy(), e_centre
+    out = []
+    for r in range(rounds):
+        t = {}
+        t0 = time.perf_counter()
+        th = centre[None, :] + sigma * rng.standard_normal((n_local, nang))
+        Jl = np.repeat(J[None, :], n_local, axis=0)
+        E = sim.energies(th, Jl)
+        t["train_data"] = time.perf_counter() - t0
+        mu, sd = float(E.mean()), float(E.std())
+        t0 = time.perf_counter()
+        mlp.set_data(features(th, Jl), ((E - mu) / sd).astype(np.float32))
+        for ep in range(epochs):
+            mlp.train_epoch(batch, lr, seed=seed * 1000 + r * 100 + ep)
+        t["train"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        cseed = seed * 7919 + r
+        pred = mlp.predict_local(cseed, 0, candidates, centre, sigma, J).astype(np.float64) * sd + mu
+        top = np.argpartition(pred, k - 1)[:k]
+        t["screen"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        th_k = local_params(cseed, 0, nang, centre, sigma, top)
+        E_k = sim.energies(th_k, np.repeat(J[None, :], k, axis=0))
+        t["exact_kept"] = time.perf_counter() - t0
+        rnd = rng.choice(candidates, size=k, replace=False)
+        th_r = local_params(cseed, 0, nang, centre, sigma, rnd)
+        E_r = sim.energies(th_r, np.repeat(J[None, :], k, axis=0))
+        r2 = float(1.0 - np.mean((pred[rnd] - E_r) ** 2) / np.var(E_r))
+        err = np.abs(pred[top] - E_k)
+        best = int(np.argmin(E_k))
+        # the chain without the network, same number of exact evaluations
+        nb = n_local + 2 * k
+        thb = base_centre[None, :] + sigma * rng.standard_normal((nb, nang))
+        Eb = sim.energies(thb, np.repeat(J[None, :], nb, axis=0))
+        if Eb.min() < base_best:
+            base_best, base_centre = float(Eb.min()), thb[int(np.argmin(Eb))].copy()
+        out.append(RoundResult(r, e_centre, n_local, r2, float(E_k.mean()), float(E_r.mean()), float(E_r.std()),
+                               float((E_r.mean() - E_k.mean()) / max(E_r.std(), 1e-300)), float(E_k[best]),
+                               float(err.mean()), float(err.max()), base_best, t))
+        pool_e = np.concatenate([E, E_k, E_r])
+        j = int(np.argmin(pool_e))
+        if pool_e[j] < e_centre:
+            centre = np.concatenate([th, th_k, th_r])[j].copy()
+            e_centre = float(pool_e[j])
+        out[-1].centre_after = e_centre
+    return out

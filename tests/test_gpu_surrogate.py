@@ -1,6 +1,8 @@
 """The learned surrogate: hand-written HIP forward/backward checked against the
-FP64 CPU oracle, training on simulator-generated energies, and what screening
-with the trained network gains over screening without it."""
+FP64 CPU oracle, training checked on a known teacher function, training on
+simulator-generated energies, and what screening with the trained network
+gains over screening without it (one layer at uniform angles; two layers and
+the 16-qubit, 6-layer ansatz in local optimisation rounds)."""
 import math
 
 import numpy as np
@@ -82,3 +84,72 @@ def test_training_on_simulated_energies_gains_over_unscreened_selection(ctx):
     res = S.screen(net, sim, th2, J2, 0.01, mu, sd, np.random.default_rng(1))
     print(res)
     assert res.mean_exact_kept < res.mean_exact_random - 0.5 * sd
+
+
+def test_training_recovers_a_known_teacher_network(ctx):
+    """Rules out training bugs: a fixed random 195 -> 512 -> 1 tanh network is
+    learned from 64 k samples to held-out R^2 > 0.95."""
+    rng = np.random.default_rng(0)
+    n_in = 195
+    w1 = rng.normal(size=(n_in, 512)) / math.sqrt(n_in)
+    w2 = rng.normal(size=512) / math.sqrt(512)
+
+    def teacher(x):
+        return np.tanh(x @ w1) @ w2
+
+    x = rng.uniform(-1, 1, size=(65536, n_in)).astype(np.float32)
+    y = teacher(x.astype(np.float64))
+    mu, sd = y.mean(), y.std()
+    net = S.MLP(n_in, 512, seed=3)
+    net.set_data(x, ((y - mu) / sd).astype(np.float32))
+    losses = [net.train_epoch(256, 1e-3, seed=e) for e in range(12)]
+    xv = rng.uniform(-1, 1, size=(8192, n_in)).astype(np.float32)
+    yv = teacher(xv.astype(np.float64))
+    r2 = 1 - np.mean((net.predict(xv) * sd + mu - yv) ** 2) / np.var(yv)
+    print("losses", [round(v, 4) for v in losses], f"held-out R^2 {r2:.4f}")
+    assert r2 > 0.95
+
+
+def test_device_local_candidates_match_host_regeneration(ctx):
+    """The network's predictions for device-drawn local candidates equal its
+    predictions for the host-regenerated candidates' features."""
+    nang = 24
+    net = S.MLP(S.n_features(nang), 256, seed=2)
+    th0 = np.random.default_rng(4).uniform(0, 2 * np.pi, nang)
+    J = np.array([0.7, 1.1, 1.3])
+    y = net.predict_local(99, 0, 4096, th0, 0.2, J)
+    idx = np.array([0, 17, 4095])
+    th = S.local_params(99, 0, nang, th0, 0.2, idx)
+    yh = net.predict(S.features(th, np.repeat(J[None, :], len(idx), axis=0)))
+    np.testing.assert_allclose(y[idx], yh, rtol=1e-2, atol=1e-3)
+
+
+def test_local_rounds_gain_beyond_one_layer(ctx):
+    """Two ansatz layers (6 qubits): in local optimisation rounds the network's
+    top 1% is at least 1 sd lower in exact energy than a random 1% of the
+    same candidates, every round, with held-out R^2 above 0.5."""
+    n, layers = 6, 2
+    sim = S.AnsatzEnergies(n, layers, slots=4096, precision="c64")
+    net = S.MLP(S.n_features(sim.nang), 256, seed=5)
+    th0 = 2 * math.pi * np.random.default_rng(5).uniform(size=sim.nang)
+    rounds = S.optimise_local(net, sim, np.array([1.0, 1.0, 1.0]), th0, rounds=3, sigma=0.3, n_local=16384,
+                              candidates=65536, keep_frac=0.01, epochs=8, lr=1e-3, seed=3, batch=256)
+    for r in rounds:
+        print(r)
+        assert r.gain_sd > 1.0 and r.heldout_r2 > 0.5
+    assert rounds[-1].centre_after < rounds[0].centre_energy
+
+
+def test_local_rounds_gain_at_16_qubits_6_layers(ctx):
+    """The 16-qubit, 6-layer ansatz (192 angles): one local round screening
+    2^20 device-drawn candidates; the kept 1% beats a random 1% by at least
+    1 sd in exact energy."""
+    n, layers = 16, 6
+    sim = S.AnsatzEnergies(n, layers, slots=4096, precision="c64")
+    net = S.MLP(S.n_features(sim.nang), 512, seed=7)
+    th0 = 2 * math.pi * np.random.default_rng(5).uniform(size=sim.nang)
+    J = 0.5 + np.random.default_rng(9).uniform(size=3)
+    (r,) = S.optimise_local(net, sim, J, th0, rounds=1, sigma=0.1, n_local=65536, candidates=1 << 20,
+                            keep_frac=0.01, epochs=8, lr=1e-3, seed=11)
+    print(r)
+    assert r.gain_sd > 1.0 and r.heldout_r2 > 0.5
