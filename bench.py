@@ -271,7 +271,7 @@ def bench_surrogate(W, args):
     seed, draw0, cand = W.SEED_BASE + 6, 0, args.candidates
     net.predict_stream(seed + 1, 0, 4096, nang)  # first launches of the screening kernels, untimed
     res = S.screen_stream(net, sim, seed, draw0, cand, 0.01, mu, sd, np.random.default_rng(3))
-    dev_screen = net.last_ms() * 1e-3
+    dev_screen = res.surrogate_device_seconds
     train_tflops = 3.0 * fwd_flop * args.epochs * args.train_samples / dev_train * 1e-12
     screen_tflops = fwd_flop * cand / dev_screen * 1e-12
     round0 = dict(train_samples=args.train_samples, gen_seconds=t_gen, gen_energies_per_s=args.train_samples / t_gen,

@@ -103,6 +103,7 @@ struct RegPass {
     uint8_t qt_in[12];   // qubit of each thread bit at load
     uint8_t qr_out[8];   // at store (SWAPs relabel qubits, the final layout differs)
     uint8_t qt_out[12];
+    uint64_t flip;       // qubits (inside the tile) whose bit the store address flips: the X part of the pass's Pauli frame
 };
 
 // A dense 2^k x 2^k block applied to tiles of 2^(k + kc) amplitudes.

@@ -304,7 +304,183 @@ struct RegPlanner {
             else op.tcm |= 1u << thread_index(lb);
         }
     }
+    // ---- Pauli frame: uncontrolled X, Y and Z gates cost no work --------
+    // The tile's true amplitudes are F phi, phi what the registers hold and
+    // F = i^fph X^fx Z^fz (masks over local bits, the X part left of the Z
+    // part). X, Y and Z multiply into F; H and single-control CNOT and CZ
+    // carry F through themselves (Clifford gates); any other gate U on a bit
+    // with a frame runs conjugated, F_t^-1 U F_t, or the frame is applied
+    // first (controls with an X part, controlled targets, parametric and LDS
+    // gates). When the pass closes, the X part is an address flip at the
+    // store, the Z part diagonal ops (folded into earlier diagonals where
+    // possible) and i^fph joins the global phase.
+    bool frames = false;
+    uint32_t fx = 0, fz = 0;
+    int fph = 0;
+    Status flush_z(int b, uint32_t src) {  // registers get Z_b: F = ... X_b^x Z_b
+        if (b < 0 || !((fz >> b) & 1)) return Status{};
+        fz &= ~(1u << b);
+        Lowered Z;
+        Z.kind = LK_Z;
+        Z.t0 = (uint32_t)qof[b];
+        return emit_op(Z, src);
+    }
+    Status flush_x(int b, uint32_t src) {  // registers get X_b: X_b Z_b^z = (-1)^z Z_b^z X_b
+        if (b < 0 || !((fx >> b) & 1)) return Status{};
+        fx &= ~(1u << b);
+        fph += 2 * ((fz >> b) & 1);
+        Lowered X;
+        X.kind = LK_X;
+        X.t0 = (uint32_t)qof[b];
+        return emit_op(X, src);
+    }
+    Status flush_bit(int b, uint32_t src) {
+        Status e = flush_z(b, src);
+        return e.ok() ? flush_x(b, src) : e;
+    }
+    // The 2x2 matrix of a one-qubit lowered gate (row-major, (re, im) pairs).
+    static void matrix_of(const Lowered &G, double *m) {
+        for (int k = 0; k < 8; k++) m[k] = 0;
+        const double *c = G.c;
+        switch (G.kind) {
+        case LK_H: m[0] = m[2] = m[4] = c[0]; m[6] = -c[0]; break;
+        case LK_X: m[2] = m[4] = 1; break;
+        case LK_Y: m[2] = c[0]; m[3] = c[1]; m[4] = c[2]; m[5] = c[3]; break;
+        case LK_RX: m[0] = m[6] = c[0]; m[2] = -c[1]; m[3] = -c[2]; m[4] = c[3]; m[5] = c[4]; break;
+        case LK_RY: m[0] = m[6] = c[0]; m[2] = -c[1]; m[4] = c[1]; break;
+        default: for (int k = 0; k < 8; k++) m[k] = c[k]; break;  // LK_U2
+        }
+    }
     Status emit(const Lowered &G, uint32_t src) {
+        if (!frames || is_relabel(G)) return emit_op(G, src);
+        const int t = lbit[G.t0];
+        uint32_t cl = 0;  // controls inside the tile (local bits)
+        bool ctl_out = false;
+        for (uint64_t cm = G.ctrl_mask; cm; cm &= cm - 1) {
+            const int q = __builtin_ctzll(cm);
+            if (lbit[q] >= 0) cl |= 1u << lbit[q];
+            else ctl_out = true;
+        }
+        const uint32_t tb = t >= 0 ? 1u << t : 0u;
+        const int xt = t >= 0 ? (int)((fx >> t) & 1) : 0, zt = t >= 0 ? (int)((fz >> t) & 1) : 0;
+        Status e;
+        if (t < 0) {  // a diagonal on a qubit outside the tile: no frame there; X parts of controls out
+            for (uint32_t c = cl; c; c &= c - 1)
+                if (!(e = flush_x(__builtin_ctz(c), src)).ok()) return e;
+            return emit_op(G, src);
+        }
+        if (G.kind == LK_SWAP || G.kind == LK_MATCH || G.slot >= 0) {
+            for (uint32_t c = cl; c; c &= c - 1)
+                if (!(e = flush_x(__builtin_ctz(c), src)).ok()) return e;
+            if (!(e = flush_bit(t, src)).ok()) return e;
+            if (G.kind == LK_SWAP || G.kind == LK_MATCH)
+                if (!(e = flush_bit(lbit[G.t1], src)).ok()) return e;
+            return emit_op(G, src);
+        }
+        if (G.ctrl_mask) {
+            if (__builtin_popcountll(G.ctrl_mask) == 1 && !ctl_out && t >= 0 && (G.kind == LK_X || G.kind == LK_Z)) {
+                const int c = __builtin_ctz(cl), xc = (int)((fx >> c) & 1);
+                if (G.kind == LK_X) {  // CNOT: X_c -> X_c X_t, Z_t -> Z_c Z_t
+                    if (xc) fx ^= tb;
+                    if (zt) fz ^= 1u << c;
+                } else {  // CZ: X_c -> X_c Z_t, X_t -> Z_c X_t, sign (-1)^(x_c x_t)
+                    if (xc) fz ^= tb;
+                    if (xt) fz ^= 1u << c;
+                    fph += 2 * (xc & xt);
+                }
+                return emit_op(G, src);
+            }
+            // Z parts of controls commute with a controlled gate, X parts are
+            // applied first. On the target, F_t C(U) F_t = C(F_t U F_t): the
+            // part of F_t that commutes with U stays (X with X, Z with a
+            // diagonal), the rest is applied first.
+            for (uint32_t c = cl; c; c &= c - 1)
+                if (!(e = flush_x(__builtin_ctz(c), src)).ok()) return e;
+            const bool diag = G.kind == LK_Z || G.kind == LK_PHASE || G.kind == LK_RZ;
+            if (G.kind == LK_X) e = flush_z(t, src);
+            else if (diag) e = flush_x(t, src);
+            else e = flush_bit(t, src);
+            if (!e.ok()) return e;
+            return emit_op(G, src);
+        }
+        switch (G.kind) {
+        case LK_X: fx ^= tb; return Status{};
+        case LK_Z: fph += 2 * xt; fz ^= tb; return Status{};  // Z_t X^x = (-1)^x X^x Z_t
+        case LK_Y:  // Y = i X Z: Y F = i^(1 + 2 x_t) X^(x ^ t) Z^(z ^ t)
+            if (G.c[0] == 0 && G.c[1] == -1 && G.c[2] == 0 && G.c[3] == 1) {
+                fph += 1 + 2 * xt;
+                fx ^= tb;
+                fz ^= tb;
+                return Status{};
+            }
+            break;
+        case LK_H:  // H X^a Z^b H = (-1)^(ab) X^b Z^a
+            fx = (fx & ~tb) | (zt ? tb : 0u);
+            fz = (fz & ~tb) | (xt ? tb : 0u);
+            fph += 2 * (xt & zt);
+            return emit_op(G, src);
+        default: break;
+        }
+        if (!xt && !zt) return emit_op(G, src);
+        // conjugated: the registers get Z^z X^x U X^x Z^z
+        Lowered C = G;
+        if (G.kind == LK_Z || G.kind == LK_PHASE || G.kind == LK_RZ) {  // diagonal: X swaps the entries, Z keeps them
+            if (xt) {
+                double d[4];
+                if (G.kind == LK_Z) { d[0] = 1; d[1] = 0; d[2] = -1; d[3] = 0; }
+                else if (G.kind == LK_PHASE) { d[0] = 1; d[1] = 0; d[2] = G.c[0]; d[3] = G.c[1]; }
+                else for (int k = 0; k < 4; k++) d[k] = G.c[k];
+                C.kind = LK_RZ;
+                C.c[0] = d[2]; C.c[1] = d[3]; C.c[2] = d[0]; C.c[3] = d[1];
+            }
+            return emit_op(C, src);
+        }
+        double m[8];
+        matrix_of(G, m);
+        if (xt) {  // X M X: rows and columns swapped
+            double n[8] = {m[6], m[7], m[4], m[5], m[2], m[3], m[0], m[1]};
+            memcpy(m, n, sizeof n);
+        }
+        if (zt) { m[2] = -m[2]; m[3] = -m[3]; m[4] = -m[4]; m[5] = -m[5]; }  // Z M Z: off-diagonals negated
+        C.kind = LK_U2;
+        for (int k = 0; k < 8; k++) C.c[k] = m[k];
+        return emit_op(C, src);
+    }
+    // The frame out at the end of the pass: Z part as diagonals (folded into
+    // the last foldable diagonal on the bit where there is one), the phase
+    // into the global phase; returns the X part as a qubit mask (the store's
+    // address flip).
+    Status close_frame(uint32_t src, uint64_t &flip) {
+        flip = 0;
+        if (!frames) return Status{};
+        for (uint32_t z = fz; z; z &= z - 1) {
+            const int b = __builtin_ctz(z);
+            const int j = reg_index(b), i = thread_index(b);
+            const int loc = j >= 0 ? 0 : 1, pos = j >= 0 ? j : i;
+            if (pos >= 0 && last_diag[loc][pos] >= 0) {
+                RegOp zop{};
+                zop.kind = RO_Z;
+                fold_diagonal(out[(size_t)last_diag[loc][pos]], zop);
+                fz &= ~(1u << b);
+                continue;
+            }
+            Status e = flush_z(b, src);
+            if (!e.ok()) return e;
+        }
+        for (uint32_t x = fx; x; x &= x - 1) flip |= uint64_t(1) << qof[__builtin_ctz(x)];
+        fx = 0;
+        const int k = ((fph % 4) + 4) % 4;
+        static const double ipow[4][2] = {{1, 0}, {0, 1}, {-1, 0}, {0, -1}};
+        if (k && gphase) {
+            const double re = gphase[0] * ipow[k][0] - gphase[1] * ipow[k][1];
+            const double im = gphase[0] * ipow[k][1] + gphase[1] * ipow[k][0];
+            gphase[0] = re;
+            gphase[1] = im;
+        }
+        fph = 0;
+        return Status{};
+    }
+    Status emit_op(const Lowered &G, uint32_t src) {
         RegOp op{};
         op.src = src;
         const bool param = G.slot >= 0;  // no rewrite that depends on the angle's value
@@ -554,6 +730,9 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         P.unscaled = &unscaled;
         P.hfactor = &hfactor;
         P.forget_diagonals();
+        // one-wave tiles only: every local bit is a lane or register bit there,
+        // so a frame can be applied wherever it sits (not so on wave bits)
+        P.frames = gphase != nullptr && TB == kLaneBits;
         for (int b = 0; b < 64; b++) P.lbit[b] = -1;
         RegPass rp{};
         rp.qmask = q;
@@ -642,7 +821,9 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
             for (size_t p = 0; p < phases.size(); p++)
                 for (size_t i : phases[p].ops) {
                     const Lowered &G = gates[members[i]];
-                    tgt_lb.push_back(is_pair(G) && !is_relabel(G) ? lb[G.t0] : -1);
+                    // (uncontrolled X and Y only change the frame: no op)
+                    const bool framed = P.frames && G.ctrl_mask == 0 && G.slot < 0 && (G.kind == LK_X || G.kind == LK_Y);
+                    tgt_lb.push_back(is_pair(G) && !is_relabel(G) && !framed ? lb[G.t0] : -1);
                     if (is_relabel(G)) std::swap(lb[G.t0], lb[G.t1]);
                 }
         }
@@ -660,6 +841,10 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
                 if (!e.ok()) return e;
                 seq++;
             }
+        }
+        {
+            Status e = P.close_frame(members.empty() ? 0u : (uint32_t)members.back(), rp.flip);
+            if (!e.ok()) return e;
         }
         // ---- store: row qubits back on lanes 0..L-1 (relabels may have moved them)
         if (TB == kLaneBits) {  // one-wave tiles: lane/register exchanges, lane k in turn
@@ -687,7 +872,7 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         for (int i = 0; i < TB; i++) rp.qt_out[i] = (uint8_t)P.qof[P.ct[i]];
         // SWAP relabels or transposes may send an amplitude to an address another
         // thread loads: the kernel then waits for every load before storing
-        rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0;
+        rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0 || rp.flip != 0;
         P.out.erase(std::remove_if(P.out.begin(), P.out.end(), [](const RegOp &op) { return op.kind == RO_NOP; }),
                     P.out.end());
         rp.op0 = (int)plan.reg_ops.size();

@@ -959,19 +959,24 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
     }
     {
         // store: thread and register bits hold the qubits of the final layout
-        uint64_t at = 0;
+        // (the frame's address flip: lane part per thread, register part uniform)
+        uint64_t at = 0, tmask = 0;
 #pragma unroll
-        for (int i = 0; i < TB; i++) at |= uint64_t((tid >> i) & 1u) << pass.qt_out[i];
+        for (int i = 0; i < TB; i++) {
+            at |= uint64_t((tid >> i) & 1u) << pass.qt_out[i];
+            tmask |= uint64_t(1) << pass.qt_out[i];
+        }
         uint64_t ar[RB];
 #pragma unroll
         for (int j = 0; j < RB; j++) ar[j] = uint64_t(1) << pass.qr_out[j];
-        T *dst = psi + (base | at);
+        const uint64_t rflip = pass.flip & ~tmask;
+        T *dst = psi + (base | (at ^ (pass.flip & tmask)));
 #pragma unroll
         for (int r = 0; r < E; r++) {
             uint64_t o = 0;
 #pragma unroll
             for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= ar[j];
-            st_stream(v.get(r), dst + o);
+            st_stream(v.get(r), dst + (o ^ rflip));
         }
     }
 }
@@ -988,12 +993,13 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
     constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
     const uint32_t lane = threadIdx.x;
     const uint64_t base = insert_zeros64(xcd_block_order(blockIdx.x, pass.chunk_log2), pass.qmask);
-    uint64_t at_in = 0, at_out = 0;
+    uint64_t at_in = 0, at_out = 0, tmask = 0;
 #pragma unroll
     for (int i = 0; i < TB; i++) {
         const uint32_t bit = (lane >> i) & 1u;
         at_in |= uint64_t(bit) << pass.qt_in[i];
         at_out |= uint64_t(bit) << pass.qt_out[i];
+        tmask |= uint64_t(1) << pass.qt_out[i];
     }
     Regs<R, E> v;
     {
@@ -1019,13 +1025,15 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.
     if (pass.sync_store) __builtin_amdgcn_s_waitcnt(0);
-    T *dst = psi + (base | at_out);
+    // the frame's address flip: lane part per lane, register part uniform
+    const uint64_t rflip = pass.flip & ~tmask;
+    T *dst = psi + (base | (at_out ^ (pass.flip & tmask)));
 #pragma unroll
     for (int r = 0; r < E; r++) {
         uint64_t o = 0;
 #pragma unroll
         for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_out[j];
-        st_stream(v.get(r), dst + o);
+        st_stream(v.get(r), dst + (o ^ rflip));
     }
 }
 

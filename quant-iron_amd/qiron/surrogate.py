@@ -259,6 +259,7 @@ class ScreenResult:
     heldout_r2: float = float("nan")  # on the random subset (not trained on)
     best_theta: Optional[np.ndarray] = None
     best_J: Optional[np.ndarray] = None
+    surrogate_device_seconds: float = float("nan")  # the candidate screen's kernels alone
 
 
 def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: int, keep_frac: float, t_mean: float,
@@ -270,6 +271,7 @@ def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: i
     nang = sim.nang
     t0 = time.perf_counter()
     pred = mlp.predict_stream(seed, draw0, count, nang).astype(np.float64) * t_std + t_mean
+    dev = mlp.last_ms() * 1e-3
     k = max(1, int(round(keep_frac * count)))
     top = np.argpartition(pred, k - 1)[:k]
     t1 = time.perf_counter()
@@ -286,7 +288,7 @@ def screen_stream(mlp: MLP, sim: AnsatzEnergies, seed: int, draw0: int, count: i
     err = np.abs(pred[top] - exact)
     b = int(np.argmin(exact))
     return ScreenResult(count, k, t1 - t0, t2 - t1, float(exact.min()), float(exact.mean()), mean_rnd,
-                        float(err.mean()), float(err.max()), sd_rnd, r2, th[b].copy(), J[b].copy())
+                        float(err.mean()), float(err.max()), sd_rnd, r2, th[b].copy(), J[b].copy(), dev)
 
 
 def screen(mlp: MLP, sim: AnsatzEnergies, thetas: np.ndarray, J: np.ndarray, keep_frac: float, t_mean: float,

@@ -306,10 +306,10 @@ void run_pass(std::vector<cd> &psi, int n, const RegPass &p, const std::vector<R
         }
         for (uint32_t t = 0; t < (uint32_t)X.T; t++)
             for (uint32_t r = 0; r < (uint32_t)X.E; r++) {
-                uint64_t a = base;
+                uint64_t a = 0;
                 for (int i = 0; i < X.TB; i++) a |= uint64_t((t >> i) & 1u) << p.qt_out[i];
                 for (int j = 0; j < X.RB; j++) a |= uint64_t((r >> j) & 1u) << p.qr_out[j];
-                psi[a] = X.at(t, r);
+                psi[base | (a ^ p.flip)] = X.at(t, r);  // the Pauli frame's X part flips tile bits
             }
     }
 }
