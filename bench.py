@@ -50,24 +50,42 @@ def run_circuit(qiron, L, n, prec, gates, opts, steps, warmup):
     return wall, dev_ms, nsteps, norm
 
 
-def cpu_baseline(qiron_workloads, n_cpu: int, budget_s: float, threads: int):
-    """The reference-structured CPU path (clone per gate, update list + serial
-    scatter, `threads` workers) on the first layer of the same circuit family
-    at n_cpu qubits, complex128, for about budget_s seconds."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
+def host_cpu():
+    """The host's CPU model and core counts (the baseline states what it ran on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
 
+
+def cpu_baseline(qiron_workloads, n: int, depth: int, budget_s: float, threads: int):
+    """The reference-structured CPU path (clone per gate, update list + serial
+    scatter, `threads` workers) on the first gates of the benchmark circuit
+    itself (n qubits, complex128 as the reference computes), timed directly:
+    gates are run until about budget_s seconds of the C call have passed (at
+    least one). Returns (gates done, seconds)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
-    gates = qiron_workloads.random_circuit(n=n_cpu, depth=2, seed=qiron_workloads.SEED_BASE + 2)
-    psi = O.init(n_cpu, 2)
-    done = 0
-    t0 = time.perf_counter()
-    while done < gates.size and time.perf_counter() - t0 < budget_s:
-        psi = O.reference_structured_apply(psi, gates[done:done + 1], threads)
+    gates = qiron_workloads.random_circuit(n=n, depth=depth, seed=qiron_workloads.SEED_BASE + 2)
+    psi = O.init(n, 2)
+    done, spent = 0, 0.0
+    while done < gates.size and (done == 0 or spent + spent / done <= budget_s):
+        _, dt = O.reference_structured_timed(psi, gates[done:done + 1], threads)
+        spent += dt
         done += 1
-    dt = time.perf_counter() - t0
-    return done, dt
+    del psi
+    return done, spent
 
 
 def bench_config1(qiron, L, W, args):
@@ -99,10 +117,9 @@ def bench_config1(qiron, L, W, args):
                max_abs_amp_diff=float(np.max(np.abs(psi - ref))), abs_dE=abs(e.real - e_ref.real),
                oracle_seconds=t_oracle)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        t0 = time.perf_counter()
-        rs = O.reference_structured_apply(O.init(n, 2), gates, threads)
-        out["reference_structured_seconds"] = time.perf_counter() - t0
+        threads = min(host_cpu()["usable"], 16)
+        rs, dt = O.reference_structured_timed(O.init(n, 2), gates, threads, keep=True)
+        out["reference_structured_seconds"] = dt
         out["reference_structured_threads"] = threads
         out["reference_structured_max_diff"] = float(np.max(np.abs(rs - ref)))
     log("config 1 (20q c128):", out)
@@ -132,6 +149,7 @@ def bench_trotter(qiron, W, args):
             st.ctx.synchronize()
             if prec == "c64":
                 t_c64 += time.perf_counter() - t0
+                step_passes = st.ctx.last_exec_steps()  # launches of one Trotter step
         a, b = states["c64"].state_vector, states["c128"].state_vector
         na, nb = float(np.vdot(a, a).real), float(np.vdot(b, b).real)
         fid = float(abs(np.vdot(b, a)) ** 2 / (na * nb))
@@ -143,10 +161,37 @@ def bench_trotter(qiron, W, args):
                          dE_normalised_c64_vs_c128=e["c64"] / na - e["c128"] / nb))
     apps = 200 * terms.size
     S = (1 << n) * 8
+    # HBM rate of the passes: each pass reads and writes the state once
     out = dict(terms=int(terms.size), steps=200, exp_applications=apps, c64_seconds=t_c64,
-               c64_exp_per_s=apps / t_c64, c64_effective_GBps=2 * S * apps / t_c64 / 1e9, checkpoints=rows)
-    log(f"trotter 25q: {apps / t_c64:.0f} exp(P)/s (c64), final fidelity {rows[-1]['fidelity_c64_vs_c128']:.9f}, "
+               c64_exp_per_s=apps / t_c64, c64_passes_per_step=step_passes,
+               c64_pass_GBps=2 * S * step_passes * 200 / t_c64 / 1e9, checkpoints=rows)
+    log(f"trotter 25q: {apps / t_c64:.0f} exp(P)/s (c64), {step_passes} passes per step at "
+        f"{out['c64_pass_GBps']:.0f} GB/s, final fidelity {rows[-1]['fidelity_c64_vs_c128']:.9f}, "
         f"drift c64 {rows[-1]['drift_c64']:.2e}")
+    return out
+
+
+def bench_measure(qiron, W):
+    """measure_n on the 30-qubit state after the benchmark circuit's first
+    layer: 2^20 shots of 20 measured qubits (one marginal histogram over 2^20
+    bins in FP64, one cumulative scan, one sampling pass), complex64."""
+    import numpy as np
+
+    n, m, shots = 30, 20, 1 << 20
+    s = qiron.State.new_plus(n, "c64")
+    s.apply_gates_inplace(W.random_circuit(n=n, depth=1, seed=W.SEED_BASE + 2))
+    u = W.SplitMix64(W.SEED_BASE + 9).uniforms(shots)
+    qs = list(range(m))
+    s.measure_outcomes(qs, u[:16])  # first launches, untimed
+    s.ctx.synchronize()
+    t0 = time.perf_counter()
+    outs = s.measure_outcomes(qs, u)
+    s.ctx.synchronize()
+    dt = time.perf_counter() - t0
+    del s
+    out = dict(qubits=n, measured=m, shots=shots, seconds=dt, shots_per_s=shots / dt,
+               distinct_outcomes=int(np.unique(outs).size))
+    log("measure_n:", out)
     return out
 
 
@@ -529,19 +574,21 @@ def main():
     if not args.only and not args.no_extras:
         results["config1"] = bench_config1(qiron, L, W, args)
         results["trotter"] = bench_trotter(qiron, W, args)
+        results["measure"] = bench_measure(qiron, W)
         results["ceiling"] = bench_ceiling(qiron, L, W)
 
     cpu = None
     if not args.no_cpu and not args.only:
-        threads = min(os.cpu_count() or 1, 16)
-        n_cpu = 26
-        done, dt = cpu_baseline(W, n_cpu, args.cpu_seconds, threads)
+        host = host_cpu()
+        threads = min(host["usable"], 16)
+        done, dt = cpu_baseline(W, n, depth, args.cpu_seconds, threads)
         gps = done / dt
-        cpu = {"value": round(gps * 2.0 ** (n_cpu - n), 6), "unit": "gates/s", "cores": threads, "kind": "port",
+        cpu = {"value": round(gps, 6), "unit": "gates/s", "cores": threads, "kind": "port",
                "sample": f"reference-structured CPU path (clone per gate, update list + serial scatter, {threads} "
-                         f"threads), complex128, first {done} gates of the same circuit family at {n_cpu} qubits in "
-                         f"{dt:.1f} s = {gps:.3f} gates/s; value scaled by 2^({n_cpu}-{n}) to {n} qubits "
-                         f"(cost per gate is linear in the state size)"}
+                         f"threads), complex128, the first {done} gates of this {n}-qubit circuit run directly "
+                         f"({dt:.1f} s of CPU path time, host buffer copies excluded); host {host['model']}, "
+                         f"nproc {host['nproc']}, {host['usable']} usable",
+               "host": host}
         log("cpu baseline", cpu)
 
     out = {

@@ -7,6 +7,7 @@ numpy vectors (interleaved re, im doubles, qubit k = bit k of the index).
 from __future__ import annotations
 
 import ctypes as C
+import time
 import subprocess
 from pathlib import Path
 
@@ -160,6 +161,30 @@ def reference_structured_apply(psi, gates: np.ndarray, threads: int) -> np.ndarr
     libc.free(C.cast(pp, C.c_void_p))
     _check(rc, e)
     return out
+
+
+def reference_structured_timed(psi, gates: np.ndarray, threads: int, keep: bool = False):
+    """reference_structured_apply with only the C call timed (the host-side
+    buffer copies of this wrapper stay outside): (result or None, seconds)."""
+    n = n_of(psi)
+    src = _vec(psi)
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    libc.free.argtypes = [C.c_void_p]
+    p = libc.malloc(src.size * 16)
+    if not p:
+        raise MemoryError("host buffer for the reference-structured path")
+    C.memmove(p, src.ctypes.data, src.size * 16)
+    pp = C.cast(p, C.POINTER(C.c_double))
+    e = _err()
+    t0 = time.perf_counter()
+    rc = lib().rf_apply_gates(C.byref(pp), n, gates.ctypes.data_as(C.c_void_p), gates.size, threads, C.byref(e))
+    dt = time.perf_counter() - t0
+    out = np.ctypeslib.as_array(pp, shape=(2 * src.size,)).copy().view(np.complex128) if keep and rc == 0 else None
+    libc.free(C.cast(pp, C.c_void_p))
+    _check(rc, e)
+    return out, dt
 
 
 def _term(t):

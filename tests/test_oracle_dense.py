@@ -255,6 +255,9 @@ def test_reference_structured_baseline_equals_oracle():
     a = O.apply_gates(psi, gates)
     b = O.reference_structured_apply(psi, gates, threads=3)
     np.testing.assert_allclose(a, b, atol=1e-13)
+    c, dt = O.reference_structured_timed(psi, gates, threads=3, keep=True)  # the benchmark's timed form
+    np.testing.assert_array_equal(b, c)
+    assert dt > 0
 
 
 def _dense_basis_measure(psi, n, qubits, U, u):
