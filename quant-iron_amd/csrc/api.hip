@@ -376,7 +376,7 @@ static Status run_maybe_graph(Ctx &C, State &s, bool use_graph, const void *key,
 // host-orchestrated steps between them.
 // `slots` (may be NULL): the angle slot of each gate, -1 for none (parametric circuits).
 static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, const qi_exec_opts &o, ExecPlan &plan,
-                         const int32_t *slots = nullptr) {
+                         const int32_t *slots = nullptr, bool allow_oop = false) {
     size_t k = 0;
     while (k < ng) {
         if (!is_operator(gates[k].kind)) { plan.steps.push_back(ExecStep::host(k)); k++; continue; }
@@ -391,7 +391,7 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
             j++;
         }
         Status e;
-        if (mode == QI_EXEC_TILED) e = plan_tiled(s, seg, o.tile_qubits, o.tile_threads_log2, plan);
+        if (mode == QI_EXEC_TILED) e = plan_tiled(s, seg, o.tile_qubits, o.tile_threads_log2, plan, allow_oop);
         else if (mode == QI_EXEC_FUSED_MFMA) e = plan_fused(s, seg, o.block_qubits, plan);
         else for (const Lowered &L : seg) plan.steps.push_back(ExecStep::gate(L));
         if (!e.ok()) return e;
@@ -401,21 +401,42 @@ static Status build_plan(State &s, const qi_gate *gates, size_t ng, int mode, co
 }
 
 // `blob`: the device copy of the plan's op tables (reg_ops, then matrices).
-static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates, const void *blob) {
+// `alt`: the second state buffer of out-of-place plans (Ctx::alt; may be null
+// for plans without moving passes): a moving pass switches the current buffer,
+// the steps between run on it, and every plan ends in the state's own buffer.
+static Status launch_plan(State &s, const ExecPlan &plan, const qi_gate *gates, const void *blob, void *alt = nullptr) {
     Ctx &C = *s.ctx;
     hipStream_t st = C.stream;
     const RegOp *dops = (const RegOp *)blob;
     const unsigned char *dmats = (const unsigned char *)blob + plan.mats_offset();
+    void *cur = s.d;
     for (const ExecStep &step : plan.steps) {
         Status e;
+        State on = s;
+        on.d = cur;
         switch (step.type) {
-        case ExecStep::GATE: e = launch_gate(s, step.L, st); break;
-        case ExecStep::REG: e = launch_reg_pass(s, step.reg, dops, st); break;
-        case ExecStep::FUSED: e = launch_fused_block(s, step.fused, dmats + step.fused.mat_off, st); break;
-        default: e = apply_validated(s, gates[step.host_gate]); break;
+        case ExecStep::GATE: e = launch_gate(on, step.L, st); break;
+        case ExecStep::REG: {
+            void *dst = cur;
+            if (step.reg.oop) {
+                if (!alt) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "out-of-place plan without a second buffer");
+                dst = cur == s.d ? alt : s.d;
+            }
+            e = launch_reg_pass(on, step.reg, dops, st, cur, dst);
+            cur = dst;
+        } break;
+        case ExecStep::FUSED:
+            if (cur != s.d) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "dense block on the second buffer");
+            e = launch_fused_block(s, step.fused, dmats + step.fused.mat_off, st);
+            break;
+        default:
+            if (cur != s.d) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "host step on the second buffer");
+            e = apply_validated(s, gates[step.host_gate]);
+            break;
         }
         if (!e.ok()) return e;
     }
+    if (cur != s.d) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "plan ends in the second buffer");
     return Status{};
 }
 
@@ -439,19 +460,37 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
     for (size_t k = 0; k < ng; k++) all_ops = all_ops && is_operator(gates[k].kind);
     Ctx &C = *s.ctx;
     const bool use_graph = o.use_graph && all_ops;
-    // plan key: options, gates and state shape; the same gate list again reuses
-    // the last plan and its op tables in the plan buffer (no planning, no upload)
-    std::vector<unsigned char> key(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 8);
+    // A second state buffer lets a tiled plan with relabelled SWAPs end in one
+    // permuting pass instead of a chain of in-place ones: taken for one-wave
+    // tiles when the list has uncontrolled SWAPs and the device keeps 16 GiB
+    // free beside it (the buffer stays with the context).
+    bool allow_oop = false;
+    if (mode == QI_EXEC_TILED && (o.tile_threads_log2 <= 0 || o.tile_threads_log2 == kLaneBits) && s.n >= 16) {
+        bool swaps = false;
+        for (size_t k = 0; k < ng && !swaps; k++) swaps = gates[k].kind == QI_GATE_SWAP && gates[k].ncontrols == 0;
+        if (swaps && C.alt.bytes >= s.bytes()) {
+            allow_oop = true;
+        } else if (swaps) {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > s.bytes() + (size_t(16) << 30))
+                allow_oop = ensure(C.alt, s.bytes()).ok();
+        }
+    }
+    // plan key: options, gates, state shape and the buffer choice; the same gate
+    // list again reuses the last plan and its op tables in the plan buffer (no
+    // planning, no upload)
+    std::vector<unsigned char> key(sizeof(qi_exec_opts) + ng * sizeof(qi_gate) + 9);
     memcpy(key.data(), &o, sizeof o);
     if (ng) memcpy(key.data() + sizeof o, gates, ng * sizeof(qi_gate));
     const uint64_t np = (uint64_t)s.n | ((uint64_t)s.prec << 32);
     memcpy(key.data() + sizeof o + ng * sizeof(qi_gate), &np, 8);
+    key[sizeof o + ng * sizeof(qi_gate) + 8] = allow_oop ? 1 : 0;
     std::shared_ptr<ExecPlan> plan;
     if (C.cached_plan && C.plan_key == key) {
         plan = std::static_pointer_cast<ExecPlan>(C.cached_plan);
     } else {
         plan = std::make_shared<ExecPlan>();
-        Status e = build_plan(s, gates, ng, mode, o, *plan);
+        Status e = build_plan(s, gates, ng, mode, o, *plan, nullptr, allow_oop);
         if (!e.ok()) return e;
         C.cached_plan.reset();
         C.plan_key.clear();
@@ -469,10 +508,11 @@ static Status apply_gates(State &s, uint32_t n_circuit, const qi_gate *gates, si
         C.plan_key = key;
     }
     C.last_plan_steps = plan->steps.size();
-    // graph key: the plan key and the op-table buffer the graph reads
-    const uint64_t pp = (uint64_t)(uintptr_t)C.plan.p;
-    key.insert(key.end(), (const unsigned char *)&pp, (const unsigned char *)&pp + 8);
-    auto launch = [&]() -> Status { return launch_plan(s, *plan, gates, C.plan.p); };
+    // graph key: the plan key and the op-table and second-state buffers the graph reads
+    const uint64_t pp[2] = {(uint64_t)(uintptr_t)C.plan.p, (uint64_t)(uintptr_t)(allow_oop ? C.alt.p : nullptr)};
+    key.insert(key.end(), (const unsigned char *)pp, (const unsigned char *)pp + sizeof pp);
+    void *alt = allow_oop ? C.alt.p : nullptr;
+    auto launch = [&]() -> Status { return launch_plan(s, *plan, gates, C.plan.p, alt); };
     return run_maybe_graph(C, s, use_graph, key.data(), key.size(), launch);
 }
 
@@ -586,7 +626,7 @@ qi_status qi_ctx_destroy(qi_ctx_t ctx) {
     (void)hipSetDevice(C.device);
     (void)hipStreamSynchronize(C.stream);
     if (C.graph_exec) (void)hipGraphExecDestroy(C.graph_exec);
-    for (DevBuf *b : {&C.red, &C.red_out, &C.ops, &C.probs, &C.aux, &C.plan, &C.trot})
+    for (DevBuf *b : {&C.red, &C.red_out, &C.ops, &C.probs, &C.aux, &C.plan, &C.trot, &C.alt})
         if (b->p) (void)hipFree(b->p);
     if (C.host_pinned) (void)hipHostFree(C.host_pinned);
     if (C.ev0) (void)hipEventDestroy(C.ev0);

@@ -104,6 +104,9 @@ struct RegPass {
     uint8_t qr_out[8];   // at store (SWAPs relabel qubits, the final layout differs)
     uint8_t qt_out[12];
     uint64_t flip;       // qubits (inside the tile) whose bit the store address flips: the X part of the pass's Pauli frame
+    int32_t oop;         // 1: stores go to the other of the state's two buffers (one-wave tiles only)
+    int32_t permute;     // 1: the store address's bits outside the tile move to perm[bit] (a qubit permutation)
+    uint8_t perm[64];
 };
 
 // A dense 2^k x 2^k block applied to tiles of 2^(k + kc) amplitudes.
@@ -149,8 +152,13 @@ int default_tile_qubits(const State &s);
 int default_tile_threads_log2(int prec, int K);
 int tile_low_qubits(int prec);
 // Register-resident passes for one state (ExecStep::REG, plan.reg_ops).
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan);
-Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st);
+// allow_oop: the plan may use a second state-sized buffer (Ctx::alt): with
+// SWAPs relabelled, the first pass then stores to it and a last pass brings the
+// state back through any qubit permutation in its store addresses.
+Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan,
+                  bool allow_oop = false);
+// src / dst: the buffers the pass loads from and stores to (equal unless p.oop).
+Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st, void *src, void *dst);
 // LDS passes over the same pass selection (ExecStep::TILE, plan.tile_ops),
 // used by the batched engine.
 Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan);

@@ -62,6 +62,7 @@ struct Ctx {
     DevBuf aux;      // misc (sample outcomes, uniforms)
     DevBuf plan;     // op tables and fused matrices of the last gate-list plan
     DevBuf trot;     // phase tables of the diagonal runs of a Trotter step
+    DevBuf alt;      // second state buffer of out-of-place tiled plans (grows with the largest such state)
     size_t last_plan_steps = 0;
     double *host_pinned = nullptr;  // pinned host words for results
     size_t host_pinned_words = 0;

@@ -926,7 +926,8 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
 // descending order of the logical qubit put back in place. A pass then never
 // needs both qubits of a SWAP in its tile; the price is the final
 // permutation (a few passes).
-static std::vector<Lowered> relabel_swaps(const std::vector<Lowered> &in, int n, bool descending) {
+static std::vector<Lowered> relabel_swaps(const std::vector<Lowered> &in, int n, bool descending,
+                                          int *holds_out = nullptr) {
     int at[64], holds[64];  // logical -> physical, physical -> logical
     for (int q = 0; q < 64; q++) at[q] = holds[q] = q;
     std::vector<Lowered> out;
@@ -944,6 +945,10 @@ static std::vector<Lowered> relabel_swaps(const std::vector<Lowered> &in, int n,
         M.ctrl_mask = 0;
         for (uint64_t c = L.ctrl_mask; c; c &= c - 1) M.ctrl_mask |= uint64_t(1) << at[__builtin_ctzll(c)];
         out.push_back(M);
+    }
+    if (holds_out) {  // the caller moves the data: physical bit p holds logical qubit holds_out[p]
+        for (int p = 0; p < 64; p++) holds_out[p] = holds[p];
+        return out;
     }
     for (int i = 0; i < n; i++) {
         const int l = descending ? n - 1 - i : i;
@@ -972,16 +977,104 @@ static void append_plan(ExecPlan &dst, const ExecPlan &src) {
 // The gate list as given and with SWAPs relabeled (both emission orders of
 // the final permutation): the plan with the fewest launches wins (the given
 // list on ties).
-Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan) {
+// The last pass of an out-of-place plan: the state moves back to its own
+// buffer with physical bit p stored at bit pi[p]. Its tile holds qubits 0..3
+// (lanes 0..3 at the load: contiguous reads) and the four qubits that go to bits
+// 0..3 (brought onto lanes 0..3 by lane/register exchanges: contiguous writes);
+// the other tile qubits land where pi sends them through the store layout and
+// the bits outside the tile through the store's base permutation.
+static void permutation_pass(int n, int K, const int *pi, ExecPlan &plan) {
+    int src_of[64];  // src_of[d]: the physical bit stored at bit d
+    for (int p = 0; p < n; p++) src_of[pi[p]] = p;
+    uint64_t q = 0xFull;
+    for (int d = 0; d < 4; d++) q |= uint64_t(1) << src_of[d];
+    for (int b = 0; b < n && __builtin_popcountll(q) < K; b++) q |= uint64_t(1) << b;
+    int qof[16], lb[64];
+    for (int b = 0, j = 0; b < n; b++) {
+        lb[b] = -1;
+        if ((q >> b) & 1) { lb[b] = j; qof[j] = b; j++; }
+    }
+    constexpr int TB = kLaneBits;
+    const int RB = K - TB;
+    // load layout: qubits 0..3 on lanes 0..3; the qubits headed for bits 0..3 next
+    // (registers first, where an exchange reaches them), then the rest
+    uint8_t ct[12] = {0}, cr[8] = {0};
+    std::vector<int> rest;
+    for (int d = 0; d < 4; d++)
+        if (src_of[d] > 3) rest.push_back(lb[src_of[d]]);
+    for (int j = 0; j < K; j++)
+        if (qof[j] > 3 && std::find(rest.begin(), rest.end(), j) == rest.end()) rest.push_back(j);
+    for (int i = 0; i < 4; i++) ct[i] = (uint8_t)lb[i];
+    size_t r = 0;
+    for (int j = 0; j < RB; j++) cr[j] = (uint8_t)rest[r++];
+    for (int i = 4; i < TB; i++) ct[i] = (uint8_t)rest[r++];
+    RegPass rp{};
+    rp.qmask = q;
+    rp.K = K;
+    rp.tb = TB;
+    memcpy(rp.lay_r, cr, 8);
+    memcpy(rp.lay_t, ct, 12);
+    for (int j = 0; j < RB; j++) rp.qr_in[j] = (uint8_t)qof[cr[j]];
+    for (int i = 0; i < TB; i++) rp.qt_in[i] = (uint8_t)qof[ct[i]];
+    std::vector<RegOp> ops;
+    for (int i = 0; i < 4; i++) {  // lane i gets the qubit stored at bit i
+        const int want = lb[src_of[i]];
+        if (ct[i] == want) continue;
+        RegOp op{};
+        op.tgt = i;
+        int k = -1, j = -1;
+        for (int x = 0; x < TB; x++) if (ct[x] == want) k = x;
+        for (int x = 0; x < RB; x++) if (cr[x] == want) j = x;
+        if (k >= 0) { op.kind = RO_XLL; op.dbit = k; std::swap(ct[i], ct[k]); }
+        else { op.kind = RO_XLR; op.dbit = j; std::swap(ct[i], cr[j]); }
+        memcpy(op.lay_t, ct, sizeof op.lay_t);
+        ops.push_back(op);
+    }
+    for (int j = 0; j < RB; j++) rp.qr_out[j] = (uint8_t)pi[qof[cr[j]]];
+    for (int i = 0; i < TB; i++) rp.qt_out[i] = (uint8_t)pi[qof[ct[i]]];
+    rp.oop = 1;
+    rp.permute = 1;
+    for (int p = 0; p < 64; p++) rp.perm[p] = (uint8_t)(p < n ? pi[p] : p);
+    rp.op0 = (int)plan.reg_ops.size();
+    rp.nops = (int)ops.size();
+    plan.reg_ops.insert(plan.reg_ops.end(), ops.begin(), ops.end());
+    RegOp end{};
+    end.kind = RO_NOP;
+    plan.reg_ops.push_back(end);
+    plan.steps.push_back(ExecStep::reg_step(rp));
+}
+
+Status plan_tiled(const State &s, const std::vector<Lowered> &gates, int K, int TB, ExecPlan &plan, bool allow_oop) {
     ExecPlan best;
     Status st = plan_tiled_list(s, gates, K, TB, best);
     if (!st.ok()) return st;
     const bool any = std::any_of(gates.begin(), gates.end(), is_relabel);
-    if (any && (int)s.n >= (K <= 0 ? default_tile_qubits(s) : K)) {
+    const int Kr = K <= 0 ? default_tile_qubits(s) : K;
+    if (any && (int)s.n >= Kr) {
         for (int order = 0; order < 2; order++) {
             ExecPlan alt;
             if (!plan_tiled_list(s, relabel_swaps(gates, (int)s.n, order == 1), K, TB, alt).ok()) continue;
             if (alt.steps.size() < best.steps.size()) best = std::move(alt);
+        }
+        const int TBr = TB <= 0 ? default_tile_threads_log2(s.prec, Kr) : TB;
+        if (allow_oop && TBr == kLaneBits && (int)s.n >= Kr + 4) {
+            // relabelled, the data moved by the first pass (to the second
+            // buffer) and the last (back, through the final permutation)
+            int holds[64];
+            ExecPlan alt;
+            if (plan_tiled_list(s, relabel_swaps(gates, (int)s.n, false, holds), K, TB, alt).ok()) {
+                bool identity = true;
+                for (int p = 0; p < (int)s.n; p++) identity = identity && holds[p] == p;
+                size_t first = alt.steps.size();
+                for (size_t k = 0; k < alt.steps.size() && first == alt.steps.size(); k++)
+                    if (alt.steps[k].type == ExecStep::REG) first = k;
+                if (!identity && first < alt.steps.size()) {
+                    alt.steps[first].reg.oop = 1;
+                    alt.steps[first].reg.sync_store = 0;  // loads and stores in different buffers
+                    permutation_pass((int)s.n, Kr, holds, alt);
+                    if (alt.steps.size() < best.steps.size()) best = std::move(alt);
+                }
+            }
         }
     }
     append_plan(plan, best);

@@ -986,9 +986,12 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
 // and needs no LDS and no barriers, and the per-op scalar work is paid once
 // per tile. (A second register set to prefetch the next tile costs more in
 // occupancy than it gains: measured slower at 2048 and 1024 amplitudes.)
+// src / dst: the pass's load and store buffers (the same buffer, or the
+// state's two buffers for a pass that moves the state: tiles are disjoint in
+// both, so no wave stores where another loads).
 template <typename R, int K>
-__global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict__ psi, const RegPass pass,
-                                                  const RegOp *__restrict__ ops) {
+__global__ __launch_bounds__(64) void k_wave_pass(const typename V2<R>::T *src_psi, typename V2<R>::T *dst_psi,
+                                                  const RegPass pass, const RegOp *__restrict__ ops) {
     using T = typename V2<R>::T;
     constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
     const uint32_t lane = threadIdx.x;
@@ -1003,7 +1006,7 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
     }
     Regs<R, E> v;
     {
-        const T *src = psi + (base | at_in);
+        const T *src = src_psi + (base | at_in);
 #pragma unroll
         for (int r = 0; r < E; r++) {
             uint64_t o = 0;
@@ -1025,9 +1028,15 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.
     if (pass.sync_store) __builtin_amdgcn_s_waitcnt(0);
-    // the frame's address flip: lane part per lane, register part uniform
+    // the frame's address flip: lane part per lane, register part uniform; a
+    // permuting pass also moves the tile's base bits (uniform per tile)
+    uint64_t base_out = base;
+    if (pass.permute) {
+        base_out = 0;
+        for (uint64_t m = base; m; m &= m - 1) base_out |= uint64_t(1) << pass.perm[__builtin_ctzll(m)];
+    }
     const uint64_t rflip = pass.flip & ~tmask;
-    T *dst = psi + (base | (at_out ^ (pass.flip & tmask)));
+    T *dst = dst_psi + (base_out | (at_out ^ (pass.flip & tmask)));
 #pragma unroll
     for (int r = 0; r < E; r++) {
         uint64_t o = 0;
@@ -1041,7 +1050,8 @@ __global__ __launch_bounds__(64) void k_wave_pass(typename V2<R>::T *__restrict_
 // 2-3% faster passes than the identity order on 30-qubit states.
 constexpr int kTileChunkLog2 = 8;
 
-template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
+template <typename R, int K, int TB>
+void launch_pass(State &s, const RegPass &p, const RegOp *ops, hipStream_t st, void *src, void *dst) {
     using T = typename V2<R>::T;
     const int tile_bits = (int)s.n - K;
     const uint64_t tiles = uint64_t(1) << tile_bits;
@@ -1049,43 +1059,47 @@ template <typename R, int K, int TB> void launch_pass(State &s, const RegPass &p
     q.chunk_log2 = xcd_chunk_log2(tiles, kTileChunkLog2);
     if constexpr (TB == kLaneBits) {
         static_assert(K <= (sizeof(R) == 4 ? 12 : 11), "one-wave tiles: at most 64 amplitude components per lane");
-        hipLaunchKernelGGL((k_wave_pass<R, K>), dim3((unsigned)tiles), dim3(64), 0, st, (T *)s.d, q, ops);
+        hipLaunchKernelGGL((k_wave_pass<R, K>), dim3((unsigned)tiles), dim3(64), 0, st, (const T *)src, (T *)dst, q, ops);
     } else {
-        hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)s.d, q, ops);
+        hipLaunchKernelGGL((k_reg_pass<R, K, TB>), dim3((unsigned)tiles), dim3(1 << TB), 0, st, (T *)dst, q, ops);
     }
 }
 
-template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st) {
+template <typename R, int K> bool launch_k(State &s, const RegPass &p, const RegOp *ops, hipStream_t st, void *src, void *dst) {
     switch (p.tb) {
     case 6:
-        if constexpr (K <= (sizeof(R) == 4 ? 12 : 11)) { launch_pass<R, K, 6>(s, p, ops, st); return true; }
+        if constexpr (K <= (sizeof(R) == 4 ? 12 : 11)) { launch_pass<R, K, 6>(s, p, ops, st, src, dst); return true; }
         return false;
-    case 8: launch_pass<R, K, 8>(s, p, ops, st); return true;
-    case 9: launch_pass<R, K, 9>(s, p, ops, st); return true;
+    case 8: launch_pass<R, K, 8>(s, p, ops, st, src, dst); return true;
+    case 9: launch_pass<R, K, 9>(s, p, ops, st, src, dst); return true;
     default: return false;
     }
 }
 
 }  // namespace
 
-Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st) {
+Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_t st, void *src, void *dst) {
     // one workgroup per tile: the grid's work-item count must stay below 2^32
     if (p.K < 10 || (int)s.n - p.K + p.tb >= 32)
         return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width too small for this state (grid over 2^32 threads)");
+    // moving or permuting passes exist for one-wave tiles only
+    if ((p.oop || p.permute) && (p.tb != kLaneBits || src == dst || !src || !dst))
+        return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "out-of-place pass: one-wave tiles and two buffers");
+    if (!p.oop && src != dst) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "in-place pass with two buffers");
     bool ok = false;
     if (s.prec == QI_C128) {
         switch (p.K) {
-        case 10: ok = launch_k<double, 10>(s, p, dops, st); break;
-        case 11: ok = launch_k<double, 11>(s, p, dops, st); break;
-        case 12: ok = launch_k<double, 12>(s, p, dops, st); break;
+        case 10: ok = launch_k<double, 10>(s, p, dops, st, src, dst); break;
+        case 11: ok = launch_k<double, 11>(s, p, dops, st, src, dst); break;
+        case 12: ok = launch_k<double, 12>(s, p, dops, st, src, dst); break;
         default: break;
         }
     } else {
         switch (p.K) {
-        case 10: ok = launch_k<float, 10>(s, p, dops, st); break;
-        case 11: ok = launch_k<float, 11>(s, p, dops, st); break;
-        case 12: ok = launch_k<float, 12>(s, p, dops, st); break;
-        case 13: ok = launch_k<float, 13>(s, p, dops, st); break;
+        case 10: ok = launch_k<float, 10>(s, p, dops, st, src, dst); break;
+        case 11: ok = launch_k<float, 11>(s, p, dops, st, src, dst); break;
+        case 12: ok = launch_k<float, 12>(s, p, dops, st, src, dst); break;
+        case 13: ok = launch_k<float, 13>(s, p, dops, st, src, dst); break;
         default: break;
         }
     }

@@ -11,13 +11,14 @@
 // op, a wrong store layout) shows as a difference far above rounding when
 // the output is compared with the oracle on the same input.
 //
-//   tile_emu <gates.bin> <n> <prec 64|128> <K> <TB> <in.bin> <out.bin> [angles.bin]
+//   tile_emu <gates.bin> <n> <prec 64|128> <K> <TB> <in.bin> <out.bin> [--angles a.bin] [--oop]
 //
-// gates.bin: qi_gate records; in/out: 2^n complex128; angles.bin (optional):
-// float64 angle per slot, where gate g's slot is g's index among the RX /
-// RY / RZ / P gates; the plan is built with every such gate parametric and
-// its ops are rewritten with qi::param_op_coeffs before execution.
-// Prints the plan's step count. No HIP call is made.
+// gates.bin: qi_gate records; in/out: 2^n complex128; --angles: float64
+// angle per slot, where gate g's slot is g's index among the RX / RY / RZ / P
+// gates; the plan is built with every such gate parametric and its ops are
+// rewritten with qi::param_op_coeffs before execution. --oop: the planner may
+// use a second state buffer (moving and permuting passes). Prints the plan's
+// step count and its number of moving passes. No HIP call is made.
 #include <complex>
 #include <cstdint>
 #include <cstdio>
@@ -259,7 +260,8 @@ void lds_gate(std::vector<cd> &loc, const RegOp &op) {
     }
 }
 
-void run_pass(std::vector<cd> &psi, int n, const RegPass &p, const std::vector<RegOp> &ops) {
+// Loads from src, stores to dst (the same vector for an in-place pass).
+void run_pass(const std::vector<cd> &src, std::vector<cd> &dst, int n, const RegPass &p, const std::vector<RegOp> &ops) {
     Tile X;
     X.TB = p.tb; X.RB = p.K - p.tb; X.T = 1 << X.TB; X.E = 1 << X.RB;
     X.v.assign(size_t(1) << p.K, cd());
@@ -275,7 +277,7 @@ void run_pass(std::vector<cd> &psi, int n, const RegPass &p, const std::vector<R
                 uint64_t a = base;
                 for (int i = 0; i < X.TB; i++) a |= uint64_t((t >> i) & 1u) << p.qt_in[i];
                 for (int j = 0; j < X.RB; j++) a |= uint64_t((r >> j) & 1u) << p.qr_in[j];
-                X.at(t, r) = psi[a];
+                X.at(t, r) = src[a];
             }
         for (int o = 0; o < p.nops; o++) {
             const RegOp &op = ops[size_t(p.op0) + o];
@@ -309,7 +311,12 @@ void run_pass(std::vector<cd> &psi, int n, const RegPass &p, const std::vector<R
                 uint64_t a = 0;
                 for (int i = 0; i < X.TB; i++) a |= uint64_t((t >> i) & 1u) << p.qt_out[i];
                 for (int j = 0; j < X.RB; j++) a |= uint64_t((r >> j) & 1u) << p.qr_out[j];
-                psi[base | (a ^ p.flip)] = X.at(t, r);  // the Pauli frame's X part flips tile bits
+                uint64_t b = base;
+                if (p.permute) {  // a permuting pass moves the base bits too
+                    b = 0;
+                    for (uint64_t m = base; m; m &= m - 1) b |= uint64_t(1) << p.perm[__builtin_ctzll(m)];
+                }
+                dst[b | (a ^ p.flip)] = X.at(t, r);  // the Pauli frame's X part flips tile bits
             }
     }
 }
@@ -320,8 +327,15 @@ bool angled(int kind) { return kind == QI_GATE_RX || kind == QI_GATE_RY || kind 
 
 int main(int argc, char **argv) {
     if (argc < 8) {
-        fprintf(stderr, "usage: tile_emu gates.bin n prec K TB in.bin out.bin [angles.bin]\n");
+        fprintf(stderr, "usage: tile_emu gates.bin n prec K TB in.bin out.bin [--angles a.bin] [--oop]\n");
         return 2;
+    }
+    const char *angles_path = nullptr;
+    bool oop = false;
+    for (int a = 8; a < argc; a++) {
+        if (!strcmp(argv[a], "--oop")) oop = true;
+        else if (!strcmp(argv[a], "--angles") && a + 1 < argc) angles_path = argv[++a];
+        else { fprintf(stderr, "argument %s\n", argv[a]); return 2; }
     }
     const std::vector<qi_gate> gates = read_all<qi_gate>(argv[1]);
     const int n = atoi(argv[2]);
@@ -332,7 +346,7 @@ int main(int argc, char **argv) {
     std::vector<cd> psi = read_all<cd>(argv[6]);
     if (psi.size() != (size_t(1) << n)) { fprintf(stderr, "state size\n"); return 2; }
     std::vector<double> angles;
-    if (argc > 8) angles = read_all<double>(argv[8]);
+    if (angles_path) angles = read_all<double>(angles_path);
     std::vector<Lowered> seg;
     int32_t slot = 0;
     for (const qi_gate &g : gates) {
@@ -343,7 +357,7 @@ int main(int argc, char **argv) {
         if (L.kind != LK_NOP) seg.push_back(L);
     }
     ExecPlan plan;
-    const Status st = plan_tiled(s, seg, K, TB, plan);
+    const Status st = plan_tiled(s, seg, K, TB, plan, oop);
     if (!st.ok()) { fprintf(stderr, "plan_tiled: %s\n", st.msg.c_str()); return 3; }
     if (!angles.empty())
         for (RegOp &op : plan.reg_ops)
@@ -351,14 +365,26 @@ int main(int argc, char **argv) {
                 if (op.param > (int)angles.size()) { fprintf(stderr, "slot out of range\n"); return 3; }
                 param_op_coeffs(op, angles[size_t(op.param) - 1]);
             }
+    // the executor's buffer tracking: a moving pass switches to the other buffer
+    std::vector<cd> other(psi.size());
+    std::vector<cd> *cur = &psi, *alt = &other;
+    int moves = 0;
     for (const ExecStep &step : plan.steps) {
-        if (step.type == ExecStep::GATE) apply_lowered(psi, step.L);
-        else if (step.type == ExecStep::REG) run_pass(psi, n, step.reg, plan.reg_ops);
-        else { fprintf(stderr, "step type %d\n", (int)step.type); return 3; }
+        if (step.type == ExecStep::GATE) apply_lowered(*cur, step.L);
+        else if (step.type == ExecStep::REG) {
+            if (step.reg.oop) {
+                run_pass(*cur, *alt, n, step.reg, plan.reg_ops);
+                std::swap(cur, alt);
+                moves++;
+            } else {
+                run_pass(*cur, *cur, n, step.reg, plan.reg_ops);
+            }
+        } else { fprintf(stderr, "step type %d\n", (int)step.type); return 3; }
     }
+    if (cur != &psi) { fprintf(stderr, "plan ends in the second buffer\n"); return 3; }
     FILE *f = fopen(argv[7], "wb");
     if (!f || fwrite(psi.data(), sizeof(cd), psi.size(), f) != psi.size()) { fprintf(stderr, "write\n"); return 2; }
     fclose(f);
-    printf("%zu\n", plan.steps.size());
+    printf("%zu %d\n", plan.steps.size(), moves);
     return 0;
 }

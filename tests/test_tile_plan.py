@@ -35,7 +35,7 @@ def emu(tmp_path_factory):
     return exe
 
 
-def run_emu(emu, tmp_path, gates, psi, n, prec, K, TB, angles=None):
+def run_emu(emu, tmp_path, gates, psi, n, prec, K, TB, angles=None, oop=False, full=False):
     g, i, o = tmp_path / "g.bin", tmp_path / "in.bin", tmp_path / "out.bin"
     g.write_bytes(np.ascontiguousarray(gates).tobytes())
     i.write_bytes(np.ascontiguousarray(psi, dtype=np.complex128).tobytes())
@@ -43,10 +43,14 @@ def run_emu(emu, tmp_path, gates, psi, n, prec, K, TB, angles=None):
     if angles is not None:
         a = tmp_path / "angles.bin"
         a.write_bytes(np.asarray(angles, dtype=np.float64).tobytes())
-        args.append(str(a))
+        args += ["--angles", str(a)]
+    if oop:
+        args.append("--oop")
     r = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
-    return np.frombuffer(o.read_bytes(), dtype=np.complex128), int(r.stdout.split()[0])
+    out = np.frombuffer(o.read_bytes(), dtype=np.complex128)
+    steps, moves = (int(x) for x in r.stdout.split()[:2])
+    return (out, steps, moves) if full else (out, steps)
 
 
 # (K, TB) pairs the executor accepts: one-wave tiles (TB 6) up to 12 / 11
@@ -105,3 +109,20 @@ def test_parametric_ops_rewritten_for_new_angles(emu, tmp_path, prec):
     psi = random_state(n, rng)
     got, _ = run_emu(emu, tmp_path, gates, psi, n, prec, 12 if prec == "c64" else 11, 6, angles=angles)
     assert np.max(np.abs(got - O.apply_gates(psi, moved))) <= 1e-11
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_out_of_place_plan_with_final_permutation(emu, tmp_path, prec):
+    """With a second buffer, the SWAP-relabelled plan moves the state to it in
+    its first pass and back through the final qubit permutation in one last
+    pass (base bits permuted at the store): the result is the oracle's, and the
+    plan is shorter than the in-place one."""
+    n = 20
+    gates = W.random_circuit(n=n, depth=40, seed=W.SEED_BASE + 2)
+    psi = random_state(n, np.random.default_rng(9))
+    K = 10  # 10-qubit tiles: the in-place final permutation needs several passes at 20 qubits
+    got, steps, moves = run_emu(emu, tmp_path, gates, psi, n, prec, K, 6, oop=True, full=True)
+    _, steps_inplace, moves_inplace = run_emu(emu, tmp_path, gates, psi, n, prec, K, 6, full=True)
+    assert moves == 2 and moves_inplace == 0
+    assert steps < steps_inplace, (steps, steps_inplace)
+    assert np.max(np.abs(got - O.apply_gates(psi, gates))) <= 1e-10
