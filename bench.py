@@ -462,9 +462,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--only", default="", choices=["", "tiled", "gate", "copy", "surrogate"],
-                    help="profiling runs: only the headline executor, only the single-gate kernels, only a plain "
-                         "copy of known size (the reference for HBM byte counters), or only the surrogate")
+    ap.add_argument("--only", default="", choices=["", "tiled", "tiled128", "gate", "copy", "surrogate"],
+                    help="profiling runs: only the headline executor (complex64, or complex128 with tiled128), only "
+                         "the single-gate kernels, only a plain copy of known size (the reference for HBM byte "
+                         "counters), or only the surrogate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
@@ -506,6 +507,12 @@ def main():
         return
 
     tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)
+    if args.only == "tiled128":
+        wall, dev_ms, passes, norm = run_circuit(qiron, L, n, "c128", gates, tiled, args.steps, args.warmup)
+        pass_ms = dev_ms / args.steps / max(passes, 1)
+        log(f"c128 tiled: {wall / args.steps * 1e3:.1f} ms/step, {G * args.steps / wall:.1f} gates/s, {passes} passes, "
+            f"{pass_ms:.3f} ms/pass, {2 * (1 << n) * 16 / (pass_ms * 1e-3) / 1e9:.0f} GB/s per pass, norm {norm:.6f}")
+        return
     # the library's complex64 defaults: 4096-amplitude tiles on one wave (512 threads for 8192 amplitudes)
     tile_k = args.tile_qubits or 12
     tile_tb = 9 if tile_k >= 13 else 6

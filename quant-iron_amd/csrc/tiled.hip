@@ -14,8 +14,9 @@
 // 4096 amplitudes, 64 per lane) has no wave bits: no LDS and no barriers;
 // its only layout changes are lane/register exchanges at the end of a pass.
 //
-// Arithmetic: complex128 keeps each product of the reference's formulas (no
-// contraction); complex64 uses fused multiply-adds.
+// Arithmetic: both precisions use fused multiply-adds in the register ops
+// (complex128 tiles agree with the reference's unfused products to about
+// 1e-15 per gate); the LDS gates keep the reference's products.
 #include "qi_internal.h"
 #include "exec_plan.h"
 
@@ -101,29 +102,8 @@ __device__ __forceinline__ double xlane(int addr, double x) {
 }
 template <typename T> __device__ __forceinline__ T xlane2(int addr, T x) { return mk<T>(xlane(addr, x.x), xlane(addr, x.y)); }
 
-// y = m x for real m
-template <typename T, typename R> __device__ __forceinline__ T rmul_add(R m0, T a, R m1, T b) {
-    if constexpr (sizeof(R) == 4) return mk<T>(__builtin_fmaf(m1, b.x, m0 * a.x), __builtin_fmaf(m1, b.y, m0 * a.y));
-    else return mk<T>(m0 * a.x + m1 * b.x, m0 * a.y + m1 * b.y);
-}
-// m0 a + m1 b for complex m0 = (k0, k1), m1 = (k2, k3)
-template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R *k, T a, T b) {
-    if constexpr (sizeof(R) == 4) {
-        float x = __builtin_fmaf(-k[1], a.y, k[0] * a.x);
-        x = __builtin_fmaf(k[2], b.x, x);
-        x = __builtin_fmaf(-k[3], b.y, x);
-        float y = __builtin_fmaf(k[1], a.x, k[0] * a.y);
-        y = __builtin_fmaf(k[2], b.y, y);
-        y = __builtin_fmaf(k[3], b.x, y);
-        return mk<T>(x, y);
-    } else {
-        return mk<T>((k[0] * a.x - k[1] * a.y) + (k[2] * b.x - k[3] * b.y),
-                     (k[0] * a.y + k[1] * a.x) + (k[2] * b.y + k[3] * b.x));
-    }
-}
-
-// ------------------------------------------------ in-place float updates --
-// A complex64 tile lives in registers across a loop whose body dispatches on
+// ------------------------------------------------------ in-place updates --
+// A tile lives in registers across a loop whose body dispatches on
 // the op kind. Each op writes every amplitude component back into the
 // register it was read from (tied asm operands), so all op paths leave the
 // tile in the same registers and no copies are needed where they merge.
@@ -165,39 +145,112 @@ __device__ __forceinline__ double selm(uint64_t mask, double x, double y) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 __device__ __forceinline__ void ip_xor(float &d, uint32_t m) { asm("v_xor_b32 %0, %1, %0" : "+v"(d) : "v"(m)); }
-__device__ __forceinline__ void ip_xor3(float &d, float x, uint32_t m) {  // d = x ^ m (old d dead)
-    asm("v_xor_b32 %0, %1, %2" : "+v"(d) : "v"(m), "v"(x));
+__device__ __forceinline__ void ip_xor3(float &d, float x, float m) {  // d = x ^ m (bit patterns; old d dead)
+    asm("v_xor_b32 %0, %1, %2" : "+v"(d) : "v"(__float_as_uint(m)), "v"(x));
+}
+__device__ __forceinline__ float xor_bits(float x, float m) { return __uint_as_float(__float_as_uint(x) ^ __float_as_uint(m)); }
+__device__ __forceinline__ float xor_sign(float x, uint32_t m) { return __uint_as_float(__float_as_uint(x) ^ m); }
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// The same in-place updates for complex128 tiles. FP64 arithmetic works on
+// register pairs; moves, selects and sign flips work on the 32-bit halves
+// (each half tied to its own register, so the pair stays in place).
+__device__ __forceinline__ uint32_t lo32(double x) { return (uint32_t)(uint64_t)__double_as_longlong(x); }
+__device__ __forceinline__ uint32_t hi32(double x) { return (uint32_t)((uint64_t)__double_as_longlong(x) >> 32); }
+__device__ __forceinline__ double mk64(uint32_t lo, uint32_t hi) {
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+__device__ __forceinline__ void ip_fma(double &d, double k, double t) {  // d = k * d + t
+    asm("v_fma_f64 %0, %1, %0, %2" : "+v"(d) : "v"(k), "v"(t));
+}
+__device__ __forceinline__ void ip_fma3(double &d, double k, double x, double t) {  // d = k * x + t (old d dead)
+    asm("v_fma_f64 %0, %1, %2, %3" : "+v"(d) : "v"(k), "v"(x), "v"(t));
+}
+__device__ __forceinline__ void ip_fmac(double &d, double k, double x) {  // d += k * x
+    asm("v_fmac_f64 %0, %1, %2" : "+v"(d) : "v"(k), "v"(x));
+}
+__device__ __forceinline__ void ip_set(double &d, double x) { asm("v_mov_b64 %0, %1" : "+v"(d) : "v"(x)); }
+__device__ __forceinline__ void ip_swap(double &x, double &y) {
+    uint32_t xl = lo32(x), xh = hi32(x), yl = lo32(y), yh = hi32(y);
+    asm("v_swap_b32 %0, %1" : "+v"(xl), "+v"(yl));
+    asm("v_swap_b32 %0, %1" : "+v"(xh), "+v"(yh));
+    x = mk64(xl, xh);
+    y = mk64(yl, yh);
+}
+__device__ __forceinline__ void ip_xor(double &d, uint32_t m) {  // sign-word XOR
+    uint32_t h = hi32(d);
+    asm("v_xor_b32 %0, %1, %0" : "+v"(h) : "v"(m));
+    d = mk64(lo32(d), h);
+}
+__device__ __forceinline__ void ip_neg(double &d) {
+    uint32_t h = hi32(d);
+    asm("v_xor_b32 %0, 0x80000000, %0" : "+v"(h));
+    d = mk64(lo32(d), h);
+}
+__device__ __forceinline__ void ip_sel(double &d, double x, uint64_t mask) {  // d = lane in mask ? x : d
+    uint32_t l = lo32(d), h = hi32(d);
+    asm("v_cndmask_b32 %0, %0, %1, %2" : "+v"(l) : "v"(lo32(x)), "s"(mask));
+    asm("v_cndmask_b32 %0, %0, %1, %2" : "+v"(h) : "v"(hi32(x)), "s"(mask));
+    d = mk64(l, h);
+}
+__device__ __forceinline__ void ip_xor3(double &d, double x, double m) {  // d = x ^ m (bit patterns; old d dead)
+    uint32_t l = lo32(d), h = hi32(d);
+    asm("v_xor_b32 %0, %1, %2" : "+v"(l) : "v"(lo32(m)), "v"(lo32(x)));
+    asm("v_xor_b32 %0, %1, %2" : "+v"(h) : "v"(hi32(m)), "v"(hi32(x)));
+    d = mk64(l, h);
+}
+__device__ __forceinline__ double xor_bits(double x, double m) {
+    return __longlong_as_double(__double_as_longlong(x) ^ __double_as_longlong(m));
+}
+__device__ __forceinline__ double xor_sign(double x, uint32_t m) {
+    return __longlong_as_double(__double_as_longlong(x) ^ (long long)((uint64_t)m << 32));
+}
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// the sign-flip pattern of a component as a value of its own type
+template <typename R> __device__ __forceinline__ R sign_bits(bool neg) {
+    if constexpr (sizeof(R) == 4) return __uint_as_float(neg ? 0x80000000u : 0u);
+    else return __longlong_as_double(neg ? (long long)0x8000000000000000ull : 0ll);
 }
 
+// m0 a + m1 b for complex m0 = (k0, k1), m1 = (k2, k3), as a new value
+template <typename T, typename R> __device__ __forceinline__ T cmul_add(const R *k, T a, T b) {
+    R x = fma_r(-k[1], a.y, k[0] * a.x);
+    x = fma_r(k[2], b.x, x);
+    x = fma_r(-k[3], b.y, x);
+    R y = fma_r(k[1], a.x, k[0] * a.y);
+    y = fma_r(k[2], b.y, y);
+    y = fma_r(k[3], b.x, y);
+    return mk<T>(x, y);
+}
 // (a, b) <- (m0 a + m1 b, m2 a + m3 b), real m, one component
-__device__ __forceinline__ void ip_rr(float &a, float &b, const float *k) {
-    const float t1 = k[2] * a, t2 = k[1] * b;
+template <typename R> __device__ __forceinline__ void ip_rr(R &a, R &b, const R *k) {
+    const R t1 = k[2] * a, t2 = k[1] * b;
     ip_fma(b, k[3], t1);
     ip_fma(a, k[0], t2);
 }
 // (a, b) <- (c0 a + c1 b, c2 a + c3 b), complex c_i = (k[2i], k[2i+1]); nk = -k
-__device__ __forceinline__ void ip_gen(float &ax, float &ay, float &bx, float &by, const float *k, const float *nk) {
-    const float p = __builtin_fmaf(nk[1], ay, k[0] * ax), q = __builtin_fmaf(k[0], ay, k[1] * ax);
-    const float r = __builtin_fmaf(nk[5], ay, k[4] * ax), s = __builtin_fmaf(k[4], ay, k[5] * ax);
+template <typename R> __device__ __forceinline__ void ip_gen(R &ax, R &ay, R &bx, R &by, const R *k, const R *nk) {
+    const R p = fma_r(nk[1], ay, k[0] * ax), q = fma_r(k[0], ay, k[1] * ax);
+    const R r = fma_r(nk[5], ay, k[4] * ax), s = fma_r(k[4], ay, k[5] * ax);
     ip_fma3(ax, k[2], bx, p);
     ip_fmac(ax, nk[3], by);
     ip_fma3(ay, k[2], by, q);
     ip_fmac(ay, k[3], bx);
-    const float z2 = __builtin_fmaf(nk[7], by, r), z = __builtin_fmaf(k[6], by, s);
+    const R z2 = fma_r(nk[7], by, r), z = fma_r(k[6], by, s);
     ip_fma3(by, k[7], bx, z);
     ip_fma(bx, k[6], z2);
 }
 // (a, b) <- (c0 a + i c1 b, i c2 a + c3 b), real c; nk = -k
-__device__ __forceinline__ void ip_rxi(float &ax, float &ay, float &bx, float &by, const float *k, const float *nk) {
-    const float t1 = nk[1] * by, t2 = k[1] * bx, t3 = nk[2] * ay, t4 = k[2] * ax;
+template <typename R> __device__ __forceinline__ void ip_rxi(R &ax, R &ay, R &bx, R &by, const R *k, const R *nk) {
+    const R t1 = nk[1] * by, t2 = k[1] * bx, t3 = nk[2] * ay, t4 = k[2] * ax;
     ip_fma(ax, k[0], t1);
     ip_fma(ay, k[0], t2);
     ip_fma(bx, k[3], t3);
     ip_fma(by, k[3], t4);
 }
 // a <- (px, py) a
-__device__ __forceinline__ void ip_cmul(float &ax, float &ay, float px, float py, float npy) {
-    const float t1 = npy * ay, t2 = py * ax;
+template <typename R> __device__ __forceinline__ void ip_cmul(R &ax, R &ay, R px, R py, R npy) {
+    const R t1 = npy * ay, t2 = py * ax;
     ip_fma(ax, px, t1);
     ip_fma(ay, px, t2);
 }
@@ -216,75 +269,50 @@ template <typename R> __device__ __forceinline__ R coef(const RegOp &op, int q) 
 // thread-held controls are all 1; ops without controls compute straight.
 template <int B, int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rcm, bool tok) {
-    if constexpr (sizeof(R) == 4) {
-        float nk[8];
+    R nk[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) nk[q] = -k[q];
-        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
-#pragma unroll
-        for (int r = 0; r < E; r++) {
-            if (r & (1 << B)) continue;
-            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
-            float &bx = v.d[2 * (r | (1 << B))], &by = v.d[2 * (r | (1 << B)) + 1];
-            if constexpr (!CT) {
-                if constexpr (KIND == RO_X) { ip_swap(ax, bx); ip_swap(ay, by); }
-                else if constexpr (KIND == RO_RR) { ip_rr(ax, bx, k); ip_rr(ay, by, k); }
-                else if constexpr (KIND == RO_RXI) ip_rxi(ax, ay, bx, by, k, nk);
-                else if constexpr (KIND == RO_Y) {  // (a, b) <- (-i b, i a)
-                    ip_swap(ax, by); ip_swap(ay, bx);
-                    ip_neg(ay); ip_neg(bx);
-                } else ip_gen(ax, ay, bx, by, k, nk);
-            } else {
-                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;  // uniform per register
-                if constexpr (KIND == RO_X) {  // controlled swap: new b first, then a and b in place
-                    const float tx = selm(m, ax, bx), ty = selm(m, ay, by);
-                    ip_sel(ax, bx, m); ip_sel(ay, by, m);
-                    ip_set(bx, tx); ip_set(by, ty);
-                    continue;
-                }
-                if constexpr (KIND == RO_Y) {  // all four new values first, then in place
-                    const float n0 = selm(m, by, ax), n1 = selm(m, -bx, ay), n2 = selm(m, -ay, bx), n3 = selm(m, ax, by);
-                    ip_set(ax, n0); ip_set(ay, n1); ip_set(bx, n2); ip_set(by, n3);
-                    continue;
-                }
-                float nax, nay, nbx, nby;
-                if constexpr (KIND == RO_RR) {
-                    nax = __builtin_fmaf(k[1], bx, k[0] * ax); nbx = __builtin_fmaf(k[3], bx, k[2] * ax);
-                    nay = __builtin_fmaf(k[1], by, k[0] * ay); nby = __builtin_fmaf(k[3], by, k[2] * ay);
-                } else if constexpr (KIND == RO_RXI) {
-                    nax = __builtin_fmaf(nk[1], by, k[0] * ax); nay = __builtin_fmaf(k[1], bx, k[0] * ay);
-                    nbx = __builtin_fmaf(nk[2], ay, k[3] * bx); nby = __builtin_fmaf(k[2], ax, k[3] * by);
-                } else {
-                    const float2 na = cmul_add<float2, float>(k, mk<float2>(ax, ay), mk<float2>(bx, by));
-                    const float2 nb = cmul_add<float2, float>(k + 4, mk<float2>(ax, ay), mk<float2>(bx, by));
-                    nax = na.x; nay = na.y; nbx = nb.x; nby = nb.y;
-                }
-                ip_sel(ax, nax, m); ip_sel(ay, nay, m); ip_sel(bx, nbx, m); ip_sel(by, nby, m);
-            }
-        }
-        return;
-    }
+    for (int q = 0; q < 8; q++) nk[q] = -k[q];
+    const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
 #pragma unroll
     for (int r = 0; r < E; r++) {
         if (r & (1 << B)) continue;
-        const T a = v.get(r), b = v.get(r | (1 << B));
-        T na, nb;
-        if constexpr (KIND == RO_X) { na = b; nb = a; }
-        else if constexpr (KIND == RO_RR) { na = rmul_add<T, R>(k[0], a, k[1], b); nb = rmul_add<T, R>(k[2], a, k[3], b); }
-        else if constexpr (KIND == RO_RXI) {
-            na = mk<T>(k[0] * a.x - k[1] * b.y, k[0] * a.y + k[1] * b.x);
-            nb = mk<T>(k[3] * b.x - k[2] * a.y, k[3] * b.y + k[2] * a.x);
-        } else if constexpr (KIND == RO_Y) { na = mk<T>(b.y, -b.x); nb = mk<T>(-a.y, a.x); }
-        else { na = cmul_add<T, R>(k, a, b); nb = cmul_add<T, R>(k + 4, a, b); }
-        if constexpr (CT) {
-            const bool on = tok && ((uint32_t(r) & rcm) == rcm);
-            na = sel(on, na, a);
-            nb = sel(on, nb, b);
+        R &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+        R &bx = v.d[2 * (r | (1 << B))], &by = v.d[2 * (r | (1 << B)) + 1];
+        if constexpr (!CT) {
+            if constexpr (KIND == RO_X) { ip_swap(ax, bx); ip_swap(ay, by); }
+            else if constexpr (KIND == RO_RR) { ip_rr(ax, bx, k); ip_rr(ay, by, k); }
+            else if constexpr (KIND == RO_RXI) ip_rxi(ax, ay, bx, by, k, nk);
+            else if constexpr (KIND == RO_Y) {  // (a, b) <- (-i b, i a)
+                ip_swap(ax, by); ip_swap(ay, bx);
+                ip_neg(ay); ip_neg(bx);
+            } else ip_gen(ax, ay, bx, by, k, nk);
+        } else {
+            const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;  // uniform per register
+            if constexpr (KIND == RO_X) {  // controlled swap: new b first, then a and b in place
+                const R tx = selm(m, ax, bx), ty = selm(m, ay, by);
+                ip_sel(ax, bx, m); ip_sel(ay, by, m);
+                ip_set(bx, tx); ip_set(by, ty);
+                continue;
+            }
+            if constexpr (KIND == RO_Y) {  // all four new values first, then in place
+                const R n0 = selm(m, by, ax), n1 = selm(m, -bx, ay), n2 = selm(m, -ay, bx), n3 = selm(m, ax, by);
+                ip_set(ax, n0); ip_set(ay, n1); ip_set(bx, n2); ip_set(by, n3);
+                continue;
+            }
+            R nax, nay, nbx, nby;
+            if constexpr (KIND == RO_RR) {
+                nax = fma_r(k[1], bx, k[0] * ax); nbx = fma_r(k[3], bx, k[2] * ax);
+                nay = fma_r(k[1], by, k[0] * ay); nby = fma_r(k[3], by, k[2] * ay);
+            } else if constexpr (KIND == RO_RXI) {
+                nax = fma_r(nk[1], by, k[0] * ax); nay = fma_r(k[1], bx, k[0] * ay);
+                nbx = fma_r(nk[2], ay, k[3] * bx); nby = fma_r(k[2], ax, k[3] * by);
+            } else {
+                const T na = cmul_add<T, R>(k, mk<T>(ax, ay), mk<T>(bx, by));
+                const T nb = cmul_add<T, R>(k + 4, mk<T>(ax, ay), mk<T>(bx, by));
+                nax = na.x; nay = na.y; nbx = nb.x; nby = nb.y;
+            }
+            ip_sel(ax, nax, m); ip_sel(ay, nay, m); ip_sel(bx, nbx, m); ip_sel(by, nby, m);
         }
-        v.set(r, na);
-        v.set(r | (1 << B), nb);
-        // one pair at a time: each result can take its input's register
-        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -292,67 +320,51 @@ __device__ __forceinline__ void pairs_reg(Regs<R, E> &v, const R *k, uint32_t rc
 // the matrix (own coefficient first).
 template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void pairs_lane(Regs<R, E> &v, int addr, const R *k, uint32_t rcm, bool tok) {
-    if constexpr (sizeof(R) == 4) {
-        const float nk1 = -k[1], nk3 = -k[3];
-        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
-        // partners fetched a group of registers ahead: the exchange latency
-        // overlaps the arithmetic of the previous registers
-        constexpr int G = E < 4 ? E : 4;
+    const R nk1 = -k[1], nk3 = -k[3];
+    const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
+    // partners fetched a group of registers ahead: the exchange latency
+    // overlaps the arithmetic of the previous registers
+    constexpr int G0 = sizeof(R) == 4 ? 4 : 2, G = E < G0 ? E : G0;  // 8 partner registers
 #pragma unroll
-        for (int r0 = 0; r0 < E; r0 += G) {
-            float px[G], py[G];
+    for (int r0 = 0; r0 < E; r0 += G) {
+        R px[G], py[G];
 #pragma unroll
-            for (int g = 0; g < G; g++) { px[g] = xlane(addr, v.d[2 * (r0 + g)]); py[g] = xlane(addr, v.d[2 * (r0 + g) + 1]); }
+        for (int g = 0; g < G; g++) { px[g] = xlane(addr, v.d[2 * (r0 + g)]); py[g] = xlane(addr, v.d[2 * (r0 + g) + 1]); }
 #pragma unroll
-            for (int g = 0; g < G; g++) {
-                const int r = r0 + g;
-                float &sx = v.d[2 * r], &sy = v.d[2 * r + 1];
-                if constexpr (!CT) {
-                    if constexpr (KIND == RO_X) { ip_set(sx, px[g]); ip_set(sy, py[g]); }
-                    else if constexpr (KIND == RO_RR) { ip_fma(sx, k[0], k[1] * px[g]); ip_fma(sy, k[0], k[1] * py[g]); }
-                    else if constexpr (KIND == RO_RXI) { ip_fma(sx, k[0], nk1 * py[g]); ip_fma(sy, k[0], k[1] * px[g]); }
-                    else if constexpr (KIND == RO_Y) {  // own = (p.y, -p.x) on target-0 lanes, (-p.y, p.x) on target-1
-                        ip_xor3(sx, py[g], __float_as_uint(k[0]));
-                        ip_xor3(sy, px[g], __float_as_uint(k[1]));
-                    } else {
-                        const float ux = __builtin_fmaf(nk3, py[g], k[2] * px[g]), uy = __builtin_fmaf(k[3], px[g], k[2] * py[g]);
-                        const float z = __builtin_fmaf(nk1, sy, ux), w = __builtin_fmaf(k[1], sx, uy);
-                        ip_fma(sy, k[0], w);
-                        ip_fma(sx, k[0], z);
-                    }
+        for (int g = 0; g < G; g++) {
+            const int r = r0 + g;
+            R &sx = v.d[2 * r], &sy = v.d[2 * r + 1];
+            if constexpr (!CT) {
+                if constexpr (KIND == RO_X) { ip_set(sx, px[g]); ip_set(sy, py[g]); }
+                else if constexpr (KIND == RO_RR) { ip_fma(sx, k[0], k[1] * px[g]); ip_fma(sy, k[0], k[1] * py[g]); }
+                else if constexpr (KIND == RO_RXI) { ip_fma(sx, k[0], nk1 * py[g]); ip_fma(sy, k[0], k[1] * px[g]); }
+                else if constexpr (KIND == RO_Y) {  // own = (p.y, -p.x) on target-0 lanes, (-p.y, p.x) on target-1
+                    ip_xor3(sx, py[g], k[0]);
+                    ip_xor3(sy, px[g], k[1]);
                 } else {
-                    const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
-                    float nx, ny;
-                    if constexpr (KIND == RO_X) { nx = px[g]; ny = py[g]; }
-                    else if constexpr (KIND == RO_RR) {
-                        nx = __builtin_fmaf(k[1], px[g], k[0] * sx); ny = __builtin_fmaf(k[1], py[g], k[0] * sy);
-                    } else if constexpr (KIND == RO_RXI) {
-                        nx = __builtin_fmaf(nk1, py[g], k[0] * sx); ny = __builtin_fmaf(k[1], px[g], k[0] * sy);
-                    } else if constexpr (KIND == RO_Y) {
-                        nx = __uint_as_float(__float_as_uint(py[g]) ^ __float_as_uint(k[0]));
-                        ny = __uint_as_float(__float_as_uint(px[g]) ^ __float_as_uint(k[1]));
-                    } else {
-                        const float2 n = cmul_add<float2, float>(k, mk<float2>(sx, sy), mk<float2>(px[g], py[g]));
-                        nx = n.x; ny = n.y;
-                    }
-                    ip_sel(sx, nx, m); ip_sel(sy, ny, m);
+                    const R ux = fma_r(nk3, py[g], k[2] * px[g]), uy = fma_r(k[3], px[g], k[2] * py[g]);
+                    const R z = fma_r(nk1, sy, ux), w = fma_r(k[1], sx, uy);
+                    ip_fma(sy, k[0], w);
+                    ip_fma(sx, k[0], z);
                 }
+            } else {
+                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+                R nx, ny;
+                if constexpr (KIND == RO_X) { nx = px[g]; ny = py[g]; }
+                else if constexpr (KIND == RO_RR) {
+                    nx = fma_r(k[1], px[g], k[0] * sx); ny = fma_r(k[1], py[g], k[0] * sy);
+                } else if constexpr (KIND == RO_RXI) {
+                    nx = fma_r(nk1, py[g], k[0] * sx); ny = fma_r(k[1], px[g], k[0] * sy);
+                } else if constexpr (KIND == RO_Y) {
+                    nx = xor_bits(py[g], k[0]);
+                    ny = xor_bits(px[g], k[1]);
+                } else {
+                    const T n = cmul_add<T, R>(k, mk<T>(sx, sy), mk<T>(px[g], py[g]));
+                    nx = n.x; ny = n.y;
+                }
+                ip_sel(sx, nx, m); ip_sel(sy, ny, m);
             }
         }
-        return;
-    }
-#pragma unroll
-    for (int r = 0; r < E; r++) {
-        const T s = v.get(r);
-        const T p = xlane2<T>(addr, s);
-        T n;
-        if constexpr (KIND == RO_X) n = p;
-        else if constexpr (KIND == RO_RR) n = rmul_add<T, R>(k[0], s, k[1], p);
-        else if constexpr (KIND == RO_RXI) n = mk<T>(k[0] * s.x - k[1] * p.y, k[0] * s.y + k[1] * p.x);
-        else if constexpr (KIND == RO_Y) n = k[0] > 0 ? mk<T>(p.y, -p.x) : mk<T>(-p.y, p.x);
-        else n = cmul_add<T, R>(k, s, p);
-        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, s);
-        v.set(r, n);
     }
 }
 
@@ -366,13 +378,9 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
         if constexpr (KIND == RO_RR || KIND == RO_RXI) {  // row 0: (m00, m01); row 1: (m11, m10)
             k[0] = bit ? coef<R>(op, 3) : coef<R>(op, 0);
             k[1] = bit ? coef<R>(op, 2) : coef<R>(op, 1);
-        } else if constexpr (KIND == RO_Y) {
-            if constexpr (sizeof(R) == 4) {  // sign masks for the two components
-                k[0] = __uint_as_float(bit ? 0x80000000u : 0u);
-                k[1] = __uint_as_float(bit ? 0u : 0x80000000u);
-            } else {
-                k[0] = bit ? R(-1) : R(1);
-            }
+        } else if constexpr (KIND == RO_Y) {  // sign patterns for the two components
+            k[0] = sign_bits<R>(bit);
+            k[1] = sign_bits<R>(!bit);
         } else if constexpr (KIND == RO_GEN) {
             k[0] = bit ? coef<R>(op, 6) : coef<R>(op, 0);
             k[1] = bit ? coef<R>(op, 7) : coef<R>(op, 1);
@@ -395,89 +403,49 @@ __device__ __forceinline__ void pair_op(Regs<R, E> &v, const RegOp &op, uint32_t
 // compile time (RO_Z and RO_P1 leave the target-0 registers alone).
 template <int B, int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void diag_reg(Regs<R, E> &v, T p0, T p1, uint32_t rcm, bool tok) {
-    if constexpr (sizeof(R) == 4) {
-        const float n0 = -p0.y, n1 = -p1.y;
-        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
-#pragma unroll
-        for (int r = 0; r < E; r++) {
-            const bool one = ((r >> B) & 1) != 0;
-            if (!one && KIND != RO_DG) continue;
-            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
-            const T p = one ? p1 : p0;
-            if constexpr (!CT) {
-                if constexpr (KIND == RO_Z) { ip_neg(ax); ip_neg(ay); }
-                else ip_cmul(ax, ay, p.x, p.y, one ? n1 : n0);
-            } else {
-                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
-                if constexpr (KIND == RO_Z) { ip_sel(ax, -ax, m); ip_sel(ay, -ay, m); }
-                else {
-                    const float2 n = cmul(mk<float2>(ax, ay), mk<float2>(p.x, p.y));
-                    ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
-                }
-            }
-        }
-        return;
-    }
+    const R n0 = -p0.y, n1 = -p1.y;
+    const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
 #pragma unroll
     for (int r = 0; r < E; r++) {
         const bool one = ((r >> B) & 1) != 0;
         if (!one && KIND != RO_DG) continue;
-        const T a = v.get(r);
-        T n;
-        if constexpr (KIND == RO_Z) n = mk<T>(-a.x, -a.y);
-        else n = cmul(a, one ? p1 : p0);
-        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, a);
-        v.set(r, n);
+        R &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+        const T p = one ? p1 : p0;
+        if constexpr (!CT) {
+            if constexpr (KIND == RO_Z) { ip_neg(ax); ip_neg(ay); }
+            else ip_cmul(ax, ay, p.x, p.y, one ? n1 : n0);
+        } else {
+            const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+            if constexpr (KIND == RO_Z) { ip_sel(ax, -ax, m); ip_sel(ay, -ay, m); }
+            else {
+                const T n = cmul(mk<T>(ax, ay), p);
+                ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
+            }
+        }
     }
 }
 // Target fixed for the whole thread (thread bit or outside the tile).
 template <int KIND, bool CT, int E, typename T, typename R>
 __device__ __forceinline__ void diag_thr(Regs<R, E> &v, T f, bool one, uint32_t rcm, bool tok) {
-    if constexpr (sizeof(R) == 4) {
-        const uint32_t sgn = one ? 0x80000000u : 0u;  // Z: sign flip on the target-1 lanes
-        const float nfy = -f.y;
-        const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
-#pragma unroll
-        for (int r = 0; r < E; r++) {
-            float &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
-            if constexpr (!CT) {
-                if constexpr (KIND == RO_Z) { ip_xor(ax, sgn); ip_xor(ay, sgn); }
-                else ip_cmul(ax, ay, f.x, f.y, nfy);
-            } else {
-                const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
-                if constexpr (KIND == RO_Z) {
-                    ip_sel(ax, __uint_as_float(__float_as_uint(ax) ^ sgn), m);
-                    ip_sel(ay, __uint_as_float(__float_as_uint(ay) ^ sgn), m);
-                } else {
-                    const float2 n = cmul(mk<float2>(ax, ay), mk<float2>(f.x, f.y));
-                    ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
-                }
-            }
-        }
-        return;
-    }
-    if constexpr (KIND == RO_Z && !CT) {
-        // sign flip as an XOR of the sign bits
-        const uint32_t m = one ? 0x80000000u : 0u;
-#pragma unroll
-        for (int r = 0; r < E; r++) {
-            const T a = v.get(r);
-            if constexpr (sizeof(R) == 4)
-                v.set(r, mk<T>(__uint_as_float(__float_as_uint(a.x) ^ m), __uint_as_float(__float_as_uint(a.y) ^ m)));
-            else
-                v.set(r, mk<T>(__longlong_as_double(__double_as_longlong(a.x) ^ (long long)(uint64_t(m) << 32)),
-                               __longlong_as_double(__double_as_longlong(a.y) ^ (long long)(uint64_t(m) << 32))));
-        }
-        return;
-    }
+    const uint32_t sgn = one ? 0x80000000u : 0u;  // Z: sign flip on the target-1 lanes
+    const R nfy = -f.y;
+    const uint64_t tokm = CT ? __builtin_amdgcn_ballot_w64(tok) : 0;
 #pragma unroll
     for (int r = 0; r < E; r++) {
-        const T a = v.get(r);
-        T n;
-        if constexpr (KIND == RO_Z) n = sel(one, mk<T>(-a.x, -a.y), a);
-        else n = cmul(a, f);
-        if constexpr (CT) n = sel(tok && ((uint32_t(r) & rcm) == rcm), n, a);
-        v.set(r, n);
+        R &ax = v.d[2 * r], &ay = v.d[2 * r + 1];
+        if constexpr (!CT) {
+            if constexpr (KIND == RO_Z) { ip_xor(ax, sgn); ip_xor(ay, sgn); }
+            else ip_cmul(ax, ay, f.x, f.y, nfy);
+        } else {
+            const uint64_t m = (uint32_t(r) & rcm) == rcm ? tokm : 0;
+            if constexpr (KIND == RO_Z) {
+                ip_sel(ax, xor_sign(ax, sgn), m);
+                ip_sel(ay, xor_sign(ay, sgn), m);
+            } else {
+                const T n = cmul(mk<T>(ax, ay), f);
+                ip_sel(ax, n.x, m); ip_sel(ay, n.y, m);
+            }
+        }
     }
 }
 
@@ -494,13 +462,13 @@ __device__ __forceinline__ void diag_op(Regs<R, E> &v, const RegOp &op, uint32_t
     diag_thr<KIND, CT, E, T, R>(v, one ? p1 : p0, one, rcm, tok);
 }
 
-// ------------------------------- single-register-control X and Z (c64) --
+// ------------------------------------ single-register-control X and Z --
 // CNOT, Toffoli and CZ whose one in-tile control sits on register bit C (no
 // lane controls): registers without bit C are skipped at compile time and
 // the rest is updated uncontrolled, instead of computing every register and
 // selecting under a mask.
-template <int B, int C, int E>
-__device__ __forceinline__ void x_reg_c1(Regs<float, E> &v) {
+template <int B, int C, int E, typename R>
+__device__ __forceinline__ void x_reg_c1(Regs<R, E> &v) {
 #pragma unroll
     for (int r = 0; r < E; r++) {
         if (((r >> B) & 1) || !((r >> C) & 1)) continue;
@@ -509,9 +477,9 @@ __device__ __forceinline__ void x_reg_c1(Regs<float, E> &v) {
         ip_swap(v.d[2 * r + 1], v.d[2 * s + 1]);
     }
 }
-template <int C, int E>
-__device__ __forceinline__ void x_lane_c1(Regs<float, E> &v, int addr) {
-    constexpr int H = E / 2, G = H < 4 ? H : 4;
+template <int C, int E, typename R>
+__device__ __forceinline__ void x_lane_c1(Regs<R, E> &v, int addr) {
+    constexpr int G0 = sizeof(R) == 4 ? 4 : 2, H = E / 2, G = H < G0 ? H : G0;
     int rs[H];
     {
         int n = 0;
@@ -520,75 +488,75 @@ __device__ __forceinline__ void x_lane_c1(Regs<float, E> &v, int addr) {
     }
 #pragma unroll
     for (int g0 = 0; g0 < H; g0 += G) {
-        float px[G], py[G];
+        R px[G], py[G];
 #pragma unroll
         for (int g = 0; g < G; g++) { px[g] = xlane(addr, v.d[2 * rs[g0 + g]]); py[g] = xlane(addr, v.d[2 * rs[g0 + g] + 1]); }
 #pragma unroll
         for (int g = 0; g < G; g++) { ip_set(v.d[2 * rs[g0 + g]], px[g]); ip_set(v.d[2 * rs[g0 + g] + 1], py[g]); }
     }
 }
-template <int B, int C, int E>
-__device__ __forceinline__ void z_reg_c1(Regs<float, E> &v) {
+template <int B, int C, int E, typename R>
+__device__ __forceinline__ void z_reg_c1(Regs<R, E> &v) {
 #pragma unroll
     for (int r = 0; r < E; r++)
         if (((r >> B) & 1) && ((r >> C) & 1)) { ip_neg(v.d[2 * r]); ip_neg(v.d[2 * r + 1]); }
 }
-template <int C, int E>
-__device__ __forceinline__ void z_thr_c1(Regs<float, E> &v, uint32_t sgn) {
+template <int C, int E, typename R>
+__device__ __forceinline__ void z_thr_c1(Regs<R, E> &v, uint32_t sgn) {
 #pragma unroll
     for (int r = 0; r < E; r++)
         if ((r >> C) & 1) { ip_xor(v.d[2 * r], sgn); ip_xor(v.d[2 * r + 1], sgn); }
 }
-template <int B, int E> __device__ __forceinline__ void x_reg_c1_any(Regs<float, E> &v, int c) {
+template <int B, int E, typename R> __device__ __forceinline__ void x_reg_c1_any(Regs<R, E> &v, int c) {
     constexpr int RB = __builtin_ctz(E);
     switch (c) {
-    case 0: if constexpr (0 < RB && B != 0) x_reg_c1<B, 0, E>(v); break;
-    case 1: if constexpr (1 < RB && B != 1) x_reg_c1<B, 1, E>(v); break;
-    case 2: if constexpr (2 < RB && B != 2) x_reg_c1<B, 2, E>(v); break;
-    case 3: if constexpr (3 < RB && B != 3) x_reg_c1<B, 3, E>(v); break;
-    case 4: if constexpr (4 < RB && B != 4) x_reg_c1<B, 4, E>(v); break;
-    case 5: if constexpr (5 < RB && B != 5) x_reg_c1<B, 5, E>(v); break;
+    case 0: if constexpr (0 < RB && B != 0) x_reg_c1<B, 0, E, R>(v); break;
+    case 1: if constexpr (1 < RB && B != 1) x_reg_c1<B, 1, E, R>(v); break;
+    case 2: if constexpr (2 < RB && B != 2) x_reg_c1<B, 2, E, R>(v); break;
+    case 3: if constexpr (3 < RB && B != 3) x_reg_c1<B, 3, E, R>(v); break;
+    case 4: if constexpr (4 < RB && B != 4) x_reg_c1<B, 4, E, R>(v); break;
+    case 5: if constexpr (5 < RB && B != 5) x_reg_c1<B, 5, E, R>(v); break;
     default: break;
     }
 }
-template <int B, int E> __device__ __forceinline__ void z_reg_c1_any(Regs<float, E> &v, int c) {
+template <int B, int E, typename R> __device__ __forceinline__ void z_reg_c1_any(Regs<R, E> &v, int c) {
     constexpr int RB = __builtin_ctz(E);
     switch (c) {
-    case 0: if constexpr (0 < RB && B != 0) z_reg_c1<B, 0, E>(v); break;
-    case 1: if constexpr (1 < RB && B != 1) z_reg_c1<B, 1, E>(v); break;
-    case 2: if constexpr (2 < RB && B != 2) z_reg_c1<B, 2, E>(v); break;
-    case 3: if constexpr (3 < RB && B != 3) z_reg_c1<B, 3, E>(v); break;
-    case 4: if constexpr (4 < RB && B != 4) z_reg_c1<B, 4, E>(v); break;
-    case 5: if constexpr (5 < RB && B != 5) z_reg_c1<B, 5, E>(v); break;
+    case 0: if constexpr (0 < RB && B != 0) z_reg_c1<B, 0, E, R>(v); break;
+    case 1: if constexpr (1 < RB && B != 1) z_reg_c1<B, 1, E, R>(v); break;
+    case 2: if constexpr (2 < RB && B != 2) z_reg_c1<B, 2, E, R>(v); break;
+    case 3: if constexpr (3 < RB && B != 3) z_reg_c1<B, 3, E, R>(v); break;
+    case 4: if constexpr (4 < RB && B != 4) z_reg_c1<B, 4, E, R>(v); break;
+    case 5: if constexpr (5 < RB && B != 5) z_reg_c1<B, 5, E, R>(v); break;
     default: break;
     }
 }
-template <int E> __device__ __forceinline__ void xc1_op(Regs<float, E> &v, const RegOp &op, uint32_t lane) {
+template <int E, typename R> __device__ __forceinline__ void xc1_op(Regs<R, E> &v, const RegOp &op, uint32_t lane) {
     constexpr int RB = __builtin_ctz(E);
     const int c = op.lp0;
     if (op.tgt >= kLaneTarget) {
         const int addr = int((lane ^ (1u << (op.tgt - kLaneTarget))) << 2);
         switch (c) {
-        case 0: if constexpr (0 < RB) x_lane_c1<0, E>(v, addr); break;
-        case 1: if constexpr (1 < RB) x_lane_c1<1, E>(v, addr); break;
-        case 2: if constexpr (2 < RB) x_lane_c1<2, E>(v, addr); break;
-        case 3: if constexpr (3 < RB) x_lane_c1<3, E>(v, addr); break;
-        case 4: if constexpr (4 < RB) x_lane_c1<4, E>(v, addr); break;
-        case 5: if constexpr (5 < RB) x_lane_c1<5, E>(v, addr); break;
+        case 0: if constexpr (0 < RB) x_lane_c1<0, E, R>(v, addr); break;
+        case 1: if constexpr (1 < RB) x_lane_c1<1, E, R>(v, addr); break;
+        case 2: if constexpr (2 < RB) x_lane_c1<2, E, R>(v, addr); break;
+        case 3: if constexpr (3 < RB) x_lane_c1<3, E, R>(v, addr); break;
+        case 4: if constexpr (4 < RB) x_lane_c1<4, E, R>(v, addr); break;
+        case 5: if constexpr (5 < RB) x_lane_c1<5, E, R>(v, addr); break;
         default: break;
         }
         return;
     }
-#define QI_RB(BB) case BB: if constexpr (BB < RB) x_reg_c1_any<BB, E>(v, c); break;
+#define QI_RB(BB) case BB: if constexpr (BB < RB) x_reg_c1_any<BB, E, R>(v, c); break;
     switch (op.tgt) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
 }
-template <int E>
-__device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint32_t tid, uint64_t base) {
+template <int E, typename R>
+__device__ __forceinline__ void zc1_op(Regs<R, E> &v, const RegOp &op, uint32_t tid, uint64_t base) {
     constexpr int RB = __builtin_ctz(E);
     const int c = op.lp0;
     if (op.dloc == 0) {
-#define QI_RB(BB) case BB: if constexpr (BB < RB) z_reg_c1_any<BB, E>(v, c); break;
+#define QI_RB(BB) case BB: if constexpr (BB < RB) z_reg_c1_any<BB, E, R>(v, c); break;
         switch (op.dbit) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
         return;
@@ -596,12 +564,12 @@ __device__ __forceinline__ void zc1_op(Regs<float, E> &v, const RegOp &op, uint3
     const bool one = op.dloc == 1 ? ((tid >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
     const uint32_t sgn = one ? 0x80000000u : 0u;
     switch (c) {
-    case 0: if constexpr (0 < RB) z_thr_c1<0, E>(v, sgn); break;
-    case 1: if constexpr (1 < RB) z_thr_c1<1, E>(v, sgn); break;
-    case 2: if constexpr (2 < RB) z_thr_c1<2, E>(v, sgn); break;
-    case 3: if constexpr (3 < RB) z_thr_c1<3, E>(v, sgn); break;
-    case 4: if constexpr (4 < RB) z_thr_c1<4, E>(v, sgn); break;
-    case 5: if constexpr (5 < RB) z_thr_c1<5, E>(v, sgn); break;
+    case 0: if constexpr (0 < RB) z_thr_c1<0, E, R>(v, sgn); break;
+    case 1: if constexpr (1 < RB) z_thr_c1<1, E, R>(v, sgn); break;
+    case 2: if constexpr (2 < RB) z_thr_c1<2, E, R>(v, sgn); break;
+    case 3: if constexpr (3 < RB) z_thr_c1<3, E, R>(v, sgn); break;
+    case 4: if constexpr (4 < RB) z_thr_c1<4, E, R>(v, sgn); break;
+    case 5: if constexpr (5 < RB) z_thr_c1<5, E, R>(v, sgn); break;
     default: break;
     }
 }
@@ -684,11 +652,11 @@ template <int E> __device__ __forceinline__ void hs_op(Regs<float, E> &v, const 
 // the amplitude at (lane bit i = a, register bit B = b) moves to
 // (lane bit i = b, register bit B = a); lanes with lane bit 1 send their
 // register-bit-0 value, lanes with lane bit 0 their register-bit-1 value.
-template <int B, int E>
-__device__ __forceinline__ void xlr_reg(Regs<float, E> &v, uint32_t lane, int i) {
+template <int B, int E, typename R>
+__device__ __forceinline__ void xlr_reg(Regs<R, E> &v, uint32_t lane, int i) {
     const int addr = int((lane ^ (1u << i)) << 2);
     const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0), lo = ~hi;
-    constexpr int G = 4;  // exchanges issued a group ahead of their use
+    constexpr int G = sizeof(R) == 4 ? 4 : 2;  // exchanges issued a group ahead of their use
     int rs[E / 2];
     {
         int n = 0;
@@ -697,7 +665,7 @@ __device__ __forceinline__ void xlr_reg(Regs<float, E> &v, uint32_t lane, int i)
     }
 #pragma unroll
     for (int g0 = 0; g0 < E / 2; g0 += G) {
-        float y[G][2];
+        R y[G][2];
 #pragma unroll
         for (int g = 0; g < G && g0 + g < E / 2; g++)
 #pragma unroll
@@ -715,60 +683,27 @@ __device__ __forceinline__ void xlr_reg(Regs<float, E> &v, uint32_t lane, int i)
             }
     }
 }
-template <int E>
-__device__ __forceinline__ void xlr_op(Regs<float, E> &v, uint32_t lane, int i, int j) {
+template <int E, typename R>
+__device__ __forceinline__ void xlr_op(Regs<R, E> &v, uint32_t lane, int i, int j) {
     constexpr int RB = __builtin_ctz(E);
-#define QI_RB(BB) case BB: if constexpr (BB < RB) xlr_reg<BB, E>(v, lane, i); break;
+#define QI_RB(BB) case BB: if constexpr (BB < RB) xlr_reg<BB, E, R>(v, lane, i); break;
     switch (j) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
 #undef QI_RB
 }
 // Lane bits i <-> i2: lanes whose two bits differ trade with lane ^ (2^i | 2^i2).
-template <int E>
-__device__ __forceinline__ void xll_op(Regs<float, E> &v, uint32_t lane, int i, int i2) {
+template <int E, typename R>
+__device__ __forceinline__ void xll_op(Regs<R, E> &v, uint32_t lane, int i, int i2) {
     const int addr = int((lane ^ ((1u << i) | (1u << i2))) << 2);
     const uint64_t m = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != ((lane >> i2) & 1u));
-    constexpr int G = 8;
+    constexpr int G = sizeof(R) == 4 ? 8 : 4;
 #pragma unroll
     for (int k0 = 0; k0 < 2 * E; k0 += G) {
-        float y[G];
+        R y[G];
 #pragma unroll
         for (int g = 0; g < G; g++) y[g] = xlane(addr, v.d[k0 + g]);
 #pragma unroll
         for (int g = 0; g < G; g++) ip_sel(v.d[k0 + g], y[g], m);
     }
-}
-
-// The same exchanges for complex128 one-wave tiles (plain selects: 32
-// amplitudes per lane).
-template <int B, typename R, int E>
-__device__ __forceinline__ void xlr_reg_g(Regs<R, E> &v, uint32_t lane, int i) {
-    const int addr = int((lane ^ (1u << i)) << 2);
-    const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0);
-#pragma unroll
-    for (int r = 0; r < E; r++) {
-        if (r & (1 << B)) continue;
-        const int s = r | (1 << B);
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const R y = xlane(addr, selm(hi, v.d[2 * r + c], v.d[2 * s + c]));
-            v.d[2 * r + c] = selm(hi, y, v.d[2 * r + c]);
-            v.d[2 * s + c] = selm(hi, v.d[2 * s + c], y);
-        }
-    }
-}
-template <typename R, int E>
-__device__ __forceinline__ void xlr_op_g(Regs<R, E> &v, uint32_t lane, int i, int j) {
-    constexpr int RB = __builtin_ctz(E);
-#define QI_RB(BB) case BB: if constexpr (BB < RB) xlr_reg_g<BB, R, E>(v, lane, i); break;
-    switch (j) { QI_RB(0) QI_RB(1) QI_RB(2) QI_RB(3) QI_RB(4) QI_RB(5) default: break; }
-#undef QI_RB
-}
-template <typename R, int E>
-__device__ __forceinline__ void xll_op_g(Regs<R, E> &v, uint32_t lane, int i, int i2) {
-    const int addr = int((lane ^ ((1u << i) | (1u << i2))) << 2);
-    const uint64_t m = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != ((lane >> i2) & 1u));
-#pragma unroll
-    for (int k = 0; k < 2 * E; k++) v.d[k] = selm(m, xlane(addr, v.d[k]), v.d[k]);
 }
 
 // ------------------------------------------------------------- LDS ops --
@@ -824,14 +759,8 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
 #define ARGS tid, base
         QI_OP(RO_Z, diag_op) QI_OP(RO_DG, diag_op) QI_OP(RO_P1, diag_op)
 #undef ARGS
-    case RO_XC1:
-        if constexpr (sizeof(R) == 4) xc1_op<E>(v, op, lane);
-        else pair_op<RO_X, true, RB, E, T, R>(v, op, lane, op.rcm, tok);
-        break;
-    case RO_ZC1:
-        if constexpr (sizeof(R) == 4) zc1_op<E>(v, op, tid, base);
-        else diag_op<RO_Z, true, RB, E, T, R>(v, op, tid, base, op.rcm, tok);
-        break;
+    case RO_XC1: xc1_op<E, R>(v, op, lane); break;
+    case RO_ZC1: zc1_op<E, R>(v, op, tid, base); break;
     case RO_RRS:
         if constexpr (sizeof(R) == 4) shear_op<false, E>(v, op);
         break;
@@ -845,12 +774,10 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         if constexpr (sizeof(R) == 4) p1s_op<E>(v, op);
         break;
     case RO_XLR:
-        if constexpr (TB == kLaneBits && sizeof(R) == 4) xlr_op<E>(v, lane, op.tgt, op.dbit);
-        else if constexpr (TB == kLaneBits) xlr_op_g<R, E>(v, lane, op.tgt, op.dbit);
+        if constexpr (TB == kLaneBits) xlr_op<E, R>(v, lane, op.tgt, op.dbit);
         break;
     case RO_XLL:
-        if constexpr (TB == kLaneBits && sizeof(R) == 4) xll_op<E>(v, lane, op.tgt, op.dbit);
-        else if constexpr (TB == kLaneBits) xll_op_g<R, E>(v, lane, op.tgt, op.dbit);
+        if constexpr (TB == kLaneBits) xll_op<E, R>(v, lane, op.tgt, op.dbit);
         break;
     default: break;
     }
