@@ -65,25 +65,31 @@ enum RegOpKind : int32_t {
                        // uncontrolled diagonal
     RO_XLR,            // exchange the local bits of lane bit tgt and register bit dbit
     RO_XLL,            // exchange the local bits of lane bits tgt and dbit
+    RO_LPH,            // one-wave tiles: multiply the per-lane pending phase by p0 = (c0, c1) where the target
+                       // (lane bit dbit for dloc 1, qubit dbit for dloc 2) is 0 and p1 = (c2, c3) where it is 1,
+                       // on the lanes whose thread-index bits tcm are all 1 (a diagonal gate deferred)
+    RO_LPF,            // multiply every amplitude by the per-lane pending phase and reset it (lp0 = 1: the
+                       // phase is a sign, applied as a sign-bit XOR)
     RO_TRANSPOSE,      // new layout through LDS
     RO_LDS,            // gate on the LDS copy (controlled SWAP, Matchgate)
     RO_NOP             // closes each pass's slice (read ahead, never run)
 };
 
 struct RegOp {            // every field dword-sized: the kernel reads ops with scalar loads
+    // fields the register kernels read come first (one 32-byte scalar load, then gcm)
     int32_t kind;        // RegOpKind
-    int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
     int32_t tgt;         // pair ops: register bit, or kLaneTarget + lane bit
     int32_t dloc;        // diagonal ops: 0 register bit (dbit = its index), 1 thread bit (dbit = thread-index bit),
                          //               2 outside the tile (dbit = qubit)
     int32_t dbit;
-    int32_t param;       // angle slot + 1 of a parametric op (RO_RR: RY, RO_RXI: RX, RO_DG: RZ, RO_P1: P); 0: none
-    int32_t lp0, lp1;    // RO_LDS: local bits of the targets
+    int32_t lp0, lp1;    // RO_LDS: local bits of the targets; RO_XC1 / RO_ZC1: lp0 = the control's register bit
     uint32_t rcm;        // controls on register bits (register-bit mask)
     uint32_t tcm;        // controls on thread bits (mask over thread-index bits)
+    uint64_t gcm;        // controls outside the tile (qubit mask)
+    int32_t lkind;       // RO_LDS: LK_SWAP or LK_MATCH
+    int32_t param;       // angle slot + 1 of a parametric op (RO_RR: RY, RO_RXI: RX, RO_DG: RZ, RO_P1: P); 0: none
     uint32_t lcm;        // RO_LDS: controls inside the tile (local-bit mask)
     uint32_t src;        // source gate index
-    uint64_t gcm;        // controls outside the tile (qubit mask)
     uint8_t lay_r[8];    // RO_TRANSPOSE: local bit of each register bit
     uint8_t lay_t[12];   //               local bit of each thread bit (also after RO_XLR / RO_XLL)
     uint32_t pad1;

@@ -679,6 +679,69 @@ struct RegPlanner {
     }
 };
 
+// One-wave tiles: a diagonal op whose factor depends on the lane alone
+// (target on a lane bit or outside the tile, no register controls, not
+// parametric) can multiply a per-lane pending phase (RO_LPH: a few
+// instructions per wave) instead of every amplitude of the lane. The phase
+// commutes with every op that moves no amplitude between lanes it depends on,
+// so it is applied to the amplitudes (RO_LPF) just before the first op that
+// does (a 2x2 op on such a lane bit, an exchange of one) or before the store.
+// A run of deferrable ops between two such points is deferred when that costs
+// less than applying each op (in practice: two or more ops).
+static void defer_lane_phases(std::vector<RegOp> &ops) {
+    auto deferrable = [](const RegOp &op) {
+        return (op.kind == RO_Z || op.kind == RO_P1 || op.kind == RO_DG) && op.dloc != 0 && op.rcm == 0 && op.param == 0;
+    };
+    // cost per amplitude in VALU instructions: sign flips or a complex product, plus the selects of lane controls
+    auto cost = [](const RegOp &op) { return (op.kind == RO_Z ? 2.0 : 4.0) + (op.tcm ? 2.0 : 0.0); };
+    auto lanes_moved = [](const RegOp &op) -> uint32_t {
+        if ((op.kind <= RO_Y || op.kind == RO_XC1) && op.tgt >= kLaneTarget) return 1u << (op.tgt - kLaneTarget);
+        if (op.kind == RO_XLR) return 1u << op.tgt;
+        if (op.kind == RO_XLL) return (1u << op.tgt) | (1u << op.dbit);
+        return 0;
+    };
+    std::vector<RegOp> out;
+    out.reserve(ops.size() + 8);
+    std::vector<size_t> run;  // positions in `out` of the run's deferrable ops
+    uint32_t dep = 0;         // lane bits the run's factors depend on
+    bool sign = true;         // every factor of the run is +-1
+    double saved = 0;
+    auto close = [&](size_t at) {
+        const double flush = sign ? 2.0 : 4.0, per_op = 0.25;
+        if (run.size() >= 2 && saved > flush + per_op * (double)run.size()) {
+            for (size_t k : run) {
+                RegOp &o = out[k];
+                double d[4];
+                RegPlanner::diag_of(o, d);
+                o.kind = RO_LPH;
+                for (int q = 0; q < 4; q++) o.c[q] = d[q];
+            }
+            RegOp f{};
+            f.kind = RO_LPF;
+            f.lp0 = sign ? 1 : 0;
+            f.src = out[run.back()].src;
+            out.insert(out.begin() + (std::ptrdiff_t)at, f);
+        }
+        run.clear();
+        dep = 0;
+        sign = true;
+        saved = 0;
+    };
+    for (const RegOp &op : ops) {
+        if (dep & lanes_moved(op)) close(out.size());
+        if (deferrable(op)) {
+            run.push_back(out.size());
+            saved += cost(op);
+            if (op.dloc == 1) dep |= 1u << op.dbit;
+            dep |= op.tcm;
+            sign = sign && op.kind == RO_Z;
+        }
+        out.push_back(op);
+    }
+    close(out.size());
+    ops.swap(out);
+}
+
 struct Phase {
     std::vector<size_t> ops;  // indices into the pass members, in execution order
     uint32_t need = 0;        // local bits the group's 2x2 gates target
@@ -875,6 +938,7 @@ static Status plan_tiled_list(const State &s, const std::vector<Lowered> &gates,
         rp.sync_store = memcmp(rp.qt_out, rp.qt_in, 12) != 0 || memcmp(rp.qr_out, rp.qr_in, 8) != 0 || rp.flip != 0;
         P.out.erase(std::remove_if(P.out.begin(), P.out.end(), [](const RegOp &op) { return op.kind == RO_NOP; }),
                     P.out.end());
+        if (TB == kLaneBits) defer_lane_phases(P.out);
         rp.op0 = (int)plan.reg_ops.size();
         rp.nops = (int)P.out.size();
         for (RegOp &op : P.out) {

@@ -738,11 +738,45 @@ __device__ __forceinline__ void lds_gate(typename V2<R>::T *tile, const RegOp &o
     }
 }
 
+// ------------------------------------------------------ lane phases --
+// One-wave tiles carry a per-lane pending phase (ph, two registers): diagonal
+// gates whose factor depends on the lane alone multiply into it, and it is
+// applied to the lane's amplitudes before anything moves them between lanes
+// (the planner places RO_LPF).
+template <typename R> __device__ __forceinline__ uint32_t sign_word(R x) {
+    if constexpr (sizeof(R) == 4) return __float_as_uint(x) & 0x80000000u;
+    else return hi32(x) & 0x80000000u;
+}
+template <typename R>
+__device__ __forceinline__ void lph_op(R *ph, const RegOp &op, uint64_t base, uint32_t lane) {
+    const bool one = op.dloc == 1 ? ((lane >> op.dbit) & 1u) != 0 : ((base >> op.dbit) & 1u) != 0;
+    const bool on = (lane & op.tcm) == op.tcm;
+    const R fx = on ? (one ? coef<R>(op, 2) : coef<R>(op, 0)) : R(1);
+    const R fy = on ? (one ? coef<R>(op, 3) : coef<R>(op, 1)) : R(0);
+    const R nx = fma_r(ph[0], fx, -(ph[1] * fy)), ny = fma_r(ph[0], fy, ph[1] * fx);
+    ph[0] = nx;
+    ph[1] = ny;
+}
+template <typename R, int E>
+__device__ __forceinline__ void lpf_op(Regs<R, E> &v, R *ph, const RegOp &op) {
+    if (op.lp0) {  // +-1: flip sign bits
+        const uint32_t sgn = sign_word(ph[0]);
+#pragma unroll
+        for (int k = 0; k < 2 * E; k++) ip_xor(v.d[k], sgn);
+    } else {
+        const R npy = -ph[1];
+#pragma unroll
+        for (int r = 0; r < E; r++) ip_cmul(v.d[2 * r], v.d[2 * r + 1], ph[0], ph[1], npy);
+    }
+    ph[0] = R(1);
+    ph[1] = R(0);
+}
+
 // ------------------------------------------------------------ the pass --
 // One in-register op on the thread's share of the tile.
 template <typename R, int K, int TB>
 __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegOp &op, uint64_t base, uint32_t tid,
-                                         uint32_t lane) {
+                                         uint32_t lane, R *ph) {
     using T = typename V2<R>::T;
     constexpr int RB = K - TB, E = 1 << RB;
     const bool tc = (op.tcm | op.rcm) != 0;   // controls inside the tile (uniform)
@@ -778,6 +812,12 @@ __device__ __forceinline__ void apply_op(Regs<R, (1 << (K - TB))> &v, const RegO
         break;
     case RO_XLL:
         if constexpr (TB == kLaneBits) xll_op<E, R>(v, lane, op.tgt, op.dbit);
+        break;
+    case RO_LPH:
+        if constexpr (TB == kLaneBits) lph_op<R>(ph, op, base, lane);
+        break;
+    case RO_LPF:
+        if constexpr (TB == kLaneBits) lpf_op<R, E>(v, ph, op);
         break;
     default: break;
     }
@@ -833,7 +873,7 @@ __global__ __launch_bounds__(1 << TB) void k_reg_pass(typename V2<R>::T *__restr
             if (kind >= RO_TRANSPOSE) break;
             nxt = ops[pass.op0 + o + 1];
             if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-            apply_op<R, K, TB>(v, op, base, tid, lane);
+            apply_op<R, K, TB>(v, op, base, tid, lane, nullptr);  // (no lane phases in LDS tiles)
         }
         if (o >= pass.nops) break;
         // through LDS: a transpose to a new layout, or a gate on the LDS copy
@@ -944,13 +984,13 @@ __global__ __launch_bounds__(64) void k_wave_pass(const typename V2<R>::T *src_p
     }
     // The op table of a pass ends with a no-op, so op o + 1 can be fetched
     // while op o runs. One-wave passes hold no LDS ops (the planner streams them).
+    R ph[2] = {R(1), R(0)};  // the lane's pending phase
     RegOp nxt = ops[pass.op0];
     for (int o = 0; o < pass.nops; o++) {
         const RegOp op = nxt;
         nxt = ops[pass.op0 + o + 1];
-        if (op.kind >= RO_TRANSPOSE) continue;
-        if ((base & op.gcm) != op.gcm) continue;  // controls outside the tile (uniform)
-        apply_op<R, K, TB>(v, op, base, lane, lane);
+        // controls outside the tile are uniform per tile
+        if (op.kind < RO_TRANSPOSE && (base & op.gcm) == op.gcm) apply_op<R, K, TB>(v, op, base, lane, lane, ph);
     }
     // In-place permutation: a store may go where this wave loaded another
     // amplitude, so every load must be complete first.

@@ -272,6 +272,7 @@ void run_pass(const std::vector<cd> &src, std::vector<cd> &dst, int n, const Reg
         const uint64_t base = insert_zeros(b, p.qmask);
         memcpy(X.lay_t, p.lay_t, sizeof X.lay_t);
         memcpy(X.lay_r, p.lay_r, sizeof X.lay_r);
+        std::vector<cd> ph(size_t(X.T), cd(1));  // each lane's pending phase (RO_LPH / RO_LPF)
         for (uint32_t t = 0; t < (uint32_t)X.T; t++)
             for (uint32_t r = 0; r < (uint32_t)X.E; r++) {
                 uint64_t a = base;
@@ -291,6 +292,19 @@ void run_pass(const std::vector<cd> &src, std::vector<cd> &dst, int n, const Reg
             case RO_P1S: p1s_op(X, op); break;
             case RO_XLR: xlr_op(X, op); break;
             case RO_XLL: xll_op(X, op); break;
+            case RO_LPH:
+                for (uint32_t t = 0; t < (uint32_t)X.T; t++) {
+                    if ((t & op.tcm) != op.tcm) continue;
+                    const bool one = op.dloc == 1 ? ((t >> op.dbit) & 1u) : ((base >> op.dbit) & 1u);
+                    ph[t] *= one ? cd(op.c[2], op.c[3]) : cd(op.c[0], op.c[1]);
+                }
+                break;
+            case RO_LPF:
+                for (uint32_t t = 0; t < (uint32_t)X.T; t++) {
+                    for (uint32_t r = 0; r < (uint32_t)X.E; r++) X.at(t, r) *= ph[t];
+                    ph[t] = 1;
+                }
+                break;
             case RO_TRANSPOSE:
                 to_local(X, loc);
                 memcpy(X.lay_t, op.lay_t, sizeof X.lay_t);
