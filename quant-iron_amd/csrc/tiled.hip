@@ -648,12 +648,41 @@ template <int E> __device__ __forceinline__ void hs_op(Regs<float, E> &v, const 
 }
 
 // ------------------------------------------------------- exchange ops --
-// One-wave tiles change layout without LDS. Lane bit i <-> register bit B:
-// the amplitude at (lane bit i = a, register bit B = b) moves to
-// (lane bit i = b, register bit B = a); lanes with lane bit 1 send their
-// register-bit-0 value, lanes with lane bit 0 their register-bit-1 value.
+// a's lanes with lane bit 4 (5) set trade with b's lanes where it is clear:
+// v_permlane16_swap / v_permlane32_swap, one VALU instruction per 32-bit
+// component, both registers updated in place (tied operands, like the other
+// in-place ops). A VALU write of a register that a permlane swap reads needs
+// two wait states before the swap: each swap carries its own s_nop 1, so no
+// instruction the compiler places in between can break that.
+__device__ __forceinline__ void swap_rows32(uint32_t &a, uint32_t &b, bool b5) {
+    if (b5) asm volatile("s_nop 1\n\tv_permlane32_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+    else asm volatile("s_nop 1\n\tv_permlane16_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void swap_rows(float &a, float &b, bool b5) {
+    uint32_t x = __float_as_uint(a), y = __float_as_uint(b);
+    swap_rows32(x, y, b5);
+    a = __uint_as_float(x);
+    b = __uint_as_float(y);
+}
+__device__ __forceinline__ void swap_rows(double &a, double &b, bool b5) {
+    uint32_t al = lo32(a), ah = hi32(a), bl = lo32(b), bh = hi32(b);
+    swap_rows32(al, bl, b5);
+    swap_rows32(ah, bh, b5);
+    a = mk64(al, ah);
+    b = mk64(bl, bh);
+}
 template <int B, int E, typename R>
 __device__ __forceinline__ void xlr_reg(Regs<R, E> &v, uint32_t lane, int i) {
+    if (i >= 4) {  // lane bits 4 and 5: one cross-row swap per component pair, no lane addressing
+        const bool b5 = i == 5;
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            if (r & (1 << B)) continue;
+#pragma unroll
+            for (int c = 0; c < 2; c++) swap_rows(v.d[2 * r + c], v.d[2 * (r | (1 << B)) + c], b5);
+        }
+        return;
+    }
     const int addr = int((lane ^ (1u << i)) << 2);
     const uint64_t hi = __builtin_amdgcn_ballot_w64(((lane >> i) & 1u) != 0), lo = ~hi;
     constexpr int G = sizeof(R) == 4 ? 4 : 2;  // exchanges issued a group ahead of their use
