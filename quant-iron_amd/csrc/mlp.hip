@@ -415,6 +415,43 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ ws
     }
 }
 
+// The epilogue of a layer GEMM whose K was split (few output tiles: small
+// batches): the slices' FP32 partials (ws, slice z at z * M * ldc) summed in
+// slice order, then EPI_GELU (bias, FP32 pre-activation if Cf, BF16 GELU) or
+// EPI_GELU_BWD (times gelu'(Zin), FP32 and BF16), four columns per thread.
+template <int EPI>
+__global__ __launch_bounds__(256) void k_split_epilogue(const float *__restrict__ ws, int splits, int M, int N, int ldc,
+                                                        float *__restrict__ Cf, bf16 *__restrict__ Cb,
+                                                        const float *__restrict__ bias, const float *__restrict__ Zin) {
+    const int q = N / 4;  // N % 4 == 0 (the caller checks)
+    const size_t slice = size_t(M) * ldc;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < size_t(M) * q; i += size_t(gridDim.x) * blockDim.x) {
+        const int row = int(i / q), col = 4 * int(i % q);
+        const size_t o = size_t(row) * ldc + col;
+        float4 a = *reinterpret_cast<const float4 *>(ws + o);
+        for (int z = 1; z < splits; z++) {
+            const float4 b = *reinterpret_cast<const float4 *>(ws + size_t(z) * slice + o);
+            a.x += b.x; a.y += b.y; a.z += b.z; a.w += b.w;
+        }
+        float v[4] = {a.x, a.y, a.z, a.w};
+        if (EPI == EPI_GELU) {
+            const float4 bb = *reinterpret_cast<const float4 *>(bias + col);
+            v[0] += bb.x; v[1] += bb.y; v[2] += bb.z; v[3] += bb.w;
+            if (Cf) *reinterpret_cast<float4 *>(Cf + o) = make_float4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+            for (int r = 0; r < 4; r++) v[r] = gelu(v[r]);
+        } else {
+            const float4 z = *reinterpret_cast<const float4 *>(Zin + o);
+            v[0] *= gelu_grad(z.x); v[1] *= gelu_grad(z.y); v[2] *= gelu_grad(z.z); v[3] *= gelu_grad(z.w);
+            *reinterpret_cast<float4 *>(Cf + o) = make_float4(v[0], v[1], v[2], v[3]);
+        }
+        uint2 pk;
+        pk.x = uint32_t(f2bf(v[0])) | (uint32_t(f2bf(v[1])) << 16);
+        pk.y = uint32_t(f2bf(v[2])) | (uint32_t(f2bf(v[3])) << 16);
+        *reinterpret_cast<uint2 *>(Cb + o) = pk;
+    }
+}
+
 // gather rows idx[0..rows) of a BF16 matrix (cols wide) into a contiguous block
 __global__ __launch_bounds__(256) void k_gather_rows(const bf16 *__restrict__ X, const uint32_t *__restrict__ idx, int rows,
                                                      int cols, bf16 *__restrict__ out, const float *__restrict__ t,
@@ -531,6 +568,7 @@ struct Mlp {
     float *y = nullptr, *t = nullptr, *dy = nullptr, *part = nullptr, *lossp = nullptr;
     bf16 *T1 = nullptr, *T2 = nullptr;                             // transposed GEMM operands (weight gradients)
     float *ws = nullptr;                                           // split-K partials
+    size_t ws_floats() const { return ws ? size_t(kMaxSplits) * size_t(std::max(hid, k_in)) * size_t(hid) : 0; }
     double *lacc = nullptr;                                        // epoch loss accumulator
     float *yall = nullptr;                                         // predictions of a candidate screen
     size_t ycap = 0;
@@ -580,10 +618,36 @@ static Status mlp_alloc(Mlp &N, size_t rows) {
     return Status{};
 }
 
+// Few output tiles (small batches): each tile's K loop is a chain of
+// dependent global-load round trips, so K is split over more workgroups
+// (their FP32 partials in ws, ws_floats long) and k_split_epilogue applies
+// the layer's epilogue.
+constexpr int kSplitBelowTiles = 128;
+
 template <bool TA, bool TB, int EPI>
 static void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int Nn, int K, float *Cf, bf16 *Cb,
-                 int ldc, const float *bias = nullptr, const float *Zin = nullptr) {
+                 int ldc, const float *bias = nullptr, const float *Zin = nullptr, float *ws = nullptr,
+                 size_t ws_floats = 0) {
     dim3 grid((Nn + TN - 1) / TN, (M + TM - 1) / TM);
+    const int tiles = int(grid.x * grid.y);
+    if constexpr (EPI == EPI_GELU || EPI == EPI_GELU_BWD) {
+        if (ws && tiles < kSplitBelowTiles && Nn % 4 == 0 && ldc % 4 == 0 && Cb) {
+            const size_t slice = size_t(M) * ldc;
+            int splits = std::min({kMaxSplits, 256 / tiles, K / (2 * TK), int(ws_floats / slice)});
+            if (splits >= 2) {
+                const int kper = ((K + splits - 1) / splits + 2 * TK - 1) / (2 * TK) * (2 * TK);
+                splits = (K + kper - 1) / kper;
+                grid.z = splits;
+                hipLaunchKernelGGL((k_gemm<TA, TB, EPI_STORE>), grid, dim3(256), 0, st, A, lda, B, ldb, M, Nn, K, ws,
+                                   (bf16 *)nullptr, ldc, (const float *)nullptr, (const float *)nullptr, kper);
+                const size_t items = size_t(M) * (Nn / 4);
+                const unsigned blocks = (unsigned)std::min<size_t>(1024, (items + 255) / 256);
+                hipLaunchKernelGGL((k_split_epilogue<EPI>), dim3(blocks), dim3(256), 0, st, (const float *)ws, splits, M,
+                                   Nn, ldc, Cf, Cb, bias, Zin);
+                return;
+            }
+        }
+    }
     hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), grid, dim3(256), 0, st, A, lda, B, ldb, M, Nn, K, Cf, Cb, ldc, bias, Zin, K);
 }
 
@@ -641,9 +705,13 @@ static void predict_rows(Mlp &N, hipStream_t st, int rows, float *y) {
 static void forward(Mlp &N, hipStream_t st, int rows, bool keepZ) {
     const bf16 *W1t = N.wt + N.off[0], *W2t = N.wt + N.off[2], *W3t = N.wt + N.off[4];
     const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
-    gemm<false, true, EPI_GELU>(st, N.X, N.k_in, W1t, N.k_in, rows, N.hid, N.k_in, keepZ ? N.Z[0] : nullptr, N.H[0], N.hid, b1);
-    gemm<false, true, EPI_GELU>(st, N.H[0], N.hid, W2t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[1] : nullptr, N.H[1], N.hid, b2);
-    gemm<false, true, EPI_GELU>(st, N.H[1], N.hid, W3t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[2] : nullptr, N.H[2], N.hid, b3);
+    const size_t wsf = N.ws_floats();
+    gemm<false, true, EPI_GELU>(st, N.X, N.k_in, W1t, N.k_in, rows, N.hid, N.k_in, keepZ ? N.Z[0] : nullptr, N.H[0], N.hid, b1,
+                                nullptr, N.ws, wsf);
+    gemm<false, true, EPI_GELU>(st, N.H[0], N.hid, W2t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[1] : nullptr, N.H[1], N.hid, b2,
+                                nullptr, N.ws, wsf);
+    gemm<false, true, EPI_GELU>(st, N.H[1], N.hid, W3t, N.hid, rows, N.hid, N.hid, keepZ ? N.Z[2] : nullptr, N.H[2], N.hid, b3,
+                                nullptr, N.ws, wsf);
 }
 
 // Gradients of the mean squared error over `rows` staged rows (targets in N.t) into N.g.
@@ -666,11 +734,13 @@ static void backward(Mlp &N, hipStream_t st, int rows) {
     const bf16 *W2 = N.wb + N.off[2], *W3 = N.wb + N.off[4];
     gemm_at_b(N, st, N.H[1], H, N.dZb[2], H, rows, N.g + N.off[4]);
     colsum(N, st, N.dZf[2], rows, H, N.g + N.off[5]);
-    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[2], H, W3, H, rows, H, H, N.dZf[1], N.dZb[1], H, nullptr, N.Z[1]);
+    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[2], H, W3, H, rows, H, H, N.dZf[1], N.dZb[1], H, nullptr, N.Z[1], N.ws,
+                                    N.ws_floats());
     // layer 2
     gemm_at_b(N, st, N.H[0], H, N.dZb[1], H, rows, N.g + N.off[2]);
     colsum(N, st, N.dZf[1], rows, H, N.g + N.off[3]);
-    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[1], H, W2, H, rows, H, H, N.dZf[0], N.dZb[0], H, nullptr, N.Z[0]);
+    gemm<false, true, EPI_GELU_BWD>(st, N.dZb[1], H, W2, H, rows, H, H, N.dZf[0], N.dZb[0], H, nullptr, N.Z[0], N.ws,
+                                    N.ws_floats());
     // layer 1: dW1 = X^T dZ1, db1
     gemm_at_b(N, st, N.X, N.k_in, N.dZb[0], H, rows, N.g + N.off[0]);
     colsum(N, st, N.dZf[0], rows, H, N.g + N.off[1]);
