@@ -44,6 +44,27 @@ def test_mlp_gradients_match_fp64_oracle(ctx):
         assert rel < 5e-2 and cos > 0.998, name
 
 
+def test_mlp_predictions_match_fp64_oracle(ctx):
+    """The prediction path (two GELU layers, then the third with the head in
+    its epilogue) against the FP64 forward pass on the same BF16 weights and
+    inputs; the engine also rounds the hidden activations to BF16."""
+    n_in, H, rows = 195, 512, 1024
+    net = S.MLP(n_in, H, seed=4)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, size=(rows, n_in)).astype(np.float32)
+    y = net.predict(x).astype(np.float64)
+    p = net.params().astype(np.float64)
+    k_in = net.k_in
+    sizes = [k_in * H, H, H * H, H, H * H, H, H, 1]
+    offs = np.cumsum([0] + sizes)
+    for blk in (0, 2, 4):
+        p[offs[blk]:offs[blk + 1]] = M.bf16_round(p[offs[blk]:offs[blk + 1]])
+    yo = M.forward(p, M.bf16_round(x.astype(np.float64)), k_in, H)[0]
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo - yo.mean())
+    print(f"prediction rel L2 err {rel:.3e}, max abs {np.max(np.abs(y - yo)):.3e}, sd {yo.std():.3e}")
+    assert rel < 2e-2
+
+
 def test_ansatz_energies_match_oracle(ctx):
     n, layers = 5, 2
     sim = S.AnsatzEnergies(n, layers, slots=16, precision="c64")
