@@ -93,25 +93,25 @@ __device__ __forceinline__ float gelu_grad(float z) {
 
 // EPI_GELU_HEAD (prediction only): the last hidden layer's GELU output is not
 // stored; its dot product with the head weights (passed as Zin) over this
-// wave's 64 columns goes to slot 2 bx + wc of the row's partials (Cf, 2 *
+// wave's columns goes to slot WN bx + wc of the row's partials (Cf, WN *
 // gridDim.x per row), summed with the head bias by k_head_fold.
 enum Epi { EPI_GELU = 0, EPI_STORE = 1, EPI_GELU_BWD = 2, EPI_GELU_HEAD = 3 };
 
 constexpr int TM = 128, TN = 128, TK = 64, LDK = TK + 8;  // LDS row: 72 BF16 = 144 bytes (9 x 16)
-constexpr int kChunks = TM * TK / 8 / 256;
-constexpr int kMaxSplits = 16;  // K slices of a weight-gradient GEMM                // 16-byte chunks per thread per operand stage
+constexpr int kMaxSplits = 16;  // K slices of a weight-gradient GEMM
 
-// One operand tile (128 rows of the output dimension x 64 of K) as 16-byte
-// chunks in registers. KC (K contiguous in memory: X row-major M x K, or W^T):
+// One operand tile (ROWS rows of the output dimension x 64 of K) as 16-byte
+// chunks in registers, ROWS * 8 / NT per thread of NT. KC (K contiguous in memory: X row-major M x K, or W^T):
 // chunk c = row c / 8, k = 8 (c % 8). Otherwise (K strided: the operand is
 // stored K x rows): chunk c = k c / 16, rows 8 (c % 16) .. + 7, transposed
 // into LDS on store. Out-of-range rows and k read as zero (K % 8 == 0).
-template <bool KC>
-__device__ __forceinline__ void load_tile(const bf16 *__restrict__ P, int ld, int rows, int K, int r0, int k0, uint4 (&reg)[kChunks]) {
+template <bool KC, int ROWS, int NT>
+__device__ __forceinline__ void load_tile(const bf16 *__restrict__ P, int ld, int rows, int K, int r0, int k0,
+                                          uint4 (&reg)[ROWS * TK / 8 / NT]) {
     const int tid = threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kChunks; i++) {
-        const int c = tid + 256 * i;
+    for (int i = 0; i < ROWS * TK / 8 / NT; i++) {
+        const int c = tid + NT * i;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (KC) {
             const int r = r0 + c / 8, k = k0 + 8 * (c % 8);
@@ -133,12 +133,12 @@ __device__ __forceinline__ void load_tile(const bf16 *__restrict__ P, int ld, in
     }
 }
 
-template <bool KC>
-__device__ __forceinline__ void store_tile(bf16 (*S)[LDK], const uint4 (&reg)[kChunks]) {
+template <bool KC, int ROWS, int NT>
+__device__ __forceinline__ void store_tile(bf16 (*S)[LDK], const uint4 (&reg)[ROWS * TK / 8 / NT]) {
     const int tid = threadIdx.x;
 #pragma unroll
-    for (int i = 0; i < kChunks; i++) {
-        const int c = tid + 256 * i;
+    for (int i = 0; i < ROWS * TK / 8 / NT; i++) {
+        const int c = tid + NT * i;
         if (KC) {
             *reinterpret_cast<uint4 *>(&S[c / 8][8 * (c % 8)]) = reg[i];
         } else {
@@ -153,13 +153,18 @@ __device__ __forceinline__ void store_tile(bf16 (*S)[LDK], const uint4 (&reg)[kC
 
 // C[M x N] = op(A)[M x K] op(B)[K x N]; op(A) = A (row-major M x K, lda) or
 // A^T (A row-major K x M); op(B) = B (row-major K x N) or B^T (B row-major N x K).
-template <bool TA, bool TB, int EPI>
-__global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B, int ldb,
-                                              int M, int N, int K, float *__restrict__ Cf, bf16 *__restrict__ Cb,
-                                              int ldc, const float *__restrict__ bias, const float *__restrict__ Zin,
-                                              int kper) {
-    __shared__ __attribute__((aligned(16))) bf16 As[2][TM][LDK];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[2][TN][LDK];
+// Workgroup tile BM x BN on WM x WN waves (each BM / WM x BN / WN). DEPTH 2:
+// two register sets take the global loads of stages s + 2 and s + 3 (a
+// 128 x 128 tile fits twice per CU); DEPTH 1: one set, stage s + 1 (the
+// 256 x 256 tile of 8 waves, once per CU: 147 KiB of LDS).
+template <bool TA, bool TB, int EPI, int BM = TM, int BN = TN, int WM = 2, int WN = 2, int DEPTH = 2>
+__global__ __launch_bounds__(64 * WM * WN) void k_gemm(const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ B,
+                                                       int ldb, int M, int N, int K, float *__restrict__ Cf,
+                                                       bf16 *__restrict__ Cb, int ldc, const float *__restrict__ bias,
+                                                       const float *__restrict__ Zin, int kper) {
+    constexpr int NT = 64 * WM * WN, FI = BM / WM / 16, FJ = BN / WN / 16, CA = BM * TK / 8 / NT, CB = BN * TK / 8 / NT;
+    __shared__ __attribute__((aligned(16))) bf16 As[2][BM][LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN][LDK];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Workgroup w runs on XCD w % 8. The N tiles of one M tile read the same
     // A rows: they go to one XCD (consecutive r below), so A crosses HBM once
@@ -170,97 +175,120 @@ __global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int ld
         bx = int(r % gridDim.x);
         by = int((r / gridDim.x) * 8 + (w & 7));
     }
-    const int m0 = by * TM, n0 = bx * TN;
+    const int m0 = by * BM, n0 = bx * BN;
     // split K: slice z takes K range [kb, ke) and writes its partial product
     // to slice z of the output (EPI_STORE into an FP32 workspace)
     const int kb = blockIdx.z * kper, ke = min(K, kb + kper);
     if (gridDim.z > 1) Cf += size_t(blockIdx.z) * M * ldc;
-    const int wr = wave >> 1, wc = wave & 1;
-    floatx4 acc[4][4];
+    const int wr = wave / WN, wc = wave % WN;
+    floatx4 acc[FI][FJ];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < FI; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0, 0, 0, 0};
-    // Stage s's global loads are issued two stages ahead (register sets 0 and
-    // 1 alternate, the loop runs two stages per trip so the set is static),
-    // then stored to LDS buffer s % 2 one stage ahead of its use.
-    // The stage count is rounded up to even: stages past K load zeros.
-    uint4 ra0[kChunks], rb0[kChunks], ra1[kChunks], rb1[kChunks];
-    const int nst = ((ke - kb + 2 * TK - 1) / (2 * TK)) * 2;
-    load_tile<!TA>(A, lda, M, ke, m0, kb, ra0);
-    load_tile<TB>(B, ldb, N, ke, n0, kb, rb0);
-    load_tile<!TA>(A, lda, M, ke, m0, kb + TK, ra1);
-    load_tile<TB>(B, ldb, N, ke, n0, kb + TK, rb1);
-    store_tile<!TA>(As[0], ra0);
-    store_tile<TB>(Bs[0], rb0);
-    __syncthreads();
+        for (int j = 0; j < FJ; j++) acc[i][j] = floatx4{0, 0, 0, 0};
     auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 32) {
-            bf16x8 a[4], b[4];
+            bf16x8 a[FI], b[FJ];
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                a[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][wr * 64 + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+            for (int i = 0; i < FI; i++)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&As[cur][wr * (BM / WM) + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-                b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wc * 64 + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+            for (int j = 0; j < FJ; j++)
+                b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wc * (BN / WN) + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < FI; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < FJ; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
     };
-    for (int s = 0; s < nst; s += 2) {
-        // even stage s: LDS buffer 0; set 0 (stage s, stored) takes stage s + 2
-        if (s + 2 < nst) {
-            load_tile<!TA>(A, lda, M, ke, m0, kb + (s + 2) * TK, ra0);
-            load_tile<TB>(B, ldb, N, ke, n0, kb + (s + 2) * TK, rb0);
-        }
-        compute(0);
-        store_tile<!TA>(As[1], ra1);
-        store_tile<TB>(Bs[1], rb1);
+    if constexpr (DEPTH == 1) {
+        // stages past K load zeros
+        const int nst = (ke - kb + TK - 1) / TK;
+        uint4 ra[CA], rb[CB];
+        load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb, ra);
+        load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb, rb);
+        store_tile<!TA, BM, NT>(As[0], ra);
+        store_tile<TB, BN, NT>(Bs[0], rb);
         __syncthreads();
-        // odd stage s + 1: LDS buffer 1; set 1 takes stage s + 3
-        if (s + 3 < nst) {
-            load_tile<!TA>(A, lda, M, ke, m0, kb + (s + 3) * TK, ra1);
-            load_tile<TB>(B, ldb, N, ke, n0, kb + (s + 3) * TK, rb1);
+        for (int s = 0; s < nst; s++) {
+            if (s + 1 < nst) {
+                load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb + (s + 1) * TK, ra);
+                load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb + (s + 1) * TK, rb);
+            }
+            compute(s & 1);
+            if (s + 1 < nst) {
+                store_tile<!TA, BM, NT>(As[(s + 1) & 1], ra);
+                store_tile<TB, BN, NT>(Bs[(s + 1) & 1], rb);
+            }
+            __syncthreads();
         }
-        compute(1);
-        if (s + 2 < nst) {
-            store_tile<!TA>(As[0], ra0);
-            store_tile<TB>(Bs[0], rb0);
-        }
+    } else {
+        // Stage s's global loads are issued two stages ahead (register sets 0
+        // and 1 alternate, the loop runs two stages per trip so the set is
+        // static), then stored to LDS buffer s % 2 one stage ahead of its use.
+        // The stage count is rounded up to even: stages past K load zeros.
+        uint4 ra0[CA], rb0[CB], ra1[CA], rb1[CB];
+        const int nst = ((ke - kb + 2 * TK - 1) / (2 * TK)) * 2;
+        load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb, ra0);
+        load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb, rb0);
+        load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb + TK, ra1);
+        load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb + TK, rb1);
+        store_tile<!TA, BM, NT>(As[0], ra0);
+        store_tile<TB, BN, NT>(Bs[0], rb0);
         __syncthreads();
+        for (int s = 0; s < nst; s += 2) {
+            // even stage s: LDS buffer 0; set 0 (stage s, stored) takes stage s + 2
+            if (s + 2 < nst) {
+                load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb + (s + 2) * TK, ra0);
+                load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb + (s + 2) * TK, rb0);
+            }
+            compute(0);
+            store_tile<!TA, BM, NT>(As[1], ra1);
+            store_tile<TB, BN, NT>(Bs[1], rb1);
+            __syncthreads();
+            // odd stage s + 1: LDS buffer 1; set 1 takes stage s + 3
+            if (s + 3 < nst) {
+                load_tile<!TA, BM, NT>(A, lda, M, ke, m0, kb + (s + 3) * TK, ra1);
+                load_tile<TB, BN, NT>(B, ldb, N, ke, n0, kb + (s + 3) * TK, rb1);
+            }
+            compute(1);
+            if (s + 2 < nst) {
+                store_tile<!TA, BM, NT>(As[0], ra0);
+                store_tile<TB, BN, NT>(Bs[0], rb0);
+            }
+            __syncthreads();
+        }
     }
     // acc[i][j] holds C^T of its 16 x 16 tile (B fed as the first operand):
     // this lane has row 16 i + lane % 16 and columns 16 j + 4 (lane / 16) + r,
     // four consecutive columns stored as one 16-byte FP32 / 8-byte BF16 access
     const bool vec = (ldc & 3) == 0;
     if constexpr (EPI == EPI_GELU_HEAD) {
-        const int slots = 2 * gridDim.x;
+        const int slots = WN * gridDim.x;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = m0 + wr * 64 + i * 16 + (lane & 15);
+        for (int i = 0; i < FI; i++) {
+            const int row = m0 + wr * (BM / WM) + i * 16 + (lane & 15);
             float hs = 0.f;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int col = n0 + wc * 64 + j * 16 + 4 * (lane >> 4);
+            for (int j = 0; j < FJ; j++) {
+                const int col = n0 + wc * (BN / WN) + j * 16 + 4 * (lane >> 4);
 #pragma unroll
                 for (int r = 0; r < 4; r++)
                     if (col + r < N) hs += gelu_fwd(acc[i][j][r] + bias[col + r]) * Zin[col + r];
             }
             hs += __shfl_xor(hs, 16);
             hs += __shfl_xor(hs, 32);
-            if (lane < 16 && row < M) Cf[size_t(row) * slots + 2 * bx + wc] = hs;
+            if (lane < 16 && row < M) Cf[size_t(row) * slots + WN * bx + wc] = hs;
         }
         return;
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < FI; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int row = m0 + wr * 64 + i * 16 + (lane & 15);
-            const int col = n0 + wc * 64 + j * 16 + 4 * (lane >> 4);
+        for (int j = 0; j < FJ; j++) {
+            const int row = m0 + wr * (BM / WM) + i * 16 + (lane & 15);
+            const int col = n0 + wc * (BN / WN) + j * 16 + 4 * (lane >> 4);
             if (row >= M) continue;
             const size_t o = size_t(row) * ldc + col;
             float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
@@ -640,10 +668,26 @@ static Status mlp_alloc(Mlp &N, size_t rows) {
 // the layer's epilogue.
 constexpr int kSplitBelowTiles = 128;
 
+// Many rows (screening chunks): 256 x 256 workgroup tiles of 8 waves (each
+// 64 x 128), one per CU, loads one stage ahead; their fragment reads per
+// matrix-core instruction are 3/4 of the 128 x 128 tile's.
+constexpr int kBigBM = 256, kBigBN = 256, kBigWM = 4, kBigWN = 2;
+static bool big_tiles(int M, int Nn) { return size_t(M) * size_t(Nn) >= size_t(512) * kBigBM * kBigBN; }
+// partial sums per row of the EPI_GELU_HEAD layer
+static int head_slots(int M, int Nn) {
+    return big_tiles(M, Nn) ? kBigWN * ((Nn + kBigBN - 1) / kBigBN) : 2 * ((Nn + TN - 1) / TN);
+}
+
 template <bool TA, bool TB, int EPI>
 static void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int Nn, int K, float *Cf, bf16 *Cb,
                  int ldc, const float *bias = nullptr, const float *Zin = nullptr, float *ws = nullptr,
                  size_t ws_floats = 0) {
+    if (big_tiles(M, Nn)) {
+        const dim3 big((Nn + kBigBN - 1) / kBigBN, (M + kBigBM - 1) / kBigBM);
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, kBigBM, kBigBN, kBigWM, kBigWN, 1>), big, dim3(64 * kBigWM * kBigWN), 0, st,
+                           A, lda, B, ldb, M, Nn, K, Cf, Cb, ldc, bias, Zin, K);
+        return;
+    }
     dim3 grid((Nn + TN - 1) / TN, (M + TM - 1) / TM);
     const int tiles = int(grid.x * grid.y);
     if constexpr (EPI == EPI_GELU || EPI == EPI_GELU_BWD) {
@@ -711,7 +755,7 @@ static void predict_rows(Mlp &N, hipStream_t st, int rows, float *y) {
     const float *b1 = N.w + N.off[1], *b2 = N.w + N.off[3], *b3 = N.w + N.off[5];
     gemm<false, true, EPI_GELU>(st, N.X, N.k_in, W1t, N.k_in, rows, N.hid, N.k_in, nullptr, N.H[0], N.hid, b1);
     gemm<false, true, EPI_GELU>(st, N.H[0], N.hid, W2t, N.hid, rows, N.hid, N.hid, nullptr, N.H[1], N.hid, b2);
-    const int slots = 2 * ((N.hid + TN - 1) / TN);
+    const int slots = head_slots(rows, N.hid);
     gemm<false, true, EPI_GELU_HEAD>(st, N.H[1], N.hid, W3t, N.hid, rows, N.hid, N.hid, N.Z[2], nullptr, slots, b3,
                                      N.w + N.off[6]);
     hipLaunchKernelGGL(k_head_fold, dim3((rows + 255) / 256), dim3(256), 0, st, (const float *)N.Z[2], slots, rows,

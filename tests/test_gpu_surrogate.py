@@ -65,6 +65,20 @@ def test_mlp_predictions_match_fp64_oracle(ctx):
     assert rel < 2e-2
 
 
+def test_mlp_large_batch_tiles_match_small_batch_tiles(ctx):
+    """65536 rows take the 256 x 256-tile GEMMs; the same rows in chunks of
+    1024 take the 128 x 128 tiles (pinned against the oracle above). Per
+    output both accumulate K in the same order: only the head's partial sums
+    differ in grouping."""
+    net = S.MLP(195, 512, seed=6)
+    x = np.random.default_rng(7).uniform(-1, 1, size=(65536, 195)).astype(np.float32)
+    y_big = net.predict(x)
+    y_small = np.concatenate([net.predict(x[i:i + 1024]) for i in range(0, len(x), 1024)])
+    err = np.max(np.abs(y_big - y_small))
+    print(f"max |big - small| {err:.3e}, max |y| {np.abs(y_small).max():.3e}")
+    assert err <= 1e-5 * max(1.0, float(np.abs(y_small).max()))
+
+
 def test_ansatz_energies_match_oracle(ctx):
     n, layers = 5, 2
     sim = S.AnsatzEnergies(n, layers, slots=16, precision="c64")
