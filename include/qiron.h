@@ -425,9 +425,11 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
                                 float *y, qi_error *err);
 
 /* Predictions for `count` candidates of a local optimisation round: angles
- * theta_j = theta0[j] + sigma z_j, z_j = sqrt(-2 ln(1 - u1)) cos(2 pi u2) with
- * u1, u2 the SplitMix64 draws draw0 + 2 (c nang + j) + 1 and + 2 of the stream
- * `seed`, couplings J[0..2] fixed for the round. */
+ * theta_j = theta0[j] + sigma z_j with z_2p = rho cos(2 pi u2), z_2p+1 =
+ * rho sin(2 pi u2), rho = sqrt(-2 ln(1 - u1)), u1, u2 the SplitMix64 draws
+ * d + 1 and d + 2 of the stream `seed`, d = draw0 + 2 (c P + p),
+ * P = ceil(nang / 2) (one Box-Muller pair per two angles); couplings J[0..2]
+ * fixed for the round. */
 qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint64_t count, uint32_t nang,
                                       const double *theta0, double sigma, const double *J, float *y,
                                       qi_error *err);

@@ -533,26 +533,60 @@ __device__ __forceinline__ double sm64_uniform(uint64_t seed, uint64_t k) {
     return double(z >> 11) * 0x1.0p-53;
 }
 
+// cos and sin of 2 pi x (x in revolutions, reduced to [-1/2, 1/2], inside
+// the range of the hardware v_cos_f32 / v_sin_f32; absolute error about
+// 1e-6, far below the BF16 rounding of the network inputs).
+__device__ __forceinline__ void sincos_rev(float x, float &s, float &c) {
+    x -= rintf(x);
+    s = __builtin_amdgcn_sinf(x);
+    c = __builtin_amdgcn_cosf(x);
+}
+__device__ __forceinline__ uint32_t pack_bf16x2(float a, float b) {
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    bf16x2 v;
+    v.x = (__bf16)a;
+    v.y = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
+// A row's features of angles j and j + 1 (cos at j, sin at nang + j; one
+// 4-byte store each when both exist and nang is even, else 2-byte stores).
+__device__ __forceinline__ void store_angle_pair(bf16 *row, int nang, int j, float ca, float sa, float cb, float sb,
+                                                 bool two) {
+    if (two && !(nang & 1)) {
+        *reinterpret_cast<uint32_t *>(row + j) = pack_bf16x2(ca, cb);
+        *reinterpret_cast<uint32_t *>(row + nang + j) = pack_bf16x2(sa, sb);
+        return;
+    }
+    row[j] = f2bf(ca);
+    row[nang + j] = f2bf(sa);
+    if (two) {
+        row[j + 1] = f2bf(cb);
+        row[nang + j + 1] = f2bf(sb);
+    }
+}
+
 // Network inputs of ansatz candidates c0 .. c0 + rows drawn from a SplitMix64
 // stream: candidate c's angles are draws draw0 + c nang + j + 1 (theta = 2 pi u),
 // its couplings draws draw0 + count nang + 3 c + i + 1 (J = 0.5 + u). Row
 // layout: cos theta_j, sin theta_j, J_i - 1, zero padding to k_in. One item
-// per (row, angle), the last item of a row writes its couplings and padding.
-// The inputs are rounded to BF16, so the trigonometry runs in FP32 (the host
-// regenerates the kept candidates' angles in FP64 for the exact evaluation).
+// per (row, pair of angles), the last item of a row writes its couplings and
+// padding. The inputs are rounded to BF16, so the trigonometry runs in FP32
+// (the host regenerates the kept candidates' angles in FP64 for the exact
+// evaluation).
 __global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t draw0, uint64_t count, uint64_t c0, int rows,
                                                          int nang, int k_in, bf16 *__restrict__ X) {
-    const uint32_t per = uint32_t(nang) + 1, items = uint32_t(rows) * per;
+    const uint32_t pairs = (uint32_t(nang) + 1) / 2, per = pairs + 1, items = uint32_t(rows) * per;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
-        const uint32_t r = i / per, j = i - r * per;
+        const uint32_t r = i / per, p = i - r * per;
         const uint64_t c = c0 + r;
         bf16 *row = X + size_t(r) * k_in;
-        if (j < uint32_t(nang)) {
-            const float th = (float)(2.0 * 3.14159265358979323846 * sm64_uniform(seed, draw0 + c * nang + j + 1));
-            float sn, cs;
-            sincosf(th, &sn, &cs);
-            row[j] = f2bf(cs);
-            row[nang + j] = f2bf(sn);
+        if (p < pairs) {
+            const int j = 2 * int(p);
+            const bool two = j + 1 < nang;
+            float sa, ca, sb = 0.f, cb = 0.f;
+            sincos_rev((float)sm64_uniform(seed, draw0 + c * nang + j + 1), sa, ca);
+            if (two) sincos_rev((float)sm64_uniform(seed, draw0 + c * nang + j + 2), sb, cb);
+            store_angle_pair(row, nang, j, ca, sa, cb, sb, two);
         } else {
             for (int q = 0; q < 3; q++)
                 row[2 * nang + q] = f2bf((float)(0.5 + sm64_uniform(seed, draw0 + count * nang + 3 * c + q + 1) - 1.0));
@@ -562,26 +596,33 @@ __global__ __launch_bounds__(256) void k_ansatz_features(uint64_t seed, uint64_t
 }
 
 // Candidates of an optimisation round around a centre theta0 (a local
-// proposal): theta_j = theta0_j + sigma z_j, z_j a standard normal by the
-// Box-Muller transform of draws draw0 + 2 (c nang + j) + 1 and + 2 of the
-// stream `seed` (z = sqrt(-2 ln(1 - u1)) cos(2 pi u2)); the couplings are the
+// proposal): theta_j = theta0_j + sigma z_j with standard normals from the
+// Box-Muller transform, both of its outputs used: for the pair p = j / 2 of
+// candidate c, u1 and u2 are draws d + 1 and d + 2 of the stream `seed`,
+// d = draw0 + 2 (c P + p), P = ceil(nang / 2), rho = sqrt(-2 ln(1 - u1)),
+// z_2p = rho cos(2 pi u2), z_2p+1 = rho sin(2 pi u2); the couplings are the
 // fixed J of the round. Same row layout and items as k_ansatz_features.
 __global__ __launch_bounds__(256) void k_ansatz_features_local(uint64_t seed, uint64_t draw0, uint64_t c0, int rows, int nang,
                                                                int k_in, const double *__restrict__ theta0, double sigma,
                                                                double j0, double j1, double j2, bf16 *__restrict__ X) {
-    const uint32_t per = uint32_t(nang) + 1, items = uint32_t(rows) * per;
+    const uint32_t pairs = (uint32_t(nang) + 1) / 2, per = pairs + 1, items = uint32_t(rows) * per;
+    const float inv2pi = 0.15915494309189535f, sg = (float)sigma;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < items; i += gridDim.x * blockDim.x) {
-        const uint32_t r = i / per, j = i - r * per;
+        const uint32_t r = i / per, p = i - r * per;
         const uint64_t c = c0 + r;
         bf16 *row = X + size_t(r) * k_in;
-        if (j < uint32_t(nang)) {
-            const uint64_t d = draw0 + 2 * (c * nang + j);
+        if (p < pairs) {
+            const int j = 2 * int(p);
+            const bool two = j + 1 < nang;
+            const uint64_t d = draw0 + 2 * (c * pairs + p);
             const double u1 = sm64_uniform(seed, d + 1), u2 = sm64_uniform(seed, d + 2);
-            const float z = sqrtf(-2.f * logf((float)(1.0 - u1))) * cosf(6.2831853071795865f * (float)u2);
-            float sn, cs;
-            sincosf((float)(theta0[j] + sigma * (double)z), &sn, &cs);
-            row[j] = f2bf(cs);
-            row[nang + j] = f2bf(sn);
+            const float rho = sqrtf(-2.f * __logf((float)(1.0 - u1)));
+            float zs, zc;
+            sincos_rev((float)u2, zs, zc);
+            float sa, ca, sb = 0.f, cb = 0.f;
+            sincos_rev(((float)theta0[j] + sg * rho * zc) * inv2pi, sa, ca);
+            if (two) sincos_rev(((float)theta0[j + 1] + sg * rho * zs) * inv2pi, sb, cb);
+            store_angle_pair(row, nang, j, ca, sa, cb, sb, two);
         } else {
             row[2 * nang] = f2bf((float)(j0 - 1.0));
             row[2 * nang + 1] = f2bf((float)(j1 - 1.0));
@@ -1060,7 +1101,7 @@ qi_status qi_mlp_predict_ansatz(qi_mlp_t m, uint64_t seed, uint64_t draw0, uint6
     (void)hipEventRecord(N.ctx->ev0, st);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
         const int nr = (int)std::min<uint64_t>(chunk, count - c0);
-        const size_t items = size_t(nr) * (nang + 1);
+        const size_t items = size_t(nr) * ((nang + 1) / 2 + 1);
         hipLaunchKernelGGL(k_ansatz_features, dim3((unsigned)std::min<size_t>((items + 255) / 256, 16384)), dim3(256), 0, st, seed,
                            draw0, count, c0, nr, (int)nang, N.k_in, N.X);
         predict_rows(N, st, nr, N.yall + c0);
@@ -1106,7 +1147,7 @@ qi_status qi_mlp_predict_ansatz_local(qi_mlp_t m, uint64_t seed, uint64_t draw0,
     (void)hipEventRecord(N.ctx->ev0, st);
     for (uint64_t c0 = 0; c0 < count; c0 += chunk) {
         const int nr = (int)std::min<uint64_t>(chunk, count - c0);
-        const size_t items = size_t(nr) * (nang + 1);
+        const size_t items = size_t(nr) * ((nang + 1) / 2 + 1);
         hipLaunchKernelGGL(k_ansatz_features_local, dim3((unsigned)std::min<size_t>((items + 255) / 256, 16384)), dim3(256), 0, st,
                            seed, draw0, c0, nr, (int)nang, N.k_in, (const double *)N.theta0, sigma, J[0], J[1], J[2], N.X);
         predict_rows(N, st, nr, N.yall + c0);

@@ -211,15 +211,21 @@ def params_at(seed: int, draw0: int, count: int, nang: int, idx: np.ndarray):
 
 def local_params(seed: int, draw0: int, nang: int, theta0: np.ndarray, sigma: float, idx: np.ndarray) -> np.ndarray:
     """The angles qi_mlp_predict_ansatz_local draws for candidates `idx`:
-    theta0[j] + sigma * z, z = sqrt(-2 ln(1 - u1)) cos(2 pi u2) with u1, u2
-    the draws draw0 + 2 (c nang + j) + 1 and + 2 of the stream `seed`."""
+    theta0[j] + sigma * z_j with one Box-Muller pair per two angles: for
+    p = j // 2, u1 and u2 are the draws d + 1 and d + 2 of the stream `seed`,
+    d = draw0 + 2 (c P + p), P = ceil(nang / 2), rho = sqrt(-2 ln(1 - u1)),
+    z_2p = rho cos(2 pi u2), z_2p+1 = rho sin(2 pi u2)."""
     idx = np.asarray(idx, dtype=np.uint64)
-    j = np.arange(nang, dtype=np.uint64)
-    d = np.uint64(draw0) + np.uint64(2) * (idx[:, None] * np.uint64(nang) + j)
+    pairs = (nang + 1) // 2
+    p = np.arange(pairs, dtype=np.uint64)
+    d = np.uint64(draw0) + np.uint64(2) * (idx[:, None] * np.uint64(pairs) + p)
     u1 = W.splitmix_at(seed, d + np.uint64(1))
     u2 = W.splitmix_at(seed, d + np.uint64(2))
-    z = np.sqrt(-2.0 * np.log(1.0 - u1)) * np.cos(2.0 * math.pi * u2)
-    return np.asarray(theta0, dtype=np.float64)[None, :] + sigma * z
+    rho = np.sqrt(-2.0 * np.log(1.0 - u1))
+    z = np.empty((idx.size, 2 * pairs))
+    z[:, 0::2] = rho * np.cos(2.0 * math.pi * u2)
+    z[:, 1::2] = rho * np.sin(2.0 * math.pi * u2)
+    return np.asarray(theta0, dtype=np.float64)[None, :] + sigma * z[:, :nang]
 
 
 def ansatz_batch(specs, precision: str = "c64", ctx: Optional[Context] = None) -> CircuitBatch:
