@@ -370,7 +370,10 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float *__restrict__ X
     const int c = blockIdx.x * 64 + (threadIdx.x & 63), g = threadIdx.x >> 6;
     const int r0 = blockIdx.y * 256;
     float s = 0.f;
+    // (unrolled: the loads of eight rows are in flight together; the sum
+    // keeps its row order)
     if (c < cols)
+#pragma unroll 8
         for (int r = r0 + g; r < r0 + 256 && r < rows; r += 4) s += X[size_t(r) * cols + c];
     sh[g][threadIdx.x & 63] = s;
     __syncthreads();
@@ -403,6 +406,7 @@ __global__ __launch_bounds__(256) void k_head_grad_part(const bf16 *__restrict__
     const int r0 = blockIdx.y * 256;
     float s = 0.f;
     if (c < hid)
+#pragma unroll 8
         for (int r = r0 + g; r < r0 + 256 && r < rows; r += 4) s += bf2f(H[size_t(r) * hid + c]) * dy[r];
     sh[g][threadIdx.x & 63] = s;
     __syncthreads();
