@@ -119,20 +119,19 @@ __device__ __forceinline__ f64x2 unamp(double2 a) { f64x2 r; r.x = a.x; r.y = a.
 //   MODE 0: bit 0 is free: both halves pair with the partner unit's halves.
 //   MODE 1: bit 0 is a control: only the odd halves change; the even halves
 //           are written back as read (each unit has exactly one owner).
-//   MODE 2: bit 0 is the target: every unit holds one pair; two units per
-//           thread, a block apart, so each load instruction stays contiguous.
+//   MODE 2: bit 0 is the target: every unit holds one pair; one unit per
+//           thread (a plain stream: one 16-byte unit per thread measured
+//           85% of HBM for a copy, two 78%).
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, GateK<float> g) {
     if constexpr (MODE == 2) {
-        const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
-        const bool h0 = k0 < g.items, h1 = k1 < g.items;
-        uint64_t j0 = 0, j1 = 0;
-        f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
-        if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
-        if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
+        const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+        if (k >= g.items) return;
+        const uint64_t j = insert_zeros(k, g.fixed) | g.set;
+        const f32x4 u = ldnt(psi + j);
         float2 na, nb;
-        if (h0) { pair_update<float, KIND>(g, lo(u0), hi(u0), na, nb); stnt(pack(na, nb), psi + j0); }
-        if (h1) { pair_update<float, KIND>(g, lo(u1), hi(u1), na, nb); stnt(pack(na, nb), psi + j1); }
+        pair_update<float, KIND>(g, lo(u), hi(u), na, nb);
+        stnt(pack(na, nb), psi + j);
     } else {
         const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
         if (k >= g.items) return;
@@ -188,14 +187,59 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64_lane(f32x4 *__restrict__ 
 // the target), so only the odd halves change.
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_diag_c64(f32x4 *__restrict__ psi, GateK<float> g) {
-    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
-    const bool h0 = k0 < g.items, h1 = k1 < g.items;
-    uint64_t j0 = 0, j1 = 0;
-    f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
-    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
-    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
-    if (h0) stnt(pack(MODE == 0 ? diag_update<float, KIND>(g, lo(u0)) : lo(u0), diag_update<float, KIND>(g, hi(u0))), psi + j0);
-    if (h1) stnt(pack(MODE == 0 ? diag_update<float, KIND>(g, lo(u1)) : lo(u1), diag_update<float, KIND>(g, hi(u1))), psi + j1);
+    const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+    if (k >= g.items) return;
+    const uint64_t j = insert_zeros(k, g.fixed) | g.set;
+    const f32x4 u = ldnt(psi + j);
+    stnt(pack(MODE == 0 ? diag_update<float, KIND>(g, lo(u)) : lo(u), diag_update<float, KIND>(g, hi(u))), psi + j);
+}
+
+// Pair gates whose fixed bits (target, controls) all sit at bit 5 of the unit
+// index or above, one 16-byte unit per thread: lanes 0..31 of a wave load the
+// a-units of 32 consecutive pairs (512 contiguous bytes), lanes 32..63 their
+// partners, and the two halves trade through v_permlane32_swap (the four
+// 32-bit components of a unit). With one pair (two units) per thread the same
+// gate measured 79% of HBM at target 15 and 71% at the widest strides; with
+// one unit per thread 83% and 80%. MODE as k_pair_c64 (complex128: 0).
+template <typename U> __device__ __forceinline__ U lane_partner32(U v) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 w = __builtin_bit_cast(u32x4, v);
+    u32x4 o;
+    const bool upper = (threadIdx.x & 32) != 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        // lanes < 32 keep their value in the first result and get lane + 32's
+        // in the second; lanes >= 32 the other way round
+        const auto r = __builtin_amdgcn_permlane32_swap(w[c], w[c], false, false);
+        o[c] = upper ? r[0] : r[1];
+    }
+    return __builtin_bit_cast(U, o);
+}
+template <int KIND, int MODE>
+__global__ __launch_bounds__(kThreads) void k_pair_c64_split(f32x4 *__restrict__ psi, GateK<float> g) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t k = g.base + (xcd_block_order(blockIdx.x, g.chunk) * (kThreads / 64) + (threadIdx.x >> 6)) * 32 + (lane & 31);
+    if (k >= g.items) return;  // lanes l and l + 32 share k
+    const bool upper = lane >= 32;
+    const uint64_t ja = insert_zeros(k, g.fixed) | g.set, j = upper ? (ja ^ g.pmask) : ja;
+    const f32x4 v = ldnt(psi + j), p = lane_partner32(v);
+    const f32x4 x = upper ? p : v, y = upper ? v : p;  // the a- and b-units
+    float2 a0 = lo(x), b0 = lo(y), a1, b1;
+    pair_update<float, KIND>(g, hi(x), hi(y), a1, b1);
+    if constexpr (MODE == 0) pair_update<float, KIND>(g, lo(x), lo(y), a0, b0);
+    stnt(upper ? pack(b0, b1) : pack(a0, a1), psi + j);
+}
+template <int KIND>
+__global__ __launch_bounds__(kThreads) void k_pair_c128_split(f64x2 *__restrict__ psi, GateK<double> g) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t k = g.base + (xcd_block_order(blockIdx.x, g.chunk) * (kThreads / 64) + (threadIdx.x >> 6)) * 32 + (lane & 31);
+    if (k >= g.items) return;
+    const bool upper = lane >= 32;
+    const uint64_t ia = insert_zeros(k, g.fixed) | g.set, j = upper ? (ia ^ g.pmask) : ia;
+    const f64x2 v = ldnt(psi + j), p = lane_partner32(v);
+    double2 na, nb;
+    pair_update<double, KIND>(g, amp(upper ? p : v), amp(upper ? v : p), na, nb);
+    stnt(unamp(upper ? nb : na), psi + j);
 }
 
 // complex128: one amplitude is already a 16-byte unit; one pair per thread.
@@ -231,14 +275,10 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128_lane(f64x2 *__restrict__
 
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_diag_c128(f64x2 *__restrict__ psi, GateK<double> g) {
-    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + threadIdx.x, k1 = k0 + kThreads;
-    const bool h0 = k0 < g.items, h1 = k1 < g.items;
-    uint64_t j0 = 0, j1 = 0;
-    f64x2 u0 = {0, 0}, u1 = {0, 0};
-    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
-    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
-    if (h0) stnt(unamp(diag_update<double, KIND>(g, amp(u0))), psi + j0);
-    if (h1) stnt(unamp(diag_update<double, KIND>(g, amp(u1))), psi + j1);
+    const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+    if (k >= g.items) return;
+    const uint64_t j = insert_zeros(k, g.fixed) | g.set;
+    stnt(unamp(diag_update<double, KIND>(g, amp(ldnt(psi + j)))), psi + j);
 }
 
 // complex64 SWAP with qubit 0 as one of its targets: the exchanged
@@ -367,7 +407,7 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     u.pmask = g.pmask >> 1;
     u.items = uint64_t(1) << (s.n - 1 - __builtin_popcountll(u.fixed));
     f32x4 *psi = reinterpret_cast<f32x4 *>(s.d);
-    const uint64_t b1 = grid_for(u.items, kThreads), b2 = grid_for(u.items, 2 * kThreads);
+    const uint64_t b1 = grid_for(u.items, kThreads);
     const int kind = L.kind == LK_SWAP ? LK_X : L.kind;  // SWAP: exchange base|tb with base|tb2
     GateK<float> ul;
     if (L.kind != LK_SWAP && mode != 2 && !diag && lane_args(u, u.pmask, u.fixed & ~u.pmask, kind, ul)) {
@@ -387,8 +427,22 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
         return Status{};
     }
     bool ok = true;
-    const bool two = mode == 2 || diag;  // two units per thread
-    split_launch(two ? b2 : b1, two ? 2 * kThreads : kThreads, u, [&](dim3 gr, const GateK<float> &h) {
+    if (!diag && mode != 2 && (u.fixed & 31) == 0) {  // one unit per thread, partners in the other half-wave
+        split_launch(grid_for(u.items, kThreads / 2), kThreads / 2, u, [&](dim3 gr, const GateK<float> &h) {
+            switch (kind * 4 + mode) {
+#define QI_SPLIT(K) \
+            case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64_split<K, 0>), gr, block, 0, st, psi, h); break; \
+            case K * 4 + 1: hipLaunchKernelGGL((k_pair_c64_split<K, 1>), gr, block, 0, st, psi, h); break;
+            QI_PAIR_KINDS(QI_SPLIT)
+#undef QI_SPLIT
+            default: ok = false;
+            }
+        });
+        if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
+    split_launch(b1, kThreads, u, [&](dim3 gr, const GateK<float> &h) {
         switch (kind * 4 + mode) {
 #define QI_PAIR(K) \
         case K * 4 + 0: hipLaunchKernelGGL((k_pair_c64<K, 0>), gr, block, 0, st, psi, h); break; \
@@ -411,7 +465,7 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
 Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
     GateK<double> g = make_args<double>(L, s.n);
     const dim3 block(kThreads);
-    const uint64_t b1 = grid_for(g.items, kThreads), b2 = grid_for(g.items, 2 * kThreads);
+    const uint64_t b1 = grid_for(g.items, kThreads);
     f64x2 *psi = reinterpret_cast<f64x2 *>(s.d);
     GateK<double> gl;
     bool ok = true;
@@ -430,8 +484,20 @@ Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
         return Status{};
     }
     const int kind = L.kind == LK_SWAP ? LK_X : L.kind;
-    const bool two = kind == LK_Z || kind == LK_PHASE;  // two amplitudes per thread
-    split_launch(two ? b2 : b1, two ? 2 * kThreads : kThreads, g, [&](dim3 gr, const GateK<double> &h) {
+    if (kind != LK_Z && kind != LK_PHASE && kind != LK_MATCH && (g.fixed & 31) == 0) {  // one amplitude per thread
+        split_launch(grid_for(g.items, kThreads / 2), kThreads / 2, g, [&](dim3 gr, const GateK<double> &h) {
+            switch (kind) {
+#define QI_SPLIT(K) case K: hipLaunchKernelGGL((k_pair_c128_split<K>), gr, block, 0, st, psi, h); break;
+            QI_PAIR_KINDS(QI_SPLIT)
+#undef QI_SPLIT
+            default: ok = false;
+            }
+        });
+        if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
+        QI_HIP_TRY(hipGetLastError());
+        return Status{};
+    }
+    split_launch(b1, kThreads, g, [&](dim3 gr, const GateK<double> &h) {
         switch (kind) {
 #define QI_PAIR(K) case K: hipLaunchKernelGGL((k_pair_c128<K>), gr, block, 0, st, psi, h); break;
         QI_PAIR_KINDS(QI_PAIR)
