@@ -501,6 +501,7 @@ def main():
     results = {}
     if args.only == "gate":
         single_gates(qiron, L, W, ctx, n, results)
+        print(json.dumps(results), flush=True)
         return
     if args.only == "surrogate":
         print(json.dumps(bench_surrogate(W, args)), flush=True)

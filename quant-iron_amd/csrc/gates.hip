@@ -64,7 +64,7 @@ template <typename R> struct GateK {
     uint64_t cmask;   // control mask tested per element (matchgate only)
     uint64_t base;    // first item of this launch (a gate over more items than one grid holds is split)
     int32_t chunk;    // workgroup order (xcd_block_order) of the pair and diagonal kernels
-    int32_t lane;     // lane-exchange kernels: lane offset of the partner (1, 2 or 4)
+    int32_t lane;     // lane-exchange kernels: lane offset of the partner (1, 2, 4, 8 or 16)
     R c[10];
 };
 
@@ -145,11 +145,12 @@ __global__ __launch_bounds__(kThreads) void k_pair_c64(f32x4 *__restrict__ psi, 
     }
 }
 
-// Low targets (partner 16-64 bytes away): in the pair layout above each load
-// instruction would touch twice the cache lines it uses. Here a wave reads
-// 128 consecutive units (index space without the control bits), two fully
-// used 1 KiB rows, and takes each partner from the lane that holds it. `g`:
-// fixed = controls only, items = every unit with the controls satisfied.
+// Low targets (partner 16-256 bytes away, unit bits 0..4 of the index without
+// the control bits): a wave reads 64 consecutive units, one fully used 1 KiB
+// row, one unit per lane, and takes each partner from the lane that holds it
+// (in the pair layout above each load instruction would touch twice the cache
+// lines it uses, or move two units per thread). `g`: fixed = controls only,
+// items = every unit with the controls satisfied.
 template <typename T, typename R, int KIND>
 __device__ __forceinline__ T lane_pick(const GateK<R> &g, T own, T other, bool upper) {
     T na, nb;
@@ -166,20 +167,16 @@ __device__ __forceinline__ f32x4 shfl_xor4(f32x4 v, int m) {
 template <int KIND, int MODE>
 __global__ __launch_bounds__(kThreads) void k_pair_c64_lane(f32x4 *__restrict__ psi, GateK<float> g) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
-                   k1 = k0 + 64;
-    const bool h0 = k0 < g.items, h1 = k1 < g.items;  // a partner is in range iff its lane's unit is
-    uint64_t j0 = 0, j1 = 0;
-    f32x4 u0 = {0, 0, 0, 0}, u1 = {0, 0, 0, 0};
-    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
-    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
-    const f32x4 p0 = shfl_xor4(u0, g.lane), p1 = shfl_xor4(u1, g.lane);
+    const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+    const bool h = k < g.items;  // a partner is in range iff its lane's unit is
+    uint64_t j = 0;
+    f32x4 u = {0, 0, 0, 0};
+    if (h) { j = insert_zeros(k, g.fixed) | g.set; u = ldnt(psi + j); }
+    const f32x4 p = shfl_xor4(u, g.lane);
     const bool upper = (lane & uint32_t(g.lane)) != 0;  // this lane holds the target-1 units
     // MODE 0: both halves of a unit pair up; MODE 1: qubit 0 is a control, only odd halves change
-    const float2 a0 = MODE == 0 ? lane_pick<float2, float, KIND>(g, lo(u0), lo(p0), upper) : lo(u0);
-    const float2 a1 = MODE == 0 ? lane_pick<float2, float, KIND>(g, lo(u1), lo(p1), upper) : lo(u1);
-    if (h0) stnt(pack(a0, lane_pick<float2, float, KIND>(g, hi(u0), hi(p0), upper)), psi + j0);
-    if (h1) stnt(pack(a1, lane_pick<float2, float, KIND>(g, hi(u1), hi(p1), upper)), psi + j1);
+    const float2 a = MODE == 0 ? lane_pick<float2, float, KIND>(g, lo(u), lo(p), upper) : lo(u);
+    if (h) stnt(pack(a, lane_pick<float2, float, KIND>(g, hi(u), hi(p), upper)), psi + j);
 }
 
 // complex64 diagonal gates (Z, phases) on two-amplitude units. MODE 0: bit 0
@@ -259,18 +256,14 @@ __global__ __launch_bounds__(kThreads) void k_pair_c128(f64x2 *__restrict__ psi,
 template <int KIND>
 __global__ __launch_bounds__(kThreads) void k_pair_c128_lane(f64x2 *__restrict__ psi, GateK<double> g) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t k0 = g.base + xcd_block_order(blockIdx.x, g.chunk) * (2 * kThreads) + (threadIdx.x >> 6) * 128 + lane,
-                   k1 = k0 + 64;
-    const bool h0 = k0 < g.items, h1 = k1 < g.items;
-    uint64_t j0 = 0, j1 = 0;
-    f64x2 u0 = {0, 0}, u1 = {0, 0};
-    if (h0) { j0 = insert_zeros(k0, g.fixed) | g.set; u0 = ldnt(psi + j0); }
-    if (h1) { j1 = insert_zeros(k1, g.fixed) | g.set; u1 = ldnt(psi + j1); }
-    const double2 p0 = make_double2(__shfl_xor(u0.x, g.lane), __shfl_xor(u0.y, g.lane));
-    const double2 p1 = make_double2(__shfl_xor(u1.x, g.lane), __shfl_xor(u1.y, g.lane));
+    const uint64_t k = g.base + xcd_block_order(blockIdx.x, g.chunk) * kThreads + threadIdx.x;
+    const bool h = k < g.items;
+    uint64_t j = 0;
+    f64x2 u = {0, 0};
+    if (h) { j = insert_zeros(k, g.fixed) | g.set; u = ldnt(psi + j); }
+    const double2 p = make_double2(__shfl_xor(u.x, g.lane), __shfl_xor(u.y, g.lane));
     const bool upper = (lane & uint32_t(g.lane)) != 0;
-    if (h0) stnt(unamp(lane_pick<double2, double, KIND>(g, amp(u0), p0, upper)), psi + j0);
-    if (h1) stnt(unamp(lane_pick<double2, double, KIND>(g, amp(u1), p1, upper)), psi + j1);
+    if (h) stnt(unamp(lane_pick<double2, double, KIND>(g, amp(u), p, upper)), psi + j);
 }
 
 template <int KIND>
@@ -352,18 +345,18 @@ constexpr int kChunkLog2 = 8;
 #define QI_LANE_KINDS(M) M(LK_H) M(LK_X) M(LK_Y) M(LK_RX) M(LK_RY) M(LK_U2)
 
 // Lane-exchange form of a one-target gate whose (unit-space) target bit `tb`
-// lands on bit 0..2 of the index with the controls `ctrl` removed: returns
+// lands on bit 0..4 of the index with the controls `ctrl` removed: returns
 // the launch arguments, or false when the pair layout is the better one.
 template <typename R> bool lane_args(const GateK<R> &g, uint64_t tb, uint64_t ctrl, int kind, GateK<R> &out) {
     if (kind == LK_RZ || (tb & (tb - 1)) || !tb) return false;
     const int b = __builtin_ctzll(tb);
     const int kb = b - __builtin_popcountll(ctrl & (tb - 1));
-    if (kb > 2) return false;
+    if (kb > 4) return false;
     out = g;
     out.fixed = ctrl;
     out.items = g.items * 2;
     out.lane = 1 << kb;
-    out.chunk = xcd_chunk_log2(grid_for(out.items, 2 * kThreads), kChunkLog2);
+    out.chunk = xcd_chunk_log2(grid_for(out.items, kThreads), kChunkLog2);
     return true;
 }
 
@@ -412,7 +405,7 @@ Status launch_c64(State &s, const Lowered &L, hipStream_t st) {
     GateK<float> ul;
     if (L.kind != LK_SWAP && mode != 2 && !diag && lane_args(u, u.pmask, u.fixed & ~u.pmask, kind, ul)) {
         bool ok = true;
-        split_launch(grid_for(ul.items, 2 * kThreads), 2 * kThreads, ul, [&](dim3 gl, const GateK<float> &h) {
+        split_launch(grid_for(ul.items, kThreads), kThreads, ul, [&](dim3 gl, const GateK<float> &h) {
             switch (kind * 2 + mode) {
 #define QI_LANE(K) \
             case K * 2 + 0: hipLaunchKernelGGL((k_pair_c64_lane<K, 0>), gl, block, 0, st, psi, h); break; \
@@ -471,7 +464,7 @@ Status launch_c128(State &s, const Lowered &L, hipStream_t st) {
     bool ok = true;
     if (L.kind != LK_SWAP && L.kind != LK_MATCH && L.kind != LK_Z && L.kind != LK_PHASE &&
         lane_args(g, g.pmask, g.fixed & ~g.pmask, L.kind, gl)) {
-        split_launch(grid_for(gl.items, 2 * kThreads), 2 * kThreads, gl, [&](dim3 gg, const GateK<double> &h) {
+        split_launch(grid_for(gl.items, kThreads), kThreads, gl, [&](dim3 gg, const GateK<double> &h) {
             switch (L.kind) {
 #define QI_LANE(K) case K: hipLaunchKernelGGL((k_pair_c128_lane<K>), gg, block, 0, st, psi, h); break;
             QI_LANE_KINDS(QI_LANE)
