@@ -43,7 +43,8 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-Wno-unused-parameter", "-D__HIP_PLATFORM_AMD__",
+              "-I/opt/rocm/include"]
 
 
 def _compile_host(src: Path) -> Path:

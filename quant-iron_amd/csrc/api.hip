@@ -13,34 +13,6 @@
 
 namespace qi {
 
-// ----------------------------------------------------------------- errors --
-// Display strings of errors.rs:4-97.
-Status make_err(int code, uint64_t a, uint64_t b, double re, double im, const char *detail) {
-    Status s;
-    s.code = code; s.a = a; s.b = b; s.re = re; s.im = im;
-    char buf[256];
-    switch (code) {
-    case QI_ERR_INVALID_NUMBER_OF_MEASUREMENTS: snprintf(buf, sizeof buf, "Invalid number of measurements: %llu", (unsigned long long)a); break;
-    case QI_ERR_OVERLAPPING_CONTROL_AND_TARGET:
-        snprintf(buf, sizeof buf, "Control qubit index %llu overlaps with target qubit index %llu", (unsigned long long)a, (unsigned long long)b); break;
-    case QI_ERR_INVALID_NUMBER_OF_QUBITS: snprintf(buf, sizeof buf, "Invalid number of qubits: %llu", (unsigned long long)a); break;
-    case QI_ERR_INVALID_QUBIT_INDEX:
-        snprintf(buf, sizeof buf, "Invalid qubit index: %llu for %llu qubits", (unsigned long long)a, (unsigned long long)b); break;
-    case QI_ERR_STATE_NOT_NORMALISED: snprintf(buf, sizeof buf, "State vector is not normalised"); break;
-    case QI_ERR_NON_UNITARY_MATRIX: snprintf(buf, sizeof buf, "Non-unitary matrix"); break;
-    case QI_ERR_UNKNOWN: snprintf(buf, sizeof buf, "An unknown error occurred"); break;
-    case QI_ERR_DEVICE: snprintf(buf, sizeof buf, "Device error: %s", detail ? detail : "unknown"); break;
-    case QI_ERR_LOCK: snprintf(buf, sizeof buf, "Failed to lock GPU context"); break;
-    case QI_ERR_ZERO_NORM: snprintf(buf, sizeof buf, "The state cannot be normalised because it has zero norm."); break;
-    case QI_ERR_INVALID_PAULI_COEFFICIENT: snprintf(buf, sizeof buf, "Invalid Pauli String coefficient: %g+%gi", re, im); break;
-    case QI_ERR_OUT_OF_MEMORY: snprintf(buf, sizeof buf, "Device out of memory%s%s", detail ? ": " : "", detail ? detail : ""); break;
-    case QI_ERR_INVALID_ARGUMENT: snprintf(buf, sizeof buf, "Invalid argument%s%s", detail ? ": " : "", detail ? detail : ""); break;
-    case QI_ERR_PRECISION_MISMATCH: snprintf(buf, sizeof buf, "States differ in precision"); break;
-    default: snprintf(buf, sizeof buf, "error %d", code); break;
-    }
-    s.msg = buf;
-    return s;
-}
 
 void fill_err(qi_error *err, const Status &st) {
     if (!err) return;

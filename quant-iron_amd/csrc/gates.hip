@@ -578,50 +578,6 @@ int pick_blocks(uint64_t items, int per_block) {
     return (int)b;
 }
 
-Status lower_gate(uint32_t n, const qi_gate &g, Lowered &L) {
-    (void)n;
-    L = Lowered{};
-    L.t0 = (uint32_t)g.targets[0];
-    L.t1 = g.ntargets > 1 ? (uint32_t)g.targets[1] : 0;
-    for (uint32_t k = 0; k < g.ncontrols; k++) L.ctrl_mask |= uint64_t(1) << g.controls[k];
-    const double r2 = 1.0 / std::sqrt(2.0);
-    const double h = g.params[0] / 2.0;
-    switch (g.kind) {
-    case QI_GATE_H: L.kind = LK_H; L.c[0] = r2; break;
-    case QI_GATE_X: case QI_GATE_CNOT: case QI_GATE_TOFFOLI: L.kind = LK_X; break;
-    case QI_GATE_Y:  // -i_complex = (-0.0, -1.0), i_complex = (0, 1) (operator.rs:489)
-        L.kind = LK_Y; L.c[0] = -0.0; L.c[1] = -1.0; L.c[2] = 0.0; L.c[3] = 1.0; break;
-    case QI_GATE_Z: L.kind = LK_Z; break;
-    case QI_GATE_I: L.kind = LK_NOP; break;
-    case QI_GATE_S: L.kind = LK_PHASE; L.c[0] = 0.0; L.c[1] = 1.0; break;
-    case QI_GATE_SDAG: L.kind = LK_PHASE; L.c[0] = 0.0; L.c[1] = -1.0; break;
-    case QI_GATE_T: L.kind = LK_PHASE; L.c[0] = r2; L.c[1] = r2; break;
-    case QI_GATE_TDAG: L.kind = LK_PHASE; L.c[0] = r2; L.c[1] = -r2; break;
-    case QI_GATE_P: L.kind = LK_PHASE; L.c[0] = std::cos(g.params[0]); L.c[1] = std::sin(g.params[0]); break;
-    case QI_GATE_RX: {  // i*sin, (-i)*sin as Complex * f64 (operator.rs:1756-1757)
-        double ch = std::cos(h), sh = std::sin(h);
-        L.kind = LK_RX; L.c[0] = ch; L.c[1] = 0.0 * sh; L.c[2] = 1.0 * sh; L.c[3] = -0.0 * sh; L.c[4] = -1.0 * sh;
-    } break;
-    case QI_GATE_RY: L.kind = LK_RY; L.c[0] = std::cos(h); L.c[1] = std::sin(h); break;
-    case QI_GATE_RZ:
-        L.kind = LK_RZ; L.c[0] = std::cos(h); L.c[1] = -std::sin(h); L.c[2] = std::cos(h); L.c[3] = std::sin(h); break;
-    case QI_GATE_U2: L.kind = LK_U2; for (int q = 0; q < 8; q++) L.c[q] = g.matrix[q]; break;
-    case QI_GATE_SWAP: L.kind = LK_SWAP; break;
-    case QI_GATE_MATCHGATE: {
-        double th = g.params[0] / 2.0, ch = std::cos(th), sh = std::sin(th);
-        // exp(i phi) = from_polar(exp(0), phi) (num-complex)
-        double e1r = std::exp(0.0) * std::cos(g.params[1]), e1i = std::exp(0.0) * std::sin(g.params[1]);
-        double e2r = std::exp(0.0) * std::cos(g.params[2]), e2i = std::exp(0.0) * std::sin(g.params[2]);
-        L.kind = LK_MATCH; L.t1 = L.t0 + 1;
-        L.c[0] = ch; L.c[1] = sh;
-        L.c[2] = e1r * sh; L.c[3] = e1i * sh;  // exp_i_phi1 * sin_theta_half
-        L.c[4] = e1r * ch; L.c[5] = e1i * ch;  // exp_i_phi1 * cos_theta_half
-        L.c[6] = e2r; L.c[7] = e2i;
-    } break;
-    default: return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "gate kind has no streaming kernel");
-    }
-    return Status{};
-}
 
 Status launch_gate(State &s, const Lowered &L, hipStream_t st) {
     if (L.kind == LK_NOP) return Status{};

@@ -1,4 +1,4 @@
-// tile_plan.hip — host planning of cache-blocked passes.
+// tile_plan.cpp — host planning of cache-blocked passes.
 //
 // A pass loads tiles of 2^K amplitudes spanned by a qubit set Q, applies a
 // group of gates inside each tile and writes the tile back, so one HBM round

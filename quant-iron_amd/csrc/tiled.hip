@@ -1,5 +1,5 @@
 // tiled.hip — register-resident execution of a tile pass (planned in
-// tile_plan.hip).
+// tile_plan.cpp).
 //
 // 2^TB threads hold a tile of 2^K amplitudes, 2^(K-TB) per thread in
 // registers. A layout assigns the K local index bits to TB thread bits (6

@@ -163,3 +163,57 @@ def test_errors_name_the_line(text, line, what):
     with pytest.raises(qiron.QuantIronError) as ei:
         parse(text)
     assert ei.value.variant == "InvalidInputValue" and ei.value.a == line and what in str(ei.value), str(ei.value)
+
+
+@pytest.fixture(scope="module")
+def qasm_san(tmp_path_factory):
+    """The reader (csrc/qasm.cpp, plain C++) with a small driver, built with the
+    address and undefined-behaviour sanitizers (any finding fatal)."""
+    import subprocess
+    root = Path(__file__).resolve().parent.parent
+    exe = tmp_path_factory.mktemp("qasm_san") / "qasm_san"
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", str(root / "quant-iron_amd" / "csrc" / "qasm.cpp"),
+           str(root / "tests" / "cpp" / "qasm_san.cpp"), "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
+def test_reader_under_sanitizers_on_truncated_and_corrupted_programs(qasm_san, tmp_path):
+    """Every prefix of an exporter program (cut at each byte of its first 600
+    and at each line end) and 400 single-byte corruptions parse to a result or
+    an error, with AddressSanitizer and UndefinedBehaviorSanitizer clean; the
+    intact programs give the library's own gate counts."""
+    import os
+    import subprocess
+    rng = np.random.default_rng(17)
+    texts = [(QASM / c["file"]).read_bytes() for c in CASES]
+    files = []
+
+    def add(data):
+        p = tmp_path / f"p{len(files)}.qasm"
+        p.write_bytes(data)
+        files.append(p)
+
+    for t in texts:
+        add(t)
+    base = texts[-1]
+    cuts = set(range(0, min(600, len(base)))) | {i + 1 for i, b in enumerate(base) if b == ord("\n")}
+    for c in sorted(cuts):
+        add(base[:c])
+    alphabet = b";()[]{}@,.-+*/^0123456789abcxyzq \n\t\"'#"
+    for _ in range(400):
+        t = bytearray(texts[int(rng.integers(len(texts)))])
+        i = int(rng.integers(len(t)))
+        t[i] = alphabet[int(rng.integers(len(alphabet)))] if rng.random() < 0.8 else int(rng.integers(256))
+        add(bytes(t))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run([str(qasm_san)] + [str(p) for p in files], capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0 and "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
+    out = r.stdout.splitlines()
+    assert len(out) == len(files)
+    for line, t in zip(out[:len(texts)], texts):
+        n, arr, _, _ = parse(t.decode())
+        assert line == f"ok {arr.size} {n} 0", line
+    print(f"{sum(l.startswith('ok') for l in out)} parsed, {sum(l.startswith('err') for l in out)} rejected")

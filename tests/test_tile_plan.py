@@ -8,6 +8,7 @@ lane/register exchange, a diagonal folded across a non-diagonal op or a
 wrong store layout all move amplitudes by O(1). These run without a GPU and
 cover every tile width and thread count, both precisions' op forms,
 relabelled SWAPs, streamed matchgates and parametric rewrites."""
+import os
 import subprocess
 from pathlib import Path
 
@@ -35,6 +36,23 @@ def emu(tmp_path_factory):
     return exe
 
 
+@pytest.fixture(scope="module")
+def emu_san(tmp_path_factory):
+    """The planner (tile_plan.cpp, host_common.cpp: plain C++) and the CPU model
+    built with the address and undefined-behaviour sanitizers, any finding
+    fatal: out-of-range layout / op-table indexing or integer overflow in the
+    planner aborts the run."""
+    exe = tmp_path_factory.mktemp("emu_san") / "tile_emu_san"
+    csrc = ROOT / "quant-iron_amd" / "csrc"
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           str(csrc / "tile_plan.cpp"), str(csrc / "host_common.cpp"), str(ROOT / "tests" / "cpp" / "tile_emu.cpp"),
+           "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return exe
+
+
 def run_emu(emu, tmp_path, gates, psi, n, prec, K, TB, angles=None, oop=False, full=False):
     g, i, o = tmp_path / "g.bin", tmp_path / "in.bin", tmp_path / "out.bin"
     g.write_bytes(np.ascontiguousarray(gates).tobytes())
@@ -46,8 +64,9 @@ def run_emu(emu, tmp_path, gates, psi, n, prec, K, TB, angles=None, oop=False, f
         args += ["--angles", str(a)]
     if oop:
         args.append("--oop")
-    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=600, env=env)
+    assert r.returncode == 0, r.stderr[-4000:]
     out = np.frombuffer(o.read_bytes(), dtype=np.complex128)
     steps, moves = (int(x) for x in r.stdout.split()[:2])
     return (out, steps, moves) if full else (out, steps)
@@ -125,4 +144,22 @@ def test_out_of_place_plan_with_final_permutation(emu, tmp_path, prec):
     _, steps_inplace, moves_inplace = run_emu(emu, tmp_path, gates, psi, n, prec, K, 6, full=True)
     assert moves == 2 and moves_inplace == 0
     assert steps < steps_inplace, (steps, steps_inplace)
+    assert np.max(np.abs(got - O.apply_gates(psi, gates))) <= 1e-10
+
+
+SAN_CASES = [("c64", 12, 6, False), ("c128", 11, 6, False), ("c64", 13, 9, False), ("c64", 10, 6, True), ("c128", 10, 6, True)]
+
+
+@pytest.mark.parametrize("prec,K,TB,oop", SAN_CASES, ids=[f"{p}-K{k}-T{t}{'-oop' if o else ''}" for p, k, t, o in SAN_CASES])
+def test_planner_under_sanitizers(emu_san, tmp_path, prec, K, TB, oop):
+    """Plans of a random circuit with SWAP relabelling, controlled gates and
+    matchgates (one-wave and LDS tiles, in place and out of place) run with
+    AddressSanitizer and UndefinedBehaviorSanitizer and still match the
+    oracle."""
+    n = 15
+    rng = np.random.default_rng(77 + K + TB)
+    gates = np.concatenate([W.random_circuit(n=n, depth=6, seed=5 + K),
+                            gates_array([random_gate(n, rng, max_controls=2) for _ in range(60)])])
+    psi = random_state(n, rng)
+    got, _ = run_emu(emu_san, tmp_path, gates, psi, n, prec, K, TB, oop=oop)
     assert np.max(np.abs(got - O.apply_gates(psi, gates))) <= 1e-10
