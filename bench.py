@@ -462,10 +462,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--only", default="", choices=["", "tiled", "tiled128", "gate", "copy", "surrogate", "ceiling"],
+    ap.add_argument("--only", default="", choices=["", "tiled", "tiled128", "gate", "copy", "surrogate", "ceiling",
+                                                   "energies"],
                     help="profiling runs: only the headline executor (complex64, or complex128 with tiled128), only "
                          "the single-gate kernels, only a plain copy of known size (the reference for HBM byte "
-                         "counters), only the surrogate, or only the 128 GiB memory-ceiling runs")
+                         "counters), only the surrogate, only the 128 GiB memory-ceiling runs, or only the batched energies")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
@@ -505,6 +506,10 @@ def main():
         return
     if args.only == "surrogate":
         print(json.dumps(bench_surrogate(W, args)), flush=True)
+        return
+    if args.only == "energies":
+        eres, _ = bench_energies(qiron, L, W, args)
+        print(json.dumps(eres), flush=True)
         return
     if args.only == "ceiling":
         print(json.dumps(bench_ceiling(qiron, L, W)), flush=True)
