@@ -385,6 +385,15 @@ qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uin
  * 10 <= k <= ks <= 13 (complex64) / 12 (complex128). Default k = ks = 13 / 12. */
 qi_status qi_batch_set_schedule(qi_batch_t b, uint32_t k, uint32_t ks, qi_error *err);
 qi_status qi_batch_get_schedule(qi_batch_t b, uint32_t *k, uint32_t *ks, qi_error *err);
+/* Executor of the circuits larger than ks qubits. QI_BATCH_WAVE (default):
+ * circuits of 12+ (complex64) / 11+ (complex128) qubits run the register-
+ * resident one-wave passes of the tiled executor (a wave holds a 2^12 / 2^11
+ * amplitude tile; pass p of every circuit in one launch), the others LDS
+ * tiles of width k. QI_BATCH_LDS: every one of them runs LDS tiles of width k.
+ * No reference counterpart. */
+typedef enum qi_batch_exec { QI_BATCH_LDS = 0, QI_BATCH_WAVE = 1 } qi_batch_exec;
+qi_status qi_batch_set_executor(qi_batch_t b, qi_batch_exec exec, qi_error *err);
+qi_status qi_batch_get_executor(qi_batch_t b, qi_batch_exec *exec, qi_error *err);
 /* Autotuner: plans, captures and times every schedule on this batch (best of
  * `trials` replays after a warm-up, HIP-event device time), keeps the fastest
  * and leaves the batch planned with it. No reference counterpart. */

@@ -89,6 +89,19 @@ __global__ __launch_bounds__(256) void k_batch_resolve(const double *__restrict_
     coef[4 * i] = ch; coef[4 * i + 1] = sh; coef[4 * i + 2] = c1; coef[4 * i + 3] = s1;
 }
 
+// Per-slot coefficient rows of the one-wave passes (kParamRow values per
+// slot in the pass's precision, the format k_batch_wave reads), from the
+// slot trigonometry above.
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_ptab(const double *__restrict__ coef, uint32_t n, R *__restrict__ out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const double *k = coef + 4 * (size_t)i;
+    R *o = out + (size_t)kParamRow * i;
+    o[0] = R(k[0]); o[1] = R(-k[1]); o[2] = R(k[1]); o[3] = R(k[2]);
+    o[4] = R(k[3]); o[5] = R(1); o[6] = R(0); o[7] = R(0);
+}
+
 __device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ coef, int slot_base, double *c) {
 #pragma unroll
     for (int q = 0; q < 8; q++) c[q] = op.c[q];
@@ -460,6 +473,11 @@ struct Batch {
         uint32_t op0 = 0, nops = 0;           // small: op range
         std::vector<TilePass> passes;         // tiled: passes (op0 relative to ops_base)
         uint32_t ops_base = 0;
+        // one-wave form (planned once: it does not depend on the schedule)
+        int wave = -1;                        // -1 not planned, 0 no one-wave plan, 1 planned
+        std::vector<RegPass> wpasses;         // op0 relative to the program's wops
+        std::vector<RegOp> wops;
+        uint32_t wpass_base = 0;              // first pass in the uploaded pass table
     };
     struct Circ {
         uint32_t prog;
@@ -473,8 +491,12 @@ struct Batch {
     std::vector<ETerm> eterms;   // host copy of the term table
     // device
     DevBuf d_ops, d_terms, d_small, d_big, d_fold, d_angles, d_coef, d_part, d_energy, d_arena;
+    DevBuf d_wops, d_wpasses, d_wtasks, d_ptab;     // one-wave passes: ops, passes, tasks, slot coefficients
+    int exec = QI_BATCH_WAVE;
     std::vector<std::pair<size_t, size_t>> levels;  // [task begin, task end) of each pass level
     std::vector<uint32_t> level_blocks;
+    std::vector<std::pair<size_t, size_t>> wlevels;  // the same for one-wave passes
+    std::vector<uint32_t> wlevel_blocks;
     std::vector<std::pair<size_t, size_t>> elevels;
     std::vector<uint32_t> elevel_blocks;
     size_t nsmall = 0, nfold = 0;
@@ -486,7 +508,8 @@ struct Batch {
     hipGraphExec_t graph = nullptr;
     double last_ms = 0.0;
     ~Batch() {
-        for (DevBuf *b : {&d_ops, &d_terms, &d_small, &d_big, &d_fold, &d_angles, &d_coef, &d_part, &d_energy, &d_arena})
+        for (DevBuf *b : {&d_ops, &d_terms, &d_small, &d_big, &d_fold, &d_angles, &d_coef, &d_part, &d_energy, &d_arena,
+                          &d_wops, &d_wpasses, &d_wtasks, &d_ptab})
             if (b->p) (void)hipFree(b->p);
         if (graph) (void)hipGraphExecDestroy(graph);
         if (side) (void)hipStreamDestroy(side);
@@ -613,6 +636,41 @@ template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &
     return Status{};
 }
 
+// The program's one-wave plan: the tiled executor's planner (kBatchWaveK
+// tile qubits, one wave per tile) with every slotted gate kept parametric.
+// Programs whose plan needs anything but in-place one-wave passes (streamed
+// LDS gates) keep LDS tiles. This engine returns energies only, which no
+// global phase changes, so an uncontrolled parametric RZ(theta) =
+// e^{-i theta/2} diag(1, e^{i theta}) runs as the one-sided phase of P(theta)
+// (the target-1 half only).
+static void plan_wave(Batch::Program &P, int prec) {
+    P.wave = 0;
+    P.wpasses.clear();
+    P.wops.clear();
+    std::vector<Lowered> lw;
+    for (size_t g = 0; g < P.gates.size(); g++) {
+        Lowered Lg;
+        if (!lower_gate(P.n, P.gates[g], Lg).ok()) return;
+        if (Lg.kind == LK_NOP) continue;
+        Lg.slot = P.rel_slot[g];
+        lw.push_back(Lg);
+    }
+    if (lw.empty()) return;
+    const int K = prec == QI_C128 ? kBatchWaveK128 : kBatchWaveK64;
+    State shape;
+    shape.n = P.n;
+    shape.prec = prec;
+    ExecPlan plan;
+    if (!plan_tiled(shape, lw, K, kLaneBits, plan, false).ok() || plan.steps.empty()) return;
+    for (const ExecStep &st : plan.steps)
+        if (st.type != ExecStep::REG || st.reg.oop || st.reg.permute || st.reg.tb != kLaneBits || st.reg.K != K) return;
+    for (RegOp &op : plan.reg_ops)
+        if (op.param > 0 && op.kind == RO_DG && !op.rcm && !op.tcm && !op.gcm) op.kind = RO_P1;
+    for (const ExecStep &st : plan.steps) P.wpasses.push_back(st.reg);
+    P.wops = std::move(plan.reg_ops);
+    P.wave = 1;
+}
+
 // Plans every program once (LDS-resident op list or tiled passes), then lays
 // out per-circuit tasks, expectation passes and the energy fold.
 Status batch_plan(Batch &B) {
@@ -621,6 +679,10 @@ Status batch_plan(Batch &B) {
     if (!B.KS) B.KS = B.K;
     if (B.KS < B.K) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "LDS-resident limit below the tile width");
     const int L = c128 ? 3 : 4;
+    const int kw = c128 ? kBatchWaveK128 : kBatchWaveK64;
+    const auto on_waves = [&](const Batch::Program &P) {
+        return !P.small && B.exec == QI_BATCH_WAVE && (int)P.n >= kw && P.wave == 1;
+    };
     std::vector<TileOp> ops;
     for (auto &P : B.progs) {
         std::vector<Lowered> lw;
@@ -635,6 +697,10 @@ Status batch_plan(Batch &B) {
         }
         P.small = (int)P.n <= B.KS;
         P.passes.clear();
+        if (!P.small && B.exec == QI_BATCH_WAVE && (int)P.n >= kw) {
+            if (P.wave < 0) plan_wave(P, B.prec);
+            if (P.wave == 1) continue;
+        }
         if (P.small) {
             P.op0 = (uint32_t)ops.size();
             for (size_t g = 0; g < lw.size(); g++) {
@@ -674,8 +740,20 @@ Status batch_plan(Batch &B) {
         }
         ops.insert(ops.end(), packed.begin(), packed.end());
     }
+    std::vector<RegOp> wops;
+    std::vector<RegPass> wpasses;
+    for (auto &P : B.progs) {
+        if (!on_waves(P)) continue;
+        P.wpass_base = (uint32_t)wpasses.size();
+        for (RegPass rp : P.wpasses) {
+            rp.op0 += (int32_t)wops.size();
+            wpasses.push_back(rp);
+        }
+        wops.insert(wops.end(), P.wops.begin(), P.wops.end());
+    }
     std::vector<SmallTask> small;
     std::vector<std::vector<BigTask>> per_level, per_elevel;
+    std::vector<std::vector<WaveTask>> per_wlevel;
     std::vector<FoldTask> folds;
     B.eterms.clear();
     uint64_t arena_elems = 0;
@@ -705,6 +783,15 @@ Status batch_plan(Batch &B) {
         const uint64_t off = arena_elems;
         arena_elems += uint64_t(1) << P.n;
         const uint32_t tiles = uint32_t(1) << (P.n - B.K);
+        if (on_waves(P)) {
+            for (size_t p = 0; p < P.wpasses.size(); p++) {
+                WaveTask wt{};
+                wt.off = off; wt.pass = P.wpass_base + (uint32_t)p; wt.slot_base = C.slot_base; wt.init = p == 0 ? 1 : 0;
+                wt.block0 = uint32_t(1) << (P.n - kw);  // tile count; turned into offsets per level below
+                if (per_wlevel.size() <= p) per_wlevel.resize(p + 1);
+                per_wlevel[p].push_back(wt);
+            }
+        }
         size_t level = 0;
         for (const TilePass &tp : P.passes) {
             BigTask bt{};
@@ -716,7 +803,7 @@ Status batch_plan(Batch &B) {
             per_level[level].push_back(bt);
             level++;
         }
-        if (P.passes.empty()) {  // no gates: one pass that only writes |0..0>
+        if (P.passes.empty() && !on_waves(P)) {  // no gates: one pass that only writes |0..0>
             BigTask bt{};
             bt.off = off; bt.init = 1;
             uint64_t q = 0;
@@ -788,6 +875,15 @@ Status batch_plan(Batch &B) {
     };
     flatten(per_level, B.levels, B.level_blocks);
     flatten(per_elevel, B.elevels, B.elevel_blocks);
+    std::vector<WaveTask> wtasks;
+    B.wlevels.clear(); B.wlevel_blocks.clear();
+    for (auto &v : per_wlevel) {
+        const size_t b0 = wtasks.size();
+        uint32_t acc = 0;
+        for (auto &t : v) { const uint32_t tiles = t.block0; t.block0 = acc; acc += tiles; wtasks.push_back(t); }
+        B.wlevels.push_back({b0, wtasks.size()});
+        B.wlevel_blocks.push_back(acc);
+    }
     B.nsmall = small.size();
     B.nfold = folds.size();
     Status e;
@@ -798,6 +894,10 @@ Status batch_plan(Batch &B) {
     if (!(e = upload_vec(B.d_fold, folds)).ok()) return e;
     if (!(e = upload_vec(B.d_angles, B.angles)).ok()) return e;
     if (!(e = ensure(B.d_coef, std::max<size_t>(B.angles.size(), 1) * 4 * sizeof(double))).ok()) return e;
+    if (!(e = upload_vec(B.d_wops, wops)).ok()) return e;
+    if (!(e = upload_vec(B.d_wpasses, wpasses)).ok()) return e;
+    if (!(e = upload_vec(B.d_wtasks, wtasks)).ok()) return e;
+    if (!(e = ensure(B.d_ptab, std::max<size_t>(B.angles.size(), 1) * kParamRow * (c128 ? 8 : 4))).ok()) return e;
     if (!(e = ensure(B.d_part, std::max<size_t>(nparts, 1) * 4 * sizeof(double))).ok()) return e;
     if (!(e = ensure(B.d_energy, std::max<size_t>(B.circs.size(), 1) * 2 * sizeof(double))).ok()) return e;
     const size_t esz = c128 ? 16 : 8;
@@ -812,14 +912,26 @@ Status batch_plan(Batch &B) {
     return Status{};
 }
 
-template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st) {
+// Pass level l of the LDS-tiled circuits and of the one-wave ones (disjoint
+// circuits), then the expectation passes and the fold.
+template <typename R, int K> static Status launch_levels(Batch &B, hipStream_t st) {
     using T = typename V2<R>::T;
     const BigTask *big = (const BigTask *)B.d_big.p;
-    for (size_t l = 0; l < B.levels.size(); l++) {
-        auto r = B.levels[l];
-        hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kPT), 0, st, (T *)B.d_arena.p,
-                           big + r.first, (int)(r.second - r.first), (const TileOp *)B.d_ops.p,
-                           (const double *)B.d_coef.p);
+    const WaveTask *wt = (const WaveTask *)B.d_wtasks.p;
+    for (size_t l = 0; l < std::max(B.levels.size(), B.wlevels.size()); l++) {
+        if (l < B.levels.size()) {
+            auto r = B.levels[l];
+            hipLaunchKernelGGL((k_batch_pass<R, K>), dim3(B.level_blocks[l]), dim3(kPT), 0, st, (T *)B.d_arena.p,
+                               big + r.first, (int)(r.second - r.first), (const TileOp *)B.d_ops.p,
+                               (const double *)B.d_coef.p);
+        }
+        if (l < B.wlevels.size()) {
+            auto r = B.wlevels[l];
+            Status e = launch_batch_wave(B.prec, B.wlevel_blocks[l], st, B.d_arena.p, wt + r.first,
+                                         (int)(r.second - r.first), (const RegPass *)B.d_wpasses.p,
+                                         (const RegOp *)B.d_wops.p, B.d_ptab.p);
+            if (!e.ok()) return e;
+        }
     }
     for (size_t l = 0; l < B.elevels.size(); l++) {
         auto r = B.elevels[l];
@@ -829,12 +941,24 @@ template <typename R, int K> static void launch_levels(Batch &B, hipStream_t st)
     if (B.nfold)
         hipLaunchKernelGGL(k_batch_fold, dim3((unsigned)B.nfold), dim3(256), 0, st, (const FoldTask *)B.d_fold.p,
                            (const double *)B.d_part.p, (double *)B.d_energy.p);
+    return Status{};
 }
 
 static Status batch_launch(Batch &B, hipStream_t st) {
-    if (!B.angles.empty())
-        hipLaunchKernelGGL(k_batch_resolve, dim3((unsigned)((B.angles.size() + 255) / 256)), dim3(256), 0, st,
-                           (const double *)B.d_angles.p, (uint32_t)B.angles.size(), (double *)B.d_coef.p);
+    if (!B.angles.empty()) {
+        const unsigned g = (unsigned)((B.angles.size() + 255) / 256);
+        const uint32_t na = (uint32_t)B.angles.size();
+        hipLaunchKernelGGL(k_batch_resolve, dim3(g), dim3(256), 0, st, (const double *)B.d_angles.p, na,
+                           (double *)B.d_coef.p);
+        if (!B.wlevels.empty()) {
+            if (B.prec == QI_C128)
+                hipLaunchKernelGGL(k_batch_ptab<double>, dim3(g), dim3(256), 0, st, (const double *)B.d_coef.p, na,
+                                   (double *)B.d_ptab.p);
+            else
+                hipLaunchKernelGGL(k_batch_ptab<float>, dim3(g), dim3(256), 0, st, (const double *)B.d_coef.p, na,
+                                   (float *)B.d_ptab.p);
+        }
+    }
     // fork: LDS-resident circuits on the side stream, tiled levels on `st`
     QI_HIP_TRY(hipEventRecord(B.fork, st));
     QI_HIP_TRY(hipStreamWaitEvent(B.side, B.fork, 0));
@@ -848,20 +972,22 @@ static Status batch_launch(Batch &B, hipStream_t st) {
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
                                (const double *)B.d_coef.p, (double *)B.d_energy.p);
     }
+    Status le;
     if (B.prec == QI_C128) {
         switch (B.K) {
-        case 10: launch_levels<double, 10>(B, st); break;
-        case 11: launch_levels<double, 11>(B, st); break;
-        default: launch_levels<double, 12>(B, st); break;
+        case 10: le = launch_levels<double, 10>(B, st); break;
+        case 11: le = launch_levels<double, 11>(B, st); break;
+        default: le = launch_levels<double, 12>(B, st); break;
         }
     } else {
         switch (B.K) {
-        case 10: launch_levels<float, 10>(B, st); break;
-        case 11: launch_levels<float, 11>(B, st); break;
-        case 12: launch_levels<float, 12>(B, st); break;
-        default: launch_levels<float, 13>(B, st); break;
+        case 10: le = launch_levels<float, 10>(B, st); break;
+        case 11: le = launch_levels<float, 11>(B, st); break;
+        case 12: le = launch_levels<float, 12>(B, st); break;
+        default: le = launch_levels<float, 13>(B, st); break;
         }
     }
+    if (!le.ok()) return le;
     QI_HIP_TRY(hipEventRecord(B.join, B.side));
     QI_HIP_TRY(hipStreamWaitEvent(st, B.join, 0));
     QI_HIP_TRY(hipGetLastError());
@@ -1082,6 +1208,22 @@ qi_status qi_batch_get_schedule(qi_batch_t b, uint32_t *k, uint32_t *ks, qi_erro
     QB_RET(Status{});
 }
 
+qi_status qi_batch_set_executor(qi_batch_t b, qi_batch_exec exec, qi_error *err) {
+    if (!b) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    if (exec != QI_BATCH_LDS && exec != QI_BATCH_WAVE) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "executor"));
+    Batch &B = b->b;
+    std::lock_guard<std::mutex> lk(B.ctx->mu);
+    B.exec = (int)exec;
+    B.planned = false;
+    QB_RET(Status{});
+}
+
+qi_status qi_batch_get_executor(qi_batch_t b, qi_batch_exec *exec, qi_error *err) {
+    if (!b || !exec) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "NULL argument"));
+    *exec = (qi_batch_exec)b->b.exec;
+    QB_RET(Status{});
+}
+
 // Autotuner: plans, captures and times every schedule (K, ks) on this batch
 // (best of `trials` graph replays after one warm-up run each, device time
 // from HIP events) and keeps the fastest; schedules the batch cannot use
@@ -1136,7 +1278,7 @@ qi_status qi_batch_info(qi_batch_t b, uint64_t *ncircuits, uint64_t *nsmall, uin
     Batch &B = b->b;
     if (ncircuits) *ncircuits = B.circs.size();
     if (nsmall) *nsmall = B.nsmall;
-    if (nlevels) *nlevels = B.levels.size() + B.elevels.size();
+    if (nlevels) *nlevels = std::max(B.levels.size(), B.wlevels.size()) + B.elevels.size();
     if (last_ms) *last_ms = B.last_ms;
     QB_RET(Status{});
 }

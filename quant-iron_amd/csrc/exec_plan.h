@@ -169,6 +169,22 @@ Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_
 // used by the batched engine.
 Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K, ExecPlan &plan);
 
+// Batched one-wave passes (the batched engine's circuits of 12+ / 11+
+// qubits): one launch runs pass p of many circuits, one tile per wave.
+constexpr int kBatchWaveK64 = 12, kBatchWaveK128 = 11;  // tile qubits (complex64 / complex128)
+// Values per angle slot in the parametric-coefficient table the kernel reads
+// (the kernel's precision): cos(t/2), -sin(t/2), sin(t/2), cos t, sin t, 1, 0, 0.
+constexpr int kParamRow = 8;
+struct WaveTask {
+    uint64_t off;       // element offset of the circuit's state in the arena
+    uint32_t block0;    // first workgroup of this task in the launch
+    uint32_t pass;      // index of its RegPass in the pass table
+    int32_t slot_base;  // first angle slot of the circuit (RegOp::param counts from it)
+    uint32_t init;      // 1: the state is |0..0> (the pass loads nothing)
+};
+Status launch_batch_wave(int prec, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks, int ntasks,
+                         const RegPass *passes, const RegOp *ops, const void *ptab);
+
 Status plan_fused(const State &s, const std::vector<Lowered> &gates, int kb, ExecPlan &plan);
 Status launch_fused_block(State &s, const FusedBlock &b, const void *dmat, hipStream_t st);
 

@@ -1042,6 +1042,94 @@ __global__ __launch_bounds__(64) void k_wave_pass(const typename V2<R>::T *src_p
     }
 }
 
+// ------------------------------------------------ batched one-wave passes --
+// Coefficients of a parametric op from its slot's row of the table
+// (kParamRow values per angle slot, in the kernel's precision), in the forms
+// param_op_coeffs writes: RY (c, -s, s, c), RX (c, -s, -s, c), RZ (c, -s, c, s)
+// with c, s = cos, sin of half the angle, P (1, 0, cos, sin) of the angle.
+template <typename R> __device__ __forceinline__ void set_coef(RegOp &op, int q, R x) {
+    if constexpr (sizeof(R) == 4) op.cf[q] = x;
+    else op.c[q] = x;
+}
+template <typename R> __device__ __forceinline__ void param_coeffs(RegOp &op, const R *__restrict__ t) {
+    const R ch = t[0], nsh = t[1], sh = t[2];
+    switch (op.kind) {
+    case RO_RR: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, sh); set_coef<R>(op, 3, ch); break;
+    case RO_RXI: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, nsh); set_coef<R>(op, 3, ch); break;
+    case RO_DG: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, ch); set_coef<R>(op, 3, sh); break;
+    default: set_coef<R>(op, 0, t[5]); set_coef<R>(op, 1, t[6]); set_coef<R>(op, 2, t[3]); set_coef<R>(op, 3, t[4]); break;
+    }
+}
+
+__device__ __forceinline__ int find_wave_task(const WaveTask *__restrict__ tasks, int ntasks, uint32_t b) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].block0 <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Pass p of many circuits in one launch (the batched energies): workgroup b
+// is one wave holding one tile of the circuit whose task covers b (tasks in
+// block order); the task names the circuit's state in the arena, its pass
+// and its first angle slot. The tile work is k_wave_pass's; a circuit's first
+// pass builds |0..0> in registers instead of loading.
+template <typename R, int K>
+__global__ __launch_bounds__(64) void k_batch_wave(typename V2<R>::T *__restrict__ arena, const WaveTask *__restrict__ tasks,
+                                                   int ntasks, const RegPass *__restrict__ passes,
+                                                   const RegOp *__restrict__ ops, const R *__restrict__ ptab) {
+    using T = typename V2<R>::T;
+    constexpr int TB = kLaneBits, RB = K - TB, E = 1 << RB;
+    const uint32_t lane = threadIdx.x;
+    const WaveTask &tk = tasks[find_wave_task(tasks, ntasks, blockIdx.x)];
+    const RegPass &pass = passes[tk.pass];
+    const uint64_t base = insert_zeros64(blockIdx.x - tk.block0, pass.qmask);
+    T *psi = arena + tk.off;
+    uint64_t at_in = 0, at_out = 0, tmask = 0;
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        const uint32_t bit = (lane >> i) & 1u;
+        at_in |= uint64_t(bit) << pass.qt_in[i];
+        at_out |= uint64_t(bit) << pass.qt_out[i];
+        tmask |= uint64_t(1) << pass.qt_out[i];
+    }
+    Regs<R, E> v;
+    if (tk.init) {  // |0..0>: register 0 of the lane whose address is 0
+        const bool one = (base | at_in) == 0;
+#pragma unroll
+        for (int r = 0; r < E; r++) v.set(r, mk<T>(r == 0 && one ? R(1) : R(0), R(0)));
+    } else {
+        const T *src = psi + (base | at_in);
+#pragma unroll
+        for (int r = 0; r < E; r++) {
+            uint64_t o = 0;
+#pragma unroll
+            for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_in[j];
+            v.set(r, ld_stream(src + o));
+        }
+    }
+    const R *prow = ptab + size_t(kParamRow) * size_t(tk.slot_base);
+    R ph[2] = {R(1), R(0)};
+    RegOp nxt = ops[pass.op0];
+    for (int o = 0; o < pass.nops; o++) {
+        RegOp op = nxt;
+        nxt = ops[pass.op0 + o + 1];
+        if (op.param > 0) param_coeffs<R>(op, prow + size_t(kParamRow) * size_t(op.param - 1));
+        if (op.kind < RO_TRANSPOSE && (base & op.gcm) == op.gcm) apply_op<R, K, TB>(v, op, base, lane, lane, ph);
+    }
+    if (pass.sync_store) __builtin_amdgcn_s_waitcnt(0);
+    const uint64_t rflip = pass.flip & ~tmask;
+    T *dst = psi + (base | (at_out ^ (pass.flip & tmask)));
+#pragma unroll
+    for (int r = 0; r < E; r++) {
+        uint64_t o = 0;
+#pragma unroll
+        for (int j = 0; j < RB; j++) if ((r >> j) & 1) o |= uint64_t(1) << pass.qr_out[j];
+        st_stream(v.get(r), dst + (o ^ rflip));
+    }
+}
+
 // Runs of consecutive tiles per XCD (log2; shorter when the grid is small):
 // 2-3% faster passes than the identity order on 30-qubit states.
 constexpr int kTileChunkLog2 = 8;
@@ -1100,6 +1188,19 @@ Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_
         }
     }
     if (!ok) return make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "tile width / threads");
+    QI_HIP_TRY(hipGetLastError());
+    return Status{};
+}
+
+Status launch_batch_wave(int prec, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks, int ntasks,
+                         const RegPass *passes, const RegOp *ops, const void *ptab) {
+    if (!blocks || ntasks <= 0) return Status{};
+    if (prec == QI_C128)
+        hipLaunchKernelGGL((k_batch_wave<double, kBatchWaveK128>), dim3(blocks), dim3(64), 0, st, (double2 *)arena, tasks,
+                           ntasks, passes, ops, (const double *)ptab);
+    else
+        hipLaunchKernelGGL((k_batch_wave<float, kBatchWaveK64>), dim3(blocks), dim3(64), 0, st, (float2 *)arena, tasks,
+                           ntasks, passes, ops, (const float *)ptab);
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }

@@ -149,6 +149,8 @@ SIGNATURES = {
     "qi_batch_set_schedule": [P, C.c_uint32, C.c_uint32, ERR],
     "qi_batch_get_schedule": [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), ERR],
     "qi_batch_autotune": [P, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), DP, ERR],
+    "qi_batch_set_executor": [P, C.c_int, ERR],
+    "qi_batch_get_executor": [P, C.POINTER(C.c_int), ERR],
     "qi_last_exec_ms": [P, DP, ERR],
     "qi_last_exec_steps": [P, U64P, ERR],
 }

@@ -1102,6 +1102,20 @@ class CircuitBatch:
         _check(self.lib.qi_batch_get_schedule(self._h, C.byref(k), C.byref(ks), C.byref(err)), err)
         return k.value, ks.value
 
+    EXECUTORS = {"lds": 0, "wave": 1}
+
+    def set_executor(self, name: str) -> None:
+        """Executor of the circuits above ks qubits: "wave" (default; one-wave
+        register passes for 12+ / 11+ qubits) or "lds" (LDS tiles of width k)."""
+        err = L.QiError()
+        _check(self.lib.qi_batch_set_executor(self._h, self.EXECUTORS[name], C.byref(err)), err)
+
+    def executor(self) -> str:
+        e = C.c_int()
+        err = L.QiError()
+        _check(self.lib.qi_batch_get_executor(self._h, C.byref(e), C.byref(err)), err)
+        return {v: k for k, v in self.EXECUTORS.items()}[e.value]
+
     def autotune(self, trials: int = 2):
         """Times every schedule on this batch and keeps the fastest: (k, ks, ms)."""
         k, ks, ms = C.c_uint32(), C.c_uint32(), C.c_double()

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from qiron import _lib as L
+from qiron import surrogate  # noqa: F401  (registers the MLP entry points' signatures)
 
 ROOT = Path(__file__).resolve().parent.parent
 HEADER = ROOT / "include" / "qiron.h"

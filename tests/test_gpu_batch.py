@@ -5,6 +5,7 @@ import pytest
 
 import oracle as O
 import qiron
+from kat_util import ONE_Q, gates_array, random_gate, random_terms
 from qiron import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -36,10 +37,13 @@ def oracle_energies(refs):
     return np.array(out)
 
 
+@pytest.mark.parametrize("executor", ["wave", "lds"])
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_batch_energies_match_one_at_a_time_oracle(ctx, prec):
+def test_batch_energies_match_one_at_a_time_oracle(ctx, prec, executor):
     specs = W.batched_ansatz(count=48, seed=77, nmin=3, nmax=17, lmin=1, lmax=4)
     b = qiron.CircuitBatch(prec)
+    b.set_executor(executor)
+    assert b.executor() == executor
     angles, refs = build(b, specs)
     e = b.run(use_graph=True)
     ref = oracle_energies(refs)
@@ -101,12 +105,14 @@ def test_every_schedule_and_the_autotuned_one_match_the_oracle(ctx, prec):
     ref = oracle_energies(refs)
     tol = 1e-11 if prec == "c128" else 2e-5
     kmax = 12 if prec == "c128" else 13
-    for k in range(10, kmax + 1):
-        for ks in (k, kmax):
-            b.set_schedule(k, ks)
-            e = b.run(use_graph=True)
-            assert np.max(np.abs(e - ref)) <= tol, (k, ks, np.max(np.abs(e - ref)))
-            assert b.schedule() == (k, ks)
+    for executor in ("lds", "wave"):
+        b.set_executor(executor)
+        for k in range(10, kmax + 1):
+            for ks in (k, kmax):
+                b.set_schedule(k, ks)
+                e = b.run(use_graph=True)
+                assert np.max(np.abs(e - ref)) <= tol, (executor, k, ks, np.max(np.abs(e - ref)))
+                assert b.schedule() == (k, ks)
     k, ks, ms = b.autotune(trials=1)
     assert 10 <= k <= ks <= kmax and ms > 0
     assert b.schedule() == (k, ks)
@@ -120,3 +126,57 @@ def test_schedule_out_of_range_is_rejected(ctx):
         b.set_schedule(13, 13)
     with pytest.raises(qiron.QuantIronError):
         b.set_schedule(12, 11)
+
+
+@pytest.mark.parametrize("prec", ["c64", "c128"])
+def test_one_wave_passes_on_general_circuits_match_the_oracle(ctx, prec):
+    """Circuits of every operator kind with up to two controls on 12..15
+    qubits, RX / RY / RZ / P (controlled or not) reading the angle table, and
+    random Pauli terms with complex coefficients: the one-wave executor and
+    the LDS one both give the oracle's energies, before and after new angles.
+    Uncontrolled SWAPs make one-wave plans with in-place permutation passes;
+    circuits with controlled SWAPs or matchgates (streamed gates in a one-wave
+    plan) run LDS tiles inside the same batch."""
+    rng = np.random.default_rng(91)
+    rot_kinds = [W.K[k] for k in ("RX", "RY", "RZ", "P")]
+
+    def circuits(angle_rng):
+        out, angles = [], []
+        for c in range(12):
+            n = 12 + c % 4
+            kinds = ONE_Q + ["CNOT", "TOFFOLI"] + (["SWAP", "MATCHGATE"] if c % 3 == 0 else [])
+            if c % 3 == 1:
+                kinds = kinds + ["SWAP"]
+            g = gates_array([random_gate(n, np.random.default_rng(1000 + c), kinds=kinds, max_controls=2)
+                             for _ in range(150)])
+            if c % 3 == 1:  # uncontrolled SWAPs only (the relabelled form)
+                g = g[~((g["kind"] == W.K["SWAP"]) & (g["ncontrols"] > 0))]
+            rot = np.where(np.isin(g["kind"], rot_kinds))[0]
+            slots = np.full(g.size, -1, np.int32)
+            slots[rot] = len(angles) + np.arange(rot.size)
+            g["params"][rot, 0] = angle_rng.uniform(-2 * np.pi, 2 * np.pi, size=rot.size)
+            angles.extend(g["params"][rot, 0].tolist())
+            out.append((n, g, slots))
+        return out, np.array(angles)
+
+    circs, angles = circuits(rng)
+    terms = [random_terms(n, 10, np.random.default_rng(50 + i)) for i, (n, _, _) in enumerate(circs)]
+    b = qiron.CircuitBatch(prec)
+    for (n, g, slots), t in zip(circs, terms):
+        b.add(n, g, t, slots)
+    b.set_angles(angles)
+    tol = 1e-11 if prec == "c128" else 5e-5
+    ref = oracle_energies([(n, g, t) for (n, g, _), t in zip(circs, terms)])
+    for executor in ("wave", "lds"):
+        b.set_executor(executor)
+        e = b.run(use_graph=True)
+        assert np.max(np.abs(e - ref)) <= tol, (executor, np.max(np.abs(e - ref)))
+    # new angles through the table: one upload and a graph replay
+    circs2, angles2 = circuits(np.random.default_rng(92))
+    b.set_angles(angles2)
+    ref2 = oracle_energies([(n, g, t) for (n, g, _), t in zip(circs2, terms)])
+    e2 = b.run(use_graph=True)
+    assert np.max(np.abs(e2 - ref2)) <= tol
+    b.set_executor("wave")
+    e3 = b.run(use_graph=True)
+    assert np.max(np.abs(e3 - ref2)) <= tol
