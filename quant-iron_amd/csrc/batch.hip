@@ -4,12 +4,14 @@
 // Circuits that fit a workgroup's LDS (2^13 complex64 / 2^12 complex128
 // amplitudes) run LDS-resident: one workgroup builds |0..0>, applies every gate
 // and reduces the energy without touching HBM. Larger circuits live in one
-// device arena; their gates are planned into LDS-tiled passes (tiled.hip's
-// planner), and pass p of every circuit runs in one launch ("level" p), so a
-// level boundary is the only synchronisation a circuit needs. Energies are
-// then accumulated by tiled expectation passes (FP64, two-sum compensated per
-// block, folded in block order). Parametric gates carry a slot into a device
-// angle table, so new parameters are one upload and the graph is replayed.
+// device arena and run tile passes, pass p of every circuit in one launch
+// ("level" p), so a level boundary is the only synchronisation a circuit
+// needs: circuits of 12+ (complex64) / 11+ (complex128) qubits run the tiled
+// executor's one-wave register passes (k_batch_wave in tiled.hip), smaller
+// ones LDS-tiled passes (k_batch_pass). Energies are then accumulated by
+// tiled expectation passes (FP64, two-sum compensated per block, folded in
+// block order). Parametric gates carry a slot into a device angle table, so
+// new parameters are one upload and the graph is replayed.
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -58,6 +60,8 @@ __device__ __forceinline__ uint64_t deposit_pos(uint32_t e, const uint8_t *pos, 
 }
 
 constexpr int kBT = 256;
+// Largest LDS-resident circuit (qubits) of the complex64 / complex128 kernels.
+constexpr int kSmallK64 = 13, kSmallK128 = 12;
 // Op kind of two parametric, uncontrolled RY on different qubits (lp0, lp1)
 // in one sweep of the tile, each optionally followed by its folded phase
 // (slot2, slot4); the first RY's angle in pad, the second's in slot3.
@@ -264,49 +268,97 @@ __device__ __forceinline__ void tile_apply(typename V2<R>::T *tile, const TileOp
     }
 }
 
-// An energy term: coefficient with (-i)^{nY} folded into quarter turns.
+// An energy term: coefficient with (-i)^{nY} folded into quarter turns. A
+// circuit's terms are stored in groups of one x mask; within a group the
+// terms whose sign depends on the element slot (see tile_energy) come first,
+// and the group's first term holds the counts.
 struct ETerm {
     uint64_t x, z;
-    uint32_t q0, pad;
-    double c[2];  // the coefficient times i^q0
+    uint32_t q0;
+    uint32_t lx;         // the x mask over the tile's local bits
+    uint32_t zl;         // the z mask over the tile's local bits (z bits outside the tile act through its base)
+    uint16_t ng, nslot;  // first term of a group: terms in the group, and how many of them come first as slot terms
+    double c[2];         // the coefficient times i^q0
 };
 
-// sum over tile elements e of conj(psi_e) * c * i^{q(e)} * psi_{e ^ x}
-template <typename R, int KMAX>
+// Elements per thread and chunk of the energy sums over a 2^K-element tile,
+// and the local index bits that select the element slot within a thread.
+constexpr int energy_per(int K) { return (1 << K) / kBT < 8 ? (1 << K) / kBT : 8; }
+constexpr uint32_t energy_slot_bits(int per) { return uint32_t(per - 1) * kBT; }
+
+// Sum over the tile's elements e < nel of conj(psi_e) * c * i^{q(e)} * psi_{e ^ x}.
+// x lies inside the tile, so flipping its global bits flips the same local
+// bits (ETerm::lx). Thread tid takes elements e0 + tid + it * kBT (it < PER)
+// of each chunk of PER * kBT elements. The terms of one x mask share
+// w = conj(a) b: their signed coefficients sum first (g) and meet w once. The
+// sign (-1)^popcount(index & z) is the product of the thread's part (tile
+// base, chunk and thread index: folded into the coefficient once per term and
+// thread) and the element slot's part (uniform: a scalar +-1). Terms whose z
+// mask misses the slot bits have one sign for the thread's PER elements and
+// are summed once per thread (gc).
+template <typename R, int PER>
 __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const ETerm *__restrict__ terms, int nt,
-                                            uint64_t base, const uint8_t *pos, int nloc, DD &sr, DD &si) {
-    const uint32_t tid = threadIdx.x, nel = 1u << nloc;
-    for (uint32_t e = tid; e < nel; e += kBT) {
-        const uint64_t gi = base | deposit_pos<KMAX>(e, pos, nloc);
-        const double ar = tile[e].x, ai = tile[e].y;
-        double hr = 0.0, hi = 0.0;
-        double wr = 0.0, wi = 0.0, gr = 0.0, gi2 = 0.0;
-        uint32_t last = ~0u;
-        for (int t = 0; t < nt; t++) {
-            const ETerm tm = terms[t];
-            // local partner: x lies inside the tile, so flipping its global
-            // bits flips the same local bits (tm.pad holds the local x mask).
-            // Terms with one x mask are consecutive: they share w = conj(a) b,
-            // their signed coefficients sum first (g) and meet w once.
-            if (tm.pad != last) {
-                hr += wr * gr - wi * gi2;
-                hi += wr * gi2 + wi * gr;
-                gr = gi2 = 0.0;
-                last = tm.pad;
-                const uint32_t le = e ^ tm.pad;
-                const double br = tile[le].x, bi = tile[le].y;
-                wr = ar * br + ai * bi;
-                wi = ar * bi - ai * br;
+                                            uint64_t base, uint32_t nel, DD &sr, DD &si) {
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t e0 = 0; e0 < nel; e0 += PER * kBT) {
+        double hr[PER], hi[PER];
+#pragma unroll
+        for (int it = 0; it < PER; it++) hr[it] = hi[it] = 0.0;
+        for (int t = 0; t < nt;) {
+            const ETerm head = terms[t];
+            const uint32_t lx = head.lx;
+            const int ng = head.ng, ns = head.nslot;
+            double wr[PER], wi[PER], gr[PER], gi[PER];
+#pragma unroll
+            for (int it = 0; it < PER; it++) {
+                wr[it] = wi[it] = gr[it] = gi[it] = 0.0;
+                const uint32_t e = e0 + tid + uint32_t(it) * kBT;
+                if (e < nel) {
+                    const double ar = tile[e].x, ai = tile[e].y;
+                    const double br = tile[e ^ lx].x, bi = tile[e ^ lx].y;
+                    wr[it] = fma(ar, br, ai * bi);
+                    wi[it] = fma(ar, bi, -(ai * br));
+                }
             }
-            // c i^q with q = q0 + 2 popcount(index & z): the stored c i^q0 times +-1
-            const double sg = (__popcll(gi & tm.z) & 1) ? -1.0 : 1.0;
-            gr = fma(sg, tm.c[0], gr);
-            gi2 = fma(sg, tm.c[1], gi2);
+            // the thread's signed coefficient: c i^q0 with the sign bit of the
+            // base, chunk and thread parts of (-1)^popcount(index & z)
+            const auto thread_coeff = [&](const ETerm &tm, double &cr, double &ci) {
+                const uint32_t p = uint32_t(__popcll(base & tm.z) + __popc((e0 | tid) & tm.zl)) << 31;
+                cr = __hiloint2double(__double2hiint(tm.c[0]) ^ (int)p, __double2loint(tm.c[0]));
+                ci = __hiloint2double(__double2hiint(tm.c[1]) ^ (int)p, __double2loint(tm.c[1]));
+            };
+            for (int k = 0; k < ns; k++) {
+                const ETerm tm = terms[t + k];
+                double cr, ci;
+                thread_coeff(tm, cr, ci);
+#pragma unroll
+                for (int it = 0; it < PER; it++) {
+                    const double su = (__popc((uint32_t(it) * kBT) & tm.zl) & 1) ? -1.0 : 1.0;
+                    gr[it] = fma(su, cr, gr[it]);
+                    gi[it] = fma(su, ci, gi[it]);
+                }
+            }
+            double gcr = 0.0, gci = 0.0;
+            for (int k = ns; k < ng; k++) {
+                double cr, ci;
+                thread_coeff(terms[t + k], cr, ci);
+                gcr += cr;
+                gci += ci;
+            }
+#pragma unroll
+            for (int it = 0; it < PER; it++) {
+                const double ur = gr[it] + gcr, ui = gi[it] + gci;
+                hr[it] = fma(wr[it], ur, fma(-wi[it], ui, hr[it]));
+                hi[it] = fma(wr[it], ui, fma(wi[it], ur, hi[it]));
+            }
+            t += ng;
         }
-        hr += wr * gr - wi * gi2;
-        hi += wr * gi2 + wi * gr;
-        dd_acc(sr, hr);
-        dd_acc(si, hi);
+#pragma unroll
+        for (int it = 0; it < PER; it++)
+            if (e0 + tid + uint32_t(it) * kBT < nel) {
+                dd_acc(sr, hr[it]);
+                dd_acc(si, hi[it]);
+            }
     }
 }
 
@@ -325,10 +377,8 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
     using T = typename V2<R>::T;
     __shared__ T tile[1 << KMAX];
     __shared__ double sh[4 * 4];
-    __shared__ uint8_t pos[KMAX];
     const SmallTask tk = tasks[blockIdx.x];
     const uint32_t nel = 1u << tk.n;
-    if (threadIdx.x < KMAX) pos[threadIdx.x] = (uint8_t)threadIdx.x;
     for (uint32_t e = threadIdx.x; e < nel; e += kBT) { T z; z.x = e == 0 ? R(1) : R(0); z.y = R(0); tile[e] = z; }
     __syncthreads();
     for (uint32_t o = 0; o < tk.nops; o++) {
@@ -339,7 +389,7 @@ __global__ __launch_bounds__(kBT) void k_batch_small(const SmallTask *__restrict
         __syncthreads();
     }
     DD v[2] = {dd_zero(), dd_zero()};
-    tile_energy<R, KMAX>(tile, terms + tk.t0, (int)tk.nt, 0, pos, (int)tk.n, v[0], v[1]);
+    tile_energy<R, energy_per(KMAX)>(tile, terms + tk.t0, (int)tk.nt, 0, nel, v[0], v[1]);
     block_reduce_dd<2>(v, sh);
     if (threadIdx.x == 0) {
         energies[2 * tk.out] = v[0].s + v[0].c;
@@ -426,7 +476,7 @@ __global__ __launch_bounds__(kBT) void k_batch_expect(const typename V2<R>::T *_
         tile[threadIdx.x + it * kBT] = psi[base | off_t | deposit_pos<K>(uint32_t(it) << 8, tk.pos, K)];
     __syncthreads();
     DD v[2] = {dd_zero(), dd_zero()};
-    tile_energy<R, K>(tile, terms + tk.t0, (int)tk.nt, base, tk.pos, K, v[0], v[1]);
+    tile_energy<R, energy_per(K)>(tile, terms + tk.t0, (int)tk.nt, base, uint32_t(N), v[0], v[1]);
     block_reduce_dd<2>(v, sh);
     if (threadIdx.x == 0) {
         part[(size_t(tk.part0) + local) * 4 + 0] = v[0].s; part[(size_t(tk.part0) + local) * 4 + 1] = v[0].c;
@@ -629,6 +679,44 @@ static size_t fold_rz(TileOp *ops, size_t n) {
     return w;
 }
 
+// Appends terms idx of circuit C to the term table for a tile whose local bit
+// j is qubit pos[j] (K bits) and an energy kernel with `per` elements per
+// thread: groups of one x mask, each with its slot terms (z on the
+// element-slot bits) first.
+static void append_terms(Batch &B, Batch::Circ &C, const std::vector<size_t> &idx, const uint8_t *pos, int K, int per) {
+    const uint32_t slot_bits = energy_slot_bits(per);
+    std::vector<ETerm> ts(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        const qi_pauli_term &p = C.terms[idx[i]];
+        ETerm &t = ts[i];
+        t = make_eterm(p);
+        for (int j = 0; j < K; j++) {
+            t.lx |= uint32_t((p.x_mask >> pos[j]) & 1) << j;
+            t.zl |= uint32_t((p.z_mask >> pos[j]) & 1) << j;
+        }
+    }
+    std::vector<size_t> order(idx.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        const bool sa = (ts[a].zl & slot_bits) != 0, sb = (ts[b].zl & slot_bits) != 0;
+        return ts[a].lx != ts[b].lx ? ts[a].lx < ts[b].lx : sa > sb;
+    });
+    for (size_t g = 0; g < order.size();) {
+        size_t e = g, ns = 0;
+        while (e < order.size() && ts[order[e]].lx == ts[order[g]].lx) {
+            if (ts[order[e]].zl & slot_bits) ns++;
+            e++;
+        }
+        ts[order[g]].ng = (uint16_t)(e - g);
+        ts[order[g]].nslot = (uint16_t)ns;
+        g = e;
+    }
+    for (size_t i : order) {
+        C.eterm_idx[idx[i]] = (uint32_t)B.eterms.size();
+        B.eterms.push_back(ts[i]);
+    }
+}
+
 template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &v) {
     Status e = ensure(b, std::max<size_t>(v.size() * sizeof(T), 16));
     if (!e.ok()) return e;
@@ -766,17 +854,11 @@ Status batch_plan(Batch &B) {
             SmallTask st{};
             st.n = P.n; st.op0 = P.op0; st.nops = P.nops; st.out = (uint32_t)ci; st.slot_base = C.slot_base;
             st.t0 = (uint32_t)B.eterms.size(); st.nt = (uint32_t)C.terms.size();
-            std::vector<size_t> order(C.terms.size());
-            for (size_t k = 0; k < order.size(); k++) order[k] = k;
-            // terms sharing an x mask next to each other: they share conj(a) b
-            std::stable_sort(order.begin(), order.end(),
-                             [&](size_t a, size_t b) { return C.terms[a].x_mask < C.terms[b].x_mask; });
-            for (size_t k : order) {
-                ETerm t = make_eterm(C.terms[k]);
-                t.pad = (uint32_t)C.terms[k].x_mask;  // local = global inside a whole-state tile
-                C.eterm_idx[k] = (uint32_t)B.eterms.size();
-                B.eterms.push_back(t);
-            }
+            std::vector<size_t> all(C.terms.size());
+            for (size_t k = 0; k < all.size(); k++) all[k] = k;
+            uint8_t ident[64];  // a whole-state tile: local bit = qubit
+            for (int j = 0; j < 64; j++) ident[j] = (uint8_t)j;
+            append_terms(B, C, all, ident, (int)P.n, energy_per(c128 ? kSmallK128 : kSmallK64));
             small.push_back(st);
             continue;
         }
@@ -786,7 +868,7 @@ Status batch_plan(Batch &B) {
         if (on_waves(P)) {
             for (size_t p = 0; p < P.wpasses.size(); p++) {
                 WaveTask wt{};
-                wt.off = off; wt.pass = P.wpass_base + (uint32_t)p; wt.slot_base = C.slot_base; wt.init = p == 0 ? 1 : 0;
+                wt.off = off; wt.pass = P.wpass_base + (uint32_t)p; wt.slot_base = C.slot_base;
                 wt.block0 = uint32_t(1) << (P.n - kw);  // tile count; turned into offsets per level below
                 if (per_wlevel.size() <= p) per_wlevel.resize(p + 1);
                 per_wlevel[p].push_back(wt);
@@ -831,22 +913,10 @@ Status batch_plan(Batch &B) {
             BigTask bt{};
             bt.off = off; bt.qmask = q;
             int j = 0;
-            uint32_t local[64];
             for (int b = 0; b < (int)P.n; b++)
-                if ((q >> b) & 1) { local[b] = j; bt.pos[j++] = (uint8_t)b; }
+                if ((q >> b) & 1) bt.pos[j++] = (uint8_t)b;
             bt.t0 = (uint32_t)B.eterms.size(); bt.nt = (uint32_t)take.size();
-            // terms sharing an x mask next to each other: they share conj(a) b
-            std::stable_sort(take.begin(), take.end(),
-                             [&](size_t a, size_t b) { return C.terms[a].x_mask < C.terms[b].x_mask; });
-            for (size_t k : take) {
-                ETerm t = make_eterm(C.terms[k]);
-                uint32_t lx = 0;
-                for (int b = 0; b < (int)P.n; b++)
-                    if ((C.terms[k].x_mask >> b) & 1) lx |= 1u << local[b];
-                t.pad = lx;
-                C.eterm_idx[k] = (uint32_t)B.eterms.size();
-                B.eterms.push_back(t);
-            }
+            append_terms(B, C, take, bt.pos, B.K, energy_per(B.K));
             bt.part0 = nparts;
             nparts += tiles;
             cparts += tiles;
@@ -927,7 +997,7 @@ template <typename R, int K> static Status launch_levels(Batch &B, hipStream_t s
         }
         if (l < B.wlevels.size()) {
             auto r = B.wlevels[l];
-            Status e = launch_batch_wave(B.prec, B.wlevel_blocks[l], st, B.d_arena.p, wt + r.first,
+            Status e = launch_batch_wave(B.prec, l == 0, B.wlevel_blocks[l], st, B.d_arena.p, wt + r.first,
                                          (int)(r.second - r.first), (const RegPass *)B.d_wpasses.p,
                                          (const RegOp *)B.d_wops.p, B.d_ptab.p);
             if (!e.ok()) return e;
@@ -964,11 +1034,11 @@ static Status batch_launch(Batch &B, hipStream_t st) {
     QI_HIP_TRY(hipStreamWaitEvent(B.side, B.fork, 0));
     if (B.nsmall) {
         if (B.prec == QI_C128)
-            hipLaunchKernelGGL((k_batch_small<double, 12>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
+            hipLaunchKernelGGL((k_batch_small<double, kSmallK128>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
                                (const double *)B.d_coef.p, (double *)B.d_energy.p);
         else
-            hipLaunchKernelGGL((k_batch_small<float, 13>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
+            hipLaunchKernelGGL((k_batch_small<float, kSmallK64>), dim3((unsigned)B.nsmall), dim3(kBT), 0, B.side,
                                (const SmallTask *)B.d_small.p, (const TileOp *)B.d_ops.p, (const ETerm *)B.d_terms.p,
                                (const double *)B.d_coef.p, (double *)B.d_energy.p);
     }

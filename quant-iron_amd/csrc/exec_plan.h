@@ -180,10 +180,12 @@ struct WaveTask {
     uint32_t block0;    // first workgroup of this task in the launch
     uint32_t pass;      // index of its RegPass in the pass table
     int32_t slot_base;  // first angle slot of the circuit (RegOp::param counts from it)
-    uint32_t init;      // 1: the state is |0..0> (the pass loads nothing)
+    uint32_t pad;
 };
-Status launch_batch_wave(int prec, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks, int ntasks,
-                         const RegPass *passes, const RegOp *ops, const void *ptab);
+// init: the launch holds the first pass of each of its circuits (the state is
+// |0..0>, the pass loads nothing).
+Status launch_batch_wave(int prec, bool init, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks,
+                         int ntasks, const RegPass *passes, const RegOp *ops, const void *ptab);
 
 Status plan_fused(const State &s, const std::vector<Lowered> &gates, int kb, ExecPlan &plan);
 Status launch_fused_block(State &s, const FusedBlock &b, const void *dmat, hipStream_t st);

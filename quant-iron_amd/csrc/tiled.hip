@@ -1073,9 +1073,10 @@ __device__ __forceinline__ int find_wave_task(const WaveTask *__restrict__ tasks
 // Pass p of many circuits in one launch (the batched energies): workgroup b
 // is one wave holding one tile of the circuit whose task covers b (tasks in
 // block order); the task names the circuit's state in the arena, its pass
-// and its first angle slot. The tile work is k_wave_pass's; a circuit's first
-// pass builds |0..0> in registers instead of loading.
-template <typename R, int K>
+// and its first angle slot. The tile work is k_wave_pass's; the first pass of
+// every circuit (INIT, the first launch) builds |0..0> in registers instead of
+// loading.
+template <typename R, int K, bool INIT>
 __global__ __launch_bounds__(64) void k_batch_wave(typename V2<R>::T *__restrict__ arena, const WaveTask *__restrict__ tasks,
                                                    int ntasks, const RegPass *__restrict__ passes,
                                                    const RegOp *__restrict__ ops, const R *__restrict__ ptab) {
@@ -1095,7 +1096,7 @@ __global__ __launch_bounds__(64) void k_batch_wave(typename V2<R>::T *__restrict
         tmask |= uint64_t(1) << pass.qt_out[i];
     }
     Regs<R, E> v;
-    if (tk.init) {  // |0..0>: register 0 of the lane whose address is 0
+    if constexpr (INIT) {  // |0..0>: register 0 of the lane whose address is 0
         const bool one = (base | at_in) == 0;
 #pragma unroll
         for (int r = 0; r < E; r++) v.set(r, mk<T>(r == 0 && one ? R(1) : R(0), R(0)));
@@ -1192,15 +1193,25 @@ Status launch_reg_pass(State &s, const RegPass &p, const RegOp *dops, hipStream_
     return Status{};
 }
 
-Status launch_batch_wave(int prec, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks, int ntasks,
-                         const RegPass *passes, const RegOp *ops, const void *ptab) {
+Status launch_batch_wave(int prec, bool init, uint32_t blocks, hipStream_t st, void *arena, const WaveTask *tasks,
+                         int ntasks, const RegPass *passes, const RegOp *ops, const void *ptab) {
     if (!blocks || ntasks <= 0) return Status{};
-    if (prec == QI_C128)
-        hipLaunchKernelGGL((k_batch_wave<double, kBatchWaveK128>), dim3(blocks), dim3(64), 0, st, (double2 *)arena, tasks,
-                           ntasks, passes, ops, (const double *)ptab);
-    else
-        hipLaunchKernelGGL((k_batch_wave<float, kBatchWaveK64>), dim3(blocks), dim3(64), 0, st, (float2 *)arena, tasks,
-                           ntasks, passes, ops, (const float *)ptab);
+    const dim3 g(blocks), b(64);
+    if (prec == QI_C128) {
+        if (init)
+            hipLaunchKernelGGL((k_batch_wave<double, kBatchWaveK128, true>), g, b, 0, st, (double2 *)arena, tasks, ntasks,
+                               passes, ops, (const double *)ptab);
+        else
+            hipLaunchKernelGGL((k_batch_wave<double, kBatchWaveK128, false>), g, b, 0, st, (double2 *)arena, tasks, ntasks,
+                               passes, ops, (const double *)ptab);
+    } else {
+        if (init)
+            hipLaunchKernelGGL((k_batch_wave<float, kBatchWaveK64, true>), g, b, 0, st, (float2 *)arena, tasks, ntasks,
+                               passes, ops, (const float *)ptab);
+        else
+            hipLaunchKernelGGL((k_batch_wave<float, kBatchWaveK64, false>), g, b, 0, st, (float2 *)arena, tasks, ntasks,
+                               passes, ops, (const float *)ptab);
+    }
     QI_HIP_TRY(hipGetLastError());
     return Status{};
 }
