@@ -236,7 +236,14 @@ def bench_energies(qiron, L, W, args):
     for prec in ("c64", "c128"):
         t0 = time.perf_counter()
         b = S.ansatz_batch(specs, prec)
-        k, ks, tuned_ms = b.autotune(trials=2)  # plans, captures and times every schedule
+        fixed = getattr(args, f"schedule_{prec}")
+        if fixed:  # a given schedule instead of the autotuner (profiles of the timed runs alone)
+            k, ks = (int(x) for x in fixed.split(","))
+            b.set_schedule(k, ks)
+            b.run(use_graph=True)
+            tuned_ms = b.info()["last_ms"]
+        else:
+            k, ks, tuned_ms = b.autotune(trials=2)  # plans, captures and times every schedule
         setup = time.perf_counter() - t0
         walls = []
         for _ in range(args.steps):
@@ -463,15 +470,18 @@ def main():
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
     ap.add_argument("--only", default="", choices=["", "tiled", "tiled128", "gate", "copy", "surrogate", "ceiling",
-                                                   "energies"],
+                                                   "energies", "trotter"],
                     help="profiling runs: only the headline executor (complex64, or complex128 with tiled128), only "
                          "the single-gate kernels, only a plain copy of known size (the reference for HBM byte "
-                         "counters), only the surrogate, only the 128 GiB memory-ceiling runs, or only the batched energies")
+                         "counters), only the surrogate, only the 128 GiB memory-ceiling runs, only the batched "
+                         "energies, or only the Trotter evolution")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-qubits", type=int, default=0, help="tile width of the headline executor (0: default)")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--plain-circuits", type=int, default=64, help="circuits timed on the plain FP64 path")
+    ap.add_argument("--schedule-c64", default="", help="batched energies: 'k,ks' instead of autotuning (complex64)")
+    ap.add_argument("--schedule-c128", default="", help="the same for complex128")
     ap.add_argument("--train-samples", type=int, default=1 << 20)
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--local-rounds", type=int, default=5, help="local optimisation rounds after the uniform round")
@@ -513,6 +523,9 @@ def main():
         return
     if args.only == "ceiling":
         print(json.dumps(bench_ceiling(qiron, L, W)), flush=True)
+        return
+    if args.only == "trotter":
+        print(json.dumps(bench_trotter(qiron, W, args)), flush=True)
         return
 
     tiled = dict(mode=L.EXEC_TILED, use_graph=True, tile_qubits=args.tile_qubits)

@@ -828,46 +828,53 @@ qi_status qi_sumop_expectation(qi_state_t s, const qi_pauli_term *terms, size_t 
 }
 
 // exp(-i theta P) for a Pauli string P as gates, so the tiled executor fuses
-// a whole Trotter step into a few HBM passes: every factor turned to Z, a CNOT
-// ladder gathering the parity on the last factor's qubit, RZ(2 theta) there,
-// then the ladder and the basis change undone. The basis changes are H and
-// RX(pi/2) scaled by sqrt 2 ([[1, 1], [1, -1]], [[1, -i], [-i, 1]]) and their
-// inverses scaled by 1/2: their products are exact in binary floating point,
-// so no norm drifts from rounded 1/sqrt 2 coefficients over many steps. The
-// same unitary as the reference's cos/sin form (pauli_string.rs:237-262, real
-// coefficient), up to rounding.
+// a whole Trotter step into a few HBM passes. With X or Y factors, P is
+// conjugated to X on one pivot qubit p (the highest X/Y factor):
+// V = S^dag on each Y factor (S^dag Y S = X), CNOT(p -> q) on each other X/Y
+// factor q (X_p X_q -> X_p) and CZ(p, q) on each Z factor q (X_p Z_q -> X_p),
+// so exp(-i theta P) = V^dag RX(2 theta)_p V. A Z string takes a CNOT ladder
+// gathering its parity on its last qubit, RZ(2 theta) there and the ladder
+// undone. Every gate but the one rotation is a permutation or a diagonal of
+// units (exact in floating point), so a step adds no norm drift beyond the
+// rotations' rounding. The same unitary as the reference's cos/sin form
+// (pauli_string.rs:237-262, real coefficient), up to rounding.
 static void pauli_exp_gates(const qi_pauli_term &t, double theta, std::vector<qi_gate> &out) {
     const uint64_t sup = t.x_mask | t.z_mask;
     int qs[64], m = 0;
     for (uint64_t b = sup; b; b &= b - 1) qs[m++] = __builtin_ctzll(b);
-    auto u2 = [&](int q, const double (&mat)[8]) {
+    auto one = [&](int kind, int q, double angle) {
         qi_gate g{};
-        g.kind = QI_GATE_U2; g.ntargets = 1; g.targets[0] = (uint64_t)q;
-        memcpy(g.matrix, mat, sizeof mat);
+        g.kind = kind; g.ntargets = 1; g.targets[0] = (uint64_t)q; g.params[0] = angle;
         out.push_back(g);
     };
-    auto cnot = [&](int c, int q) {
+    auto ctl = [&](int kind, int c, int q) {
         qi_gate g{};
-        g.kind = QI_GATE_CNOT; g.ntargets = 1; g.targets[0] = (uint64_t)q; g.ncontrols = 1; g.controls[0] = (uint64_t)c;
+        g.kind = kind; g.ntargets = 1; g.targets[0] = (uint64_t)q; g.ncontrols = 1; g.controls[0] = (uint64_t)c;
         out.push_back(g);
     };
-    // row-major (re, im) pairs
-    static const double x_in[8] = {1, 0, 1, 0, 1, 0, -1, 0}, x_out[8] = {0.5, 0, 0.5, 0, 0.5, 0, -0.5, 0};
-    static const double y_in[8] = {1, 0, 0, -1, 0, -1, 1, 0}, y_out[8] = {0.5, 0, 0, 0.5, 0, 0.5, 0.5, 0};
-    auto basis = [&](bool undo) {
+    if (t.x_mask == 0) {
+        for (int k = 0; k + 1 < m; k++) ctl(QI_GATE_CNOT, qs[k], qs[m - 1]);
+        one(QI_GATE_RZ, qs[m - 1], 2.0 * theta);
+        for (int k = m - 2; k >= 0; k--) ctl(QI_GATE_CNOT, qs[k], qs[m - 1]);
+        return;
+    }
+    const int p = 63 - __builtin_clzll(t.x_mask);
+    auto y_to_x = [&](bool undo) {  // S^dag on each Y factor, S to undo
+        for (int k = 0; k < m; k++)
+            if ((t.x_mask >> qs[k]) & (t.z_mask >> qs[k]) & 1) one(undo ? QI_GATE_S : QI_GATE_SDAG, qs[k], 0.0);
+    };
+    auto gather = [&]() {  // self-inverse, mutually commuting (one control p, distinct targets)
         for (int k = 0; k < m; k++) {
-            const bool x = (t.x_mask >> qs[k]) & 1, z = (t.z_mask >> qs[k]) & 1;
-            if (x && !z) u2(qs[k], undo ? x_out : x_in);
-            else if (x && z) u2(qs[k], undo ? y_out : y_in);
+            const int q = qs[k];
+            if (q == p) continue;
+            ctl((t.x_mask >> q) & 1 ? QI_GATE_CNOT : QI_GATE_Z, p, q);  // CNOT, or CZ on a Z factor
         }
     };
-    basis(false);
-    for (int k = 0; k + 1 < m; k++) cnot(qs[k], qs[m - 1]);
-    qi_gate rz{};
-    rz.kind = QI_GATE_RZ; rz.ntargets = 1; rz.targets[0] = (uint64_t)qs[m - 1]; rz.params[0] = 2.0 * theta;
-    out.push_back(rz);
-    for (int k = m - 2; k >= 0; k--) cnot(qs[k], qs[m - 1]);
-    basis(true);
+    y_to_x(false);
+    gather();
+    one(QI_GATE_RX, p, 2.0 * theta);
+    gather();
+    y_to_x(true);
 }
 
 // States from this size on run Trotter steps through the tiled executor.

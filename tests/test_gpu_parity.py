@@ -491,8 +491,10 @@ def test_trotter_lattice_matches_oracle(ctx, order):
 @pytest.mark.parametrize("shape,order", [((4, 4), 1), ((4, 4), 2), ((4, 5), 1)])
 def test_trotter_fused_steps_match_oracle(ctx, shape, order):
     """From 16 qubits a Trotter step runs as one gate list through the tiled
-    executor (basis change, CNOT ladder, RZ per term): the same unitary as the
-    per-term cos/sin form up to rounding."""
+    executor (per term: Y factors turned to X, CNOTs and CZs gathering the
+    string onto one X, RX there and the gathering undone; Z strings a CNOT
+    ladder and RZ): the same unitary as the per-term cos/sin form up to
+    rounding."""
     terms, xs = W.trotter_2d(*shape, seed=43)
     n = shape[0] * shape[1]
     psi = O.init(n, 0)
@@ -510,6 +512,25 @@ def test_trotter_fused_steps_match_oracle(ctx, shape, order):
     d = float(np.max(np.abs(got - ref)))
     f = fidelity(got, ref)
     assert d <= 1e-4 and f >= 1 - 1e-7, (shape, order, d, 1 - f)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_trotter_fused_mixed_pauli_strings_match_oracle(ctx, order):
+    """Strings mixing X, Y and Z factors (weights 1..5, a pivot at any position
+    among them) on 17 qubits, tiled executor against the oracle's per-term
+    cos/sin product: a wrong conjugation (a CZ for an X factor, a missing
+    S / S-dagger pair, the wrong pivot) moves amplitudes by O(theta)."""
+    rng = np.random.default_rng(500 + order)
+    n = 17
+    terms = random_terms(n, 24, rng, max_weight=5)
+    terms["c_im"] = 0.0  # the fused path takes Hermitian terms
+    terms = terms[(terms["x_mask"] | terms["z_mask"]) != 0]
+    assert np.any(terms["x_mask"] & terms["z_mask"]) and np.any(terms["x_mask"] & ~terms["z_mask"])
+    psi = random_state(n, rng)
+    ref = O.trotter(psi, terms, 0.05, 3, order)
+    s = upload(psi, "c128")
+    got = qiron.trotter_evolve_state(_sumop_from_arr(terms), s, 0.05, 3, order).state_vector
+    check(got, ref, "c128", f"trotter mixed strings order {order}")
 
 
 def _sumop_from_arr(arr):
