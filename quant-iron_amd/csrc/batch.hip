@@ -304,29 +304,34 @@ __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const
         double hr[PER], hi[PER];
 #pragma unroll
         for (int it = 0; it < PER; it++) hr[it] = hi[it] = 0.0;
+        double ar[PER], ai[PER];  // the thread's elements, read once per chunk
+#pragma unroll
+        for (int it = 0; it < PER; it++) {
+            const uint32_t e = e0 + tid + uint32_t(it) * kBT;
+            ar[it] = e < nel ? double(tile[e].x) : 0.0;
+            ai[it] = e < nel ? double(tile[e].y) : 0.0;
+        }
+        // the thread's signed coefficient: c i^q0 with the sign bit of the
+        // base, chunk and thread parts of (-1)^popcount(index & z)
+        const auto thread_coeff = [&](const ETerm &tm, double &cr, double &ci) {
+            const uint32_t p = uint32_t(__popcll(base & tm.z) + __popc((e0 | tid) & tm.zl)) << 31;
+            cr = __hiloint2double(__double2hiint(tm.c[0]) ^ (int)p, __double2loint(tm.c[0]));
+            ci = __hiloint2double(__double2hiint(tm.c[1]) ^ (int)p, __double2loint(tm.c[1]));
+        };
         for (int t = 0; t < nt;) {
             const ETerm head = terms[t];
             const uint32_t lx = head.lx;
             const int ng = head.ng, ns = head.nslot;
-            double wr[PER], wi[PER], gr[PER], gi[PER];
-#pragma unroll
-            for (int it = 0; it < PER; it++) {
-                wr[it] = wi[it] = gr[it] = gi[it] = 0.0;
-                const uint32_t e = e0 + tid + uint32_t(it) * kBT;
-                if (e < nel) {
-                    const double ar = tile[e].x, ai = tile[e].y;
-                    const double br = tile[e ^ lx].x, bi = tile[e ^ lx].y;
-                    wr[it] = fma(ar, br, ai * bi);
-                    wi[it] = fma(ar, bi, -(ai * br));
-                }
+            double gcr = 0.0, gci = 0.0;  // terms with one sign for all of the thread's elements
+            for (int k = ns; k < ng; k++) {
+                double cr, ci;
+                thread_coeff(terms[t + k], cr, ci);
+                gcr += cr;
+                gci += ci;
             }
-            // the thread's signed coefficient: c i^q0 with the sign bit of the
-            // base, chunk and thread parts of (-1)^popcount(index & z)
-            const auto thread_coeff = [&](const ETerm &tm, double &cr, double &ci) {
-                const uint32_t p = uint32_t(__popcll(base & tm.z) + __popc((e0 | tid) & tm.zl)) << 31;
-                cr = __hiloint2double(__double2hiint(tm.c[0]) ^ (int)p, __double2loint(tm.c[0]));
-                ci = __hiloint2double(__double2hiint(tm.c[1]) ^ (int)p, __double2loint(tm.c[1]));
-            };
+            double gr[PER], gi[PER];
+#pragma unroll
+            for (int it = 0; it < PER; it++) { gr[it] = gcr; gi[it] = gci; }
             for (int k = 0; k < ns; k++) {
                 const ETerm tm = terms[t + k];
                 double cr, ci;
@@ -338,18 +343,16 @@ __device__ __forceinline__ void tile_energy(const typename V2<R>::T *tile, const
                     gi[it] = fma(su, ci, gi[it]);
                 }
             }
-            double gcr = 0.0, gci = 0.0;
-            for (int k = ns; k < ng; k++) {
-                double cr, ci;
-                thread_coeff(terms[t + k], cr, ci);
-                gcr += cr;
-                gci += ci;
-            }
+            // the group meets w = conj(a) b once per element
 #pragma unroll
             for (int it = 0; it < PER; it++) {
-                const double ur = gr[it] + gcr, ui = gi[it] + gci;
-                hr[it] = fma(wr[it], ur, fma(-wi[it], ui, hr[it]));
-                hi[it] = fma(wr[it], ui, fma(wi[it], ur, hi[it]));
+                const uint32_t e = e0 + tid + uint32_t(it) * kBT;
+                if (e < nel) {
+                    const double br = tile[e ^ lx].x, bi = tile[e ^ lx].y;
+                    const double wr = fma(ar[it], br, ai[it] * bi), wi = fma(ar[it], bi, -(ai[it] * br));
+                    hr[it] = fma(wr, gr[it], fma(-wi, gi[it], hr[it]));
+                    hi[it] = fma(wr, gi[it], fma(wi, gr[it], hi[it]));
+                }
             }
             t += ng;
         }

@@ -157,6 +157,12 @@ def test_one_wave_passes_on_general_circuits_match_the_oracle(ctx, prec):
             g["params"][rot, 0] = angle_rng.uniform(-2 * np.pi, 2 * np.pi, size=rot.size)
             angles.extend(g["params"][rot, 0].tolist())
             out.append((n, g, slots))
+        # edge shapes: no gates at all, and uncontrolled SWAPs only (a plan of
+        # permutation passes alone)
+        out.append((13, gates_array([]), np.zeros(0, np.int32)))
+        sw = gates_array([{"kind": "SWAP", "targets": [int(a), int(b)], "controls": []}
+                          for a, b in np.random.default_rng(7).integers(0, 12, size=(40, 2)) if a != b])
+        out.append((12, sw, np.full(sw.size, -1, np.int32)))
         return out, np.array(angles)
 
     circs, angles = circuits(rng)
