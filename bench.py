@@ -254,9 +254,10 @@ def bench_energies(qiron, L, W, args):
         wall = min(walls)
         res[f"batch_{prec}"] = dict(energies_per_s=count / wall, wall_ms=wall * 1e3, device_ms=info["last_ms"],
                                     setup_s=setup, levels=info["levels"], lds_resident=info["lds_resident"],
-                                    schedule={"tile_qubits": k, "lds_resident_max_qubits": ks, "tuned_ms": tuned_ms})
+                                    schedule={"executor": b.executor(), "tile_qubits": k, "lds_resident_max_qubits": ks,
+                                              "tuned_ms": tuned_ms})
         log(f"batched {prec}: {count} energies in {wall * 1e3:.1f} ms ({count / wall:.0f}/s), device "
-            f"{info['last_ms']:.1f} ms, autotuned schedule k={k} ks={ks}, setup {setup:.1f} s")
+            f"{info['last_ms']:.1f} ms, schedule {b.executor()} k={k} ks={ks}, setup {setup:.1f} s")
         E[prec] = e.real
         del b
     dE = float(np.max(np.abs(E["c64"] - E["c128"])))

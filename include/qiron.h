@@ -394,9 +394,10 @@ qi_status qi_batch_get_schedule(qi_batch_t b, uint32_t *k, uint32_t *ks, qi_erro
 typedef enum qi_batch_exec { QI_BATCH_LDS = 0, QI_BATCH_WAVE = 1 } qi_batch_exec;
 qi_status qi_batch_set_executor(qi_batch_t b, qi_batch_exec exec, qi_error *err);
 qi_status qi_batch_get_executor(qi_batch_t b, qi_batch_exec *exec, qi_error *err);
-/* Autotuner: plans, captures and times every schedule on this batch (best of
- * `trials` replays after a warm-up, HIP-event device time), keeps the fastest
- * and leaves the batch planned with it. No reference counterpart. */
+/* Autotuner: plans, captures and times every schedule on this batch (both
+ * executors, every k and ks; best of `trials` replays after a warm-up, HIP-event
+ * device time), keeps the fastest and leaves the batch planned with it (the
+ * executor it picked: qi_batch_get_executor). No reference counterpart. */
 qi_status qi_batch_autotune(qi_batch_t b, uint32_t trials, uint32_t *k, uint32_t *ks, double *best_ms,
                             qi_error *err);
 

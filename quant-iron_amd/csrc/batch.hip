@@ -1297,7 +1297,7 @@ qi_status qi_batch_get_executor(qi_batch_t b, qi_batch_exec *exec, qi_error *err
     QB_RET(Status{});
 }
 
-// Autotuner: plans, captures and times every schedule (K, ks) on this batch
+// Autotuner: plans, captures and times every schedule (executor, K, ks) on this batch
 // (best of `trials` graph replays after one warm-up run each, device time
 // from HIP events) and keeps the fastest; schedules the batch cannot use
 // (a Pauli term wider than the tile) are skipped. The batch is left planned
@@ -1317,23 +1317,26 @@ qi_status qi_batch_autotune(qi_batch_t b, uint32_t trials, uint32_t *k_out, uint
     const int kmax = B.prec == QI_C128 ? 12 : 13;
     std::vector<double> en(2 * B.circs.size());
     double best = 1e300;
-    int bk = 0, bks = 0;
-    for (int k = 10; k <= kmax; k++)
-        for (int ks = k; ks <= kmax; ks++) {
-            B.K = k;
-            B.KS = ks;
-            B.planned = false;
-            Status e = batch_run(B, en.data(), true);  // plan, capture, warm up
-            if (e.code == QI_ERR_INVALID_ARGUMENT) continue;
-            if (!e.ok()) QB_RET(e);
-            double t = 1e300;
-            for (uint32_t r = 0; r < std::max<uint32_t>(trials, 1); r++) {
-                if (!(e = batch_run(B, en.data(), true)).ok()) QB_RET(e);
-                t = std::min(t, B.last_ms);
+    int bk = 0, bks = 0, bex = QI_BATCH_WAVE;
+    for (int ex : {QI_BATCH_WAVE, QI_BATCH_LDS})
+        for (int k = 10; k <= kmax; k++)
+            for (int ks = k; ks <= kmax; ks++) {
+                B.exec = ex;
+                B.K = k;
+                B.KS = ks;
+                B.planned = false;
+                Status e = batch_run(B, en.data(), true);  // plan, capture, warm up
+                if (e.code == QI_ERR_INVALID_ARGUMENT) continue;
+                if (!e.ok()) QB_RET(e);
+                double t = 1e300;
+                for (uint32_t r = 0; r < std::max<uint32_t>(trials, 1); r++) {
+                    if (!(e = batch_run(B, en.data(), true)).ok()) QB_RET(e);
+                    t = std::min(t, B.last_ms);
+                }
+                if (t < best) { best = t; bk = k; bks = ks; bex = ex; }
             }
-            if (t < best) { best = t; bk = k; bks = ks; }
-        }
     if (!bk) QB_RET(make_err(QI_ERR_INVALID_ARGUMENT, 0, 0, 0, 0, "no schedule fits the batch"));
+    B.exec = bex;
     B.K = bk;
     B.KS = bks;
     B.planned = false;
