@@ -877,6 +877,67 @@ static void pauli_exp_gates(const qi_pauli_term &t, double theta, std::vector<qi
     y_to_x(true);
 }
 
+// A term X_pX_q, Y_pY_q or Z_pZ_q: 1, 2 or 3 (0: anything else).
+static int bond_kind(const qi_pauli_term &t) {
+    if (__builtin_popcountll(t.x_mask | t.z_mask) != 2) return 0;
+    if (t.z_mask == 0) return 1;
+    if (t.x_mask == t.z_mask) return 2;
+    return t.x_mask == 0 ? 3 : 0;
+}
+
+// exp(-i (a X_pX_q + b Y_pY_q + c Z_pZ_q)) for a run of such terms on one
+// qubit pair (they commute, so their product is exact in any order): with
+// C = CNOT(p -> q), C X_pX_q C = X_p, C Y_pY_q C = -X_p Z_q and C Z_pZ_q C = Z_q,
+// and CZ X_p CZ = X_p Z_q, the run is C RZ_q(2c) RX_p(2a) CZ RX_p(-2b) CZ C:
+// 7 gates for the 13 of three separate terms, every rotation uncontrolled (the
+// tile executor's norm-preserving shear forms in complex64).
+static void bond_exp_gates(int p, int q, double a, double b, double c, std::vector<qi_gate> &out) {
+    auto push = [&](int kind, int t, int ctl, double angle) {
+        qi_gate g{};
+        g.kind = kind; g.ntargets = 1; g.targets[0] = (uint64_t)t; g.params[0] = angle;
+        if (ctl >= 0) { g.ncontrols = 1; g.controls[0] = (uint64_t)ctl; }
+        out.push_back(g);
+    };
+    push(QI_GATE_CNOT, q, p, 0.0);
+    if (c != 0.0) push(QI_GATE_RZ, q, -1, 2.0 * c);
+    if (a != 0.0) push(QI_GATE_RX, p, -1, 2.0 * a);
+    if (b != 0.0) {
+        push(QI_GATE_Z, q, p, 0.0);  // CZ
+        push(QI_GATE_RX, p, -1, -2.0 * b);
+        push(QI_GATE_Z, q, p, 0.0);
+    }
+    push(QI_GATE_CNOT, q, p, 0.0);
+}
+
+// One Trotter step's exponentials as gates, terms in list order or reversed
+// (angle f times each coefficient): runs of consecutive XX / YY / ZZ terms on
+// one qubit pair become one bond_exp_gates, every other term pauli_exp_gates.
+static void trotter_step_gates(const qi_pauli_term *terms, size_t nterms, double f, bool reverse,
+                               std::vector<qi_gate> &out) {
+    const auto order = [&](size_t i) { return reverse ? nterms - 1 - i : i; };
+    for (size_t i = 0; i < nterms;) {
+        const qi_pauli_term &t = terms[order(i)];
+        const uint64_t sup = t.x_mask | t.z_mask;
+        size_t j = i;
+        double abc[4] = {0, 0, 0, 0};
+        while (j < nterms) {
+            const qi_pauli_term &u = terms[order(j)];
+            const int k = bond_kind(u);
+            if (!k || (u.x_mask | u.z_mask) != sup) break;
+            abc[k] += f * u.c_re;
+            j++;
+        }
+        if (j > i + 1) {
+            const int p = 63 - __builtin_clzll(sup), q = __builtin_ctzll(sup);
+            bond_exp_gates(p, q, abc[1], abc[2], abc[3], out);
+            i = j;
+        } else {
+            pauli_exp_gates(t, f * t.c_re, out);
+            i++;
+        }
+    }
+}
+
 // States from this size on run Trotter steps through the tiled executor.
 constexpr uint32_t kFusedTrotterQubits = 16;
 
@@ -1018,10 +1079,10 @@ qi_status qi_trotter_evolve(qi_state_t s, const qi_pauli_term *terms, size_t nte
     if (fused) {
         std::vector<qi_gate> step_gates;
         if (order == 1) {
-            for (size_t k = 0; k < nterms; k++) pauli_exp_gates(terms[k], dt * terms[k].c_re, step_gates);
+            trotter_step_gates(terms, nterms, dt, false, step_gates);
         } else {
-            for (size_t k = 0; k < nterms; k++) pauli_exp_gates(terms[k], 0.5 * dt * terms[k].c_re, step_gates);
-            for (size_t k = nterms; k-- > 0;) pauli_exp_gates(terms[k], 0.5 * dt * terms[k].c_re, step_gates);
+            trotter_step_gates(terms, nterms, 0.5 * dt, false, step_gates);
+            trotter_step_gates(terms, nterms, 0.5 * dt, true, step_gates);
         }
         qi_exec_opts o{};
         o.mode = QI_EXEC_TILED;

@@ -491,10 +491,10 @@ def test_trotter_lattice_matches_oracle(ctx, order):
 @pytest.mark.parametrize("shape,order", [((4, 4), 1), ((4, 4), 2), ((4, 5), 1)])
 def test_trotter_fused_steps_match_oracle(ctx, shape, order):
     """From 16 qubits a Trotter step runs as one gate list through the tiled
-    executor (per term: Y factors turned to X, CNOTs and CZs gathering the
-    string onto one X, RX there and the gathering undone; Z strings a CNOT
-    ladder and RZ): the same unitary as the per-term cos/sin form up to
-    rounding."""
+    executor (the XX, YY, ZZ terms of a bond as one 7-gate block; other terms:
+    Y factors turned to X, CNOTs and CZs gathering the string onto one X, RX
+    there and the gathering undone; Z strings a CNOT ladder and RZ): the same
+    unitary as the per-term cos/sin form up to rounding."""
     terms, xs = W.trotter_2d(*shape, seed=43)
     n = shape[0] * shape[1]
     psi = O.init(n, 0)
@@ -512,6 +512,11 @@ def test_trotter_fused_steps_match_oracle(ctx, shape, order):
     d = float(np.max(np.abs(got - ref)))
     f = fidelity(got, ref)
     assert d <= 1e-4 and f >= 1 - 1e-7, (shape, order, d, 1 - f)
+    # every rotation of the fused step runs in a norm-preserving form: a
+    # rotation with rounded cos / sin entries (det != 1) would drift the norm
+    # by ~4e-8 per application, about 1e-5 here
+    drift = abs(float(np.vdot(got, got).real) - 1.0)
+    assert drift <= 5e-6, (shape, order, drift)
 
 
 @pytest.mark.parametrize("order", [1, 2])
@@ -531,6 +536,33 @@ def test_trotter_fused_mixed_pauli_strings_match_oracle(ctx, order):
     s = upload(psi, "c128")
     got = qiron.trotter_evolve_state(_sumop_from_arr(terms), s, 0.05, 3, order).state_vector
     check(got, ref, "c128", f"trotter mixed strings order {order}")
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_trotter_fused_bond_runs_match_oracle(ctx, order):
+    """Runs of XX / YY / ZZ terms on one qubit pair (every subset, repeated
+    terms, XX and YY with equal coefficients) between other strings, 16
+    qubits, against the oracle's term-by-term product."""
+    rng = np.random.default_rng(600 + order)
+    n = 16
+    ts = []
+    subsets = [(1,), (2,), (3,), (1, 2), (1, 3), (2, 3), (1, 2, 3), (1, 1, 2), (3, 1, 2), (1, 2)]
+    for i, sub in enumerate(subsets):
+        p, q = (int(x) for x in rng.choice(n, size=2, replace=False))
+        b = (1 << p) | (1 << q)
+        same = i == len(subsets) - 1  # XX and YY with equal coefficients
+        c0 = float(rng.normal())
+        for k in sub:
+            c = c0 if same else float(rng.normal())
+            ts.append({1: (b, 0, c, 0.0), 2: (b, b, c, 0.0), 3: (0, b, c, 0.0)}[k])
+        r = int(rng.integers(n))
+        ts.append((1 << r, 0, float(rng.normal()), 0.0))  # an X term between runs
+    terms = term_array(ts)
+    psi = random_state(n, rng)
+    ref = O.trotter(psi, terms, 0.07, 3, order)
+    s = upload(psi, "c128")
+    got = qiron.trotter_evolve_state(_sumop_from_arr(terms), s, 0.07, 3, order).state_vector
+    check(got, ref, "c128", f"trotter bond runs order {order}")
 
 
 def _sumop_from_arr(arr):
