@@ -186,3 +186,18 @@ def test_one_wave_passes_on_general_circuits_match_the_oracle(ctx, prec):
     b.set_executor("wave")
     e3 = b.run(use_graph=True)
     assert np.max(np.abs(e3 - ref2)) <= tol
+    # new term coefficients (complex) without replanning: the grouped term
+    # table keeps its order, only the signed coefficients change
+    crng = np.random.default_rng(93)
+    terms2 = []
+    for t in terms:
+        t2 = t.copy()
+        t2["c_re"] = crng.normal(size=t.size)
+        t2["c_im"] = crng.normal(size=t.size) * (crng.random(t.size) < 0.5)
+        terms2.append(t2)
+    b.set_term_coeffs(np.concatenate([t2["c_re"] + 1j * t2["c_im"] for t2 in terms2]))
+    ref4 = oracle_energies([(n, g, t2) for (n, g, _), t2 in zip(circs2, terms2)])
+    for executor in ("wave", "lds"):
+        b.set_executor(executor)
+        e4 = b.run(use_graph=True)
+        assert np.max(np.abs(e4 - ref4)) <= tol, (executor, np.max(np.abs(e4 - ref4)))
