@@ -147,16 +147,18 @@ def test_out_of_place_plan_with_final_permutation(emu, tmp_path, prec):
     assert np.max(np.abs(got - O.apply_gates(psi, gates))) <= 1e-10
 
 
-SAN_CASES = [("c64", 12, 6, False), ("c128", 11, 6, False), ("c64", 13, 9, False), ("c64", 10, 6, True), ("c128", 10, 6, True)]
+SAN_CASES = [("c64", 12, 6, False, 15), ("c128", 11, 6, False, 15), ("c64", 13, 9, False, 15),
+             ("c64", 10, 6, True, 15), ("c128", 10, 6, True, 15), ("c64", 12, 6, True, 20)]
 
 
-@pytest.mark.parametrize("prec,K,TB,oop", SAN_CASES, ids=[f"{p}-K{k}-T{t}{'-oop' if o else ''}" for p, k, t, o in SAN_CASES])
-def test_planner_under_sanitizers(emu_san, tmp_path, prec, K, TB, oop):
+@pytest.mark.parametrize("prec,K,TB,oop,n", SAN_CASES,
+                         ids=[f"{p}-K{k}-T{t}{'-oop' if o else ''}-n{n}" for p, k, t, o, n in SAN_CASES])
+def test_planner_under_sanitizers(emu_san, tmp_path, prec, K, TB, oop, n):
     """Plans of a random circuit with SWAP relabelling, controlled gates and
-    matchgates (one-wave and LDS tiles, in place and out of place) run with
+    matchgates (one-wave and LDS tiles, in place and out of place; at 20
+    qubits with the tile qubit search and its pair exchanges) run with
     AddressSanitizer and UndefinedBehaviorSanitizer and still match the
     oracle."""
-    n = 15
     rng = np.random.default_rng(77 + K + TB)
     gates = np.concatenate([W.random_circuit(n=n, depth=6, seed=5 + K),
                             gates_array([random_gate(n, rng, max_controls=2) for _ in range(60)])])
