@@ -103,7 +103,11 @@ __global__ __launch_bounds__(256) void k_batch_ptab(const double *__restrict__ c
     const double *k = coef + 4 * (size_t)i;
     R *o = out + (size_t)kParamRow * i;
     o[0] = R(k[0]); o[1] = R(-k[1]); o[2] = R(k[1]); o[3] = R(k[2]);
-    o[4] = R(k[3]); o[5] = R(1); o[6] = R(0); o[7] = R(0);
+    o[4] = R(k[3]);
+    // the rotation by psi = t/2 as shears: psi moved by pi into [-pi/2, pi/2]
+    // (cos psi >= 0), a = -tan(psi/2) = -sin psi / (1 + cos psi), b = sin psi
+    const double cr = k[0] < 0.0 ? -k[0] : k[0], sr = k[0] < 0.0 ? -k[1] : k[1];
+    o[5] = R(-sr / (1.0 + cr)); o[6] = R(sr); o[7] = R(0);
 }
 
 __device__ __forceinline__ void resolve(const TileOp &op, const double *__restrict__ coef, int slot_base, double *c) {
@@ -733,7 +737,9 @@ template <typename T> static Status upload_vec(DevBuf &b, const std::vector<T> &
 // LDS gates) keep LDS tiles. This engine returns energies only, which no
 // global phase changes, so an uncontrolled parametric RZ(theta) =
 // e^{-i theta/2} diag(1, e^{i theta}) runs as the one-sided phase of P(theta)
-// (the target-1 half only).
+// (the target-1 half only), and in complex64 an uncontrolled parametric RY on
+// a register bit as three shears (RY(theta) = -RY(theta -+ 2 pi) keeps the
+// shear coefficients bounded).
 static void plan_wave(Batch::Program &P, int prec) {
     P.wave = 0;
     P.wpasses.clear();
@@ -755,8 +761,13 @@ static void plan_wave(Batch::Program &P, int prec) {
     if (!plan_tiled(shape, lw, K, kLaneBits, plan, false).ok() || plan.steps.empty()) return;
     for (const ExecStep &st : plan.steps)
         if (st.type != ExecStep::REG || st.reg.oop || st.reg.permute || st.reg.tb != kLaneBits || st.reg.K != K) return;
-    for (RegOp &op : plan.reg_ops)
-        if (op.param > 0 && op.kind == RO_DG && !op.rcm && !op.tcm && !op.gcm) op.kind = RO_P1;
+    for (RegOp &op : plan.reg_ops) {
+        const bool free = !op.rcm && !op.tcm && !op.gcm;  // uncontrolled: a sign or phase of it is global
+        if (op.param > 0 && op.kind == RO_DG && free) op.kind = RO_P1;
+        // complex64: an uncontrolled RY on a register bit as three shears (the
+        // rotation by pi moved out of its angle is a global sign)
+        if (op.param > 0 && op.kind == RO_RR && op.tgt < kLaneTarget && free && prec == QI_C64) op.kind = RO_RRS;
+    }
     for (const ExecStep &st : plan.steps) P.wpasses.push_back(st.reg);
     P.wops = std::move(plan.reg_ops);
     P.wave = 1;

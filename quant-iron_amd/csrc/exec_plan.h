@@ -173,7 +173,9 @@ Status plan_lds_passes(const State &s, const std::vector<Lowered> &gates, int K,
 // qubits): one launch runs pass p of many circuits, one tile per wave.
 constexpr int kBatchWaveK64 = 12, kBatchWaveK128 = 11;  // tile qubits (complex64 / complex128)
 // Values per angle slot in the parametric-coefficient table the kernel reads
-// (the kernel's precision): cos(t/2), -sin(t/2), sin(t/2), cos t, sin t, 1, 0, 0.
+// (the kernel's precision): cos(t/2), -sin(t/2), sin(t/2), cos t, sin t, then
+// the shears (a, b) of the rotation by t/2 brought into [-pi/2, pi/2] (a sign,
+// a global phase, dropped: for uncontrolled RY whose sign no result sees), 0.
 constexpr int kParamRow = 8;
 struct WaveTask {
     uint64_t off;       // element offset of the circuit's state in the arena

@@ -1046,7 +1046,8 @@ __global__ __launch_bounds__(64) void k_wave_pass(const typename V2<R>::T *src_p
 // Coefficients of a parametric op from its slot's row of the table
 // (kParamRow values per angle slot, in the kernel's precision), in the forms
 // param_op_coeffs writes: RY (c, -s, s, c), RX (c, -s, -s, c), RZ (c, -s, c, s)
-// with c, s = cos, sin of half the angle, P (1, 0, cos, sin) of the angle.
+// with c, s = cos, sin of half the angle, P (1, 0, cos, sin) of the angle; and
+// RY as shears (RO_RRS, complex64: the row's shear pair).
 template <typename R> __device__ __forceinline__ void set_coef(RegOp &op, int q, R x) {
     if constexpr (sizeof(R) == 4) op.cf[q] = x;
     else op.c[q] = x;
@@ -1057,7 +1058,8 @@ template <typename R> __device__ __forceinline__ void param_coeffs(RegOp &op, co
     case RO_RR: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, sh); set_coef<R>(op, 3, ch); break;
     case RO_RXI: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, nsh); set_coef<R>(op, 3, ch); break;
     case RO_DG: set_coef<R>(op, 0, ch); set_coef<R>(op, 1, nsh); set_coef<R>(op, 2, ch); set_coef<R>(op, 3, sh); break;
-    default: set_coef<R>(op, 0, t[5]); set_coef<R>(op, 1, t[6]); set_coef<R>(op, 2, t[3]); set_coef<R>(op, 3, t[4]); break;
+    case RO_RRS: set_coef<R>(op, 0, t[5]); set_coef<R>(op, 1, t[6]); break;
+    default: set_coef<R>(op, 0, R(1)); set_coef<R>(op, 1, R(0)); set_coef<R>(op, 2, t[3]); set_coef<R>(op, 3, t[4]); break;
     }
 }
 
