@@ -201,3 +201,27 @@ def test_one_wave_passes_on_general_circuits_match_the_oracle(ctx, prec):
         b.set_executor(executor)
         e4 = b.run(use_graph=True)
         assert np.max(np.abs(e4 - ref4)) <= tol, (executor, np.max(np.abs(e4 - ref4)))
+
+
+def test_full_size_ansatz_circuits_match_the_per_gate_path(ctx):
+    """Ansatz circuits at the batch's largest sizes (20..22 qubits, 8 layers:
+    one-wave plans with the tile qubit search), in both precisions, against
+    the per-gate complex128 path and the SumOp expectation (each checked
+    against the oracle at smaller sizes): 1e-11 in complex128, 1e-5 in
+    complex64."""
+    from qiron import _lib as L
+    specs = [(n, 8, np.random.default_rng(40 + n).uniform(0, 2 * np.pi, 2 * n * 8), (0.7, 1.1, 1.3))
+             for n in (20, 21, 22)]
+    ref = []
+    for n, layers, th, j in specs:
+        s = qiron.State.new_zero(n, "c128")
+        s.apply_gates_inplace(W.ansatz_circuit(n, layers, th), mode=L.EXEC_PER_GATE)
+        ref.append(qiron.expectation_terms(s, W.heisenberg_open_1d(n, *j)))
+        del s
+    ref = np.array(ref)
+    for prec, tol in (("c128", 1e-11), ("c64", 1e-5)):
+        b = qiron.CircuitBatch(prec)
+        build(b, specs)
+        assert b.executor() == "wave"
+        e = b.run(use_graph=True)
+        assert np.max(np.abs(e - ref)) <= tol, (prec, np.max(np.abs(e - ref)))
